@@ -1,0 +1,284 @@
+// Test infrastructure only: a driver that runs the reference solver
+// (ssadz/EnSCDynamics, compiled from its own sources by oracle/Makefile)
+// on one .inp deck and writes what it computes as golden vectors.
+//
+// Nothing in the engine links or loads this.  It exists so the oracle
+// restatement and the reader can be checked against the real reference.
+//
+// The reference writes VTU files through VTK, which is not available here,
+// so DataOut (Explicit3D/include/dataOut.h:44-97) is given empty bodies.
+// Stepping is observed by subclassing exDyna3D and overriding the virtual
+// computeSate() (Explicit3D/include/exDyna3D.h:75): the base step runs
+// unchanged, then the state is recorded.
+#include "exDyna3D.h"
+#include "dataIn.h"
+#include "dataOut.h"
+#include "spdlog/spdlog.h"
+
+#include <cstdio>
+#include <cstdint>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+#include <fstream>
+#include <sstream>
+#include <stdexcept>
+#include <unistd.h>
+
+namespace EnSC {
+DataOut::DataOut(const Types::VerticesAll<3>& v, const std::vector<Element_HexN8>& e, Types::Real t)
+    : opVerticesAll(v), opTriaHex(e), time(t) {}
+void DataOut::add_node_data(DataOutType::DataType, const Eigen::Matrix<Types::Real, Eigen::Dynamic, 1>*, std::string) {}
+void DataOut::add_ele_data(DataOutType::DataType, const Eigen::Matrix<Types::Real, Eigen::Dynamic, 1>*, std::string) {}
+void DataOut::write_vtu(const std::string&) {}
+void DataOut::write_vtu_virtualParticles(const unsigned int&, const FSI_share_data&) const {}
+}  // namespace EnSC
+
+namespace {
+
+// Minimal .npy (v1.0) writer: little-endian, C order.
+void write_npy(const std::string& path, const char* descr, size_t elsize,
+               const std::vector<size_t>& shape, const void* data) {
+    std::string shp = "(";
+    size_t n = 1;
+    for (size_t i = 0; i < shape.size(); ++i) {
+        shp += std::to_string(shape[i]);
+        shp += (shape.size() == 1) ? "," : (i + 1 < shape.size() ? ", " : "");
+        n *= shape[i];
+    }
+    shp += ")";
+    std::string hdr = std::string("{'descr': '") + descr + "', 'fortran_order': False, 'shape': " + shp + ", }";
+    size_t total = 10 + hdr.size() + 1;
+    size_t pad = (64 - total % 64) % 64;
+    hdr += std::string(pad, ' ');
+    hdr += '\n';
+    FILE* f = std::fopen(path.c_str(), "wb");
+    if (!f) throw std::runtime_error("cannot write " + path);
+    const unsigned char magic[8] = {0x93, 'N', 'U', 'M', 'P', 'Y', 1, 0};
+    std::fwrite(magic, 1, 8, f);
+    uint16_t hl = (uint16_t)hdr.size();
+    std::fwrite(&hl, 2, 1, f);
+    std::fwrite(hdr.data(), 1, hdr.size(), f);
+    if (n) std::fwrite(data, elsize, n, f);
+    std::fclose(f);
+}
+
+void npy_f64(const std::string& p, const std::vector<size_t>& s, const double* d) { write_npy(p, "<f8", 8, s, d); }
+void npy_u32(const std::string& p, const std::vector<size_t>& s, const uint32_t* d) { write_npy(p, "<u4", 4, s, d); }
+
+std::string jstr(const std::string& s) {
+    std::string o = "\"";
+    for (char c : s) {
+        if (c == '"' || c == '\\') o += '\\';
+        o += c;
+    }
+    return o + "\"";
+}
+
+std::string jnum(double v) {
+    char b[64];
+    std::snprintf(b, sizeof b, "%.17g", v);
+    return b;
+}
+
+struct StopRun {};
+
+}  // namespace
+
+class Probe : public EnSC::exDyna3D {
+public:
+    std::string out;
+    long max_steps = -1;
+    std::vector<long> snap_steps;
+    std::vector<uint32_t> probe_nodes;
+    long nstep = 0;
+    std::vector<double> hist;  // per step: time, dt_i, dt_i_1, min dtStable, then 3 u per probe node
+
+    size_t nn() const { return vertices.size(); }
+
+    void dump_model() {
+        const size_t Nn = vertices.size(), Ne = hexahedron_elements.size();
+        std::vector<double> x0(3 * Nn);
+        for (size_t i = 0; i < Nn; ++i)
+            for (int d = 0; d < 3; ++d) x0[3 * i + d] = vertices[i][d];
+        npy_f64(out + "/x0.npy", {Nn, 3}, x0.data());
+        std::vector<uint32_t> conn(8 * Ne);
+        for (size_t e = 0; e < Ne; ++e) {
+            const auto& vi = hexahedron_elements[e].get_verticesIndex();
+            for (int k = 0; k < 8; ++k) conn[8 * e + k] = vi[k];
+        }
+        npy_u32(out + "/conn.npy", {Ne, 8}, conn.data());
+
+        std::ofstream j(out + "/model.json");
+        j << "{\n";
+        j << "\"E\": " << jnum(mMatElastic.E) << ", \"nu\": " << jnum(mMatElastic.v) << ", \"rho\": " << jnum(mMatElastic.rho) << ",\n";
+        j << "\"G\": " << jnum(mMatElastic.G) << ", \"K\": " << jnum(mMatElastic.K) << ", \"lambda\": " << jnum(mMatElastic.lambda)
+          << ", \"WOS\": " << jnum(mMatElastic.WOS) << ",\n";
+        j << "\"totalTime\": " << jnum(totalTime) << ", \"Cvisl\": " << jnum(Cvisl) << ", \"Cvisq\": " << jnum(Cvisq) << ",\n";
+        j << "\"c_dr\": " << jnum(c_dr) << ", \"c_cr\": " << jnum(c_cr) << ", \"factor\": " << jnum(factor_timeStep) << ",\n";
+        j << "\"n_nodes\": " << Nn << ", \"n_elems\": " << Ne << ",\n";
+        j << "\"ini_vel\": [";
+        for (size_t i = 0; i < ini_vel_generation.size(); ++i) {
+            const auto& iv = ini_vel_generation[i];
+            j << (i ? ", " : "") << "[" << jstr(iv.first) << ", " << iv.second.first << ", " << jnum(iv.second.second) << "]";
+        }
+        j << "],\n\"node_sets\": {";
+        bool first = true;
+        for (const auto& kv : map_set_node_list) {
+            j << (first ? "" : ", ") << jstr(kv.first) << ": [";
+            for (size_t i = 0; i < kv.second.size(); ++i) j << (i ? ", " : "") << kv.second[i];
+            j << "]";
+            first = false;
+        }
+        j << "},\n\"amplitudes\": {";
+        first = true;
+        for (const auto& kv : map_amp_list) {
+            j << (first ? "" : ", ") << jstr(kv.first) << ": [";
+            for (size_t i = 0; i < kv.second.size(); ++i)
+                j << (i ? ", " : "") << "[" << jnum(kv.second[i][0]) << ", " << jnum(kv.second[i][1]) << "]";
+            j << "]";
+            first = false;
+        }
+        j << "},\n\"steps\": [\n";
+        for (size_t s = 0; s < steps.size(); ++s) {
+            const auto& st = steps[s];
+            j << (s ? ",\n" : "") << "{\"name\": " << jstr(st.name) << ", \"period\": " << jnum(st.timePeriod)
+              << ", \"reset_spc\": " << (st.resetSpcBoundary ? 1 : 0) << ", \"reset_vel\": " << (st.resetVelBoundary ? 1 : 0)
+              << ", \"reset_dload\": " << (st.resetDload ? 1 : 0) << ", \"reset_dsload\": " << (st.resetDsload ? 1 : 0);
+            j << ", \"spc\": [";
+            for (size_t i = 0; i < st.boundary.spc_nodes.size(); ++i) {
+                const auto& b = st.boundary.spc_nodes[i];
+                j << (i ? ", " : "") << "[" << jstr(b.first) << ", " << b.second[0] << ", " << b.second[1] << ", " << b.second[2] << "]";
+            }
+            j << "], \"vel\": [";
+            for (size_t i = 0; i < st.boundary.vel_nodes.size(); ++i) {
+                const auto& b = st.boundary.vel_nodes[i];
+                j << (i ? ", " : "") << "[" << jstr(b.first) << ", " << b.second.first[0] << ", " << b.second.first[1] << ", "
+                  << jnum(b.second.second) << "]";
+            }
+            const auto& g = st.gravity;
+            j << "], \"gravity\": [" << (std::get<0>(g) ? 1 : 0) << ", " << jstr(std::get<1>(g)) << ", " << jnum(std::get<2>(g)) << ", "
+              << jnum(std::get<3>(g)) << ", " << jnum(std::get<4>(g)) << ", " << jnum(std::get<5>(g)) << "]";
+            j << ", \"n_dsload\": " << st.dsload.size() << "}";
+        }
+        j << "\n]\n}\n";
+    }
+
+    void dump_setup() {
+        // State right after init(): B0, V0, inverse lumped mass, the init
+        // pass's dtStable and min edge lengths.
+        const size_t Nn = vertices.size(), Ne = hexahedron_elements.size();
+        std::vector<double> b0(24 * Ne);
+        for (size_t e = 0; e < Ne; ++e)
+            for (int a = 0; a < 8; ++a)
+                for (int d = 0; d < 3; ++d) b0[24 * e + 3 * a + d] = all_BT[e](a, d);
+        npy_f64(out + "/B0.npy", {Ne, 8, 3}, b0.data());
+        npy_f64(out + "/V0.npy", {Ne}, all_Volume.data());
+        std::vector<double> im(Nn);
+        for (size_t i = 0; i < Nn; ++i) im[i] = inv_mass_matrix[3 * i];
+        npy_f64(out + "/invM.npy", {Nn}, im.data());
+        npy_f64(out + "/dtStable0.npy", {Ne}, dtStable.data());
+        npy_f64(out + "/Length0.npy", {Ne}, all_Length.data());
+        npy_f64(out + "/v_init.npy", {Nn, 3}, solution_v.data());
+    }
+
+    void dump_state(const std::string& tag) {
+        const size_t Nn = vertices.size(), Ne = hexahedron_elements.size();
+        npy_f64(out + "/" + tag + "_u.npy", {Nn, 3}, solution_u.data());
+        npy_f64(out + "/" + tag + "_v.npy", {Nn, 3}, solution_v.data());
+        npy_f64(out + "/" + tag + "_rf.npy", {Nn, 3}, solution_rf.data());
+        std::vector<double> x(3 * Nn);
+        for (size_t i = 0; i < Nn; ++i)
+            for (int d = 0; d < 3; ++d) x[3 * i + d] = vertices[i][d];
+        npy_f64(out + "/" + tag + "_x.npy", {Nn, 3}, x.data());
+        std::vector<double> s(6 * Ne), q(12 * Ne);
+        for (size_t e = 0; e < Ne; ++e) {
+            for (int k = 0; k < 6; ++k) s[6 * e + k] = all_sigma[e][k];
+            for (int k = 0; k < 12; ++k) q[12 * e + k] = all_QIA[e][k];
+        }
+        npy_f64(out + "/" + tag + "_sigma.npy", {Ne, 6}, s.data());
+        npy_f64(out + "/" + tag + "_qia.npy", {Ne, 12}, q.data());
+        npy_f64(out + "/" + tag + "_dtStable.npy", {Ne}, dtStable.data());
+        npy_f64(out + "/" + tag + "_Length.npy", {Ne}, all_Length.data());
+        double sc[5] = {time, dt_i, dt_i_1, (double)nstep, (double)currentStepIndex};
+        npy_f64(out + "/" + tag + "_scalars.npy", {5}, sc);
+    }
+
+    void record() {
+        hist.push_back(time);
+        hist.push_back(dt_i);
+        hist.push_back(dt_i_1);
+        hist.push_back(dtStable.minCoeff());
+        for (uint32_t n : probe_nodes)
+            for (int d = 0; d < 3; ++d) hist.push_back(solution_u[3 * (size_t)n + d]);
+    }
+
+    void computeSate() override {
+        EnSC::exDyna3D::computeSate();
+        ++nstep;
+        record();
+        for (long s : snap_steps)
+            if (s == nstep) dump_state("step" + std::to_string(nstep));
+        if (max_steps > 0 && nstep >= max_steps) throw StopRun();
+    }
+
+    void finish() {
+        size_t w = 4 + 3 * probe_nodes.size();
+        npy_f64(out + "/history.npy", {hist.size() / w, w}, hist.data());
+        dump_state("final");
+    }
+
+    void do_init() { init(); }
+};
+
+static std::vector<long> parse_list(const char* s) {
+    std::vector<long> v;
+    std::stringstream ss(s);
+    std::string t;
+    while (std::getline(ss, t, ','))
+        if (!t.empty()) v.push_back(std::stol(t));
+    return v;
+}
+
+int main(int argc, char** argv) {
+    // usage: ref_harness WORKDIR DECK OUTDIR [max_steps] [snap,steps] [probe,nodes] [c_dr c_cr factor]
+    if (argc < 4) {
+        std::fprintf(stderr, "usage: %s WORKDIR DECK OUTDIR [max_steps] [snaps] [probes] [c_dr c_cr factor]\n", argv[0]);
+        return 2;
+    }
+    std::string work = argv[1], deck = argv[2], out = argv[3];
+    spdlog::set_level(spdlog::level::warn);
+    if (chdir(work.c_str()) != 0) {
+        std::perror("chdir");
+        return 2;
+    }
+    {
+        // project.txt is read from the working directory (exDyna3D.cpp:705-712).
+        std::ofstream p("project.txt");
+        p << "#deck\n" << deck << "\n";
+        if (argc >= 10) {
+            p << "#\n#c_dr\n" << argv[7] << "\n#\n#c_cr\n" << argv[8] << "\n#\n#factor\n" << argv[9] << "\n#\n#time_interval\n1.0\n";
+        }
+    }
+    Probe sim;
+    sim.out = out;
+    if (argc >= 5) sim.max_steps = std::atol(argv[4]);
+    if (argc >= 6) sim.snap_steps = parse_list(argv[5]);
+    if (argc >= 7)
+        for (long n : parse_list(argv[6])) sim.probe_nodes.push_back((uint32_t)n);
+    EnSC::DataIn reader(sim);
+    reader.read_inp(deck);
+    sim.do_init();
+    // After init(): read_project_txt has set c_dr, c_cr and factor_timeStep
+    // (exDyna3D.cpp:698-772); node coordinates have not moved yet.
+    sim.dump_model();
+    sim.dump_setup();
+    try {
+        sim.run();
+    } catch (const StopRun&) {
+    }
+    sim.finish();
+    std::printf("steps %ld\n", sim.nstep);
+    return 0;
+}
