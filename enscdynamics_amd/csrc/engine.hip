@@ -1,0 +1,541 @@
+// Host side of the step engine and its C ABI (include/ensc_engine.h).
+//
+// Replaces, for the hot path, the reference's exDyna3D::init / run /
+// setCurrentStep / computeSate (Explicit3D/source/exDyna3D.cpp:54-205,
+// 492-515, 642-696, 774-793, 1760-1940).  Setup quantities (B0, V0, lumped
+// mass) are computed once on the host in FP64 with the reference's
+// arithmetic; everything per step runs on the device (kernels.h).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <memory>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "engine_host.h"
+
+namespace ensc {
+
+thread_local std::string g_last_error;
+
+ensc_status fail(ensc_status s, const std::string& msg) {
+    g_last_error = msg;
+    return s;
+}
+
+#define HIPCHK(x)                                                                                    \
+    do {                                                                                             \
+        hipError_t e_ = (x);                                                                         \
+        if (e_ != hipSuccess) return fail(ENSC_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+// setup arithmetic (FP64, reference order)
+// ---------------------------------------------------------------------------
+
+static double det3h(const double m[3][3]) {
+    const double h0 = m[0][0] * (m[1][1] * m[2][2] - m[1][2] * m[2][1]);
+    const double h1 = m[0][1] * (m[1][0] * m[2][2] - m[1][2] * m[2][0]);
+    const double h2 = m[0][2] * (m[1][0] * m[2][1] - m[1][1] * m[2][0]);
+    return h0 - h1 + h2;
+}
+static double cofh(const double m[3][3], int i, int j) {
+    const int i1 = (i + 1) % 3, i2 = (i + 2) % 3, j1 = (j + 1) % 3, j2 = (j + 2) % 3;
+    return m[i1][j1] * m[i2][j2] - m[i1][j2] * m[i2][j1];
+}
+static void inv3h(const double m[3][3], double r[3][3]) {
+    const double c00 = cofh(m, 0, 0), c10 = cofh(m, 1, 0), c20 = cofh(m, 2, 0);
+    const double det = c00 * m[0][0] + c10 * m[1][0] + c20 * m[2][0];
+    const double invdet = 1.0 / det;
+    r[1][0] = cofh(m, 0, 1) * invdet;
+    r[1][1] = cofh(m, 1, 1) * invdet;
+    r[2][0] = cofh(m, 0, 2) * invdet;
+    r[1][2] = cofh(m, 2, 1) * invdet;
+    r[2][1] = cofh(m, 1, 2) * invdet;
+    r[2][2] = cofh(m, 2, 2) * invdet;
+    r[0][0] = c00 * invdet;
+    r[0][1] = c10 * invdet;
+    r[0][2] = c20 * invdet;
+}
+
+// FEValues<8,3>::reinit at the single Gauss point (fevalues.h:233-265) and
+// compute_invMassMatrix (exDyna3D.cpp:492-515), original element order.
+static bool setup_geometry(const std::vector<double>& x0, const std::vector<uint32_t>& conn, size_t Ne, double rho,
+                           std::vector<double>& B0, std::vector<double>& V0, std::vector<double>& invM, size_t Nn,
+                           std::string& err) {
+    static const double cs[8][3] = {{-1, -1, -1}, {1, -1, -1}, {1, 1, -1}, {-1, 1, -1},
+                                    {-1, -1, 1},  {1, -1, 1},  {1, 1, 1},  {-1, 1, 1}};
+    double dN[3][8];
+    for (int a = 0; a < 8; ++a)
+        for (int d = 0; d < 3; ++d) dN[d][a] = 1.0 / 8.0 * cs[a][d];
+    B0.assign(24 * Ne, 0.0);
+    V0.assign(Ne, 0.0);
+    std::vector<double> mass(Nn, 0.0);
+    for (size_t e = 0; e < Ne; ++e) {
+        double X[8][3];
+        for (int k = 0; k < 8; ++k)
+            for (int d = 0; d < 3; ++d) X[k][d] = x0[3 * (size_t)conn[8 * e + k] + d];
+        double Jm[3][3];
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) {
+                double s = dN[i][0] * X[0][j];
+                for (int k = 1; k < 8; ++k) s += dN[i][k] * X[k][j];
+                Jm[i][j] = s;
+            }
+        double Ji[3][3];
+        inv3h(Jm, Ji);
+        const double det = det3h(Jm);
+        V0[e] = det * 8.0;
+        if (!(V0[e] > 0.0)) {
+            err = "element " + std::to_string(e) + " has non-positive reference volume (inverted or degenerate)";
+            return false;
+        }
+        for (int k = 0; k < 8; ++k)
+            for (int i = 0; i < 3; ++i) {
+                double s = Ji[i][0] * dN[0][k];
+                s += Ji[i][1] * dN[1][k];
+                s += Ji[i][2] * dN[2][k];
+                B0[24 * e + 3 * k + i] = s;
+            }
+        const double nodeMass = rho * V0[e] / 8.0;
+        for (int k = 0; k < 8; ++k) mass[conn[8 * e + k]] += nodeMass;
+    }
+    invM.resize(Nn);
+    for (size_t i = 0; i < Nn; ++i) invM[i] = 1.0 / mass[i];
+    return true;
+}
+
+// Space-filling (Morton) element order from element centroids; nodes are
+// then numbered in first-touch order of the reordered elements.
+static void morton_order(const std::vector<double>& x0, const std::vector<uint32_t>& conn, size_t Ne, size_t Nn,
+                         std::vector<uint32_t>& e_new2old, std::vector<uint32_t>& n_new2old) {
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    for (size_t i = 0; i < Nn; ++i)
+        for (int d = 0; d < 3; ++d) {
+            lo[d] = std::min(lo[d], x0[3 * i + d]);
+            hi[d] = std::max(hi[d], x0[3 * i + d]);
+        }
+    auto spread = [](uint64_t v) {
+        v &= 0x1fffff;
+        v = (v | v << 32) & 0x1f00000000ffffULL;
+        v = (v | v << 16) & 0x1f0000ff0000ffULL;
+        v = (v | v << 8) & 0x100f00f00f00f00fULL;
+        v = (v | v << 4) & 0x10c30c30c30c30c3ULL;
+        v = (v | v << 2) & 0x1249249249249249ULL;
+        return v;
+    };
+    std::vector<std::pair<uint64_t, uint32_t>> key(Ne);
+    for (size_t e = 0; e < Ne; ++e) {
+        double c[3] = {0, 0, 0};
+        for (int k = 0; k < 8; ++k)
+            for (int d = 0; d < 3; ++d) c[d] += x0[3 * (size_t)conn[8 * e + k] + d];
+        uint64_t q[3];
+        for (int d = 0; d < 3; ++d) {
+            const double span = hi[d] - lo[d];
+            const double t = span > 0 ? (c[d] / 8.0 - lo[d]) / span : 0.0;
+            q[d] = (uint64_t)std::min(2097151.0, std::max(0.0, t * 2097151.0));
+        }
+        key[e] = {spread(q[0]) | spread(q[1]) << 1 | spread(q[2]) << 2, (uint32_t)e};
+    }
+    std::sort(key.begin(), key.end());
+    e_new2old.resize(Ne);
+    for (size_t i = 0; i < Ne; ++i) e_new2old[i] = key[i].second;
+    std::vector<uint32_t> seen(Nn, 0xFFFFFFFFu);
+    n_new2old.clear();
+    n_new2old.reserve(Nn);
+    for (size_t i = 0; i < Ne; ++i)
+        for (int k = 0; k < 8; ++k) {
+            const uint32_t n = conn[8 * (size_t)e_new2old[i] + k];
+            if (seen[n] == 0xFFFFFFFFu) {
+                seen[n] = 1;
+                n_new2old.push_back(n);
+            }
+        }
+    for (size_t n = 0; n < Nn; ++n)
+        if (seen[n] == 0xFFFFFFFFu) n_new2old.push_back((uint32_t)n);  // nodes in no element
+}
+
+// ---------------------------------------------------------------------------
+// model copy + per-job device state
+// ---------------------------------------------------------------------------
+
+size_t round_up(size_t v, size_t m) { return (v + m - 1) / m * m; }
+
+bool copy_model(const ensc_model_desc* m, ModelCopy& c, std::string& err) {
+    if (!m || !m->x0 || !m->conn || m->n_nodes == 0 || m->n_elems == 0) {
+        err = "model has no nodes or elements";
+        return false;
+    }
+    if (m->n_steps < 2 || !m->steps) {
+        err = "model needs the Initial step and at least one *Step";
+        return false;
+    }
+    c.Nn = m->n_nodes;
+    c.Ne = m->n_elems;
+    c.x0.assign(m->x0, m->x0 + 3 * (size_t)c.Nn);
+    c.conn.assign(m->conn, m->conn + 8 * (size_t)c.Ne);
+    for (uint32_t v : c.conn)
+        if (v >= c.Nn) {
+            err = "connectivity references node " + std::to_string(v) + " >= n_nodes";
+            return false;
+        }
+    for (double v : c.x0)
+        if (!std::isfinite(v)) {
+            err = "non-finite node coordinate";
+            return false;
+        }
+    c.v_init.assign(3 * (size_t)c.Nn, 0.0);
+    if (m->v_init) c.v_init.assign(m->v_init, m->v_init + 3 * (size_t)c.Nn);
+    for (double v : c.v_init)
+        if (!std::isfinite(v)) {
+            err = "non-finite initial velocity";
+            return false;
+        }
+    c.E = m->E;
+    c.nu = m->nu;
+    c.rho = m->rho;
+    if (!(c.rho > 0) || !(c.E > 0) || !(c.nu > -1.0 && c.nu < 0.5)) {
+        err = "invalid material (need rho > 0, E > 0, -1 < nu < 0.5)";
+        return false;
+    }
+    c.plastic = m->plastic;
+    c.yield0 = m->yield0;
+    c.hard = m->hard;
+    c.c_dr = m->c_dr;
+    c.c_cr = m->c_cr;
+    c.factor = m->factor;
+    c.cvisl = m->cvisl;
+    c.cvisq = m->cvisq;
+    c.total_time = m->total_time;
+    c.steps.resize(m->n_steps);
+    for (uint32_t s = 0; s < m->n_steps; ++s) {
+        const ensc_step_def& sd = m->steps[s];
+        StepCopy& sc = c.steps[s];
+        sc.period = sd.period;
+        sc.reset_spc = sd.reset_spc;
+        sc.reset_vel = sd.reset_vel;
+        sc.reset_dload = sd.reset_dload;
+        sc.grav_on = sd.grav_on;
+        sc.grav_value = sd.grav_value;
+        sc.grav_amp = sd.grav_amp;
+        for (int d = 0; d < 3; ++d) sc.grav_dir[d] = sd.grav_dir[d];
+        auto cp = [&](const ensc_bc_entry* b, uint32_t n, std::vector<BcCopy>& out) -> bool {
+            for (uint32_t i = 0; i < n; ++i) {
+                BcCopy bc;
+                bc.nodes.assign(b[i].nodes, b[i].nodes + b[i].n_nodes);
+                for (uint32_t v : bc.nodes)
+                    if (v >= c.Nn) {
+                        err = "BC references node " + std::to_string(v) + " >= n_nodes";
+                        return false;
+                    }
+                if (b[i].d0 < 0 || b[i].d1 > 2) {
+                    err = "BC dof range outside 0..2";
+                    return false;
+                }
+                bc.d0 = b[i].d0;
+                bc.d1 = b[i].d1;
+                bc.cond = b[i].cond;
+                bc.value = b[i].value;
+                bc.amp = b[i].amp;
+                out.push_back(std::move(bc));
+            }
+            return true;
+        };
+        if (!cp(sd.spc, sd.n_spc, sc.spc) || !cp(sd.vel, sd.n_vel, sc.vel)) return false;
+    }
+    c.amp_off.push_back(0);
+    for (uint32_t i = 0; i < m->n_amps; ++i) {
+        for (uint32_t q = 0; q < m->amps[i].n; ++q) {
+            c.amp_t.push_back(m->amps[i].t[q]);
+            c.amp_a.push_back(m->amps[i].a[q]);
+        }
+        c.amp_off.push_back((int32_t)c.amp_t.size());
+    }
+    // Material_elastic::update (Material_elastic.cpp:9-14)
+    c.G = c.E / ((1.0 + c.nu) * 2.0);
+    c.K = c.E / (1.0 - 2.0 * c.nu) / 3.0;
+    c.lambda = 2.0 * c.G * c.nu / (1.0 - 2.0 * c.nu);
+    c.wos = std::sqrt((c.lambda + 2.0 * c.G) / c.rho);
+    return true;
+}
+
+template <class T>
+static ensc_status upload(T** dst, const std::vector<T>& src, size_t n_alloc, DeviceArena& arena) {
+    const size_t bytes = std::max<size_t>(n_alloc, 1) * sizeof(T);
+    void* p = arena.alloc(bytes);
+    if (!p) return fail(ENSC_ERR_MEM, "device allocation of " + std::to_string(bytes) + " bytes failed");
+    *dst = static_cast<T*>(p);
+    HIPCHK(hipMemset(p, 0, bytes));
+    if (!src.empty()) HIPCHK(hipMemcpy(p, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice));
+    return ENSC_OK;
+}
+
+void* DeviceArena::alloc(size_t bytes) {
+    void* p = nullptr;
+    if (hipMalloc(&p, round_up(bytes, 256)) != hipSuccess) return nullptr;
+    ptrs.push_back(p);
+    total += round_up(bytes, 256);
+    return p;
+}
+DeviceArena::~DeviceArena() {
+    for (void* p : ptrs) (void)hipFree(p);
+}
+
+// Builds one job's device state.  The job's ctl/partials pointers are filled
+// in by the engine (they live in engine-wide arrays).
+ensc_status build_job(Job& J, int precision, int reorder, int blk_e, int blk_n) {
+    const ModelCopy& M = J.model;
+    const size_t Ne = M.Ne, Nn = M.Nn;
+    const bool mixed = precision == ENSC_MIXED;
+    std::string err;
+    std::vector<double> B0, V0, invM;
+    if (!setup_geometry(M.x0, M.conn, Ne, M.rho, B0, V0, invM, Nn, err)) return fail(ENSC_ERR_ARG, err);
+    // numbering
+    if (reorder) {
+        morton_order(M.x0, M.conn, Ne, Nn, J.e_new2old, J.n_new2old);
+    } else {
+        J.e_new2old.resize(Ne);
+        J.n_new2old.resize(Nn);
+        std::iota(J.e_new2old.begin(), J.e_new2old.end(), 0u);
+        std::iota(J.n_new2old.begin(), J.n_new2old.end(), 0u);
+    }
+    J.n_old2new.assign(Nn, 0);
+    for (size_t i = 0; i < Nn; ++i) J.n_old2new[J.n_new2old[i]] = (uint32_t)i;
+    J.e_old2new.assign(Ne, 0);
+    for (size_t i = 0; i < Ne; ++i) J.e_old2new[J.e_new2old[i]] = (uint32_t)i;
+    const size_t es = round_up(Ne, 256), ns = round_up(Nn, 256);
+    J.ne_s = es;
+    J.nn_s = ns;
+    // element SoA arrays in the new numbering
+    std::vector<uint32_t> conn_d(8 * es, 0);
+    for (size_t i = 0; i < Ne; ++i) {
+        const size_t eo = J.e_new2old[i];
+        for (int k = 0; k < 8; ++k) conn_d[(size_t)k * es + i] = J.n_old2new[M.conn[8 * eo + k]];
+    }
+    // ELL gather table: per node the (new element, corner) slots sorted by
+    // (corner, original element id) - the reference's summation order.
+    std::vector<uint32_t> cnt(Nn, 0);
+    for (size_t e = 0; e < Ne; ++e)
+        for (int k = 0; k < 8; ++k) cnt[M.conn[8 * e + k]]++;
+    uint32_t W = 0;
+    for (uint32_t v : cnt) W = std::max(W, v);
+    J.ell_w = W;
+    std::vector<uint32_t> ell((size_t)W * ns, ELL_EMPTY);
+    std::vector<uint32_t> fillc(Nn, 0);
+    for (int k = 0; k < 8; ++k)
+        for (size_t eo = 0; eo < Ne; ++eo) {
+            const uint32_t no = M.conn[8 * eo + k];
+            const uint32_t nn_ = J.n_old2new[no];
+            ell[(size_t)fillc[no]++ * ns + nn_] = (J.e_old2new[eo] << 3) | (uint32_t)k;
+        }
+    if (Ne >= (1u << 29)) return fail(ENSC_ERR_ARG, "too many elements for the 29-bit gather slot");
+    // node records
+    std::vector<double> rec((size_t)Nn * NODE_REC, 0.0);
+    for (size_t i = 0; i < Nn; ++i) {
+        const size_t o = J.n_new2old[i];
+        for (int d = 0; d < 3; ++d) {
+            rec[i * NODE_REC + d] = M.x0[3 * o + d];
+            rec[i * NODE_REC + 3 + d] = M.v_init[3 * o + d];
+        }
+        rec[i * NODE_REC + 9] = invM[o];
+    }
+    // initial kinetic energy
+    double ke0 = 0.0;
+    for (size_t n = 0; n < Nn; ++n) {
+        double k2 = 0.0;
+        for (int d = 0; d < 3; ++d) k2 += M.v_init[3 * n + d] * M.v_init[3 * n + d];
+        ke0 += 0.5 * k2 / invM[n];
+    }
+    J.ke0 = ke0;
+    DeviceArena& A = J.arena;
+    JobDev& D = J.dev;
+    std::memset(&D, 0, sizeof(D));
+    ensc_status st;
+    uint32_t* d_conn;
+    if ((st = upload(&d_conn, conn_d, conn_d.size(), A))) return st;
+    D.conn = d_conn;
+    if (mixed) {
+        std::vector<float> b(24 * es, 0.f), v(es, 0.f);
+        for (size_t i = 0; i < Ne; ++i) {
+            const size_t eo = J.e_new2old[i];
+            for (int q = 0; q < 24; ++q) b[(size_t)q * es + i] = (float)B0[24 * eo + q];
+            v[i] = (float)V0[eo];
+        }
+        float *db, *dv, *ds, *dq, *dp, *df, *dx;
+        if ((st = upload(&db, b, b.size(), A))) return st;
+        if ((st = upload(&dv, v, v.size(), A))) return st;
+        std::vector<float> none;
+        if ((st = upload(&ds, none, 6 * es, A))) return st;
+        if ((st = upload(&dq, none, 12 * es, A))) return st;
+        if ((st = upload(&dp, none, es, A))) return st;
+        if ((st = upload(&df, none, 8 * es * 4, A))) return st;
+        if ((st = upload(&dx, none, Nn * 4, A))) return st;
+        D.B0 = db;
+        D.V0 = dv;
+        D.sigma = ds;
+        D.qia = dq;
+        D.epsp = dp;
+        D.fel = df;
+        D.xcomp = dx;
+    } else {
+        std::vector<double> b(24 * es, 0.0), v(es, 0.0);
+        for (size_t i = 0; i < Ne; ++i) {
+            const size_t eo = J.e_new2old[i];
+            for (int q = 0; q < 24; ++q) b[(size_t)q * es + i] = B0[24 * eo + q];
+            v[i] = V0[eo];
+        }
+        double *db, *dv, *ds, *dq, *dp, *df;
+        if ((st = upload(&db, b, b.size(), A))) return st;
+        if ((st = upload(&dv, v, v.size(), A))) return st;
+        std::vector<double> none;
+        if ((st = upload(&ds, none, 6 * es, A))) return st;
+        if ((st = upload(&dq, none, 12 * es, A))) return st;
+        if ((st = upload(&dp, none, es, A))) return st;
+        if ((st = upload(&df, none, 8 * es * 4, A))) return st;
+        D.B0 = db;
+        D.V0 = dv;
+        D.sigma = ds;
+        D.qia = dq;
+        D.epsp = dp;
+        D.fel = df;
+        D.xcomp = nullptr;
+    }
+    double* d_node;
+    if ((st = upload(&d_node, rec, rec.size(), A))) return st;
+    D.node = d_node;
+    std::vector<double> none_d;
+    if ((st = upload(&D.rf, none_d, 3 * Nn, A))) return st;
+    std::vector<uint8_t> none_u8;
+    uint8_t* d_mask;
+    if ((st = upload(&d_mask, none_u8, Nn, A))) return st;
+    D.bcmask = d_mask;
+    J.d_bcmask = d_mask;
+    double* d_bcval;
+    if ((st = upload(&d_bcval, none_d, 3 * Nn, A))) return st;
+    D.bcval = d_bcval;
+    J.d_bcval = d_bcval;
+    std::vector<int16_t> none_i16;
+    int16_t* d_bcamp;
+    if ((st = upload(&d_bcamp, none_i16, 3 * Nn, A))) return st;
+    D.bcamp = d_bcamp;
+    J.d_bcamp = d_bcamp;
+    uint32_t* d_ell;
+    if ((st = upload(&d_ell, ell, ell.size(), A))) return st;
+    D.ell = d_ell;
+    double *d_at, *d_aa;
+    int32_t* d_ao;
+    if ((st = upload(&d_at, M.amp_t, M.amp_t.size(), A))) return st;
+    if ((st = upload(&d_aa, M.amp_a, M.amp_a.size(), A))) return st;
+    if ((st = upload(&d_ao, M.amp_off, M.amp_off.size(), A))) return st;
+    D.amp_t = d_at;
+    D.amp_a = d_aa;
+    D.amp_off = d_ao;
+    D.n_amp = (int32_t)M.amp_off.size() - 1;
+    D.lambda = M.lambda;
+    D.G = M.G;
+    D.K = M.K;
+    D.rho = M.rho;
+    D.wos = M.wos;
+    D.c_dr = M.c_dr;
+    D.c_cr = M.c_cr;
+    D.cvisl = M.cvisl;
+    D.cvisq = M.cvisq;
+    D.factor = M.factor;
+    D.yield0 = M.yield0;
+    D.hard = M.hard;
+    D.plastic = M.plastic;
+    D.ell_w = (int32_t)W;
+    D.ne = (uint32_t)Ne;
+    D.nn = (uint32_t)Nn;
+    D.ne_s = (uint32_t)es;
+    D.nn_s = (uint32_t)ns;
+    D.nblk_e = (uint32_t)((Ne + blk_e - 1) / blk_e);
+    D.nblk_n = (uint32_t)((Nn + blk_n - 1) / blk_n);
+    D.grav_amp = -1;
+    // dyn_bc: any velocity BC anywhere with an amplitude
+    for (const auto& s : M.steps)
+        for (const auto& b : s.vel)
+            if (b.amp >= 0) D.dyn_bc = 1;
+    J.grav.resize(M.steps.size());
+    for (size_t s = 0; s < M.steps.size(); ++s) {
+        J.grav[s].on = M.steps[s].grav_on;
+        J.grav[s].amp = M.steps[s].grav_amp;
+        J.grav[s].g = M.steps[s].grav_value;
+        for (int d = 0; d < 3; ++d) J.grav[s].dir[d] = M.steps[s].grav_dir[d];
+    }
+    return ENSC_OK;
+}
+
+// setCurrentStep's BC inheritance (exDyna3D.cpp:1760-1908), on the job's
+// host copy; returns false when the reference would return early.
+bool job_set_step(Job& J, uint32_t si) {
+    const ModelCopy& M = J.model;
+    if (si >= M.steps.size()) return false;
+    if (si == J.cur_step) return false;
+    if (si > 0) {
+        J.prev_spc = J.cur_spc;
+        J.prev_vel = J.cur_vel;
+    }
+    J.cur_step = si;
+    const StepCopy& st = M.steps[si];
+    auto ids = [](uint32_t s, const std::vector<BcCopy>& v, std::vector<BcRef>& out) {
+        for (uint32_t i = 0; i < v.size(); ++i) out.push_back({s, i});
+    };
+    J.cur_spc.clear();
+    J.cur_vel.clear();
+    if (si == 0) {
+        ids(0, st.spc, J.cur_spc);
+        ids(0, st.vel, J.cur_vel);
+    } else {
+        ids(si, st.spc, J.cur_spc);
+        if (!st.reset_spc) {
+            if (!J.prev_spc.empty())
+                J.cur_spc.insert(J.cur_spc.end(), J.prev_spc.begin(), J.prev_spc.end());
+            else
+                ids(si - 1, M.steps[si - 1].spc, J.cur_spc);
+        }
+        ids(si, st.vel, J.cur_vel);
+        if (!st.reset_vel) J.cur_vel.insert(J.cur_vel.end(), J.prev_vel.begin(), J.prev_vel.end());
+        if (!st.reset_dload && !J.grav[si].on && J.grav[si - 1].on) J.grav[si] = J.grav[si - 1];
+    }
+    return true;
+}
+
+// Per-node BC arrays for the job's current BC lists (new numbering).
+void job_bc_arrays(const Job& J, std::vector<uint8_t>& mask, std::vector<double>& val, std::vector<int16_t>& amp,
+                   std::vector<uint32_t>& nodes) {
+    const ModelCopy& M = J.model;
+    mask.assign(M.Nn, 0);
+    val.assign(3 * (size_t)M.Nn, 0.0);
+    amp.assign(3 * (size_t)M.Nn, (int16_t)-1);
+    for (const BcRef& r : J.cur_vel) {
+        const BcCopy& b = M.steps[r.step].vel[r.idx];
+        for (uint32_t no : b.nodes) {
+            const uint32_t n = J.n_old2new[no];
+            for (int d = b.d0; d <= b.d1; ++d) {
+                mask[n] |= (uint8_t)(BC_VEL0 << d);
+                val[3 * (size_t)n + d] = b.value;  // later lines overwrite, as sequential application would
+                amp[3 * (size_t)n + d] = (int16_t)b.amp;
+            }
+        }
+    }
+    for (const BcRef& r : J.cur_spc) {
+        const BcCopy& b = M.steps[r.step].spc[r.idx];
+        if (b.cond != 1) continue;
+        for (uint32_t no : b.nodes) {
+            const uint32_t n = J.n_old2new[no];
+            for (int d = b.d0; d <= b.d1; ++d) mask[n] |= (uint8_t)(BC_SPC0 << d);
+        }
+    }
+    nodes.clear();
+    for (uint32_t n = 0; n < M.Nn; ++n)
+        if (mask[n]) nodes.push_back(n);
+}
+
+}  // namespace ensc

@@ -1,0 +1,69 @@
+// Device-side data structures shared by the engine's kernels and host code.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace ensc {
+
+// What the element kernel did in the current step iteration ("tick").
+enum : int32_t { KIND_NONE = 0, KIND_STEP = 1, KIND_PREPASS = 2, KIND_INIT = 3 };
+
+// BC mask bits per node: bit d = velocity BC on dof d, bit 3+d = SPC on dof d.
+constexpr uint8_t BC_VEL0 = 1u, BC_SPC0 = 8u;
+constexpr uint32_t ELL_EMPTY = 0xFFFFFFFFu;
+constexpr int NODE_REC = 10;  // x0 x1 x2 v0 v1 v2 u0 u1 u2 invM (doubles)
+constexpr int MAX_PROBE = 8;
+
+// Per-job scalar state, resident on the device.  The time-step scalars are
+// the reference's (exDyna3D.h:175-179); dts_min is dtStable.minCoeff() of
+// the last element pass (exDyna3D.cpp:677).
+struct Ctl {
+    double time, dt_i, dt_i_1, step_end, total_time, dts_min, lmin_pre;
+    double ke, ie, hg, vis, wext, ke0;
+    double grav_amp;
+    long long nsteps;
+    long long step_limit;  // no step starts once nsteps reaches it
+    int32_t kind, have_lpre, init_pending, step_index;
+    uint32_t ticket_e, ticket_n, err, pad;
+};
+
+struct JobDev {
+    // element arrays, SoA with stride ne_s (precision-dependent element type)
+    const uint32_t* conn;  // [8][ne_s]
+    const void* B0;        // [24][ne_s]  B0(k,j) at row 3k+j
+    const void* V0;        // [ne_s]
+    void* sigma;           // [6][ne_s]
+    void* qia;             // [12][ne_s]
+    void* epsp;            // [ne_s]
+    void* fel;             // [8][ne_s] x (double4 | float4): element nodal forces
+    // node arrays
+    double* node;          // [nn][NODE_REC]
+    float* xcomp;          // [nn][4] compensation of x (mixed) or null
+    double* rf;            // [nn][3]
+    const uint8_t* bcmask; // [nn]
+    const double* bcval;   // [nn][3]
+    const int16_t* bcamp;  // [nn][3]
+    const uint32_t* ell;   // [ell_w][nn_s] slots (e << 3 | corner), sorted by (corner, original element)
+    // amplitude tables, packed: table i is amp_t/amp_a[amp_off[i] .. amp_off[i+1])
+    const double* amp_t;
+    const double* amp_a;
+    const int32_t* amp_off;
+    int32_t n_amp;
+    // gravity of the current step (StepData.h:22)
+    int32_t grav_on, grav_amp;
+    double grav_g, grav_dir[3];
+    // material and parameters (Material_elastic.cpp:9-14; exDyna3D.h:179-188)
+    double lambda, G, K, rho, wos, c_dr, c_cr, cvisl, cvisq, factor, yield0, hard;
+    int32_t plastic, dyn_bc, ell_w, n_probe;
+    uint32_t ne, nn, ne_s, nn_s;
+    uint32_t eblk0, nblk_e, nblk0, nblk_n;
+    uint32_t probe[MAX_PROBE];
+    double* hist;
+    long long hist_cap;
+    Ctl* ctl;
+    double* part_e;  // [nblk_e][8]: min L, min dtStable, internal, hourglass, viscous work
+    double* part_n;  // [nblk_n][2]: kinetic energy, external power
+};
+
+}  // namespace ensc

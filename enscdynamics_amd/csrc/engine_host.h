@@ -1,0 +1,90 @@
+// Host-side structures of the step engine.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/ensc_engine.h"
+#include "engine_dev.h"
+
+namespace ensc {
+
+extern thread_local std::string g_last_error;
+ensc_status fail(ensc_status s, const std::string& msg);
+size_t round_up(size_t v, size_t m);
+
+struct BcCopy {
+    std::vector<uint32_t> nodes;  // original numbering
+    int32_t d0 = 0, d1 = 0, cond = 1, amp = -1;
+    double value = 0.0;
+};
+
+struct StepCopy {
+    double period = 0.0;
+    int32_t reset_spc = 0, reset_vel = 0, reset_dload = 0, grav_on = 0, grav_amp = -1;
+    double grav_value = 0.0, grav_dir[3] = {0, 0, 0};
+    std::vector<BcCopy> spc, vel;
+};
+
+// The caller's model, copied (include/ensc_model.h).
+struct ModelCopy {
+    uint32_t Nn = 0, Ne = 0;
+    std::vector<double> x0, v_init;
+    std::vector<uint32_t> conn;
+    double E = 0, nu = 0, rho = 0, G = 0, K = 0, lambda = 0, wos = 0;
+    int32_t plastic = 0;
+    double yield0 = 0, hard = 0, c_dr = 0, c_cr = 0, factor = 0, cvisl = 0, cvisq = 0, total_time = 0;
+    std::vector<StepCopy> steps;
+    std::vector<double> amp_t, amp_a;
+    std::vector<int32_t> amp_off;
+};
+
+struct BcRef {
+    uint32_t step, idx;
+};
+
+struct GravState {
+    int32_t on = 0, amp = -1;
+    double g = 0.0, dir[3] = {0, 0, 0};
+};
+
+struct DeviceArena {
+    std::vector<void*> ptrs;
+    size_t total = 0;
+    void* alloc(size_t bytes);
+    ~DeviceArena();
+};
+
+struct Job {
+    ModelCopy model;
+    JobDev dev{};
+    DeviceArena arena;
+    std::vector<uint32_t> e_new2old, e_old2new, n_new2old, n_old2new;
+    size_t ne_s = 0, nn_s = 0;
+    uint32_t ell_w = 0;
+    double ke0 = 0.0;
+    // setCurrentStep state (exDyna3D.h:215-220)
+    uint32_t cur_step = 0;
+    std::vector<BcRef> cur_spc, cur_vel, prev_spc, prev_vel;
+    std::vector<GravState> grav;
+    uint8_t* d_bcmask = nullptr;
+    double* d_bcval = nullptr;
+    int16_t* d_bcamp = nullptr;
+    uint32_t* d_bcnodes = nullptr;
+    size_t d_bcnodes_cap = 0;
+    std::vector<uint32_t> probes;
+    double* d_hist = nullptr;
+    long long hist_cap = 0;
+    bool inited = false;
+};
+
+bool copy_model(const ensc_model_desc* m, ModelCopy& c, std::string& err);
+ensc_status build_job(Job& J, int precision, int reorder, int blk_e, int blk_n);
+bool job_set_step(Job& J, uint32_t si);
+void job_bc_arrays(const Job& J, std::vector<uint8_t>& mask, std::vector<double>& val, std::vector<int16_t>& amp,
+                   std::vector<uint32_t>& nodes);
+
+}  // namespace ensc

@@ -1,0 +1,564 @@
+// HIP kernels of the step engine (gfx950, wave64).
+//
+// One step iteration of the reference loop (exDyna3D.cpp:86-100) is one
+// "tick" = two launches over every live job:
+//   k_element : one element per lane.  Gathers node x/u/v, runs the element
+//               body (elem_body.inc), writes the element's 8 nodal forces,
+//               and reduces min edge / min stable dt / work terms per block.
+//               The last block of each job (atomic ticket) reduces the block
+//               partials in a fixed order and advances the job's time-step
+//               scalars exactly as update_minVertex_perMaterial_and_dt and
+//               the run-loop clamps do (exDyna3D.cpp:663-686, 90-97, 775-780).
+//   k_node    : one node per lane.  Sums the node's element forces in the
+//               canonical (corner, element) order, adds gravity, applies the
+//               BCs and reactions (2029-2145), and does the central-difference
+//               update a -> v -> u -> x (642-696).
+// The step size of a tick depends on the min edge of the current
+// coordinates, which the element pass itself computes.  Only the overshoot
+// clamp near the end time (682-685) needs that value before the pass; a tick
+// that can hit it (decided from the previous pass's bound) runs a min-edge
+// pre-pass instead and the next tick does the step.
+#pragma once
+
+#include "engine_dev.h"
+#include "../../include/ensc_engine.h"
+
+namespace ensc {
+
+// Rows of the reference's hourglass base vectors (exDyna3D.cpp:1003-1008).
+__device__ constexpr double TAU_[4][8] = {{1., -1., 1., -1., 1., -1., 1., -1.},
+                                          {-1., -1., 1., 1., 1., 1., -1., -1.},
+                                          {-1., 1., 1., -1., 1., -1., -1., 1.},
+                                          {-1., 1., -1., 1., 1., -1., 1., -1.}};
+
+template <class R>
+struct Vec4;
+template <>
+struct Vec4<double> {
+    typedef double4 T;
+};
+template <>
+struct Vec4<float> {
+    typedef float4 T;
+};
+
+// Eigen 3.4's 3x3 determinant (Determinant.h:19-24, 56-65).
+template <class R>
+__device__ __forceinline__ R det3(const R m[3][3]) {
+#pragma clang fp contract(off)
+    const R h0 = m[0][0] * (m[1][1] * m[2][2] - m[1][2] * m[2][1]);
+    const R h1 = m[0][1] * (m[1][0] * m[2][2] - m[1][2] * m[2][0]);
+    const R h2 = m[0][2] * (m[1][0] * m[2][1] - m[1][1] * m[2][0]);
+    return h0 - h1 + h2;
+}
+
+template <class R>
+__device__ __forceinline__ R cof3(const R m[3][3], int i, int j) {
+#pragma clang fp contract(off)
+    const int i1 = (i + 1) % 3, i2 = (i + 2) % 3, j1 = (j + 1) % 3, j2 = (j + 2) % 3;
+    return m[i1][j1] * m[i2][j2] - m[i1][j2] * m[i2][j1];
+}
+
+// Eigen 3.4's 3x3 cofactor inverse (InverseImpl.h:127-176).
+template <class R>
+__device__ __forceinline__ void inverse3(const R m[3][3], R r[3][3]) {
+#pragma clang fp contract(off)
+    const R c00 = cof3<R>(m, 0, 0), c10 = cof3<R>(m, 1, 0), c20 = cof3<R>(m, 2, 0);
+    const R det = c00 * m[0][0] + c10 * m[1][0] + c20 * m[2][0];
+    const R invdet = R(1) / det;
+    r[1][0] = cof3<R>(m, 0, 1) * invdet;
+    r[1][1] = cof3<R>(m, 1, 1) * invdet;
+    r[2][0] = cof3<R>(m, 0, 2) * invdet;
+    r[1][2] = cof3<R>(m, 2, 1) * invdet;
+    r[2][1] = cof3<R>(m, 1, 2) * invdet;
+    r[2][2] = cof3<R>(m, 2, 2) * invdet;
+    r[0][0] = c00 * invdet;
+    r[0][1] = c10 * invdet;
+    r[0][2] = c20 * invdet;
+}
+
+// Velocity-BC amplitude (extension): piecewise linear in total time, held
+// constant outside the table.
+__device__ inline double amp_value(const JobDev& J, int id, double t) {
+#pragma clang fp contract(off)
+    if (id < 0 || id >= J.n_amp) return 1.0;
+    const int b = J.amp_off[id], n = J.amp_off[id + 1] - b;
+    const double* T = J.amp_t + b;
+    const double* A = J.amp_a + b;
+    if (n <= 0) return 1.0;
+    if (t <= T[0]) return A[0];
+    if (t >= T[n - 1]) return A[n - 1];
+    int i = 1;
+    while (i < n && T[i] < t) ++i;
+    const double w = (t - T[i - 1]) / (T[i] - T[i - 1]);
+    return A[i - 1] + w * (A[i] - A[i - 1]);
+}
+
+// exDyna3D::deal_amp (exDyna3D.cpp:1222-1278): trapezoid mean of the
+// amplitude over [time, time + dt].  Tables of fewer than 2 points give 1.
+__device__ inline double deal_amp(const JobDev& J, int id, double time, double dt) {
+#pragma clang fp contract(off)
+    if (id < 0 || id >= J.n_amp) return 1.0;
+    const int b = J.amp_off[id], n = J.amp_off[id + 1] - b;
+    const double* T = J.amp_t + b;
+    const double* A = J.amp_a + b;
+    if (n < 2) return 1.0;
+    double y = 1.0;
+    for (int i = 0; i + 1 < n; ++i) {
+        const double x1 = T[i], y1 = A[i], x2 = T[i + 1], y2 = A[i + 1];
+        if (time > x2) break;
+        const double m = (y2 - y1) / (x2 - x1);
+        const double c = y1 - m * x1;
+        const double xm = time + dt / 2.0;
+        if ((xm + dt / 2.0) < x2) {
+            y = m * xm + c;
+            break;
+        }
+        const double DT1 = x2 - time;
+        const double X = time + DT1 / 2.0;
+        const double Y = m * X + c;
+        const double A1 = Y * DT1;
+        double A2 = 0.0;
+        for (int j = 2; j + i < n; ++j) {
+            const double xj = T[i + j], yj = A[i + j], xj_1 = T[i + j - 1], yj_1 = A[i + j - 1];
+            if ((xm + dt / 2.0) > xj) {
+                A2 += (xj - xj_1) * (yj + yj_1) / 2.0;
+            } else {
+                const double M = (yj - yj_1) / (xj - xj_1);
+                const double C = yj_1 - m * xj_1;
+                const double Y1 = M * (xm + dt / 2.0) + C;
+                A2 += (xm + dt / 2.0 - xj_1) * (yj_1 + Y1) / 2.0;
+                break;
+            }
+        }
+        const double x_end = T[n - 1];
+        if (xm + dt / 2.0 > x_end) A2 += (xm + dt / 2.0 - x_end) * 1.0;
+        y = (A1 + A2) / dt;
+        break;
+    }
+    return y;
+}
+
+struct Decision {
+    int32_t kind;
+    double dt_i_1;  // the increment the element pass integrates with
+    double time_n;  // time after this step's increment
+};
+
+// What this tick does for one job, from its scalars at tick start.  Every
+// block of the job evaluates it identically.
+__device__ inline Decision decide(const Ctl& c, double factor, double wos) {
+#pragma clang fp contract(off)
+    Decision d{KIND_NONE, 0.0, c.time};
+    if (c.init_pending) {
+        d.kind = KIND_INIT;
+        return d;
+    }
+    if (!(c.time < c.step_end) || c.nsteps >= c.step_limit) return d;
+    double dt1 = c.dt_i;
+    const double rem = c.total_time - (c.time + dt1);
+    // dt_i <= dtStable_min * factor always, so the overshoot clamp of
+    // exDyna3D.cpp:682-685 can only fire inside this band.
+    const bool band = (rem > 0.0) && (rem < c.dts_min * factor);
+    if (band && !c.have_lpre) {
+        d.kind = KIND_PREPASS;
+        return d;
+    }
+    if (band) {
+        double dt = c.lmin_pre / wos;
+        const double tmp = c.dts_min;
+        dt = dt < tmp ? dt : tmp;
+        dt *= factor;
+        if (rem < dt && rem > 0.0) dt1 = rem;
+    }
+    if (c.time + dt1 > c.step_end) dt1 = c.step_end - c.time;
+    d.kind = KIND_STEP;
+    d.dt_i_1 = dt1;
+    double t = c.time + dt1;
+    if (t > c.total_time) t = c.total_time;
+    d.time_n = t;
+    return d;
+}
+
+template <class T>
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+template <class T>
+__device__ __forceinline__ T wave_min(T v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const T w = __shfl_xor(v, o, 64);
+        v = w < v ? w : v;
+    }
+    return v;
+}
+
+// Element body instantiations.
+__device__ __forceinline__ void element_fp64(const JobDev& J, uint32_t e, double dt, int32_t kind, double time_n,
+                                           double& Le, double& dts, double* en, uint32_t& errf) {
+#pragma clang fp contract(off)
+    typedef double R;
+    constexpr bool MIXED = false;
+#include "elem_body.inc"
+}
+
+__device__ __forceinline__ void element_mixed(const JobDev& J, uint32_t e, double dt, int32_t kind, double time_n,
+                                            double& Le, double& dts, double* en, uint32_t& errf) {
+#pragma clang fp contract(fast)
+    typedef float R;
+    constexpr bool MIXED = true;
+#include "elem_body.inc"
+}
+
+// Fixed-order reduction of a job's block partials by the job's last block.
+template <int BLK, int NV>
+__device__ inline void reduce_partials(const double* part, uint32_t nblk, const bool* is_min, double* out,
+                                       double* lds /* [NV][BLK] */) {
+    const int t = threadIdx.x;
+    double acc[NV];
+#pragma unroll
+    for (int q = 0; q < NV; ++q) acc[q] = is_min[q] ? 1.0 / 0.0 : 0.0;
+    for (uint32_t b = t; b < nblk; b += BLK) {
+#pragma unroll
+        for (int q = 0; q < NV; ++q) {
+            const double v = __builtin_nontemporal_load(part + (size_t)b * 8 + q);
+            acc[q] = is_min[q] ? (v < acc[q] ? v : acc[q]) : acc[q] + v;
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < NV; ++q) lds[q * BLK + t] = acc[q];
+    __syncthreads();
+    for (int s = BLK / 2; s > 0; s >>= 1) {
+        if (t < s) {
+#pragma unroll
+            for (int q = 0; q < NV; ++q) {
+                const double a = lds[q * BLK + t], b = lds[q * BLK + t + s];
+                lds[q * BLK + t] = is_min[q] ? (b < a ? b : a) : a + b;
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int q = 0; q < NV; ++q) out[q] = lds[q * BLK];
+}
+
+template <bool MIXED, int BLK>
+__global__ __launch_bounds__(BLK) void k_element(const JobDev* __restrict__ jobs, const uint32_t* __restrict__ blk2job) {
+#pragma clang fp contract(off)
+    const uint32_t b = blockIdx.x;
+    const JobDev& J = jobs[blk2job[b]];
+    Ctl* ctl = J.ctl;
+    const Decision dc = decide(*ctl, J.factor, J.wos);
+    if (dc.kind == KIND_NONE) return;
+    const uint32_t lb = b - J.eblk0;
+    const uint32_t e = lb * BLK + threadIdx.x;
+    double Le = 1.0 / 0.0, dts = 1.0 / 0.0;
+    double en[3] = {0.0, 0.0, 0.0};
+    uint32_t errf = 0;
+    if (e < J.ne) {
+        if (MIXED)
+            element_mixed(J, e, dc.dt_i_1, dc.kind, dc.time_n, Le, dts, en, errf);
+        else
+            element_fp64(J, e, dc.dt_i_1, dc.kind, dc.time_n, Le, dts, en, errf);
+    }
+    // block partials: wave shuffles, then LDS across the block's waves
+    __shared__ double s_red[BLK / 64][5];
+    __shared__ uint32_t s_err;
+    __shared__ int s_last;
+    if (threadIdx.x == 0) s_err = 0;
+    Le = wave_min(Le);
+    dts = wave_min(dts);
+    en[0] = wave_sum(en[0]);
+    en[1] = wave_sum(en[1]);
+    en[2] = wave_sum(en[2]);
+    __syncthreads();
+    if (errf) atomicOr(&s_err, errf);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        s_red[w][0] = Le;
+        s_red[w][1] = dts;
+        s_red[w][2] = en[0];
+        s_red[w][3] = en[1];
+        s_red[w][4] = en[2];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double r0 = s_red[0][0], r1 = s_red[0][1], r2 = s_red[0][2], r3 = s_red[0][3], r4 = s_red[0][4];
+        for (int q = 1; q < BLK / 64; ++q) {
+            r0 = s_red[q][0] < r0 ? s_red[q][0] : r0;
+            r1 = s_red[q][1] < r1 ? s_red[q][1] : r1;
+            r2 += s_red[q][2];
+            r3 += s_red[q][3];
+            r4 += s_red[q][4];
+        }
+        double* p = J.part_e + (size_t)lb * 8;
+        p[0] = r0;
+        p[1] = r1;
+        p[2] = r2;
+        p[3] = r3;
+        p[4] = r4;
+        if (s_err) atomicOr(&ctl->err, s_err);
+        __threadfence();
+        const uint32_t t = atomicAdd(&ctl->ticket_e, 1u);
+        s_last = (t == J.nblk_e - 1);
+    }
+    __syncthreads();
+    if (!s_last) return;
+    __threadfence();
+    __shared__ double s_lds[5 * BLK];
+    const bool is_min[5] = {true, true, false, false, false};
+    double tot[5];
+    reduce_partials<BLK, 5>(J.part_e, J.nblk_e, is_min, tot, s_lds);
+    if (threadIdx.x == 0) {
+        Ctl& c = *ctl;
+        const double Lmin = tot[0], dtsn = tot[1];
+        if (dc.kind == KIND_PREPASS) {
+            c.lmin_pre = Lmin;
+            c.have_lpre = 1;
+            c.kind = KIND_PREPASS;
+        } else if (dc.kind == KIND_INIT) {
+            c.dts_min = dtsn;
+            c.init_pending = 0;
+            c.kind = KIND_INIT;
+        } else {
+            // update_minVertex_perMaterial_and_dt (663-686) + run-loop clamp (90-97)
+            double dt1 = c.dt_i;
+            const double time_prev = c.time + dt1;
+            double dt = Lmin / J.wos;
+            const double tmp = c.dts_min;
+            dt = dt < tmp ? dt : tmp;
+            dt *= J.factor;
+            if ((c.total_time - time_prev) < dt && (c.total_time - time_prev) > 0.0) {
+                dt = c.total_time - time_prev;
+                dt1 = dt;
+            }
+            if (c.time + dt1 > c.step_end) dt = dt1 = c.step_end - c.time;
+            if (dt1 != dc.dt_i_1) atomicOr(&c.err, (uint32_t)ENSC_EF_DT);  // cannot happen: same inputs as decide()
+            // computeSate (775-780)
+            double t = c.time + dt1;
+            if (t > c.total_time) t = c.total_time;
+            c.time = t;
+            c.dt_i = dt;
+            c.dt_i_1 = dt1;
+            c.dts_min = dtsn;
+            c.nsteps += 1;
+            c.have_lpre = 0;
+            c.ie += tot[2];
+            c.hg += tot[3];
+            c.vis += tot[4];
+            c.grav_amp = (J.grav_on && J.grav_amp >= 0) ? deal_amp(J, J.grav_amp, t, dt) : 1.0;
+            if (!(dtsn > 0.0) || !(dt >= 0.0)) atomicOr(&c.err, (uint32_t)ENSC_EF_DT);
+            c.kind = KIND_STEP;
+        }
+        c.ticket_e = 0;
+        __threadfence();
+    }
+}
+
+template <bool MIXED, int BLK>
+__global__ __launch_bounds__(BLK) void k_node(const JobDev* __restrict__ jobs, const uint32_t* __restrict__ blk2job) {
+#pragma clang fp contract(off)
+    const uint32_t b = blockIdx.x;
+    const JobDev& J = jobs[blk2job[b]];
+    Ctl* ctl = J.ctl;
+    if (ctl->kind != KIND_STEP) return;
+    const double dt_i = ctl->dt_i, dt_i_1 = ctl->dt_i_1, time = ctl->time;
+    const uint32_t lb = b - J.nblk0;
+    const uint32_t n = lb * BLK + threadIdx.x;
+    double ke = 0.0, pw = 0.0;
+    if (n < J.nn) {
+        double acc[3] = {0.0, 0.0, 0.0}, grp[3] = {0.0, 0.0, 0.0};
+        int gk = -1;
+        const size_t es = J.ne_s;
+        for (int w = 0; w < J.ell_w; ++w) {
+            const uint32_t sl = J.ell[(size_t)w * J.nn_s + n];
+            if (sl == ELL_EMPTY) break;
+            const int k = (int)(sl & 7u);
+            const size_t idx = (size_t)k * es + (sl >> 3);
+            double f0, f1, f2;
+            if (MIXED) {
+                const float4 q = static_cast<const float4*>(J.fel)[idx];
+                f0 = q.x;
+                f1 = q.y;
+                f2 = q.z;
+            } else {
+                const double4 q = static_cast<const double4*>(J.fel)[idx];
+                f0 = q.x;
+                f1 = q.y;
+                f2 = q.z;
+            }
+            if (k != gk) {
+                if (gk >= 0) {
+                    acc[0] += grp[0];
+                    acc[1] += grp[1];
+                    acc[2] += grp[2];
+                }
+                grp[0] = grp[1] = grp[2] = 0.0;
+                gk = k;
+            }
+            grp[0] += f0;
+            grp[1] += f1;
+            grp[2] += f2;
+        }
+        if (gk >= 0) {
+            acc[0] += grp[0];
+            acc[1] += grp[1];
+            acc[2] += grp[2];
+        }
+        double* rec = J.node + (size_t)n * NODE_REC;
+        double x[3] = {rec[0], rec[1], rec[2]};
+        double v[3] = {rec[3], rec[4], rec[5]};
+        double u[3] = {rec[6], rec[7], rec[8]};
+        const double im = rec[9];
+        double a[3] = {0.0 + acc[0], 0.0 + acc[1], 0.0 + acc[2]};
+        if (J.grav_on) {  // apply_gravity (800-826)
+            const double ag = ctl->grav_amp * J.grav_g;
+            a[0] += ag * J.grav_dir[0] / im;
+            a[1] += ag * J.grav_dir[1] / im;
+            a[2] += ag * J.grav_dir[2] / im;
+        }
+        const uint8_t mk = J.bcmask[n];
+        double vbc[3] = {0.0, 0.0, 0.0};
+        if (mk) {  // apply_boundary_condition_a (2029-2145): velocity lines, then SPC lines
+            double rfv[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                if (mk & (BC_VEL0 << d)) {
+                    vbc[d] = J.bcval[(size_t)n * 3 + d] * amp_value(J, J.bcamp[(size_t)n * 3 + d], time);
+                    pw += -a[d] * vbc[d];
+                    a[d] = 0.0;
+                }
+                if (mk & (BC_SPC0 << d)) {
+                    if (a[d] != 0.0) {
+                        rfv[d] = -a[d];
+                        a[d] = 0.0;
+                    }
+                }
+            }
+            if (mk & (7u * BC_SPC0)) {
+                double* r = J.rf + (size_t)n * 3;
+                r[0] = rfv[0];
+                r[1] = rfv[1];
+                r[2] = rfv[2];
+            }
+        }
+        const double hs = dt_i + dt_i_1;
+        double k2 = 0.0;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const double ad = a[d] * im;  // compute_a (642-644)
+            double vd = v[d];
+            if (mk & (BC_VEL0 << d)) vd = vbc[d];
+            if (mk & (BC_SPC0 << d)) vd = 0.0;
+            vd += hs * ad / 2.0;          // update_velocity (646-649)
+            u[d] += dt_i * vd;            // update_displacement (651-654)
+            const double dx = dt_i * vd;  // move_mesh (688-696)
+            if (MIXED) {
+                // compensated (Kahan) position accumulation
+                float4* xc = reinterpret_cast<float4*>(J.xcomp) + n;
+                const float cf = (&xc->x)[d];
+                const double y = dx - (double)cf;
+                const double t = x[d] + y;
+                (&xc->x)[d] = (float)((t - x[d]) - y);
+                x[d] = t;
+            } else {
+                x[d] += dx;
+            }
+            v[d] = vd;
+            k2 += vd * vd;
+        }
+        ke = 0.5 * k2 / im;
+        rec[0] = x[0];
+        rec[1] = x[1];
+        rec[2] = x[2];
+        rec[3] = v[0];
+        rec[4] = v[1];
+        rec[5] = v[2];
+        rec[6] = u[0];
+        rec[7] = u[1];
+        rec[8] = u[2];
+    }
+    __shared__ double s_red[BLK / 64][2];
+    __shared__ int s_last;
+    ke = wave_sum(ke);
+    pw = wave_sum(pw);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        s_red[w][0] = ke;
+        s_red[w][1] = pw;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double r0 = s_red[0][0], r1 = s_red[0][1];
+        for (int q = 1; q < BLK / 64; ++q) {
+            r0 += s_red[q][0];
+            r1 += s_red[q][1];
+        }
+        double* p = J.part_n + (size_t)lb * 8;
+        p[0] = r0;
+        p[1] = r1;
+        __threadfence();
+        const uint32_t t = atomicAdd(&ctl->ticket_n, 1u);
+        s_last = (t == J.nblk_n - 1);
+    }
+    __syncthreads();
+    if (!s_last) return;
+    __threadfence();
+    __shared__ double s_lds[2 * BLK];
+    const bool is_min[2] = {false, false};
+    double tot[2];
+    reduce_partials<BLK, 2>(J.part_n, J.nblk_n, is_min, tot, s_lds);
+    if (threadIdx.x == 0) {
+        Ctl& c = *ctl;
+        c.ke = tot[0];
+        c.wext += tot[1] * (0.5 * (dt_i + dt_i_1));
+        const long long row = c.nsteps - 1;
+        if (J.hist && row >= 0 && row < J.hist_cap) {
+            double* h = J.hist + row * (ENSC_H_BASE + 3 * J.n_probe);
+            h[ENSC_H_TIME] = c.time;
+            h[ENSC_H_DT_I] = c.dt_i;
+            h[ENSC_H_DT_I_1] = c.dt_i_1;
+            h[ENSC_H_DTS_MIN] = c.dts_min;
+            h[ENSC_H_KE] = c.ke;
+            h[ENSC_H_IE] = c.ie;
+            h[ENSC_H_HG] = c.hg;
+            h[ENSC_H_VIS] = c.vis;
+            h[ENSC_H_WEXT] = c.wext;
+            for (int p = 0; p < J.n_probe; ++p) {
+                const double* r = J.node + (size_t)J.probe[p] * NODE_REC + 6;
+                h[ENSC_H_BASE + 3 * p + 0] = __builtin_nontemporal_load(r + 0);
+                h[ENSC_H_BASE + 3 * p + 1] = __builtin_nontemporal_load(r + 1);
+                h[ENSC_H_BASE + 3 * p + 2] = __builtin_nontemporal_load(r + 2);
+            }
+        }
+        c.kind = KIND_NONE;
+        c.ticket_n = 0;
+        __threadfence();
+    }
+}
+
+// setCurrentStep's apply_boundary_condition_vec (exDyna3D.cpp:1911, 379-490)
+// at the current time, plus stepEndTime = time + timePeriod (83).  One
+// thread per listed BC node.
+__global__ void k_apply_bc(JobDev* __restrict__ jobs, uint32_t job, const uint32_t* __restrict__ nodes, uint32_t n,
+                           double period, int set_end) {
+#pragma clang fp contract(off)
+    JobDev& J = jobs[job];
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const double time = J.ctl->time;
+    if (i < n) {
+        const uint32_t nd = nodes[i];
+        const uint8_t mk = J.bcmask[nd];
+        double* rec = J.node + (size_t)nd * NODE_REC;
+        for (int d = 0; d < 3; ++d) {
+            if (mk & (BC_VEL0 << d)) rec[3 + d] = J.bcval[(size_t)nd * 3 + d] * amp_value(J, J.bcamp[(size_t)nd * 3 + d], time);
+            if (mk & (BC_SPC0 << d)) rec[3 + d] = 0.0;
+        }
+    }
+    if (set_end && i == 0) J.ctl->step_end = time + period;
+}
+
+}  // namespace ensc

@@ -1,0 +1,199 @@
+"""ctypes binding of libensc_hip.so (include/ensc_engine.h).
+
+``Engine`` mirrors the reference solver class exDyna3D
+(Explicit3D/include/exDyna3D.h:36-80): ``init()``, ``set_current_step()``,
+``run()`` and a step-loop ``advance()``, with the job's state resident on
+the GPU.  There is no CPU fallback: constructing an Engine without the built
+library or without a GPU raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Dict, Optional, Sequence, Tuple
+
+import numpy as np
+
+from .model import Model
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libensc_hip.so")
+
+FP64 = 0
+MIXED = 1
+
+S_NAMES = ["time", "dt_i", "dt_i_1", "dts_min", "ke", "ie", "hg", "vis", "wext", "ke0", "nsteps",
+           "step_index", "step_end", "err", "done"]
+H_BASE = 9
+H_NAMES = ["time", "dt_i", "dt_i_1", "dts_min", "ke", "ie", "hg", "vis", "wext"]
+
+EXPORTS = [
+    "ensc_last_error", "ensc_device_count", "ensc_engine_create", "ensc_engine_destroy", "ensc_engine_init",
+    "ensc_engine_set_step", "ensc_engine_advance", "ensc_engine_run", "ensc_engine_enqueue_steps",
+    "ensc_engine_sync", "ensc_engine_download", "ensc_engine_scalars", "ensc_engine_history",
+    "ensc_engine_kernel_times", "ensc_engine_set_timing", "ensc_engine_device_bytes",
+]
+
+
+class EnscError(RuntimeError):
+    pass
+
+
+class Options(C.Structure):
+    _fields_ = [
+        ("precision", C.c_int32),
+        ("reorder", C.c_int32),
+        ("block_elem", C.c_int32),
+        ("block_node", C.c_int32),
+        ("device", C.c_int32),
+        ("n_probe", C.c_uint32),
+        ("probe_nodes", C.POINTER(C.c_uint32)),
+        ("hist_cap", C.c_int64),
+    ]
+
+
+_lib = None
+
+
+def load_library(path: str = LIB_PATH):
+    """Load libensc_hip.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise EnscError(f"{path} is missing: build it with `make -C {HERE}` (or __graft_entry__.build())")
+    L = C.CDLL(path)
+    L.ensc_last_error.restype = C.c_char_p
+    L.ensc_device_count.restype = C.c_int32
+    vp = C.c_void_p
+    for name, args in [
+        ("ensc_engine_create", [vp, vp, C.POINTER(vp)]),
+        ("ensc_engine_destroy", [vp]),
+        ("ensc_engine_init", [vp]),
+        ("ensc_engine_set_step", [vp, C.c_uint32]),
+        ("ensc_engine_advance", [vp, C.c_int64, C.POINTER(C.c_int32)]),
+        ("ensc_engine_run", [vp]),
+        ("ensc_engine_enqueue_steps", [vp, C.c_int64]),
+        ("ensc_engine_sync", [vp]),
+        ("ensc_engine_download", [vp, C.c_char_p, vp, C.c_int64]),
+        ("ensc_engine_scalars", [vp, vp]),
+        ("ensc_engine_history", [vp, vp, C.c_int64, C.POINTER(C.c_int64)]),
+        ("ensc_engine_kernel_times", [vp, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
+        ("ensc_engine_set_timing", [vp, C.c_int32]),
+    ]:
+        f = getattr(L, name)
+        f.restype = C.c_int
+        f.argtypes = args
+    L.ensc_engine_device_bytes.restype = C.c_int64
+    L.ensc_engine_device_bytes.argtypes = [vp]
+    _lib = L
+    return L
+
+
+def _check(rc: int) -> None:
+    if rc != 0:
+        msg = _lib.ensc_last_error().decode(errors="replace") if _lib else "?"
+        raise EnscError(f"ensc error {rc}: {msg}")
+
+
+def device_count() -> int:
+    return int(load_library().ensc_device_count())
+
+
+class Engine:
+    """One job resident on the GPU, driven like exDyna3D."""
+
+    def __init__(self, model: Model, precision: int = FP64, reorder: bool = False, block_elem: int = 0,
+                 block_node: int = 0, probes: Sequence[int] = (), hist_cap: int = 0, device: int = 0):
+        L = load_library()
+        self.model = model
+        self.precision = precision
+        self._desc = model.desc()
+        pr = np.ascontiguousarray(np.asarray(probes, dtype=np.uint32))
+        self._probes = pr
+        o = Options()
+        o.precision = precision
+        o.reorder = int(reorder)
+        o.block_elem = block_elem
+        o.block_node = block_node
+        o.device = device
+        o.n_probe = len(pr)
+        o.probe_nodes = pr.ctypes.data_as(C.POINTER(C.c_uint32)) if len(pr) else None
+        o.hist_cap = hist_cap
+        h = C.c_void_p()
+        _check(L.ensc_engine_create(C.byref(self._desc), C.byref(o), C.byref(h)))
+        self.h = h
+        self.Nn, self.Ne = model.n_nodes, model.n_elems
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            _lib.ensc_engine_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # --- exDyna3D-shaped interface ---
+    def init(self) -> "Engine":
+        _check(_lib.ensc_engine_init(self.h))
+        return self
+
+    def set_current_step(self, i: int) -> None:
+        _check(_lib.ensc_engine_set_step(self.h, i))
+
+    def advance(self, max_steps: int = 0) -> bool:
+        done = C.c_int32(0)
+        _check(_lib.ensc_engine_advance(self.h, int(max_steps), C.byref(done)))
+        return bool(done.value)
+
+    def run(self) -> "Engine":
+        _check(_lib.ensc_engine_run(self.h))
+        return self
+
+    def run_steps(self, max_steps: int) -> int:
+        """init + step 1 + up to max_steps step iterations; returns steps taken."""
+        self.init()
+        self.set_current_step(1)
+        self.advance(max_steps)
+        return int(self.scalars()["nsteps"])
+
+    # --- low level ---
+    def enqueue(self, n: int) -> None:
+        _check(_lib.ensc_engine_enqueue_steps(self.h, int(n)))
+
+    def sync(self) -> None:
+        _check(_lib.ensc_engine_sync(self.h))
+
+    def set_timing(self, on: bool) -> None:
+        _check(_lib.ensc_engine_set_timing(self.h, int(on)))
+
+    def kernel_times(self) -> Tuple[float, float, int]:
+        a, b, n = C.c_double(), C.c_double(), C.c_int64()
+        _check(_lib.ensc_engine_kernel_times(self.h, C.byref(a), C.byref(b), C.byref(n)))
+        return a.value, b.value, n.value
+
+    def get(self, name: str) -> np.ndarray:
+        shapes = {"u": (self.Nn, 3), "v": (self.Nn, 3), "x": (self.Nn, 3), "rf": (self.Nn, 3),
+                  "sigma": (self.Ne, 6), "qia": (self.Ne, 12), "epsp": (self.Ne,)}
+        out = np.zeros(shapes[name])
+        _check(_lib.ensc_engine_download(self.h, name.encode(), out.ctypes.data, out.size))
+        return out
+
+    def scalars(self) -> Dict[str, float]:
+        o = np.zeros(len(S_NAMES))
+        _check(_lib.ensc_engine_scalars(self.h, o.ctypes.data))
+        return dict(zip(S_NAMES, o.tolist()))
+
+    def history(self, rows: Optional[int] = None) -> np.ndarray:
+        w = H_BASE + 3 * len(self._probes)
+        rows = int(self.scalars()["nsteps"]) if rows is None else rows
+        out = np.zeros((max(rows, 0), w))
+        got = C.c_int64()
+        _check(_lib.ensc_engine_history(self.h, out.ctypes.data if rows else None, rows, C.byref(got)))
+        return out[: got.value]
+
+    def device_bytes(self) -> int:
+        return int(_lib.ensc_engine_device_bytes(self.h))

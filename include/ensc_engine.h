@@ -1,0 +1,129 @@
+/*
+ * ensc_engine.h - C ABI of the MI355X explicit-dynamics step engine.
+ *
+ * Drop-in for the CPU step loop of ssadz/EnSCDynamics' exDyna3D
+ * (Explicit3D/include/exDyna3D.h:36-222).  The reference exposes a C++
+ * class with public run()/init()/setCurrentStep() and protected virtual
+ * computeSate()/compute_a() hooks; a binding that replaces its step loop
+ * calls these entry points instead (see INTEGRATION.md for the C++ shim).
+ *
+ * One ensc_engine holds one or more jobs (meshes) resident in device
+ * memory.  All calls are host-thread-safe per engine only when serialized by
+ * the caller.  Every call returns an ensc_status; ensc_last_error() gives
+ * the message of the last failure on the calling thread.
+ */
+#ifndef ENSC_ENGINE_H
+#define ENSC_ENGINE_H
+
+#include <stdint.h>
+#include "ensc_model.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum ensc_status {
+    ENSC_OK = 0,
+    ENSC_ERR_ARG = 1,      /* bad argument or model */
+    ENSC_ERR_HIP = 2,      /* HIP runtime error */
+    ENSC_ERR_STATE = 3,    /* call out of sequence */
+    ENSC_ERR_MEM = 4,      /* device memory exhausted / over budget */
+    ENSC_ERR_NUMERIC = 5,  /* a job raised a device error flag */
+    ENSC_ERR_NODEV = 6     /* no GPU or the kernels are missing */
+} ensc_status;
+
+typedef enum ensc_precision {
+    ENSC_FP64 = 0,   /* everything FP64: parity path (<=1e-10 vs the reference) */
+    ENSC_MIXED = 1   /* FP32 element kinematics/constitutive, FP64 nodal state,
+                        energies and dt; compensated position accumulation */
+} ensc_precision;
+
+/* Device error flags of a job (ensc_engine_scalars ENSC_S_ERR). */
+enum {
+    ENSC_EF_JACOBIAN = 1u,  /* det F <= 0 in some element */
+    ENSC_EF_NONFINITE = 2u, /* non-finite element force or stress */
+    ENSC_EF_DT = 4u         /* stable dt collapsed (<= 0 or non-finite) */
+};
+
+typedef struct ensc_options {
+    int32_t precision;      /* ensc_precision */
+    int32_t reorder;        /* 1: renumber elements and nodes for locality
+                               (space-filling order); results are returned in
+                               the caller's numbering either way */
+    int32_t block_elem;     /* element-kernel block size (0 = autotuned/default) */
+    int32_t block_node;     /* node-kernel block size (0 = default) */
+    int32_t device;         /* HIP device ordinal */
+    uint32_t n_probe;       /* nodes whose displacement is recorded per step */
+    const uint32_t* probe_nodes;
+    int64_t hist_cap;       /* rows of per-step history kept on device */
+} ensc_options;
+
+/* Scalars of one job (ensc_engine_scalars). */
+enum {
+    ENSC_S_TIME = 0, ENSC_S_DT_I, ENSC_S_DT_I_1, ENSC_S_DTS_MIN, ENSC_S_KE, ENSC_S_IE, ENSC_S_HG,
+    ENSC_S_VIS, ENSC_S_WEXT, ENSC_S_KE0, ENSC_S_NSTEPS, ENSC_S_STEP_INDEX, ENSC_S_STEP_END,
+    ENSC_S_ERR, ENSC_S_DONE, ENSC_S_COUNT
+};
+
+/* One history row: ENSC_H_BASE scalars then 3 displacements per probe. */
+enum {
+    ENSC_H_TIME = 0, ENSC_H_DT_I, ENSC_H_DT_I_1, ENSC_H_DTS_MIN, ENSC_H_KE, ENSC_H_IE, ENSC_H_HG,
+    ENSC_H_VIS, ENSC_H_WEXT, ENSC_H_BASE
+};
+
+typedef struct ensc_engine ensc_engine;
+
+const char* ensc_last_error(void);
+/* Number of visible HIP devices (0 when none). */
+int32_t ensc_device_count(void);
+
+/* Lifecycle.  create() copies the model; the caller keeps its arrays.
+ * Replaces the reference's DataIn population + init_data + compute_invMassMatrix
+ * (exDyna3D.cpp:144-205, 492-515). */
+ensc_status ensc_engine_create(const ensc_model_desc* model, const ensc_options* opts, ensc_engine** out);
+ensc_status ensc_engine_destroy(ensc_engine* eng);
+
+/* exDyna3D::init (exDyna3D.cpp:105-142): initial velocity and the
+ * zero-increment element pass that fills the stable-dt array. */
+ensc_status ensc_engine_init(ensc_engine* eng);
+
+/* exDyna3D::setCurrentStep (exDyna3D.cpp:1760-1940) for step_index >= 1,
+ * then stepEndTime = time + timePeriod (exDyna3D.cpp:83). */
+ensc_status ensc_engine_set_step(ensc_engine* eng, uint32_t step_index);
+
+/* Run the `while (time < stepEndTime)` loop of exDyna3D::run
+ * (exDyna3D.cpp:86-100) for at most max_steps iterations (<= 0: until the
+ * step ends).  *done is set to 1 when time has reached stepEndTime. */
+ensc_status ensc_engine_advance(ensc_engine* eng, int64_t max_steps, int32_t* done);
+
+/* exDyna3D::run (exDyna3D.cpp:54-102): init, then every *Step. */
+ensc_status ensc_engine_run(ensc_engine* eng);
+
+/* Enqueue n step iterations on the engine's stream without waiting
+ * (iterations after the step end are no-ops on the device). */
+ensc_status ensc_engine_enqueue_steps(ensc_engine* eng, int64_t n);
+ensc_status ensc_engine_sync(ensc_engine* eng);
+
+/* Copy a field to host memory, in the caller's node/element numbering:
+ * "u","v","x","rf" -> [n_nodes][3]; "sigma" -> [n_elems][6] (xx yy zz xy xz yz);
+ * "qia" -> [n_elems][12]; "epsp","dtStable" -> [n_elems].
+ * count is the number of doubles the caller's buffer holds. */
+ensc_status ensc_engine_download(ensc_engine* eng, const char* field, double* out, int64_t count);
+ensc_status ensc_engine_scalars(ensc_engine* eng, double* out /* ENSC_S_COUNT */);
+/* Per-step history rows recorded so far (up to rows). */
+ensc_status ensc_engine_history(ensc_engine* eng, double* out, int64_t rows, int64_t* got);
+
+/* Element/node kernel timing over the last enqueue_steps call, measured with
+ * HIP events on the engine's stream: mean ms per launch of each kernel and
+ * the number of launches timed. */
+ensc_status ensc_engine_kernel_times(ensc_engine* eng, double* elem_ms, double* node_ms, int64_t* launches);
+/* Enable (1) / disable (0) per-kernel event timing for later enqueue calls. */
+ensc_status ensc_engine_set_timing(ensc_engine* eng, int32_t on);
+/* Device bytes held by the engine. */
+int64_t ensc_engine_device_bytes(ensc_engine* eng);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ENSC_ENGINE_H */

@@ -1,0 +1,212 @@
+"""GPU parity: the HIP engine (through the C ABI) against the reference's
+golden vectors and the CPU oracle.
+
+Bars: FP64 path <= 1e-10 relative to the reference / oracle on the same
+deck; mixed path <= 1e-4 relative displacement error vs FP64 and an energy
+difference <= 1% of the reference energy scale (KE0 or external work).
+"""
+import numpy as np
+import pytest
+
+import goldens as G
+from enscdynamics_amd import decks
+from enscdynamics_amd import engine as eng
+from oracle.oracle import HIST_BASE, Oracle
+
+pytestmark = pytest.mark.gpu
+
+TOL_FP64 = 1e-10
+TOL_MIXED = 1e-4
+
+
+def rel(a, b, floor=0.0):
+    s = max(float(np.abs(b).max()) if np.size(b) else 0.0, floor)
+    d = float(np.abs(np.asarray(a) - np.asarray(b)).max()) if np.size(b) else 0.0
+    return d / s if s > 0 else d
+
+
+@pytest.fixture(scope="module", autouse=True)
+def gpu_present():
+    n = eng.device_count()
+    print(f"[gpu] devices visible: {n}", flush=True)
+    assert n > 0, "GPU tests need a HIP device"
+
+
+def test_fp64_c1_matches_reference():
+    d = G.load("c1")
+    m = G.model_from_dump(d)
+    tip = int(d["tip_node"])
+    e = eng.Engine(m, eng.FP64, probes=[tip], hist_cap=4096)
+    e.run()
+    h = e.history()
+    H = d["history"]
+    assert h.shape[0] == H.shape[0] == 2008
+    assert rel(h[:, 0], H[:, 0]) <= 1e-14
+    assert rel(h[:, 1], H[:, 1]) <= 1e-11
+    assert rel(h[:, HIST_BASE:], H[:, 4:]) <= TOL_FP64
+    for f in ["u", "v", "x", "sigma", "qia", "rf"]:
+        err = rel(e.get(f), d["final_" + f])
+        print(f"[c1 fp64] {f}: {err:.2e}", flush=True)
+        assert err <= TOL_FP64, f
+
+
+@pytest.mark.parametrize("case", G.CASES_MICRO)
+def test_fp64_micro_matches_reference(case):
+    d = G.load(case)
+    m = G.model_from_dump(d)
+    e = eng.Engine(m, eng.FP64)
+    e.init()
+    e.set_current_step(1)
+    vscale = float(np.abs(d["v_init"]).max()) + 1.0
+    taken = 0
+    for snap in (1, 2, 3, 50):
+        e.advance(snap - taken)
+        taken = snap
+        assert int(e.scalars()["nsteps"]) == snap
+        for f, fl in (("u", 0.0), ("v", 1e-9 * vscale), ("sigma", 0.0), ("rf", 0.0)):
+            assert rel(e.get(f), d[f"step{snap}_{f}"], fl) <= TOL_FP64, (snap, f)
+    # finish every remaining *Step like exDyna3D::run
+    e.advance(0)
+    for s in range(2, len(m.steps)):
+        e.set_current_step(s)
+        e.advance(0)
+    assert int(e.scalars()["nsteps"]) == d["history"].shape[0]
+    sigmax = float(np.abs(d["final_sigma"]).max())
+    for f, fl in (("u", 0.0), ("v", 1e-9 * vscale), ("x", 0.0), ("sigma", 0.0), ("rf", 1e-12 * sigmax)):
+        assert rel(e.get(f), d["final_" + f], fl) <= TOL_FP64, f
+
+
+@pytest.mark.parametrize("case", G.CASES_REF)
+def test_fp64_reference_decks(case):
+    d = G.load(case)
+    m = G.model_from_dump(d)
+    n_ref = int(d["n_steps"])
+    e = eng.Engine(m, eng.FP64, probes=[0], hist_cap=n_ref + 8)
+    e.run()
+    h = e.history()
+    assert h.shape[0] == n_ref
+    st = int(d["history_stride"])
+    assert rel(h[::st][:, 0], d["history_sub"][:, 0]) <= 1e-14
+    assert rel(h[::st][:, HIST_BASE:], d["history_sub"][:, 4:], 1e-12) <= TOL_FP64
+    vmax = max(1.0, float(np.abs(m.v_init).max()))
+    for f, fl in (("u", 1e-12), ("v", 1e-9 * vmax), ("x", 0.0), ("sigma", 1.0), ("rf", 1e-12)):
+        assert rel(e.get(f), d["final_" + f], fl) <= TOL_FP64, f
+
+
+def run_both(m, steps, precision=eng.FP64, reorder=False, probes=()):
+    e = eng.Engine(m, precision, reorder=reorder, probes=probes, hist_cap=steps + 8)
+    e.run_steps(steps)
+    return e
+
+
+def test_fp64_matches_oracle_c2_small_jittered_ramp():
+    # configuration-2 physics (jittered block, velocity ramp by amplitude) at
+    # a size the oracle finishes in seconds
+    m = decks.config2(nx=24, ny=6, nz=6, steps=400)
+    e = eng.Engine(m, eng.FP64, hist_cap=512)
+    e.run()
+    o = Oracle(m, 0).init()
+    o.run(0, [], 0)
+    assert int(e.scalars()["nsteps"]) == int(o.scalars()["nsteps"])
+    for f in ["u", "v", "x", "sigma", "qia", "rf"]:
+        err = rel(e.get(f), o.get(f), 1e-12)
+        print(f"[c2s fp64 vs oracle] {f}: {err:.2e}", flush=True)
+        assert err <= TOL_FP64, f
+
+
+def test_fp64_matches_oracle_plastic_cube():
+    m = decks.config3(n=16, steps=500)
+    e = eng.Engine(m, eng.FP64)
+    e.run()
+    o = Oracle(m, 0).init()
+    o.run(0, [], 0)
+    assert (o.get("epsp") > 0).any()
+    for f in ["u", "v", "sigma", "qia", "epsp", "rf"]:
+        err = rel(e.get(f), o.get(f), 1e-12)
+        print(f"[c3s fp64 vs oracle] {f}: {err:.2e}", flush=True)
+        assert err <= TOL_FP64, f
+    so, se = o.scalars(), e.scalars()
+    for k in ("ie", "hg", "vis", "wext", "ke"):
+        assert se[k] == pytest.approx(so[k], rel=1e-9, abs=1e-9 * abs(so["wext"])), k
+
+
+def test_reorder_is_bit_exact():
+    m = decks.config2(nx=20, ny=7, nz=5, steps=200, seed=11)
+    a = eng.Engine(m, eng.FP64, reorder=False)
+    a.run()
+    b = eng.Engine(m, eng.FP64, reorder=True)
+    b.run()
+    for f in ["u", "v", "x", "sigma", "qia", "rf"]:
+        assert np.array_equal(a.get(f), b.get(f)), f
+
+
+def test_determinism_two_runs():
+    m = decks.config3(n=12, steps=300)
+    r = []
+    for _ in range(2):
+        e = eng.Engine(m, eng.MIXED, reorder=True)
+        e.run()
+        r.append({f: e.get(f) for f in ("u", "v", "sigma", "qia")} | {"s": e.scalars()})
+    for f in ("u", "v", "sigma", "qia"):
+        assert np.array_equal(r[0][f], r[1][f]), f
+    assert r[0]["s"]["ie"] == r[1]["s"]["ie"]
+
+
+def test_mixed_vs_fp64_c1():
+    m = decks.config1()
+    tip = m.tip_node
+    a = eng.Engine(m, eng.FP64, probes=[tip], hist_cap=4096).run()
+    b = eng.Engine(m, eng.MIXED, probes=[tip], hist_cap=4096).run()
+    ha, hb = a.history(), b.history()
+    n = min(len(ha), len(hb))
+    err = rel(hb[:n, HIST_BASE:], ha[:n, HIST_BASE:])
+    print(f"[c1 mixed vs fp64] tip u history rel err {err:.2e}", flush=True)
+    assert err <= TOL_MIXED
+    assert rel(b.get("u"), a.get("u")) <= TOL_MIXED
+    sa, sb = a.scalars(), b.scalars()
+    ea = sa["ke"] + sa["ie"] + sa["hg"] + sa["vis"]
+    eb = sb["ke"] + sb["ie"] + sb["hg"] + sb["vis"]
+    assert abs(ea - eb) <= 0.01 * sa["ke0"]
+
+
+def test_mixed_vs_fp64_plastic_cube():
+    m = decks.config3(n=20, steps=800)
+    a = eng.Engine(m, eng.FP64).run()
+    b = eng.Engine(m, eng.MIXED).run()
+    err_u = rel(b.get("u"), a.get("u"))
+    err_s = rel(b.get("sigma"), a.get("sigma"))
+    sa, sb = a.scalars(), b.scalars()
+    e_err = abs((sb["ke"] + sb["ie"] + sb["hg"] + sb["vis"]) - (sa["ke"] + sa["ie"] + sa["hg"] + sa["vis"]))
+    print(f"[c3s mixed vs fp64] u {err_u:.2e} sigma {err_s:.2e} energy {e_err / sa['wext']:.2e} of W_ext",
+          flush=True)
+    assert err_u <= TOL_MIXED
+    assert e_err <= 0.01 * sa["wext"]
+
+
+def test_full_size_cube_fp64_vs_oracle_few_steps():
+    # configuration 3 at its full 1M-element size: a few steps against the
+    # oracle (size-independent check of the same arithmetic at scale)
+    m = decks.config3(n=100, steps=3000)
+    e = eng.Engine(m, eng.FP64, reorder=True)
+    e.run_steps(6)
+    o = Oracle(m, 0).init()
+    o.run(6, [], 0)
+    for f in ["u", "v", "sigma", "qia"]:
+        err = rel(e.get(f), o.get(f), 1e-12)
+        print(f"[c3 full fp64 vs oracle, 6 steps] {f}: {err:.2e}", flush=True)
+        assert err <= TOL_FP64, f
+
+
+def test_bad_models_fail_loudly():
+    m = decks.config3(n=4, steps=10)
+    bad = decks.config3(n=4, steps=10)
+    bad.conn = bad.conn.copy()
+    bad.conn[0, [0, 1]] = bad.conn[0, [1, 0]]  # swap two corners: inverted element
+    with pytest.raises(eng.EnscError):
+        eng.Engine(bad)
+    nan = decks.config3(n=4, steps=10)
+    nan.v_init = nan.v_init.copy()
+    nan.v_init[3, 1] = np.nan
+    with pytest.raises(eng.EnscError):
+        eng.Engine(nan)
+    eng.Engine(m).run()

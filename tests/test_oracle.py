@@ -1,0 +1,222 @@
+"""Pin the CPU restatement (oracle/) before trusting it.
+
+1. Against the real reference (golden fixtures written by
+   tests/golden/make_golden.py from the reference's own sources).
+2. Against closed forms and invariants that a plausible mistake would break:
+   box-element B0/V0/mass, the initial stable dt, free flight, uniaxial
+   strain (normal Jaumann terms), J2 consistency and hardening slope, and
+   energy conservation of a closed system.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import goldens as G
+from enscdynamics_amd import decks
+from enscdynamics_amd.model import Bc, Model, Step
+from oracle.oracle import HIST_BASE, Oracle
+
+TOL_REF = 1e-10  # FP64 bar vs the reference
+
+
+def check_field(a, b, tol, floor=0.0):
+    s = max(float(np.abs(b).max()) if b.size else 0.0, floor)
+    err = float(np.abs(a - b).max()) / s if s > 0 else float(np.abs(a - b).max())
+    assert err <= tol, f"rel err {err:.3e} > {tol}"
+    return err
+
+
+def test_oracle_c1_matches_reference():
+    d = G.load("c1")
+    m = G.model_from_dump(d)
+    tip = int(d["tip_node"])
+    o = Oracle(m, 0).init()
+    check_field(o.get("V0"), d["V0"], 1e-14)
+    check_field(o.get("invM"), d["invM"], 1e-14)
+    check_field(o.get("dtStable"), d["dtStable0"], 1e-14)
+    n, h = o.run(0, [tip], 10 ** 6)
+    H = d["history"]
+    assert n == H.shape[0] == 2008
+    check_field(h[:, 0], H[:, 0], 1e-14)        # time
+    check_field(h[:, 1], H[:, 1], 1e-11)        # dt_i
+    check_field(h[:, HIST_BASE:], H[:, 4:], TOL_REF)
+    for f in ["u", "v", "x", "sigma", "qia", "rf", "dtStable"]:
+        check_field(o.get(f), d["final_" + f], TOL_REF)
+
+
+@pytest.mark.parametrize("case", G.CASES_MICRO)
+def test_oracle_micro_matches_reference(case):
+    d = G.load(case)
+    m = G.model_from_dump(d)
+    o = Oracle(m, 1).init()
+    check_field(o.get("B0"), d["B0"], 1e-13)
+    check_field(o.get("V0"), d["V0"], 1e-14)
+    H = d["history"]
+    npr = (H.shape[1] - 4) // 3
+    vscale = float(np.abs(d["v_init"]).max()) + 1.0
+    taken = 0
+    for snap in (1, 2, 3, 50):
+        n, _ = o.run(snap - taken, [], 0)
+        taken += n
+        assert taken == snap
+        for f, fl in (("u", 0.0), ("v", 1e-9 * vscale), ("sigma", 0.0), ("rf", 0.0)):
+            check_field(o.get(f), d[f"step{snap}_{f}"], TOL_REF, fl)
+        sc = d[f"step{snap}_scalars"]
+        s = o.scalars()
+        assert s["time"] == pytest.approx(sc[0], rel=1e-14, abs=0)
+    n, h = o.run(0, list(range(npr)), 10 ** 6)
+    sigmax = float(np.abs(d["final_sigma"]).max())
+    for f, fl in (("u", 0.0), ("v", 1e-9 * vscale), ("x", 0.0), ("sigma", 0.0), ("rf", 1e-12 * sigmax)):
+        check_field(o.get(f), d["final_" + f], TOL_REF, fl)
+
+
+@pytest.mark.parametrize("case", G.CASES_REF)
+def test_oracle_reference_decks(case):
+    d = G.load(case)
+    m = G.model_from_dump(d)
+    o = Oracle(m, 0).init()
+    n, h = o.run(0, [0], 10 ** 7)
+    assert n == int(d["n_steps"])
+    st = int(d["history_stride"])
+    Hs = d["history_sub"]
+    check_field(h[::st][:, 0], Hs[:, 0], 1e-14)
+    check_field(h[::st][:, HIST_BASE:], Hs[:, 4:], TOL_REF, 1e-12)
+    check_field(h[-5:, 1], d["history_last"][:, 1], 1e-10)
+    vmax = max(1.0, float(np.abs(m.v_init).max()))
+    for f, fl in (("u", 1e-12), ("v", 1e-9 * vmax), ("x", 0.0), ("sigma", 1.0), ("rf", 1e-12)):
+        check_field(o.get(f), d["final_" + f], TOL_REF, fl)
+
+
+# ---------------------------------------------------------------------------
+# closed forms
+# ---------------------------------------------------------------------------
+
+def box_model(nx, ny, nz, lx, ly, lz, **kw):
+    x0, conn, ijk = decks.block_mesh(nx, ny, nz, lx, ly, lz)
+    m = Model(x0=x0, conn=conn, **kw)
+    m.v_init = np.zeros_like(x0)
+    m.steps = [Step("Initial", 0.0), Step("Step-1", 1.0)]
+    m.total_time = 1.0
+    return m, ijk
+
+
+def test_setup_closed_forms():
+    hx, hy, hz = 0.02, 0.015, 0.01
+    m, ijk = box_model(3, 2, 2, 3 * hx, 2 * hy, 2 * hz, rho=2700.0)
+    o = Oracle(m, 1).init()
+    V0 = o.get("V0")
+    assert np.allclose(V0, hx * hy * hz, rtol=1e-14)
+    B0 = o.get("B0")
+    cs = decks.CORNERS * 2 - 1
+    expect = cs / (4.0 * np.array([hx, hy, hz]))
+    assert np.allclose(B0, expect[None], rtol=1e-13, atol=0)
+    mass = 1.0 / o.get("invM")
+    assert mass.sum() == pytest.approx(2700.0 * 3 * hx * 2 * hy * 2 * hz, rel=1e-13)
+    interior = np.nonzero((ijk[:, 0] == 1) & (ijk[:, 1] == 1) & (ijk[:, 2] == 1))[0]
+    assert mass[interior] == pytest.approx(2700.0 * hx * hy * hz, rel=1e-13)
+
+
+def test_initial_dt_closed_form_c1():
+    m = decks.config1()
+    o = Oracle(m, 0).init()
+    n, h = o.run(1, [], 1)
+    G_, K_, lam, c = m.material_consts()
+    xi = m.cvisl
+    dt0 = m.factor * (math.sqrt(1 + xi * xi) - xi) * 0.01 / c
+    assert h[0, 1] == pytest.approx(dt0, rel=1e-13)
+    assert h[0, 2] == 0.0 and h[0, 0] == 0.0  # the first step integrates with dt_i_1 = 0
+
+
+def test_free_flight():
+    m, _ = box_model(3, 3, 2, 0.03, 0.03, 0.02, E=200e9, nu=0.3, rho=7800.0)
+    v0 = np.array([3.0, -1.5, 0.75])
+    m.v_init = np.tile(v0, (m.n_nodes, 1))
+    m.steps[1].period = m.total_time = 2e-5
+    o = Oracle(m, 1).init()
+    n, h = o.run(0, [], 10 ** 5)
+    # u advances with dt_i while time advances with the lagged dt_i_1
+    # (exDyna3D.cpp:652 vs 775): u = v0 * sum(dt_i)
+    assert np.allclose(o.get("u"), v0[None] * h[:, 1].sum(), rtol=1e-12)
+    assert np.abs(o.get("sigma")).max() < 1e-3  # Pa: rounding only
+    assert np.abs(o.get("v") - v0).max() < 1e-12
+
+
+def uniaxial_model(plastic=False, rate=20.0, steps=400):
+    """One column of elements, x/y fixed on every node, z=0 fixed, every node
+    driven with v_z = rate * z (uniform uniaxial strain rate)."""
+    h = 0.01
+    x0, conn, ijk = decks.block_mesh(1, 1, 1, h, h, h)
+    m = Model(x0=x0, conn=conn, E=200e9, nu=0.3, rho=7800.0, plastic=plastic, yield0=250e6, hard=1e9)
+    allnodes = np.arange(m.n_nodes, dtype=np.uint32)
+    m.v_init = np.zeros_like(x0)
+    top = np.nonzero(ijk[:, 2] == 1)[0].astype(np.uint32)
+    bot = np.nonzero(ijk[:, 2] == 0)[0].astype(np.uint32)
+    m.steps = [Step("Initial", 0.0, spc=[Bc("spc", allnodes, 0, 1), Bc("spc", bot, 2, 2)]),
+               Step("Step-1", 1.0, vel=[Bc("vel", top, 2, 2, value=-rate * h)])]
+    dt0 = decks.stable_dt0(m)
+    m.steps[1].period = m.total_time = steps * dt0
+    return m
+
+
+def test_uniaxial_strain_elastic():
+    m = uniaxial_model(plastic=False, rate=5.0, steps=50)
+    o = Oracle(m, 1).init()
+    o.run(0, [], 0)
+    s = o.get("sigma")[0]
+    G_, K_, lam, c = m.material_consts()
+    # sigma_xx / sigma_zz = lambda / (lambda + 2G) for uniaxial strain; no shear
+    assert s[0] / s[2] == pytest.approx(lam / (lam + 2 * G_), rel=1e-10)
+    assert s[0] == pytest.approx(s[1], rel=1e-12)
+    assert np.abs(s[3:]).max() <= 1e-9 * abs(s[2])
+    # sigma_zz = (lambda+2G) * integral of L_zz dt = (lambda+2G) * log-like strain
+    h = 0.01
+    uz = o.get("u")[:, 2].min()
+    eps = math.log((h + uz) / h)
+    assert s[2] == pytest.approx((lam + 2 * G_) * eps, rel=2e-2)
+
+
+def test_j2_consistency_and_hardening_slope():
+    m = uniaxial_model(plastic=True, rate=60.0, steps=600)
+    o = Oracle(m, 1).init()
+    G_, K_, lam, c = m.material_consts()
+    H = m.hard
+    path = []
+    for _ in range(60):
+        o.run(10, [], 0)
+        s = o.get("sigma")[0]
+        ep = o.get("epsp")[0]
+        p = s[:3].mean()
+        dev = s[:3] - p
+        q = math.sqrt(1.5 * (dev @ dev) + 3 * (s[3:] @ s[3:]))
+        uz = o.get("u")[:, 2].min()
+        path.append((uz, s[2], q, ep))
+    path = np.array(path)
+    plastic = path[:, 3] > 0
+    assert plastic.any() and (~plastic).any()
+    # consistency: on the yield surface after every plastic return
+    qy = m.yield0 + H * path[plastic, 3]
+    assert np.allclose(path[plastic, 2], qy, rtol=1e-10)
+    # plastic strain only grows (dissipation >= 0)
+    assert np.all(np.diff(path[:, 3]) >= 0)
+    # uniaxial-strain plastic tangent: K + 4 G H / (3 (3G + H)) (vs elastic lambda + 2G)
+    h = 0.01
+    eps = np.log((h + path[:, 0]) / h)
+    pl = np.nonzero(plastic)[0]
+    k = pl[len(pl) // 2:]
+    slope = np.polyfit(eps[k], path[k, 1], 1)[0]
+    assert slope == pytest.approx(K_ + 4 * G_ * H / (3 * (3 * G_ + H)), rel=3e-2)
+    el = np.nonzero(~plastic)[0]
+    if len(el) > 3:
+        slope_e = np.polyfit(eps[el], path[el, 1], 1)[0]
+        assert slope_e == pytest.approx(lam + 2 * G_, rel=3e-2)
+
+
+def test_energy_conservation_c1():
+    m = decks.config1()
+    o = Oracle(m, 0).init()
+    o.run(0, [], 0)
+    s = o.scalars()
+    total = s["ke"] + s["ie"] + s["hg"] + s["vis"] - s["wext"]
+    assert abs(total - s["ke0"]) <= 0.01 * s["ke0"]
+    assert s["vis"] >= 0.0
