@@ -56,8 +56,8 @@ def workload(name: str):
     raise SystemExit(f"unknown workload {name}")
 
 
-def time_engine(m, precision, warmup, steps, timed_kernel_steps):
-    e = eng.Engine(m, precision, reorder=True)
+def time_engine(m, precision, warmup, steps, timed_kernel_steps, staging=True):
+    e = eng.Engine(m, precision, reorder=True, staging=staging)
     e.init()
     e.set_current_step(1)
     e.enqueue(warmup)
@@ -108,6 +108,7 @@ def main() -> int:
     ap.add_argument("--kernel-steps", type=int, default=50)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=100)
+    ap.add_argument("--no-staging", action="store_true", help="gather element corners from HBM instead of LDS")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on one GPU only: every job of this engine fits one MI355X and the "
@@ -121,7 +122,8 @@ def main() -> int:
     r = m.n_nodes / m.n_elems
     res = {}
     for prec, name in ((eng.MIXED, "mixed"), (eng.FP64, "fp64")):
-        e, secs, kms, nms, s = time_engine(m, prec, args.warmup, args.steps, args.kernel_steps)
+        e, secs, kms, nms, s = time_engine(m, prec, args.warmup, args.steps, args.kernel_steps,
+                                           staging=not args.no_staging)
         res[name] = dict(engine=e, secs=secs, elem_ms=kms, node_ms=nms, scalars=s)
         print(f"[bench] {name}: {args.steps} steps in {secs:.4f} s; element kernel {kms:.4f} ms, "
               f"node kernel {nms:.4f} ms", file=sys.stderr, flush=True)

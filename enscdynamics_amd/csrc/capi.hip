@@ -1,7 +1,13 @@
 // C ABI of the step engine (include/ensc_engine.h).
+//
+// An engine holds jobs in stable slots.  Every live job is laid out in one
+// job table with contiguous element-block and node-block ranges, so one
+// launch of each kernel advances all of them ("tick").  The layout is
+// rebuilt between ticks whenever jobs are added, retired or finish.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <limits>
@@ -22,35 +28,65 @@ using namespace ensc;
 
 typedef void (*KernelFn)(const JobDev*, const uint32_t*);
 
+namespace {
+
+// Device buffer that grows (contents are not preserved).
+struct DevBuf {
+    void* p = nullptr;
+    size_t cap = 0;
+    bool ensure(size_t bytes) {
+        if (bytes <= cap && p) return true;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+        const size_t b = std::max<size_t>(round_up(bytes, 4096), 4096);
+        if (hipMalloc(&p, b) != hipSuccess) return false;
+        cap = b;
+        return true;
+    }
+    ~DevBuf() {
+        if (p) (void)hipFree(p);
+    }
+};
+
+}  // namespace
+
 struct ensc_engine {
-    int device = 0, precision = ENSC_FP64, blk_e = 256, blk_n = 256;
+    int device = 0, precision = ENSC_FP64, blk_e = 256, blk_n = 256, reorder = 0, staging_opt = 0;
     hipStream_t stream = nullptr;
-    std::vector<std::unique_ptr<Job>> jobs;
-    DeviceArena arena;
-    JobDev* d_jobs = nullptr;
-    uint32_t* d_b2j_e = nullptr;
-    uint32_t* d_b2j_n = nullptr;
-    uint32_t nblk_e = 0, nblk_n = 0;
+    std::vector<std::unique_ptr<Job>> slots;  // job id = slot index; null = free
+    // per-slot scalars (device array + pinned host mirror), never moved while jobs live
     Ctl* d_ctl = nullptr;
-    Ctl* h_ctl = nullptr;  // pinned host mirror
-    double* d_part_e = nullptr;
-    double* d_part_n = nullptr;
+    Ctl* h_ctl = nullptr;
+    uint32_t ctl_cap = 0;
+    // layout of the jobs currently stepped
+    std::vector<uint32_t> live;
+    DevBuf jobs_buf, b2e_buf, b2n_buf, pe_buf, pn_buf;
+    uint32_t nblk_e = 0, nblk_n = 0;
+    bool dirty = true;
     KernelFn ke = nullptr, kn = nullptr;
+    bool staged = false;
+    size_t lds_bytes = 0;
+    // timing
     bool timing = false;
     std::vector<hipEvent_t> evs;
     double elem_ms = 0.0, node_ms = 0.0;
-    int64_t launches = 0;
-    int64_t pending_ticks = 0;
+    int64_t launches = 0, pending_ticks = 0;
+    // graph of graph_ticks ticks, valid for the current layout
+    int32_t graph_ticks = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    bool graph_valid = false;
 };
 
 namespace {
 
-template <bool M>
+template <bool M, bool S>
 KernelFn pick_e(int blk) {
     switch (blk) {
-        case 64: return k_element<M, 64>;
-        case 128: return k_element<M, 128>;
-        default: return k_element<M, 256>;
+        case 64: return k_element<M, 64, S>;
+        case 128: return k_element<M, 128, S>;
+        default: return k_element<M, 256, S>;
     }
 }
 template <bool M>
@@ -62,27 +98,146 @@ KernelFn pick_n(int blk) {
     }
 }
 
-ensc_status upload_jobs(ensc_engine* E) {
-    std::vector<JobDev> h(E->jobs.size());
-    for (size_t j = 0; j < E->jobs.size(); ++j) h[j] = E->jobs[j]->dev;
-    HIPCHK(hipMemcpyAsync(E->d_jobs, h.data(), h.size() * sizeof(JobDev), hipMemcpyHostToDevice, E->stream));
-    HIPCHK(hipStreamSynchronize(E->stream));
-    return ENSC_OK;
+Job* job_at(ensc_engine* E, uint32_t id) {
+    return id < E->slots.size() ? E->slots[id].get() : nullptr;
 }
 
 ensc_status read_ctl(ensc_engine* E) {
-    HIPCHK(hipMemcpyAsync(E->h_ctl, E->d_ctl, E->jobs.size() * sizeof(Ctl), hipMemcpyDeviceToHost, E->stream));
+    if (!E->ctl_cap) return ENSC_OK;
+    HIPCHK(hipMemcpyAsync(E->h_ctl, E->d_ctl, E->ctl_cap * sizeof(Ctl), hipMemcpyDeviceToHost, E->stream));
     HIPCHK(hipStreamSynchronize(E->stream));
     return ENSC_OK;
 }
 
 ensc_status write_ctl(ensc_engine* E) {
-    HIPCHK(hipMemcpyAsync(E->d_ctl, E->h_ctl, E->jobs.size() * sizeof(Ctl), hipMemcpyHostToDevice, E->stream));
+    if (!E->ctl_cap) return ENSC_OK;
+    HIPCHK(hipMemcpyAsync(E->d_ctl, E->h_ctl, E->ctl_cap * sizeof(Ctl), hipMemcpyHostToDevice, E->stream));
     HIPCHK(hipStreamSynchronize(E->stream));
     return ENSC_OK;
 }
 
+bool job_done(const Ctl& c) { return !(c.time < c.step_end) || c.nsteps >= c.step_limit || c.err != 0; }
+
+// Grow the slot scalar array to hold `need` slots (device idle).
+ensc_status grow_ctl(ensc_engine* E, uint32_t need) {
+    if (need <= E->ctl_cap) return ENSC_OK;
+    const uint32_t cap = std::max<uint32_t>(need, std::max<uint32_t>(16, 2 * E->ctl_cap));
+    Ctl* d = nullptr;
+    Ctl* h = nullptr;
+    HIPCHK(hipMalloc((void**)&d, cap * sizeof(Ctl)));
+    HIPCHK(hipHostMalloc((void**)&h, cap * sizeof(Ctl), hipHostMallocDefault));
+    std::memset(h, 0, cap * sizeof(Ctl));
+    if (E->ctl_cap) {
+        ensc_status st = read_ctl(E);
+        if (st) return st;
+        std::memcpy(h, E->h_ctl, E->ctl_cap * sizeof(Ctl));
+        (void)hipFree(E->d_ctl);
+        (void)hipHostFree(E->h_ctl);
+    }
+    E->d_ctl = d;
+    E->h_ctl = h;
+    E->ctl_cap = cap;
+    for (size_t s = 0; s < E->slots.size(); ++s)
+        if (E->slots[s]) E->slots[s]->dev.ctl = E->d_ctl + s;
+    E->dirty = true;
+    return write_ctl(E);
+}
+
+// Rebuild the job table and block maps for the jobs that are not finished.
+ensc_status relayout(ensc_engine* E) {
+    ensc_status st;
+    if ((st = read_ctl(E))) return st;
+    E->live.clear();
+    for (uint32_t s = 0; s < E->slots.size(); ++s) {
+        Job* J = E->slots[s].get();
+        if (!J) continue;
+        const Ctl& c = E->h_ctl[s];
+        const bool idle = J->inited && !c.init_pending && !(c.time < c.step_end) && !J->pending_step;
+        if (!idle || J->keep_in_layout) E->live.push_back(s);
+    }
+    std::vector<uint32_t> b2e, b2n;
+    std::vector<JobDev> tab;
+    uint32_t lds_nodes = 0;
+    for (uint32_t i = 0; i < E->live.size(); ++i) {
+        Job& J = *E->slots[E->live[i]];
+        JobDev& D = J.dev;
+        D.eblk0 = (uint32_t)b2e.size();
+        D.nblk0 = (uint32_t)b2n.size();
+        for (uint32_t b = 0; b < D.nblk_e; ++b) b2e.push_back(i);
+        for (uint32_t b = 0; b < D.nblk_n; ++b) b2n.push_back(i);
+        lds_nodes = std::max(lds_nodes, J.lds_nodes);
+    }
+    E->nblk_e = (uint32_t)b2e.size();
+    E->nblk_n = (uint32_t)b2n.size();
+    if (!E->jobs_buf.ensure(std::max<size_t>(1, E->live.size()) * sizeof(JobDev)) ||
+        !E->b2e_buf.ensure(std::max<size_t>(1, b2e.size()) * 4) || !E->b2n_buf.ensure(std::max<size_t>(1, b2n.size()) * 4) ||
+        !E->pe_buf.ensure(std::max<size_t>(1, b2e.size()) * 8 * sizeof(double)) ||
+        !E->pn_buf.ensure(std::max<size_t>(1, b2n.size()) * 8 * sizeof(double)))
+        return fail(ENSC_ERR_MEM, "layout allocation failed");
+    for (uint32_t i = 0; i < E->live.size(); ++i) {
+        Job& J = *E->slots[E->live[i]];
+        J.dev.ctl = E->d_ctl + E->live[i];
+        J.dev.part_e = static_cast<double*>(E->pe_buf.p) + (size_t)J.dev.eblk0 * 8;
+        J.dev.part_n = static_cast<double*>(E->pn_buf.p) + (size_t)J.dev.nblk0 * 8;
+        tab.push_back(J.dev);
+    }
+    if (!tab.empty())
+        HIPCHK(hipMemcpyAsync(E->jobs_buf.p, tab.data(), tab.size() * sizeof(JobDev), hipMemcpyHostToDevice, E->stream));
+    if (!b2e.empty()) HIPCHK(hipMemcpyAsync(E->b2e_buf.p, b2e.data(), b2e.size() * 4, hipMemcpyHostToDevice, E->stream));
+    if (!b2n.empty()) HIPCHK(hipMemcpyAsync(E->b2n_buf.p, b2n.data(), b2n.size() * 4, hipMemcpyHostToDevice, E->stream));
+    HIPCHK(hipStreamSynchronize(E->stream));
+    const bool mixed = E->precision == ENSC_MIXED;
+    // LDS staging when every live job's largest block node list fits 64 KiB
+    E->staged = E->staging_opt != 2 && (size_t)lds_nodes * 9 * sizeof(double) <= 65536;
+    E->lds_bytes = E->staged ? (size_t)std::max<uint32_t>(lds_nodes, 1) * 9 * sizeof(double) : 0;
+    if (E->staged)
+        E->ke = mixed ? pick_e<true, true>(E->blk_e) : pick_e<false, true>(E->blk_e);
+    else
+        E->ke = mixed ? pick_e<true, false>(E->blk_e) : pick_e<false, false>(E->blk_e);
+    E->kn = mixed ? pick_n<true>(E->blk_n) : pick_n<false>(E->blk_n);
+    E->dirty = false;
+    E->graph_valid = false;
+    return ENSC_OK;
+}
+
+ensc_status launch_tick(ensc_engine* E) {
+    const JobDev* jobs = static_cast<const JobDev*>(E->jobs_buf.p);
+    if (E->nblk_e)
+        hipLaunchKernelGGL(E->ke, dim3(E->nblk_e), dim3(E->blk_e), E->lds_bytes, E->stream, jobs,
+                           static_cast<const uint32_t*>(E->b2e_buf.p));
+    return ENSC_OK;
+}
+ensc_status launch_node(ensc_engine* E) {
+    const JobDev* jobs = static_cast<const JobDev*>(E->jobs_buf.p);
+    if (E->nblk_n)
+        hipLaunchKernelGGL(E->kn, dim3(E->nblk_n), dim3(E->blk_n), 0, E->stream, jobs,
+                           static_cast<const uint32_t*>(E->b2n_buf.p));
+    return ENSC_OK;
+}
+
+ensc_status build_graph(ensc_engine* E) {
+    if (E->graph_exec) {
+        (void)hipGraphExecDestroy(E->graph_exec);
+        E->graph_exec = nullptr;
+    }
+    if (E->graph) {
+        (void)hipGraphDestroy(E->graph);
+        E->graph = nullptr;
+    }
+    HIPCHK(hipStreamBeginCapture(E->stream, hipStreamCaptureModeThreadLocal));
+    for (int i = 0; i < E->graph_ticks; ++i) {
+        launch_tick(E);
+        launch_node(E);
+    }
+    HIPCHK(hipStreamEndCapture(E->stream, &E->graph));
+    HIPCHK(hipGraphInstantiate(&E->graph_exec, E->graph, nullptr, nullptr, 0));
+    E->graph_valid = true;
+    return ENSC_OK;
+}
+
 ensc_status enqueue_ticks(ensc_engine* E, int64_t n) {
+    ensc_status st;
+    if (E->dirty && (st = relayout(E))) return st;
     if (E->timing) {
         const size_t need = 2 * (size_t)n + 1;
         while (E->evs.size() < need) {
@@ -91,15 +246,26 @@ ensc_status enqueue_ticks(ensc_engine* E, int64_t n) {
             E->evs.push_back(ev);
         }
         HIPCHK(hipEventRecord(E->evs[0], E->stream));
-    }
-    for (int64_t i = 0; i < n; ++i) {
-        hipLaunchKernelGGL(E->ke, dim3(E->nblk_e), dim3(E->blk_e), 0, E->stream, E->d_jobs, E->d_b2j_e);
-        if (E->timing) HIPCHK(hipEventRecord(E->evs[2 * i + 1], E->stream));
-        hipLaunchKernelGGL(E->kn, dim3(E->nblk_n), dim3(E->blk_n), 0, E->stream, E->d_jobs, E->d_b2j_n);
-        if (E->timing) HIPCHK(hipEventRecord(E->evs[2 * i + 2], E->stream));
+        for (int64_t i = 0; i < n; ++i) {
+            launch_tick(E);
+            HIPCHK(hipEventRecord(E->evs[2 * i + 1], E->stream));
+            launch_node(E);
+            HIPCHK(hipEventRecord(E->evs[2 * i + 2], E->stream));
+        }
+        E->pending_ticks = n;
+    } else {
+        int64_t i = 0;
+        if (E->graph_ticks > 0 && n >= E->graph_ticks) {
+            if (!E->graph_valid && (st = build_graph(E))) return st;
+            for (; i + E->graph_ticks <= n; i += E->graph_ticks) HIPCHK(hipGraphLaunch(E->graph_exec, E->stream));
+        }
+        for (; i < n; ++i) {
+            launch_tick(E);
+            launch_node(E);
+        }
+        E->pending_ticks = 0;
     }
     HIPCHK(hipGetLastError());
-    E->pending_ticks = E->timing ? n : 0;
     return ENSC_OK;
 }
 
@@ -125,218 +291,197 @@ ensc_status sync(ensc_engine* E) {
     return finish_timing(E);
 }
 
-bool job_done(const Ctl& c) { return !(c.time < c.step_end) || c.nsteps >= c.step_limit || c.err != 0; }
-
-}  // namespace
-
-extern "C" {
-
-const char* ensc_last_error(void) { return g_last_error.c_str(); }
-
-int32_t ensc_device_count(void) {
-    int n = 0;
-    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
-    return n;
-}
-
-ensc_status ensc_engine_create(const ensc_model_desc* model, const ensc_options* opts, ensc_engine** out) {
-    if (!out) return fail(ENSC_ERR_ARG, "out is null");
-    *out = nullptr;
+ensc_status engine_new(const ensc_options* opts, std::unique_ptr<ensc_engine>& E) {
     ensc_options o{};
     if (opts) o = *opts;
     if (o.precision != ENSC_FP64 && o.precision != ENSC_MIXED) return fail(ENSC_ERR_ARG, "unknown precision");
     if (ensc_device_count() <= 0) return fail(ENSC_ERR_NODEV, "no HIP device visible");
-    std::unique_ptr<ensc_engine> E(new ensc_engine());
+    E.reset(new ensc_engine());
     E->device = o.device;
     E->precision = o.precision;
+    E->reorder = o.reorder;
+    E->staging_opt = o.staging;
     auto okblk = [](int b) { return b == 64 || b == 128 || b == 256; };
     E->blk_e = okblk(o.block_elem) ? o.block_elem : 256;
     E->blk_n = okblk(o.block_node) ? o.block_node : 256;
     HIPCHK(hipSetDevice(E->device));
     HIPCHK(hipStreamCreateWithFlags(&E->stream, hipStreamNonBlocking));
+    return ENSC_OK;
+}
+
+// Build one job into a free slot.  The engine must be idle.
+ensc_status add_job(ensc_engine* E, const ensc_model_desc* model, const ensc_options* opts, uint32_t* slot_out) {
     std::unique_ptr<Job> J(new Job());
     std::string err;
     if (!copy_model(model, J->model, err)) return fail(ENSC_ERR_ARG, err);
-    ensc_status st = build_job(*J, E->precision, o.reorder, E->blk_e, E->blk_n);
+    ensc_status st = build_job(*J, E->precision, E->reorder, E->blk_e, E->blk_n);
     if (st) return st;
-    // probes and history
-    if (o.n_probe > MAX_PROBE) return fail(ENSC_ERR_ARG, "at most 8 probe nodes");
-    for (uint32_t i = 0; i < o.n_probe; ++i) {
-        if (o.probe_nodes[i] >= J->model.Nn) return fail(ENSC_ERR_ARG, "probe node out of range");
-        J->dev.probe[i] = J->n_old2new[o.probe_nodes[i]];
+    if (opts && opts->n_probe) {
+        if (opts->n_probe > MAX_PROBE) return fail(ENSC_ERR_ARG, "at most 8 probe nodes");
+        for (uint32_t i = 0; i < opts->n_probe; ++i) {
+            if (opts->probe_nodes[i] >= J->model.Nn) return fail(ENSC_ERR_ARG, "probe node out of range");
+            J->dev.probe[i] = J->n_old2new[opts->probe_nodes[i]];
+        }
+        J->dev.n_probe = (int32_t)opts->n_probe;
     }
-    J->dev.n_probe = (int32_t)o.n_probe;
-    if (o.hist_cap > 0) {
-        const size_t bytes = (size_t)o.hist_cap * (ENSC_H_BASE + 3 * o.n_probe) * sizeof(double);
+    if (opts && opts->hist_cap > 0) {
+        const size_t bytes = (size_t)opts->hist_cap * (ENSC_H_BASE + 3 * J->dev.n_probe) * sizeof(double);
         J->d_hist = static_cast<double*>(J->arena.alloc(bytes));
         if (!J->d_hist) return fail(ENSC_ERR_MEM, "history allocation failed");
         HIPCHK(hipMemset(J->d_hist, 0, bytes));
         J->dev.hist = J->d_hist;
-        J->dev.hist_cap = o.hist_cap;
-        J->hist_cap = o.hist_cap;
+        J->dev.hist_cap = opts->hist_cap;
+        J->hist_cap = opts->hist_cap;
     }
-    E->jobs.push_back(std::move(J));
-    // engine-wide arrays: ctl, partials, block maps, job table
-    const size_t nj = E->jobs.size();
-    E->d_ctl = static_cast<Ctl*>(E->arena.alloc(nj * sizeof(Ctl)));
-    E->d_jobs = static_cast<JobDev*>(E->arena.alloc(nj * sizeof(JobDev)));
-    if (!E->d_ctl || !E->d_jobs) return fail(ENSC_ERR_MEM, "engine allocation failed");
-    HIPCHK(hipHostMalloc((void**)&E->h_ctl, nj * sizeof(Ctl), hipHostMallocDefault));
-    std::vector<uint32_t> b2e, b2n;
-    for (size_t j = 0; j < nj; ++j) {
-        JobDev& D = E->jobs[j]->dev;
-        D.eblk0 = (uint32_t)b2e.size();
-        D.nblk0 = (uint32_t)b2n.size();
-        for (uint32_t b = 0; b < D.nblk_e; ++b) b2e.push_back((uint32_t)j);
-        for (uint32_t b = 0; b < D.nblk_n; ++b) b2n.push_back((uint32_t)j);
-    }
-    E->nblk_e = (uint32_t)b2e.size();
-    E->nblk_n = (uint32_t)b2n.size();
-    E->d_b2j_e = static_cast<uint32_t*>(E->arena.alloc(b2e.size() * 4));
-    E->d_b2j_n = static_cast<uint32_t*>(E->arena.alloc(b2n.size() * 4));
-    E->d_part_e = static_cast<double*>(E->arena.alloc((size_t)E->nblk_e * 8 * sizeof(double)));
-    E->d_part_n = static_cast<double*>(E->arena.alloc((size_t)E->nblk_n * 8 * sizeof(double)));
-    if (!E->d_b2j_e || !E->d_b2j_n || !E->d_part_e || !E->d_part_n) return fail(ENSC_ERR_MEM, "engine allocation failed");
-    HIPCHK(hipMemcpy(E->d_b2j_e, b2e.data(), b2e.size() * 4, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(E->d_b2j_n, b2n.data(), b2n.size() * 4, hipMemcpyHostToDevice));
-    for (size_t j = 0; j < nj; ++j) {
-        Job& Jb = *E->jobs[j];
-        JobDev& D = Jb.dev;
-        D.ctl = E->d_ctl + j;
-        D.part_e = E->d_part_e + (size_t)D.eblk0 * 8;
-        D.part_n = E->d_part_n + (size_t)D.nblk0 * 8;
-        Ctl& c = E->h_ctl[j];
-        std::memset(&c, 0, sizeof(Ctl));
-        c.total_time = Jb.model.total_time;
-        c.dts_min = std::numeric_limits<double>::infinity();
-        c.step_limit = std::numeric_limits<long long>::max();
-        c.ke0 = Jb.ke0;
-        c.ke = Jb.ke0;
-        c.grav_amp = 1.0;
-    }
-    if ((st = write_ctl(E.get()))) return st;
-    if ((st = upload_jobs(E.get()))) return st;
-    const bool mixed = E->precision == ENSC_MIXED;
-    E->ke = mixed ? pick_e<true>(E->blk_e) : pick_e<false>(E->blk_e);
-    E->kn = mixed ? pick_n<true>(E->blk_n) : pick_n<false>(E->blk_n);
-    *out = E.release();
+    uint32_t slot = 0;
+    while (slot < E->slots.size() && E->slots[slot]) ++slot;
+    if (slot == E->slots.size()) E->slots.emplace_back();
+    if ((st = grow_ctl(E, slot + 1))) return st;
+    if ((st = read_ctl(E))) return st;
+    Ctl& c = E->h_ctl[slot];
+    std::memset(&c, 0, sizeof(Ctl));
+    c.total_time = J->model.total_time;
+    c.dts_min = std::numeric_limits<double>::infinity();
+    c.step_limit = std::numeric_limits<long long>::max();
+    c.ke0 = J->ke0;
+    c.ke = J->ke0;
+    c.grav_amp = 1.0;
+    J->dev.ctl = E->d_ctl + slot;
+    E->slots[slot] = std::move(J);
+    E->dirty = true;
+    if ((st = write_ctl(E))) return st;
+    *slot_out = slot;
     return ENSC_OK;
 }
 
-ensc_status ensc_engine_destroy(ensc_engine* E) {
-    if (!E) return ENSC_OK;
-    (void)hipSetDevice(E->device);
-    (void)hipStreamSynchronize(E->stream);
-    for (hipEvent_t ev : E->evs) (void)hipEventDestroy(ev);
-    if (E->h_ctl) (void)hipHostFree(E->h_ctl);
-    E->jobs.clear();
-    if (E->stream) (void)hipStreamDestroy(E->stream);
-    delete E;
+// setCurrentStep for one job: host inheritance, BC arrays, device BC
+// application and the new step end time (enqueued, stream ordered).
+ensc_status job_switch_step(ensc_engine* E, uint32_t slot, uint32_t si) {
+    Job& J = *E->slots[slot];
+    if (!job_set_step(J, si)) return ENSC_OK;
+    std::vector<uint8_t> mask;
+    std::vector<double> val;
+    std::vector<int16_t> amp;
+    std::vector<uint32_t> nodes;
+    job_bc_arrays(J, mask, val, amp, nodes);
+    HIPCHK(hipMemcpyAsync(J.d_bcmask, mask.data(), mask.size(), hipMemcpyHostToDevice, E->stream));
+    HIPCHK(hipMemcpyAsync(J.d_bcval, val.data(), val.size() * sizeof(double), hipMemcpyHostToDevice, E->stream));
+    HIPCHK(hipMemcpyAsync(J.d_bcamp, amp.data(), amp.size() * sizeof(int16_t), hipMemcpyHostToDevice, E->stream));
+    HIPCHK(hipMemsetAsync(J.dev.rf, 0, 3 * (size_t)J.model.Nn * sizeof(double), E->stream));
+    if (nodes.size() > J.d_bcnodes_cap) {
+        J.d_bcnodes = static_cast<uint32_t*>(J.arena.alloc(nodes.size() * 4));
+        if (!J.d_bcnodes) return fail(ENSC_ERR_MEM, "BC node list allocation failed");
+        J.d_bcnodes_cap = nodes.size();
+    }
+    if (!nodes.empty())
+        HIPCHK(hipMemcpyAsync(J.d_bcnodes, nodes.data(), nodes.size() * 4, hipMemcpyHostToDevice, E->stream));
+    const GravState& g = J.grav[si];
+    J.dev.grav_on = g.on;
+    J.dev.grav_amp = g.amp;
+    J.dev.grav_g = g.g;
+    for (int d = 0; d < 3; ++d) J.dev.grav_dir[d] = g.dir[d];
+    // the BC kernel reads this job's table entry: upload it to a scratch slot
+    if (!J.d_self) {
+        J.d_self = static_cast<JobDev*>(J.arena.alloc(sizeof(JobDev)));
+        if (!J.d_self) return fail(ENSC_ERR_MEM, "job table allocation failed");
+    }
+    J.self_copy = J.dev;
+    HIPCHK(hipMemcpyAsync(J.d_self, &J.self_copy, sizeof(JobDev), hipMemcpyHostToDevice, E->stream));
+    const uint32_t n = (uint32_t)nodes.size();
+    const uint32_t blocks = std::max<uint32_t>(1, (n + 255) / 256);
+    hipLaunchKernelGGL(k_apply_bc, dim3(blocks), dim3(256), 0, E->stream, J.d_self, 0u, (const uint32_t*)J.d_bcnodes,
+                       n, J.model.steps[si].period, 1);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(E->stream));  // host vectors above go out of scope
+    J.pending_step = false;
+    E->dirty = true;  // table entries changed (gravity, BC arrays)
     return ENSC_OK;
 }
 
-ensc_status ensc_engine_init(ensc_engine* E) {
-    if (!E) return fail(ENSC_ERR_ARG, "null engine");
-    HIPCHK(hipSetDevice(E->device));
+ensc_status do_init(ensc_engine* E) {
     ensc_status st;
     if ((st = sync(E)) || (st = read_ctl(E))) return st;
     bool any = false;
-    for (size_t j = 0; j < E->jobs.size(); ++j) {
-        if (E->jobs[j]->inited) continue;
-        E->h_ctl[j].init_pending = 1;
-        E->jobs[j]->inited = true;
+    for (uint32_t s = 0; s < E->slots.size(); ++s) {
+        Job* J = E->slots[s].get();
+        if (!J || J->inited) continue;
+        E->h_ctl[s].init_pending = 1;
+        J->inited = true;
         any = true;
     }
     if (!any) return ENSC_OK;  // init() runs once (exDyna3D.cpp:109-116)
     if ((st = write_ctl(E))) return st;
-    if ((st = enqueue_ticks(E, 1))) return st;
+    E->dirty = true;
+    const bool t = E->timing;
+    E->timing = false;
+    st = enqueue_ticks(E, 1);
+    E->timing = t;
+    if (st) return st;
     return sync(E);
 }
 
-ensc_status ensc_engine_set_step(ensc_engine* E, uint32_t si) {
-    if (!E) return fail(ENSC_ERR_ARG, "null engine");
-    HIPCHK(hipSetDevice(E->device));
+ensc_status do_set_step(ensc_engine* E, uint32_t si) {
     ensc_status st;
     if ((st = sync(E))) return st;
-    bool changed = false;
-    for (auto& Jp : E->jobs) {
-        Job& J = *Jp;
-        if (!J.inited) return fail(ENSC_ERR_STATE, "ensc_engine_init must run before set_step");
-        if (!job_set_step(J, si)) continue;
-        changed = true;
-        std::vector<uint8_t> mask;
-        std::vector<double> val;
-        std::vector<int16_t> amp;
-        std::vector<uint32_t> nodes;
-        job_bc_arrays(J, mask, val, amp, nodes);
-        HIPCHK(hipMemcpyAsync(J.d_bcmask, mask.data(), mask.size(), hipMemcpyHostToDevice, E->stream));
-        HIPCHK(hipMemcpyAsync(J.d_bcval, val.data(), val.size() * sizeof(double), hipMemcpyHostToDevice, E->stream));
-        HIPCHK(hipMemcpyAsync(J.d_bcamp, amp.data(), amp.size() * sizeof(int16_t), hipMemcpyHostToDevice, E->stream));
-        HIPCHK(hipMemsetAsync(J.dev.rf, 0, 3 * (size_t)J.model.Nn * sizeof(double), E->stream));
-        if (nodes.size() > J.d_bcnodes_cap) {
-            J.d_bcnodes = static_cast<uint32_t*>(J.arena.alloc(nodes.size() * 4));
-            if (!J.d_bcnodes) return fail(ENSC_ERR_MEM, "BC node list allocation failed");
-            J.d_bcnodes_cap = nodes.size();
-        }
-        if (!nodes.empty())
-            HIPCHK(hipMemcpyAsync(J.d_bcnodes, nodes.data(), nodes.size() * 4, hipMemcpyHostToDevice, E->stream));
-        const GravState& g = J.grav[si];
-        J.dev.grav_on = g.on;
-        J.dev.grav_amp = g.amp;
-        J.dev.grav_g = g.g;
-        for (int d = 0; d < 3; ++d) J.dev.grav_dir[d] = g.dir[d];
-        J.dev.bcmask = J.d_bcmask;
+    for (uint32_t s = 0; s < E->slots.size(); ++s) {
+        Job* J = E->slots[s].get();
+        if (!J) continue;
+        if (!J->inited) return fail(ENSC_ERR_STATE, "ensc_engine_init must run before set_step");
+        if ((st = job_switch_step(E, s, si))) return st;
     }
-    if (!changed) return ENSC_OK;
-    if ((st = upload_jobs(E))) return st;
-    for (size_t j = 0; j < E->jobs.size(); ++j) {
-        Job& J = *E->jobs[j];
-        if (J.cur_step != si) continue;
-        uint32_t n = 0;
-        std::vector<uint8_t> dummy;
-        // recount BC nodes (same list as uploaded above)
-        std::vector<uint8_t> mask;
-        std::vector<double> val;
-        std::vector<int16_t> amp;
-        std::vector<uint32_t> nodes;
-        job_bc_arrays(J, mask, val, amp, nodes);
-        n = (uint32_t)nodes.size();
-        const uint32_t blocks = std::max<uint32_t>(1, (n + 255) / 256);
-        hipLaunchKernelGGL(k_apply_bc, dim3(blocks), dim3(256), 0, E->stream, E->d_jobs, (uint32_t)j,
-                           (const uint32_t*)J.d_bcnodes, n, J.model.steps[si].period, 1);
-    }
-    HIPCHK(hipGetLastError());
-    if ((st = sync(E)) || (st = read_ctl(E))) return st;
-    for (size_t j = 0; j < E->jobs.size(); ++j) E->h_ctl[j].step_index = (int32_t)E->jobs[j]->cur_step;
-    return write_ctl(E);
+    if ((st = read_ctl(E))) return st;
+    for (uint32_t s = 0; s < E->slots.size(); ++s)
+        if (E->slots[s]) E->h_ctl[s].step_index = (int32_t)E->slots[s]->cur_step;
+    if ((st = write_ctl(E))) return st;
+    E->dirty = true;
+    return ENSC_OK;
 }
 
-ensc_status ensc_engine_advance(ensc_engine* E, int64_t max_steps, int32_t* done) {
-    if (!E) return fail(ENSC_ERR_ARG, "null engine");
-    HIPCHK(hipSetDevice(E->device));
+// Run ticks until every job has reached its step end (or max_steps more
+// steps, or an error flag).  Finished jobs drop out of the layout.
+ensc_status do_advance(ensc_engine* E, int64_t max_steps, int32_t* done) {
     ensc_status st;
     if ((st = sync(E)) || (st = read_ctl(E))) return st;
-    for (size_t j = 0; j < E->jobs.size(); ++j)
-        E->h_ctl[j].step_limit = max_steps > 0 ? E->h_ctl[j].nsteps + max_steps : std::numeric_limits<long long>::max();
+    for (uint32_t s = 0; s < E->slots.size(); ++s)
+        if (E->slots[s])
+            E->h_ctl[s].step_limit =
+                max_steps > 0 ? E->h_ctl[s].nsteps + max_steps : std::numeric_limits<long long>::max();
     if ((st = write_ctl(E))) return st;
-    int64_t chunk = 4;
+    E->dirty = true;
+    int64_t chunk = std::max<int64_t>(4, E->graph_ticks);
+    size_t n_live_blocks = 0;
     for (;;) {
         bool all = true;
-        for (size_t j = 0; j < E->jobs.size(); ++j) all = all && job_done(E->h_ctl[j]);
+        size_t done_blocks = 0;
+        for (uint32_t s = 0; s < E->slots.size(); ++s) {
+            Job* J = E->slots[s].get();
+            if (!J) continue;
+            const bool d = job_done(E->h_ctl[s]);
+            all = all && d;
+            if (d) done_blocks += J->dev.nblk_e;
+        }
         if (all) break;
+        // drop finished jobs from the launch once they hold a quarter of the blocks
+        if (n_live_blocks && done_blocks * 4 > n_live_blocks) E->dirty = true;
+        if (E->dirty) {
+            if ((st = relayout(E))) return st;
+            n_live_blocks = E->nblk_e;
+        }
         const bool t = E->timing;
         E->timing = false;
         st = enqueue_ticks(E, chunk);
         E->timing = t;
         if (st) return st;
         if ((st = read_ctl(E))) return st;
-        chunk = std::min<int64_t>(chunk * 2, 256);
+        chunk = std::min<int64_t>(chunk * 2, std::max<int64_t>(256, E->graph_ticks));
+        if (E->graph_ticks > 0) chunk = (chunk / E->graph_ticks) * E->graph_ticks;
     }
     bool step_end = true, errs = false;
-    for (size_t j = 0; j < E->jobs.size(); ++j) {
-        step_end = step_end && !(E->h_ctl[j].time < E->h_ctl[j].step_end);
-        errs = errs || E->h_ctl[j].err != 0;
-        E->h_ctl[j].step_limit = std::numeric_limits<long long>::max();
+    for (uint32_t s = 0; s < E->slots.size(); ++s) {
+        if (!E->slots[s]) continue;
+        step_end = step_end && !(E->h_ctl[s].time < E->h_ctl[s].step_end);
+        errs = errs || E->h_ctl[s].err != 0;
+        E->h_ctl[s].step_limit = std::numeric_limits<long long>::max();
     }
     if ((st = write_ctl(E))) return st;
     if (done) *done = step_end ? 1 : 0;
@@ -344,44 +489,27 @@ ensc_status ensc_engine_advance(ensc_engine* E, int64_t max_steps, int32_t* done
     return ENSC_OK;
 }
 
-ensc_status ensc_engine_run(ensc_engine* E) {
-    if (!E) return fail(ENSC_ERR_ARG, "null engine");
-    ensc_status st;
-    if ((st = ensc_engine_init(E))) return st;
-    const size_t nsteps = E->jobs[0]->model.steps.size();
-    for (uint32_t s = 1; s < nsteps; ++s) {
-        if ((st = ensc_engine_set_step(E, s))) return st;
-        int32_t done = 0;
-        if ((st = ensc_engine_advance(E, 0, &done))) return st;
-    }
-    return ENSC_OK;
+void fill_scalars(const Ctl& c, double* out) {
+    out[ENSC_S_TIME] = c.time;
+    out[ENSC_S_DT_I] = c.dt_i;
+    out[ENSC_S_DT_I_1] = c.dt_i_1;
+    out[ENSC_S_DTS_MIN] = c.dts_min;
+    out[ENSC_S_KE] = c.ke;
+    out[ENSC_S_IE] = c.ie;
+    out[ENSC_S_HG] = c.hg;
+    out[ENSC_S_VIS] = c.vis;
+    out[ENSC_S_WEXT] = c.wext;
+    out[ENSC_S_KE0] = c.ke0;
+    out[ENSC_S_NSTEPS] = (double)c.nsteps;
+    out[ENSC_S_STEP_INDEX] = (double)c.step_index;
+    out[ENSC_S_STEP_END] = c.step_end;
+    out[ENSC_S_ERR] = (double)c.err;
+    out[ENSC_S_DONE] = (c.time < c.step_end) ? 0.0 : 1.0;
 }
 
-ensc_status ensc_engine_enqueue_steps(ensc_engine* E, int64_t n) {
-    if (!E) return fail(ENSC_ERR_ARG, "null engine");
-    HIPCHK(hipSetDevice(E->device));
-    if (E->pending_ticks) {
-        ensc_status st = sync(E);
-        if (st) return st;
-    }
-    return enqueue_ticks(E, n);
-}
-
-ensc_status ensc_engine_sync(ensc_engine* E) {
-    if (!E) return fail(ENSC_ERR_ARG, "null engine");
-    HIPCHK(hipSetDevice(E->device));
-    return sync(E);
-}
-
-ensc_status ensc_engine_download(ensc_engine* E, const char* field, double* out, int64_t count) {
-    if (!E || !field || !out) return fail(ENSC_ERR_ARG, "null argument");
-    HIPCHK(hipSetDevice(E->device));
-    ensc_status st = sync(E);
-    if (st) return st;
-    Job& J = *E->jobs[0];
+ensc_status download(ensc_engine* E, Job& J, const std::string& f, double* out, int64_t count) {
     const size_t Nn = J.model.Nn, Ne = J.model.Ne, es = J.ne_s;
     const bool mixed = E->precision == ENSC_MIXED;
-    const std::string f(field);
     if (f == "u" || f == "v" || f == "x") {
         if (count < (int64_t)(3 * Nn)) return fail(ENSC_ERR_ARG, "buffer too small");
         std::vector<double> rec(Nn * NODE_REC);
@@ -431,28 +559,217 @@ ensc_status ensc_engine_download(ensc_engine* E, const char* field, double* out,
     return ENSC_OK;
 }
 
+}  // namespace
+
+// Internal entry points used by the serving loop (server.cpp).
+namespace ensc {
+ensc_status engine_new_public(const ensc_options* o, ensc_engine** out) {
+    std::unique_ptr<ensc_engine> E;
+    ensc_status st = engine_new(o, E);
+    if (st) return st;
+    *out = E.release();
+    return ENSC_OK;
+}
+ensc_status engine_add_job(ensc_engine* E, const ensc_model_desc* m, uint32_t* slot) {
+    ensc_status st = sync(E);
+    if (st) return st;
+    return add_job(E, m, nullptr, slot);
+}
+// Stream-ordered start of a newly added job: its init pass runs in the next
+// tick together with every other live job; its step 1 is switched in after.
+ensc_status engine_start_jobs(ensc_engine* E, const std::vector<uint32_t>& slots) {
+    ensc_status st;
+    if ((st = sync(E)) || (st = read_ctl(E))) return st;
+    for (uint32_t s : slots) {
+        Job* J = job_at(E, s);
+        if (!J) return fail(ENSC_ERR_ARG, "bad job id");
+        E->h_ctl[s].init_pending = 1;
+        J->inited = true;
+        J->pending_step = true;
+    }
+    if ((st = write_ctl(E))) return st;
+    E->dirty = true;
+    // init pass for the new jobs (others step normally in the same tick)
+    if ((st = enqueue_ticks(E, 1))) return st;
+    if ((st = sync(E))) return st;
+    for (uint32_t s : slots)
+        if ((st = job_switch_step(E, s, 1))) return st;
+    if ((st = read_ctl(E))) return st;
+    for (uint32_t s : slots) E->h_ctl[s].step_index = 1;
+    if ((st = write_ctl(E))) return st;
+    E->dirty = true;
+    return ENSC_OK;
+}
+ensc_status engine_ticks(ensc_engine* E, int64_t n) { return enqueue_ticks(E, n); }
+ensc_status engine_poll(ensc_engine* E, std::vector<uint32_t>& finished) {
+    ensc_status st;
+    if ((st = sync(E)) || (st = read_ctl(E))) return st;
+    finished.clear();
+    for (uint32_t s = 0; s < E->slots.size(); ++s) {
+        Job* J = E->slots[s].get();
+        if (!J || !J->inited || J->pending_step) continue;
+        const Ctl& c = E->h_ctl[s];
+        if (c.init_pending) continue;
+        if (!(c.time < c.step_end) || c.err) finished.push_back(s);
+    }
+    return ENSC_OK;
+}
+ensc_status engine_job_scalars(ensc_engine* E, uint32_t s, double* out) {
+    ensc_status st;
+    if ((st = sync(E)) || (st = read_ctl(E))) return st;
+    if (!job_at(E, s)) return fail(ENSC_ERR_ARG, "bad job id");
+    fill_scalars(E->h_ctl[s], out);
+    return ENSC_OK;
+}
+ensc_status engine_job_download(ensc_engine* E, uint32_t s, const char* f, double* out, int64_t count) {
+    ensc_status st = sync(E);
+    if (st) return st;
+    Job* J = job_at(E, s);
+    if (!J) return fail(ENSC_ERR_ARG, "bad job id");
+    return download(E, *J, f, out, count);
+}
+ensc_status engine_remove_job(ensc_engine* E, uint32_t s) {
+    ensc_status st = sync(E);
+    if (st) return st;
+    if (!job_at(E, s)) return fail(ENSC_ERR_ARG, "bad job id");
+    E->slots[s].reset();
+    E->dirty = true;
+    return ENSC_OK;
+}
+int64_t engine_job_bytes(ensc_engine* E, uint32_t s) {
+    Job* J = job_at(E, s);
+    return J ? (int64_t)J->arena.total : 0;
+}
+ensc_status engine_set_graph(ensc_engine* E, int32_t ticks) {
+    E->graph_ticks = std::max(0, ticks);
+    E->graph_valid = false;
+    return ENSC_OK;
+}
+}  // namespace ensc
+
+extern "C" {
+
+const char* ensc_last_error(void) { return g_last_error.c_str(); }
+
+int32_t ensc_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+ensc_status ensc_engine_create(const ensc_model_desc* model, const ensc_options* opts, ensc_engine** out) {
+    if (!out) return fail(ENSC_ERR_ARG, "out is null");
+    *out = nullptr;
+    std::unique_ptr<ensc_engine> E;
+    ensc_status st = engine_new(opts, E);
+    if (st) return st;
+    uint32_t slot = 0;
+    if ((st = add_job(E.get(), model, opts, &slot))) return st;
+    E->slots[0]->keep_in_layout = true;
+    if ((st = relayout(E.get()))) return st;
+    *out = E.release();
+    return ENSC_OK;
+}
+
+ensc_status ensc_batch_create(const ensc_model_desc* const* models, uint32_t n_models, const ensc_options* opts,
+                              ensc_engine** out) {
+    if (!out || !models || n_models == 0) return fail(ENSC_ERR_ARG, "no models");
+    *out = nullptr;
+    std::unique_ptr<ensc_engine> E;
+    ensc_status st = engine_new(opts, E);
+    if (st) return st;
+    if ((st = grow_ctl(E.get(), n_models))) return st;
+    for (uint32_t i = 0; i < n_models; ++i) {
+        uint32_t slot = 0;
+        if ((st = add_job(E.get(), models[i], nullptr, &slot))) return st;
+    }
+    if ((st = relayout(E.get()))) return st;
+    *out = E.release();
+    return ENSC_OK;
+}
+
+ensc_status ensc_engine_destroy(ensc_engine* E) {
+    if (!E) return ENSC_OK;
+    (void)hipSetDevice(E->device);
+    (void)hipStreamSynchronize(E->stream);
+    for (hipEvent_t ev : E->evs) (void)hipEventDestroy(ev);
+    if (E->graph_exec) (void)hipGraphExecDestroy(E->graph_exec);
+    if (E->graph) (void)hipGraphDestroy(E->graph);
+    E->slots.clear();
+    if (E->d_ctl) (void)hipFree(E->d_ctl);
+    if (E->h_ctl) (void)hipHostFree(E->h_ctl);
+    if (E->stream) (void)hipStreamDestroy(E->stream);
+    delete E;
+    return ENSC_OK;
+}
+
+ensc_status ensc_engine_init(ensc_engine* E) {
+    if (!E) return fail(ENSC_ERR_ARG, "null engine");
+    HIPCHK(hipSetDevice(E->device));
+    return do_init(E);
+}
+
+ensc_status ensc_engine_set_step(ensc_engine* E, uint32_t si) {
+    if (!E) return fail(ENSC_ERR_ARG, "null engine");
+    HIPCHK(hipSetDevice(E->device));
+    return do_set_step(E, si);
+}
+
+ensc_status ensc_engine_advance(ensc_engine* E, int64_t max_steps, int32_t* done) {
+    if (!E) return fail(ENSC_ERR_ARG, "null engine");
+    HIPCHK(hipSetDevice(E->device));
+    return do_advance(E, max_steps, done);
+}
+
+ensc_status ensc_engine_run(ensc_engine* E) {
+    if (!E) return fail(ENSC_ERR_ARG, "null engine");
+    HIPCHK(hipSetDevice(E->device));
+    ensc_status st;
+    if ((st = do_init(E))) return st;
+    size_t nsteps = 0;
+    for (auto& j : E->slots)
+        if (j) nsteps = std::max(nsteps, j->model.steps.size());
+    for (uint32_t s = 1; s < nsteps; ++s) {
+        if ((st = do_set_step(E, s))) return st;
+        int32_t done = 0;
+        if ((st = do_advance(E, 0, &done))) return st;
+    }
+    return ENSC_OK;
+}
+
+ensc_status ensc_engine_enqueue_steps(ensc_engine* E, int64_t n) {
+    if (!E) return fail(ENSC_ERR_ARG, "null engine");
+    HIPCHK(hipSetDevice(E->device));
+    if (E->pending_ticks) {
+        ensc_status st = sync(E);
+        if (st) return st;
+    }
+    return enqueue_ticks(E, n);
+}
+
+ensc_status ensc_engine_sync(ensc_engine* E) {
+    if (!E) return fail(ENSC_ERR_ARG, "null engine");
+    HIPCHK(hipSetDevice(E->device));
+    return sync(E);
+}
+
+ensc_status ensc_engine_set_graph(ensc_engine* E, int32_t ticks) {
+    if (!E) return fail(ENSC_ERR_ARG, "null engine");
+    ensc_status st = sync(E);
+    if (st) return st;
+    return engine_set_graph(E, ticks);
+}
+
+ensc_status ensc_engine_download(ensc_engine* E, const char* field, double* out, int64_t count) {
+    if (!E || !field || !out) return fail(ENSC_ERR_ARG, "null argument");
+    HIPCHK(hipSetDevice(E->device));
+    return engine_job_download(E, 0, field, out, count);
+}
+
 ensc_status ensc_engine_scalars(ensc_engine* E, double* out) {
     if (!E || !out) return fail(ENSC_ERR_ARG, "null argument");
     HIPCHK(hipSetDevice(E->device));
-    ensc_status st;
-    if ((st = sync(E)) || (st = read_ctl(E))) return st;
-    const Ctl& c = E->h_ctl[0];
-    out[ENSC_S_TIME] = c.time;
-    out[ENSC_S_DT_I] = c.dt_i;
-    out[ENSC_S_DT_I_1] = c.dt_i_1;
-    out[ENSC_S_DTS_MIN] = c.dts_min;
-    out[ENSC_S_KE] = c.ke;
-    out[ENSC_S_IE] = c.ie;
-    out[ENSC_S_HG] = c.hg;
-    out[ENSC_S_VIS] = c.vis;
-    out[ENSC_S_WEXT] = c.wext;
-    out[ENSC_S_KE0] = c.ke0;
-    out[ENSC_S_NSTEPS] = (double)c.nsteps;
-    out[ENSC_S_STEP_INDEX] = (double)c.step_index;
-    out[ENSC_S_STEP_END] = c.step_end;
-    out[ENSC_S_ERR] = (double)c.err;
-    out[ENSC_S_DONE] = (c.time < c.step_end) ? 0.0 : 1.0;
-    return ENSC_OK;
+    return engine_job_scalars(E, 0, out);
 }
 
 ensc_status ensc_engine_history(ensc_engine* E, double* out, int64_t rows, int64_t* got) {
@@ -460,12 +777,13 @@ ensc_status ensc_engine_history(ensc_engine* E, double* out, int64_t rows, int64
     HIPCHK(hipSetDevice(E->device));
     ensc_status st;
     if ((st = sync(E)) || (st = read_ctl(E))) return st;
-    Job& J = *E->jobs[0];
-    int64_t n = std::min<int64_t>(E->h_ctl[0].nsteps, J.hist_cap);
+    Job* J = job_at(E, 0);
+    if (!J) return fail(ENSC_ERR_ARG, "no job");
+    int64_t n = std::min<int64_t>(E->h_ctl[0].nsteps, J->hist_cap);
     n = std::min<int64_t>(n, rows);
     *got = n;
     if (n > 0 && out)
-        HIPCHK(hipMemcpy(out, J.d_hist, (size_t)n * (ENSC_H_BASE + 3 * J.dev.n_probe) * sizeof(double),
+        HIPCHK(hipMemcpy(out, J->d_hist, (size_t)n * (ENSC_H_BASE + 3 * J->dev.n_probe) * sizeof(double),
                          hipMemcpyDeviceToHost));
     return ENSC_OK;
 }
@@ -490,9 +808,25 @@ ensc_status ensc_engine_set_timing(ensc_engine* E, int32_t on) {
 
 int64_t ensc_engine_device_bytes(ensc_engine* E) {
     if (!E) return 0;
-    int64_t t = (int64_t)E->arena.total;
-    for (auto& j : E->jobs) t += (int64_t)j->arena.total;
-    return t;
+    int64_t t = 0;
+    for (auto& j : E->slots)
+        if (j) t += (int64_t)j->arena.total;
+    return t + (int64_t)(E->jobs_buf.cap + E->b2e_buf.cap + E->b2n_buf.cap + E->pe_buf.cap + E->pn_buf.cap) +
+           (int64_t)E->ctl_cap * (int64_t)sizeof(Ctl);
+}
+
+uint32_t ensc_batch_size(ensc_engine* E) { return E ? (uint32_t)E->slots.size() : 0; }
+
+ensc_status ensc_batch_job_scalars(ensc_engine* E, uint32_t job, double* out) {
+    if (!E || !out) return fail(ENSC_ERR_ARG, "null argument");
+    HIPCHK(hipSetDevice(E->device));
+    return engine_job_scalars(E, job, out);
+}
+
+ensc_status ensc_batch_job_download(ensc_engine* E, uint32_t job, const char* field, double* out, int64_t count) {
+    if (!E || !field || !out) return fail(ENSC_ERR_ARG, "null argument");
+    HIPCHK(hipSetDevice(E->device));
+    return engine_job_download(E, job, field, out, count);
 }
 
 }  // extern "C"

@@ -335,6 +335,32 @@ ensc_status build_job(Job& J, int precision, int reorder, int blk_e, int blk_n) 
             ell[(size_t)fillc[no]++ * ns + nn_] = (J.e_old2new[eo] << 3) | (uint32_t)k;
         }
     if (Ne >= (1u << 29)) return fail(ENSC_ERR_ARG, "too many elements for the 29-bit gather slot");
+    // LDS staging lists: per element block, its distinct nodes (ascending new
+    // ids) and each corner's 16-bit index into that list.
+    std::vector<uint32_t> blk_ptr, blk_nodes;
+    std::vector<uint16_t> lconn(8 * es, 0);
+    J.lds_nodes = 0;
+    {
+        const size_t nb = (Ne + blk_e - 1) / blk_e;
+        blk_ptr.assign(nb + 1, 0);
+        std::vector<uint32_t> tmp;
+        for (size_t b = 0; b < nb; ++b) {
+            const size_t e0 = b * blk_e, e1 = std::min(Ne, e0 + blk_e);
+            tmp.clear();
+            for (size_t e = e0; e < e1; ++e)
+                for (int k = 0; k < 8; ++k) tmp.push_back(conn_d[(size_t)k * es + e]);
+            std::sort(tmp.begin(), tmp.end());
+            tmp.erase(std::unique(tmp.begin(), tmp.end()), tmp.end());
+            for (size_t e = e0; e < e1; ++e)
+                for (int k = 0; k < 8; ++k) {
+                    const uint32_t n = conn_d[(size_t)k * es + e];
+                    lconn[(size_t)k * es + e] = (uint16_t)(std::lower_bound(tmp.begin(), tmp.end(), n) - tmp.begin());
+                }
+            blk_nodes.insert(blk_nodes.end(), tmp.begin(), tmp.end());
+            blk_ptr[b + 1] = (uint32_t)blk_nodes.size();
+            J.lds_nodes = std::max<uint32_t>(J.lds_nodes, (uint32_t)tmp.size());
+        }
+    }
     // node records
     std::vector<double> rec((size_t)Nn * NODE_REC, 0.0);
     for (size_t i = 0; i < Nn; ++i) {
@@ -374,7 +400,7 @@ ensc_status build_job(Job& J, int precision, int reorder, int blk_e, int blk_n) 
         if ((st = upload(&ds, none, 6 * es, A))) return st;
         if ((st = upload(&dq, none, 12 * es, A))) return st;
         if ((st = upload(&dp, none, es, A))) return st;
-        if ((st = upload(&df, none, 8 * es * 4, A))) return st;
+        if ((st = upload(&df, none, 8 * es * 3, A))) return st;
         if ((st = upload(&dx, none, Nn * 4, A))) return st;
         D.B0 = db;
         D.V0 = dv;
@@ -397,7 +423,7 @@ ensc_status build_job(Job& J, int precision, int reorder, int blk_e, int blk_n) 
         if ((st = upload(&ds, none, 6 * es, A))) return st;
         if ((st = upload(&dq, none, 12 * es, A))) return st;
         if ((st = upload(&dp, none, es, A))) return st;
-        if ((st = upload(&df, none, 8 * es * 4, A))) return st;
+        if ((st = upload(&df, none, 8 * es * 3, A))) return st;
         D.B0 = db;
         D.V0 = dv;
         D.sigma = ds;
@@ -428,6 +454,14 @@ ensc_status build_job(Job& J, int precision, int reorder, int blk_e, int blk_n) 
     uint32_t* d_ell;
     if ((st = upload(&d_ell, ell, ell.size(), A))) return st;
     D.ell = d_ell;
+    uint16_t* d_lconn;
+    uint32_t *d_bptr, *d_bnodes;
+    if ((st = upload(&d_lconn, lconn, lconn.size(), A))) return st;
+    if ((st = upload(&d_bptr, blk_ptr, blk_ptr.size(), A))) return st;
+    if ((st = upload(&d_bnodes, blk_nodes, blk_nodes.size(), A))) return st;
+    D.lconn = d_lconn;
+    D.blk_ptr = d_bptr;
+    D.blk_nodes = d_bnodes;
     double *d_at, *d_aa;
     int32_t* d_ao;
     if ((st = upload(&d_at, M.amp_t, M.amp_t.size(), A))) return st;

@@ -6,6 +6,23 @@
 
 namespace ensc {
 
+// Typed address spaces for device code.  Pointers read from the job table
+// are generic; casting them to the global (1) / LDS (3) space lets the
+// compiler emit global_load/ds_read with an SGPR base and a 32-bit VGPR
+// offset instead of flat accesses with 64-bit VGPR addresses.  Byte offsets
+// are formed in 32 bits: every per-job array is kept under 4 GiB (checked at
+// job build).
+#define ENSC_GLB __attribute__((address_space(1)))
+#define ENSC_LDS __attribute__((address_space(3)))
+template <class T>
+__device__ __forceinline__ const ENSC_GLB T& gld(const T* base, uint32_t idx) {
+    return *(const ENSC_GLB T*)((const ENSC_GLB char*)base + (uint32_t)(idx * (uint32_t)sizeof(T)));
+}
+template <class T>
+__device__ __forceinline__ ENSC_GLB T& gst(T* base, uint32_t idx) {
+    return *(ENSC_GLB T*)((ENSC_GLB char*)base + (uint32_t)(idx * (uint32_t)sizeof(T)));
+}
+
 // What the element kernel did in the current step iteration ("tick").
 enum : int32_t { KIND_NONE = 0, KIND_STEP = 1, KIND_PREPASS = 2, KIND_INIT = 3 };
 
@@ -36,7 +53,11 @@ struct JobDev {
     void* sigma;           // [6][ne_s]
     void* qia;             // [12][ne_s]
     void* epsp;            // [ne_s]
-    void* fel;             // [8][ne_s] x (double4 | float4): element nodal forces
+    void* fel;             // [8][ne_s][3] element nodal forces (corner-major, packed xyz)
+    // LDS staging of each element block's nodes (null when not staged)
+    const uint16_t* lconn;     // [8][ne_s] corner -> index in the block's node list
+    const uint32_t* blk_ptr;   // [nblk_e + 1] offsets into blk_nodes
+    const uint32_t* blk_nodes; // node ids of each element block, ascending
     // node arrays
     double* node;          // [nn][NODE_REC]
     float* xcomp;          // [nn][4] compensation of x (mixed) or null

@@ -65,6 +65,7 @@ struct Job {
     std::vector<uint32_t> e_new2old, e_old2new, n_new2old, n_old2new;
     size_t ne_s = 0, nn_s = 0;
     uint32_t ell_w = 0;
+    uint32_t lds_nodes = 0;  // largest distinct-node count of an element block
     double ke0 = 0.0;
     // setCurrentStep state (exDyna3D.h:215-220)
     uint32_t cur_step = 0;
@@ -79,6 +80,10 @@ struct Job {
     double* d_hist = nullptr;
     long long hist_cap = 0;
     bool inited = false;
+    bool pending_step = false;    // started by the serving loop, step 1 not switched in yet
+    bool keep_in_layout = false;  // single-job engines keep their job in every launch
+    JobDev* d_self = nullptr;     // device copy of this job's table entry (BC kernel)
+    JobDev self_copy{};
 };
 
 bool copy_model(const ensc_model_desc* m, ModelCopy& c, std::string& err);

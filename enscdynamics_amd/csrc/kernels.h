@@ -196,32 +196,83 @@ __device__ __forceinline__ T wave_min(T v) {
     return v;
 }
 
-// Element body instantiations.
-__device__ __forceinline__ void element_fp64(const JobDev& J, uint32_t e, double dt, int32_t kind, double time_n,
-                                           double& Le, double& dts, double* en, uint32_t& errf) {
+// Velocity BCs with an amplitude (extension) change the prescribed velocity
+// every step; the element pass applies the value at this step's time to the
+// corners it reads.  Out of line: jobs without amplitudes never call it.
+__device__ __noinline__ void bc_override(const JobDev& J, uint32_t g, double t, double* v) {
+#pragma clang fp contract(off)
+    const uint8_t mk = gld(J.bcmask, g);
+    if (!mk) return;
+    for (int d = 0; d < 3; ++d) {
+        if (mk & (BC_VEL0 << d)) v[d] = gld(J.bcval, g * 3 + d) * amp_value(J, gld(J.bcamp, g * 3 + d), t);
+        if (mk & (BC_SPC0 << d)) v[d] = 0.0;
+    }
+}
+
+#define CORNER_FENCE
+// Element body instantiations: corner data gathered from HBM (G) or from
+// the block's node records staged in LDS (S).
+#define ELEM_NODES_SETUP                                                   \
+    const double* ND = J.node;                                             \
+    uint32_t nid[8];                                                       \
+    _Pragma("unroll") for (int k = 0; k < 8; ++k) nid[k] = gld(J.conn, k * es + e);
+#define NVAL(k, i) gld(ND, nid[k] * NODE_REC + (i))
+#define DYN_BC_OVERRIDE(k, v) \
+    if (J.dyn_bc && kind == KIND_STEP) bc_override(J, nid[k], time_n, v)
+__device__ __forceinline__ void element_fp64_G(const JobDev& J, uint32_t e, double dt, int32_t kind, double time_n,
+                                               double& Le, double& dts, double* en, uint32_t& errf,
+                                               const ENSC_LDS double* lds) {
 #pragma clang fp contract(off)
     typedef double R;
     constexpr bool MIXED = false;
 #include "elem_body.inc"
 }
-
-__device__ __forceinline__ void element_mixed(const JobDev& J, uint32_t e, double dt, int32_t kind, double time_n,
-                                            double& Le, double& dts, double* en, uint32_t& errf) {
+__device__ __forceinline__ void element_mixed_G(const JobDev& J, uint32_t e, double dt, int32_t kind, double time_n,
+                                                double& Le, double& dts, double* en, uint32_t& errf,
+                                                const ENSC_LDS double* lds) {
 #pragma clang fp contract(fast)
     typedef float R;
     constexpr bool MIXED = true;
 #include "elem_body.inc"
 }
+#undef ELEM_NODES_SETUP
+#undef NVAL
+#undef DYN_BC_OVERRIDE
+#define ELEM_NODES_SETUP                                                   \
+    uint32_t lid[8];                                                       \
+    _Pragma("unroll") for (int k = 0; k < 8; ++k) lid[k] = gld(J.lconn, k * es + e);
+#define NVAL(k, i) lds[lid[k] * 9 + (i)]
+#define DYN_BC_OVERRIDE(k, v)  // applied once per node while staging
+__device__ __forceinline__ void element_fp64_S(const JobDev& J, uint32_t e, double dt, int32_t kind, double time_n,
+                                               double& Le, double& dts, double* en, uint32_t& errf,
+                                               const ENSC_LDS double* lds) {
+#pragma clang fp contract(off)
+    typedef double R;
+    constexpr bool MIXED = false;
+#include "elem_body.inc"
+}
+__device__ __forceinline__ void element_mixed_S(const JobDev& J, uint32_t e, double dt, int32_t kind, double time_n,
+                                                double& Le, double& dts, double* en, uint32_t& errf,
+                                                const ENSC_LDS double* lds) {
+#pragma clang fp contract(fast)
+    typedef float R;
+    constexpr bool MIXED = true;
+#include "elem_body.inc"
+}
+#undef ELEM_NODES_SETUP
+#undef NVAL
+#undef DYN_BC_OVERRIDE
 
-// Fixed-order reduction of a job's block partials by the job's last block.
-template <int BLK, int NV>
-__device__ inline void reduce_partials(const double* part, uint32_t nblk, const bool* is_min, double* out,
-                                       double* lds /* [NV][BLK] */) {
+// Fixed-order reduction of a job's block partials by wave 0 of the job's
+// last block: lane l folds blocks l, l+64, ... in ascending order, then a
+// fixed xor-shuffle tree combines the lanes.  Same inputs, same result.
+template <int NV>
+__device__ inline void reduce_partials(const double* part, uint32_t nblk, const bool* is_min, double* out) {
     const int t = threadIdx.x;
     double acc[NV];
 #pragma unroll
-    for (int q = 0; q < NV; ++q) acc[q] = is_min[q] ? 1.0 / 0.0 : 0.0;
-    for (uint32_t b = t; b < nblk; b += BLK) {
+    for (int q = 0; q < NV; ++q) acc[q] = is_min[q] ? __builtin_inf() : 0.0;
+    for (uint32_t b = t; b < nblk; b += 64) {
 #pragma unroll
         for (int q = 0; q < NV; ++q) {
             const double v = __builtin_nontemporal_load(part + (size_t)b * 8 + q);
@@ -229,25 +280,13 @@ __device__ inline void reduce_partials(const double* part, uint32_t nblk, const 
         }
     }
 #pragma unroll
-    for (int q = 0; q < NV; ++q) lds[q * BLK + t] = acc[q];
-    __syncthreads();
-    for (int s = BLK / 2; s > 0; s >>= 1) {
-        if (t < s) {
-#pragma unroll
-            for (int q = 0; q < NV; ++q) {
-                const double a = lds[q * BLK + t], b = lds[q * BLK + t + s];
-                lds[q * BLK + t] = is_min[q] ? (b < a ? b : a) : a + b;
-            }
-        }
-        __syncthreads();
-    }
-#pragma unroll
-    for (int q = 0; q < NV; ++q) out[q] = lds[q * BLK];
+    for (int q = 0; q < NV; ++q) out[q] = is_min[q] ? wave_min(acc[q]) : wave_sum(acc[q]);
 }
 
-template <bool MIXED, int BLK>
+template <bool MIXED, int BLK, bool STAGED>
 __global__ __launch_bounds__(BLK) void k_element(const JobDev* __restrict__ jobs, const uint32_t* __restrict__ blk2job) {
 #pragma clang fp contract(off)
+    extern __shared__ double s_nodes[];  // STAGED: [block's nodes][x0 x1 x2 v0 v1 v2 u0 u1 u2]
     const uint32_t b = blockIdx.x;
     const JobDev& J = jobs[blk2job[b]];
     Ctl* ctl = J.ctl;
@@ -255,14 +294,44 @@ __global__ __launch_bounds__(BLK) void k_element(const JobDev* __restrict__ jobs
     if (dc.kind == KIND_NONE) return;
     const uint32_t lb = b - J.eblk0;
     const uint32_t e = lb * BLK + threadIdx.x;
-    double Le = 1.0 / 0.0, dts = 1.0 / 0.0;
+    ENSC_LDS double* lds = (ENSC_LDS double*)s_nodes;
+    if (STAGED) {
+        // stage the block's node records once (x, v, u); velocity BCs with an
+        // amplitude are applied here, once per node
+        const uint32_t n0 = gld(J.blk_ptr, lb), n1 = gld(J.blk_ptr, lb + 1);
+        for (uint32_t i = threadIdx.x; i < n1 - n0; i += BLK) {
+            const uint32_t g = gld(J.blk_nodes, n0 + i);
+            const uint32_t r0 = g * NODE_REC;
+            double v[3] = {gld(J.node, r0 + 3), gld(J.node, r0 + 4), gld(J.node, r0 + 5)};
+            if (J.dyn_bc && dc.kind == KIND_STEP) bc_override(J, g, dc.time_n, v);
+            ENSC_LDS double* d = lds + i * 9;
+            d[0] = gld(J.node, r0);
+            d[1] = gld(J.node, r0 + 1);
+            d[2] = gld(J.node, r0 + 2);
+            d[3] = v[0];
+            d[4] = v[1];
+            d[5] = v[2];
+            d[6] = gld(J.node, r0 + 6);
+            d[7] = gld(J.node, r0 + 7);
+            d[8] = gld(J.node, r0 + 8);
+        }
+        __syncthreads();
+    }
+    double Le = __builtin_inf(), dts = __builtin_inf();
     double en[3] = {0.0, 0.0, 0.0};
     uint32_t errf = 0;
     if (e < J.ne) {
-        if (MIXED)
-            element_mixed(J, e, dc.dt_i_1, dc.kind, dc.time_n, Le, dts, en, errf);
-        else
-            element_fp64(J, e, dc.dt_i_1, dc.kind, dc.time_n, Le, dts, en, errf);
+        if (STAGED) {
+            if (MIXED)
+                element_mixed_S(J, e, dc.dt_i_1, dc.kind, dc.time_n, Le, dts, en, errf, lds);
+            else
+                element_fp64_S(J, e, dc.dt_i_1, dc.kind, dc.time_n, Le, dts, en, errf, lds);
+        } else {
+            if (MIXED)
+                element_mixed_G(J, e, dc.dt_i_1, dc.kind, dc.time_n, Le, dts, en, errf, lds);
+            else
+                element_fp64_G(J, e, dc.dt_i_1, dc.kind, dc.time_n, Le, dts, en, errf, lds);
+        }
     }
     // block partials: wave shuffles, then LDS across the block's waves
     __shared__ double s_red[BLK / 64][5];
@@ -306,12 +375,11 @@ __global__ __launch_bounds__(BLK) void k_element(const JobDev* __restrict__ jobs
         s_last = (t == J.nblk_e - 1);
     }
     __syncthreads();
-    if (!s_last) return;
+    if (!s_last || threadIdx.x >= 64) return;
     __threadfence();
-    __shared__ double s_lds[5 * BLK];
     const bool is_min[5] = {true, true, false, false, false};
     double tot[5];
-    reduce_partials<BLK, 5>(J.part_e, J.nblk_e, is_min, tot, s_lds);
+    reduce_partials<5>(J.part_e, J.nblk_e, is_min, tot);
     if (threadIdx.x == 0) {
         Ctl& c = *ctl;
         const double Lmin = tot[0], dtsn = tot[1];
@@ -372,23 +440,23 @@ __global__ __launch_bounds__(BLK) void k_node(const JobDev* __restrict__ jobs, c
     if (n < J.nn) {
         double acc[3] = {0.0, 0.0, 0.0}, grp[3] = {0.0, 0.0, 0.0};
         int gk = -1;
-        const size_t es = J.ne_s;
+        const uint32_t es = J.ne_s;
         for (int w = 0; w < J.ell_w; ++w) {
-            const uint32_t sl = J.ell[(size_t)w * J.nn_s + n];
+            const uint32_t sl = gld(J.ell, w * J.nn_s + n);
             if (sl == ELL_EMPTY) break;
             const int k = (int)(sl & 7u);
-            const size_t idx = (size_t)k * es + (sl >> 3);
+            const uint32_t idx = k * es + (sl >> 3);
             double f0, f1, f2;
             if (MIXED) {
-                const float4 q = static_cast<const float4*>(J.fel)[idx];
-                f0 = q.x;
-                f1 = q.y;
-                f2 = q.z;
+                const float* q = static_cast<const float*>(J.fel);
+                f0 = gld(q, 3 * idx);
+                f1 = gld(q, 3 * idx + 1);
+                f2 = gld(q, 3 * idx + 2);
             } else {
-                const double4 q = static_cast<const double4*>(J.fel)[idx];
-                f0 = q.x;
-                f1 = q.y;
-                f2 = q.z;
+                const double* q = static_cast<const double*>(J.fel);
+                f0 = gld(q, 3 * idx);
+                f1 = gld(q, 3 * idx + 1);
+                f2 = gld(q, 3 * idx + 2);
             }
             if (k != gk) {
                 if (gk >= 0) {
@@ -408,11 +476,11 @@ __global__ __launch_bounds__(BLK) void k_node(const JobDev* __restrict__ jobs, c
             acc[1] += grp[1];
             acc[2] += grp[2];
         }
-        double* rec = J.node + (size_t)n * NODE_REC;
-        double x[3] = {rec[0], rec[1], rec[2]};
-        double v[3] = {rec[3], rec[4], rec[5]};
-        double u[3] = {rec[6], rec[7], rec[8]};
-        const double im = rec[9];
+        const uint32_t r0 = n * NODE_REC;
+        double x[3] = {gld(J.node, r0), gld(J.node, r0 + 1), gld(J.node, r0 + 2)};
+        double v[3] = {gld(J.node, r0 + 3), gld(J.node, r0 + 4), gld(J.node, r0 + 5)};
+        double u[3] = {gld(J.node, r0 + 6), gld(J.node, r0 + 7), gld(J.node, r0 + 8)};
+        const double im = gld(J.node, r0 + 9);
         double a[3] = {0.0 + acc[0], 0.0 + acc[1], 0.0 + acc[2]};
         if (J.grav_on) {  // apply_gravity (800-826)
             const double ag = ctl->grav_amp * J.grav_g;
@@ -420,14 +488,14 @@ __global__ __launch_bounds__(BLK) void k_node(const JobDev* __restrict__ jobs, c
             a[1] += ag * J.grav_dir[1] / im;
             a[2] += ag * J.grav_dir[2] / im;
         }
-        const uint8_t mk = J.bcmask[n];
+        const uint8_t mk = gld(J.bcmask, n);
         double vbc[3] = {0.0, 0.0, 0.0};
         if (mk) {  // apply_boundary_condition_a (2029-2145): velocity lines, then SPC lines
             double rfv[3] = {0.0, 0.0, 0.0};
 #pragma unroll
             for (int d = 0; d < 3; ++d) {
                 if (mk & (BC_VEL0 << d)) {
-                    vbc[d] = J.bcval[(size_t)n * 3 + d] * amp_value(J, J.bcamp[(size_t)n * 3 + d], time);
+                    vbc[d] = gld(J.bcval, n * 3 + d) * amp_value(J, gld(J.bcamp, n * 3 + d), time);
                     pw += -a[d] * vbc[d];
                     a[d] = 0.0;
                 }
@@ -439,10 +507,9 @@ __global__ __launch_bounds__(BLK) void k_node(const JobDev* __restrict__ jobs, c
                 }
             }
             if (mk & (7u * BC_SPC0)) {
-                double* r = J.rf + (size_t)n * 3;
-                r[0] = rfv[0];
-                r[1] = rfv[1];
-                r[2] = rfv[2];
+                gst(J.rf, n * 3) = rfv[0];
+                gst(J.rf, n * 3 + 1) = rfv[1];
+                gst(J.rf, n * 3 + 2) = rfv[2];
             }
         }
         const double hs = dt_i + dt_i_1;
@@ -458,11 +525,10 @@ __global__ __launch_bounds__(BLK) void k_node(const JobDev* __restrict__ jobs, c
             const double dx = dt_i * vd;  // move_mesh (688-696)
             if (MIXED) {
                 // compensated (Kahan) position accumulation
-                float4* xc = reinterpret_cast<float4*>(J.xcomp) + n;
-                const float cf = (&xc->x)[d];
+                const float cf = gld(J.xcomp, n * 4 + d);
                 const double y = dx - (double)cf;
                 const double t = x[d] + y;
-                (&xc->x)[d] = (float)((t - x[d]) - y);
+                gst(J.xcomp, n * 4 + d) = (float)((t - x[d]) - y);
                 x[d] = t;
             } else {
                 x[d] += dx;
@@ -471,15 +537,15 @@ __global__ __launch_bounds__(BLK) void k_node(const JobDev* __restrict__ jobs, c
             k2 += vd * vd;
         }
         ke = 0.5 * k2 / im;
-        rec[0] = x[0];
-        rec[1] = x[1];
-        rec[2] = x[2];
-        rec[3] = v[0];
-        rec[4] = v[1];
-        rec[5] = v[2];
-        rec[6] = u[0];
-        rec[7] = u[1];
-        rec[8] = u[2];
+        gst(J.node, r0) = x[0];
+        gst(J.node, r0 + 1) = x[1];
+        gst(J.node, r0 + 2) = x[2];
+        gst(J.node, r0 + 3) = v[0];
+        gst(J.node, r0 + 4) = v[1];
+        gst(J.node, r0 + 5) = v[2];
+        gst(J.node, r0 + 6) = u[0];
+        gst(J.node, r0 + 7) = u[1];
+        gst(J.node, r0 + 8) = u[2];
     }
     __shared__ double s_red[BLK / 64][2];
     __shared__ int s_last;
@@ -505,12 +571,11 @@ __global__ __launch_bounds__(BLK) void k_node(const JobDev* __restrict__ jobs, c
         s_last = (t == J.nblk_n - 1);
     }
     __syncthreads();
-    if (!s_last) return;
+    if (!s_last || threadIdx.x >= 64) return;
     __threadfence();
-    __shared__ double s_lds[2 * BLK];
     const bool is_min[2] = {false, false};
     double tot[2];
-    reduce_partials<BLK, 2>(J.part_n, J.nblk_n, is_min, tot, s_lds);
+    reduce_partials<2>(J.part_n, J.nblk_n, is_min, tot);
     if (threadIdx.x == 0) {
         Ctl& c = *ctl;
         c.ke = tot[0];

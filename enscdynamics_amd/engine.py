@@ -49,6 +49,8 @@ class Options(C.Structure):
         ("n_probe", C.c_uint32),
         ("probe_nodes", C.POINTER(C.c_uint32)),
         ("hist_cap", C.c_int64),
+        ("staging", C.c_int32),
+        ("pad_", C.c_int32),
     ]
 
 
@@ -104,7 +106,8 @@ class Engine:
     """One job resident on the GPU, driven like exDyna3D."""
 
     def __init__(self, model: Model, precision: int = FP64, reorder: bool = False, block_elem: int = 0,
-                 block_node: int = 0, probes: Sequence[int] = (), hist_cap: int = 0, device: int = 0):
+                 block_node: int = 0, probes: Sequence[int] = (), hist_cap: int = 0, device: int = 0,
+                 staging: bool = True):
         L = load_library()
         self.model = model
         self.precision = precision
@@ -120,6 +123,7 @@ class Engine:
         o.n_probe = len(pr)
         o.probe_nodes = pr.ctypes.data_as(C.POINTER(C.c_uint32)) if len(pr) else None
         o.hist_cap = hist_cap
+        o.staging = 0 if staging else 2
         h = C.c_void_p()
         _check(L.ensc_engine_create(C.byref(self._desc), C.byref(o), C.byref(h)))
         self.h = h

@@ -56,6 +56,9 @@ typedef struct ensc_options {
     uint32_t n_probe;       /* nodes whose displacement is recorded per step */
     const uint32_t* probe_nodes;
     int64_t hist_cap;       /* rows of per-step history kept on device */
+    int32_t staging;        /* 0: stage element-block nodes in LDS when they fit
+                               (default), 2: always gather corners from HBM */
+    int32_t pad_;
 } ensc_options;
 
 /* Scalars of one job (ensc_engine_scalars). */
@@ -106,7 +109,7 @@ ensc_status ensc_engine_sync(ensc_engine* eng);
 
 /* Copy a field to host memory, in the caller's node/element numbering:
  * "u","v","x","rf" -> [n_nodes][3]; "sigma" -> [n_elems][6] (xx yy zz xy xz yz);
- * "qia" -> [n_elems][12]; "epsp","dtStable" -> [n_elems].
+ * "qia" -> [n_elems][12]; "epsp" -> [n_elems].
  * count is the number of doubles the caller's buffer holds. */
 ensc_status ensc_engine_download(ensc_engine* eng, const char* field, double* out, int64_t count);
 ensc_status ensc_engine_scalars(ensc_engine* eng, double* out /* ENSC_S_COUNT */);
@@ -121,6 +124,46 @@ ensc_status ensc_engine_kernel_times(ensc_engine* eng, double* elem_ms, double* 
 ensc_status ensc_engine_set_timing(ensc_engine* eng, int32_t on);
 /* Device bytes held by the engine. */
 int64_t ensc_engine_device_bytes(ensc_engine* eng);
+/* Capture `ticks` step iterations (element + node launch each) as a
+ * hipGraph and replay it for enqueued steps (0: plain launches).  The graph
+ * is re-captured whenever the set of stepped jobs changes. */
+ensc_status ensc_engine_set_graph(ensc_engine* eng, int32_t ticks);
+
+/* ---- batches: many independent meshes in one engine ----
+ * No counterpart in the reference (it runs one mesh per process).  Every job
+ * keeps its own time step, step end and BCs; one launch of each kernel per
+ * step iteration covers all unfinished jobs, whose element and node blocks
+ * are packed back to back.  ensc_engine_init / set_step / advance / run apply
+ * to every job of the engine; jobs are numbered 0..n-1 in creation order. */
+ensc_status ensc_batch_create(const ensc_model_desc* const* models, uint32_t n_models, const ensc_options* opts,
+                              ensc_engine** out);
+uint32_t ensc_batch_size(ensc_engine* eng);
+ensc_status ensc_batch_job_scalars(ensc_engine* eng, uint32_t job, double* out /* ENSC_S_COUNT */);
+ensc_status ensc_batch_job_download(ensc_engine* eng, uint32_t job, const char* field, double* out, int64_t count);
+
+/* ---- continuous-batching solve server ----
+ * No counterpart in the reference.  Requests queue on the host; each pump
+ * admits queued jobs in arrival order while their device footprint fits
+ * hbm_budget_bytes (and at most max_live_jobs are live), runs their init
+ * pass inside the next step iteration of the live batch, advances every
+ * live job together for `ticks` step iterations, and retires jobs that
+ * reached their end time (results to the host, device memory freed).
+ * Times are seconds on the server's steady clock (ensc_server_clock). */
+typedef struct ensc_server ensc_server;
+ensc_status ensc_server_create(const ensc_options* opts, int64_t hbm_budget_bytes, int32_t max_live_jobs,
+                               ensc_server** out);
+ensc_status ensc_server_destroy(ensc_server* srv);
+ensc_status ensc_server_set_graph(ensc_server* srv, int32_t ticks);
+/* Copy a job into the queue.  keep_result != 0 keeps its final displacement
+ * field for ensc_server_result.  t_arrival < 0 means now. */
+ensc_status ensc_server_submit(ensc_server* srv, const ensc_model_desc* model, int32_t keep_result, double t_arrival,
+                               uint64_t* ticket);
+/* Admit, step, retire.  Writes up to cap retired tickets to done. */
+ensc_status ensc_server_pump(ensc_server* srv, int32_t ticks, uint64_t* done, uint32_t cap, uint32_t* n_done);
+ensc_status ensc_server_result(ensc_server* srv, uint64_t ticket, double* scalars /* ENSC_S_COUNT */, double* u,
+                               int64_t u_count, double* times /* submit, admit, finish */);
+ensc_status ensc_server_status(ensc_server* srv, uint32_t* queued, uint32_t* live, int64_t* bytes_used);
+double ensc_server_clock(ensc_server* srv);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,89 @@
+"""Batched and served execution on the GPU.
+
+* alone == in-mix: each job's result in a batch (or in the serving loop,
+  admitted while others run) equals the same job run alone in its own
+  engine, bit for bit (per-job reductions only, fixed order);
+* a hipGraph-captured step replays to the same state as eager launches;
+* every job reaches its own end time, and FP64 batch results match the
+  CPU oracle.
+"""
+import numpy as np
+import pytest
+
+from enscdynamics_amd import decks
+from enscdynamics_amd import engine as eng
+from enscdynamics_amd.batch import Batch, Server
+from oracle.oracle import Oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def small_jobs(n, seed=7, plastic_p=0.0):
+    rng = np.random.default_rng(seed)
+    return [decks.random_job(rng, 300, 4000, plastic_p, k_lo=150, k_hi=400, name=f"t{i}") for i in range(n)]
+
+
+@pytest.mark.parametrize("precision", [eng.FP64, eng.MIXED])
+def test_batch_alone_equals_in_mix(precision):
+    jobs = small_jobs(10, seed=3, plastic_p=0.5)
+    b = Batch(jobs, precision, reorder=True).run()
+    for i, m in enumerate(jobs):
+        s = b.scalars(i)
+        assert s["done"] == 1.0 and s["err"] == 0.0
+        assert s["time"] == pytest.approx(m.total_time, rel=1e-12)
+        a = eng.Engine(m, precision, reorder=True).run()
+        for f in ("u", "v", "sigma", "qia"):
+            assert np.array_equal(b.get(i, f), a.get(f)), (i, f)
+        sa = a.scalars()
+        for k in ("nsteps", "ie", "hg", "vis", "ke"):
+            assert s[k] == sa[k], (i, k)
+    print(f"[batch] {len(jobs)} jobs equal alone and in the mix", flush=True)
+
+
+def test_batch_fp64_matches_oracle():
+    jobs = small_jobs(4, seed=5, plastic_p=0.5)
+    b = Batch(jobs, eng.FP64).run()
+    for i, m in enumerate(jobs):
+        o = Oracle(m, 0).init()
+        o.run(0, [], 0)
+        for f in ("u", "sigma"):
+            ref = o.get(f)
+            err = np.abs(b.get(i, f) - ref).max() / max(np.abs(ref).max(), 1e-30)
+            assert err <= 1e-10, (i, f, err)
+
+
+def test_graph_replay_matches_eager():
+    jobs = small_jobs(6, seed=9)
+    a = Batch(jobs, eng.MIXED, graph_ticks=0).run()
+    g = Batch(jobs, eng.MIXED, graph_ticks=16).run()
+    for i in range(len(jobs)):
+        for f in ("u", "sigma"):
+            assert np.array_equal(a.get(i, f), g.get(i, f)), (i, f)
+        assert a.scalars(i)["nsteps"] == g.scalars(i)["nsteps"]
+
+
+def test_server_alone_equals_in_mix_and_latency():
+    jobs = small_jobs(8, seed=21, plastic_p=0.5)
+    srv = Server(eng.MIXED, budget_bytes=64 << 20, max_live=4, graph_ticks=8)
+    tickets = []
+    done = []
+    # staggered arrivals: submit two jobs, pump, submit more while they run
+    for i, m in enumerate(jobs):
+        tickets.append(srv.submit(m, keep=True))
+        if i % 2 == 1:
+            done += srv.pump(24)
+    guard = 0
+    while len(done) < len(jobs):
+        done += srv.pump(64)
+        guard += 1
+        assert guard < 10000
+    assert sorted(done) == sorted(tickets)
+    q, live, used = srv.status()
+    assert q == 0 and live == 0 and used == 0
+    for t, m in zip(tickets, jobs):
+        s, u, times = srv.result(t, with_u=True)
+        assert times[0] <= times[1] <= times[2]
+        assert s["time"] == pytest.approx(m.total_time, rel=1e-12)
+        a = eng.Engine(m, eng.MIXED, reorder=True).run()
+        assert np.array_equal(u, a.get("u")), t
+        assert s["nsteps"] == a.scalars()["nsteps"]

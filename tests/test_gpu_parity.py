@@ -201,7 +201,7 @@ def test_bad_models_fail_loudly():
     m = decks.config3(n=4, steps=10)
     bad = decks.config3(n=4, steps=10)
     bad.conn = bad.conn.copy()
-    bad.conn[0, [0, 1]] = bad.conn[0, [1, 0]]  # swap two corners: inverted element
+    bad.conn[0] = bad.conn[0][[4, 5, 6, 7, 0, 1, 2, 3]]  # top and bottom faces swapped: inverted element
     with pytest.raises(eng.EnscError):
         eng.Engine(bad)
     nan = decks.config3(n=4, steps=10)
