@@ -52,7 +52,7 @@ def _lib():
     return L
 
 
-def _options(precision: int, reorder: bool, block_elem: int, staging: bool) -> _eng.Options:
+def _options(precision: int, reorder: bool, block_elem: int, staging: bool, waves: int = 0) -> _eng.Options:
     o = _eng.Options()
     o.precision = precision
     o.reorder = int(reorder)
@@ -63,6 +63,7 @@ def _options(precision: int, reorder: bool, block_elem: int, staging: bool) -> _
     o.probe_nodes = None
     o.hist_cap = 0
     o.staging = 0 if staging else 2
+    o.waves = waves
     return o
 
 
@@ -70,12 +71,12 @@ class Batch:
     """Many meshes stepped together, each to its own end time."""
 
     def __init__(self, models: Sequence[Model], precision: int = _eng.FP64, reorder: bool = True,
-                 block_elem: int = 0, graph_ticks: int = 0, staging: bool = True):
+                 block_elem: int = 0, graph_ticks: int = 0, staging: bool = True, waves: int = 0):
         L = _lib()
         self.models = list(models)
         self._descs = [m.desc() for m in self.models]
         arr = (C.c_void_p * len(self._descs))(*[C.addressof(d) for d in self._descs])
-        self._opts = _options(precision, reorder, block_elem, staging)
+        self._opts = _options(precision, reorder, block_elem, staging, waves)
         h = C.c_void_p()
         _eng._check(L.ensc_batch_create(arr, len(self._descs), C.byref(self._opts), C.byref(h)))
         self.h = h
@@ -118,9 +119,10 @@ class Server:
     """Continuous-batching solve server."""
 
     def __init__(self, precision: int = _eng.FP64, budget_bytes: int = 64 << 30, max_live: int = 1024,
-                 reorder: bool = True, graph_ticks: int = 0, block_elem: int = 0, staging: bool = True):
+                 reorder: bool = True, graph_ticks: int = 0, block_elem: int = 0, staging: bool = True,
+                 waves: int = 0):
         L = _lib()
-        self._opts = _options(precision, reorder, block_elem, staging)
+        self._opts = _options(precision, reorder, block_elem, staging, waves)
         h = C.c_void_p()
         _eng._check(L.ensc_server_create(C.byref(self._opts), int(budget_bytes), int(max_live), C.byref(h)))
         self.h = h

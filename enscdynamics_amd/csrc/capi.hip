@@ -52,7 +52,7 @@ struct DevBuf {
 }  // namespace
 
 struct ensc_engine {
-    int device = 0, precision = ENSC_FP64, blk_e = 256, blk_n = 256, reorder = 0, staging_opt = 0;
+    int device = 0, precision = ENSC_FP64, blk_e = 256, blk_n = 256, reorder = 0, staging_opt = 0, waves = 2;
     hipStream_t stream = nullptr;
     std::vector<std::unique_ptr<Job>> slots;  // job id = slot index; null = free
     // per-slot scalars (device array + pinned host mirror), never moved while jobs live
@@ -70,7 +70,7 @@ struct ensc_engine {
     // timing
     bool timing = false;
     std::vector<hipEvent_t> evs;
-    double elem_ms = 0.0, node_ms = 0.0;
+    double elem_ms = 0.0, node_ms = 0.0, phase_ms[4] = {0.0, 0.0, 0.0, 0.0};
     int64_t launches = 0, pending_ticks = 0;
     // graph of graph_ticks ticks, valid for the current layout
     int32_t graph_ticks = 0;
@@ -81,12 +81,30 @@ struct ensc_engine {
 
 namespace {
 
-template <bool M, bool S>
-KernelFn pick_e(int blk) {
+template <bool M, int W>
+KernelFn pick_staged(int blk) {
     switch (blk) {
-        case 64: return k_element<M, 64, S>;
-        case 128: return k_element<M, 128, S>;
-        default: return k_element<M, 256, S>;
+        case 64: return k_element<M, 64, true, W>;
+        case 128: return k_element<M, 128, true, W>;
+        default: return k_element<M, 256, true, W>;
+    }
+}
+// Element kernel for (precision, block size, staging, occupancy target).
+// The occupancy target (waves per SIMD) caps the kernel's registers; it is
+// a tuning choice only and never changes results.
+template <bool M>
+KernelFn pick_e(int blk, bool staged, int waves) {
+    if (!staged) {
+        switch (blk) {
+            case 64: return k_element<M, 64, false, 1>;
+            case 128: return k_element<M, 128, false, 1>;
+            default: return k_element<M, 256, false, 1>;
+        }
+    }
+    switch (waves) {
+        case 1: return pick_staged<M, 1>(blk);
+        case 3: return pick_staged<M, 3>(blk);
+        default: return pick_staged<M, 2>(blk);
     }
 }
 template <bool M>
@@ -190,29 +208,36 @@ ensc_status relayout(ensc_engine* E) {
     // LDS staging when every live job's largest block node list fits 64 KiB
     E->staged = E->staging_opt != 2 && (size_t)lds_nodes * 9 * sizeof(double) <= 65536;
     E->lds_bytes = E->staged ? (size_t)std::max<uint32_t>(lds_nodes, 1) * 9 * sizeof(double) : 0;
-    if (E->staged)
-        E->ke = mixed ? pick_e<true, true>(E->blk_e) : pick_e<false, true>(E->blk_e);
-    else
-        E->ke = mixed ? pick_e<true, false>(E->blk_e) : pick_e<false, false>(E->blk_e);
+    E->ke = mixed ? pick_e<true>(E->blk_e, E->staged, E->waves) : pick_e<false>(E->blk_e, E->staged, E->waves);
     E->kn = mixed ? pick_n<true>(E->blk_n) : pick_n<false>(E->blk_n);
     E->dirty = false;
     E->graph_valid = false;
     return ENSC_OK;
 }
 
-ensc_status launch_tick(ensc_engine* E) {
+// One tick = four launches: element pass, per-job element finalize, nodal
+// update, per-job node finalize.  `ev` (timing only) gets an event after
+// each launch.
+constexpr int FIN_BLK = 256;
+void launch_phase(ensc_engine* E, int phase) {
     const JobDev* jobs = static_cast<const JobDev*>(E->jobs_buf.p);
-    if (E->nblk_e)
-        hipLaunchKernelGGL(E->ke, dim3(E->nblk_e), dim3(E->blk_e), E->lds_bytes, E->stream, jobs,
-                           static_cast<const uint32_t*>(E->b2e_buf.p));
-    return ENSC_OK;
+    const uint32_t nj = (uint32_t)E->live.size();
+    if (!nj) return;
+    switch (phase) {
+        case 0:
+            hipLaunchKernelGGL(E->ke, dim3(E->nblk_e), dim3(E->blk_e), E->lds_bytes, E->stream, jobs,
+                               static_cast<const uint32_t*>(E->b2e_buf.p));
+            break;
+        case 1: hipLaunchKernelGGL(k_fin_element<FIN_BLK>, dim3(nj), dim3(FIN_BLK), 0, E->stream, jobs); break;
+        case 2:
+            hipLaunchKernelGGL(E->kn, dim3(E->nblk_n), dim3(E->blk_n), 0, E->stream, jobs,
+                               static_cast<const uint32_t*>(E->b2n_buf.p));
+            break;
+        default: hipLaunchKernelGGL(k_fin_node<FIN_BLK>, dim3(nj), dim3(FIN_BLK), 0, E->stream, jobs); break;
+    }
 }
-ensc_status launch_node(ensc_engine* E) {
-    const JobDev* jobs = static_cast<const JobDev*>(E->jobs_buf.p);
-    if (E->nblk_n)
-        hipLaunchKernelGGL(E->kn, dim3(E->nblk_n), dim3(E->blk_n), 0, E->stream, jobs,
-                           static_cast<const uint32_t*>(E->b2n_buf.p));
-    return ENSC_OK;
+void launch_tick(ensc_engine* E) {
+    for (int p = 0; p < 4; ++p) launch_phase(E, p);
 }
 
 ensc_status build_graph(ensc_engine* E) {
@@ -225,10 +250,7 @@ ensc_status build_graph(ensc_engine* E) {
         E->graph = nullptr;
     }
     HIPCHK(hipStreamBeginCapture(E->stream, hipStreamCaptureModeThreadLocal));
-    for (int i = 0; i < E->graph_ticks; ++i) {
-        launch_tick(E);
-        launch_node(E);
-    }
+    for (int i = 0; i < E->graph_ticks; ++i) launch_tick(E);
     HIPCHK(hipStreamEndCapture(E->stream, &E->graph));
     HIPCHK(hipGraphInstantiate(&E->graph_exec, E->graph, nullptr, nullptr, 0));
     E->graph_valid = true;
@@ -239,19 +261,18 @@ ensc_status enqueue_ticks(ensc_engine* E, int64_t n) {
     ensc_status st;
     if (E->dirty && (st = relayout(E))) return st;
     if (E->timing) {
-        const size_t need = 2 * (size_t)n + 1;
+        const size_t need = 4 * (size_t)n + 1;
         while (E->evs.size() < need) {
             hipEvent_t ev;
             HIPCHK(hipEventCreate(&ev));
             E->evs.push_back(ev);
         }
         HIPCHK(hipEventRecord(E->evs[0], E->stream));
-        for (int64_t i = 0; i < n; ++i) {
-            launch_tick(E);
-            HIPCHK(hipEventRecord(E->evs[2 * i + 1], E->stream));
-            launch_node(E);
-            HIPCHK(hipEventRecord(E->evs[2 * i + 2], E->stream));
-        }
+        for (int64_t i = 0; i < n; ++i)
+            for (int p = 0; p < 4; ++p) {
+                launch_phase(E, p);
+                HIPCHK(hipEventRecord(E->evs[4 * i + p + 1], E->stream));
+            }
         E->pending_ticks = n;
     } else {
         int64_t i = 0;
@@ -259,10 +280,7 @@ ensc_status enqueue_ticks(ensc_engine* E, int64_t n) {
             if (!E->graph_valid && (st = build_graph(E))) return st;
             for (; i + E->graph_ticks <= n; i += E->graph_ticks) HIPCHK(hipGraphLaunch(E->graph_exec, E->stream));
         }
-        for (; i < n; ++i) {
-            launch_tick(E);
-            launch_node(E);
-        }
+        for (; i < n; ++i) launch_tick(E);
         E->pending_ticks = 0;
     }
     HIPCHK(hipGetLastError());
@@ -271,16 +289,16 @@ ensc_status enqueue_ticks(ensc_engine* E, int64_t n) {
 
 ensc_status finish_timing(ensc_engine* E) {
     if (!E->timing || E->pending_ticks == 0) return ENSC_OK;
-    double se = 0.0, sn = 0.0;
-    for (int64_t i = 0; i < E->pending_ticks; ++i) {
-        float a = 0.f, b = 0.f;
-        HIPCHK(hipEventElapsedTime(&a, E->evs[2 * i], E->evs[2 * i + 1]));
-        HIPCHK(hipEventElapsedTime(&b, E->evs[2 * i + 1], E->evs[2 * i + 2]));
-        se += a;
-        sn += b;
-    }
-    E->elem_ms = se / E->pending_ticks;
-    E->node_ms = sn / E->pending_ticks;
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int64_t i = 0; i < E->pending_ticks; ++i)
+        for (int p = 0; p < 4; ++p) {
+            float a = 0.f;
+            HIPCHK(hipEventElapsedTime(&a, E->evs[4 * i + p], E->evs[4 * i + p + 1]));
+            acc[p] += a;
+        }
+    for (int p = 0; p < 4; ++p) E->phase_ms[p] = acc[p] / E->pending_ticks;
+    E->elem_ms = E->phase_ms[0];
+    E->node_ms = E->phase_ms[2];
     E->launches = E->pending_ticks;
     E->pending_ticks = 0;
     return ENSC_OK;
@@ -304,6 +322,8 @@ ensc_status engine_new(const ensc_options* opts, std::unique_ptr<ensc_engine>& E
     auto okblk = [](int b) { return b == 64 || b == 128 || b == 256; };
     E->blk_e = okblk(o.block_elem) ? o.block_elem : 256;
     E->blk_n = okblk(o.block_node) ? o.block_node : 256;
+    if (o.waves < 0 || o.waves > 3) return fail(ENSC_ERR_ARG, "waves must be 0 (default), 1, 2 or 3");
+    E->waves = o.waves ? o.waves : 2;
     HIPCHK(hipSetDevice(E->device));
     HIPCHK(hipStreamCreateWithFlags(&E->stream, hipStreamNonBlocking));
     return ENSC_OK;
@@ -795,6 +815,14 @@ ensc_status ensc_engine_kernel_times(ensc_engine* E, double* elem_ms, double* no
     if (elem_ms) *elem_ms = E->elem_ms;
     if (node_ms) *node_ms = E->node_ms;
     if (launches) *launches = E->launches;
+    return ENSC_OK;
+}
+
+ensc_status ensc_engine_phase_times(ensc_engine* E, double* ms4) {
+    if (!E || !ms4) return fail(ENSC_ERR_ARG, "null argument");
+    ensc_status st = sync(E);
+    if (st) return st;
+    for (int p = 0; p < 4; ++p) ms4[p] = E->phase_ms[p];
     return ENSC_OK;
 }
 

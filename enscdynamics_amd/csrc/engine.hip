@@ -66,15 +66,20 @@ static void inv3h(const double m[3][3], double r[3][3]) {
 
 // FEValues<8,3>::reinit at the single Gauss point (fevalues.h:233-265) and
 // compute_invMassMatrix (exDyna3D.cpp:492-515), original element order.
+//
+// B0 = Ji * dN with every dN entry +-1/8, so B0(k,i) = sum_j G(i,j) * sign(k,j)
+// for G = Ji / 8; multiplying by 1/8 is exact, and the kernels form that sum
+// in the reference's order (j = 0, 1, 2), so storing G (9 values) instead of
+// B0 (24 values) reproduces B0 bit for bit.
 static bool setup_geometry(const std::vector<double>& x0, const std::vector<uint32_t>& conn, size_t Ne, double rho,
-                           std::vector<double>& B0, std::vector<double>& V0, std::vector<double>& invM, size_t Nn,
+                           std::vector<double>& G8, std::vector<double>& V0, std::vector<double>& invM, size_t Nn,
                            std::string& err) {
     static const double cs[8][3] = {{-1, -1, -1}, {1, -1, -1}, {1, 1, -1}, {-1, 1, -1},
                                     {-1, -1, 1},  {1, -1, 1},  {1, 1, 1},  {-1, 1, 1}};
     double dN[3][8];
     for (int a = 0; a < 8; ++a)
         for (int d = 0; d < 3; ++d) dN[d][a] = 1.0 / 8.0 * cs[a][d];
-    B0.assign(24 * Ne, 0.0);
+    G8.assign(9 * Ne, 0.0);
     V0.assign(Ne, 0.0);
     std::vector<double> mass(Nn, 0.0);
     for (size_t e = 0; e < Ne; ++e) {
@@ -96,13 +101,8 @@ static bool setup_geometry(const std::vector<double>& x0, const std::vector<uint
             err = "element " + std::to_string(e) + " has non-positive reference volume (inverted or degenerate)";
             return false;
         }
-        for (int k = 0; k < 8; ++k)
-            for (int i = 0; i < 3; ++i) {
-                double s = Ji[i][0] * dN[0][k];
-                s += Ji[i][1] * dN[1][k];
-                s += Ji[i][2] * dN[2][k];
-                B0[24 * e + 3 * k + i] = s;
-            }
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) G8[9 * e + 3 * i + j] = Ji[i][j] * 0.125;
         const double nodeMass = rho * V0[e] / 8.0;
         for (int k = 0; k < 8; ++k) mass[conn[8 * e + k]] += nodeMass;
     }
@@ -294,8 +294,8 @@ ensc_status build_job(Job& J, int precision, int reorder, int blk_e, int blk_n) 
     const size_t Ne = M.Ne, Nn = M.Nn;
     const bool mixed = precision == ENSC_MIXED;
     std::string err;
-    std::vector<double> B0, V0, invM;
-    if (!setup_geometry(M.x0, M.conn, Ne, M.rho, B0, V0, invM, Nn, err)) return fail(ENSC_ERR_ARG, err);
+    std::vector<double> G8, V0, invM;
+    if (!setup_geometry(M.x0, M.conn, Ne, M.rho, G8, V0, invM, Nn, err)) return fail(ENSC_ERR_ARG, err);
     // numbering
     if (reorder) {
         morton_order(M.x0, M.conn, Ne, Nn, J.e_new2old, J.n_new2old);
@@ -387,10 +387,10 @@ ensc_status build_job(Job& J, int precision, int reorder, int blk_e, int blk_n) 
     if ((st = upload(&d_conn, conn_d, conn_d.size(), A))) return st;
     D.conn = d_conn;
     if (mixed) {
-        std::vector<float> b(24 * es, 0.f), v(es, 0.f);
+        std::vector<float> b(9 * es, 0.f), v(es, 0.f);
         for (size_t i = 0; i < Ne; ++i) {
             const size_t eo = J.e_new2old[i];
-            for (int q = 0; q < 24; ++q) b[(size_t)q * es + i] = (float)B0[24 * eo + q];
+            for (int q = 0; q < 9; ++q) b[(size_t)q * es + i] = (float)G8[9 * eo + q];
             v[i] = (float)V0[eo];
         }
         float *db, *dv, *ds, *dq, *dp, *df, *dx;
@@ -402,7 +402,7 @@ ensc_status build_job(Job& J, int precision, int reorder, int blk_e, int blk_n) 
         if ((st = upload(&dp, none, es, A))) return st;
         if ((st = upload(&df, none, 8 * es * 3, A))) return st;
         if ((st = upload(&dx, none, Nn * 4, A))) return st;
-        D.B0 = db;
+        D.Gi = db;
         D.V0 = dv;
         D.sigma = ds;
         D.qia = dq;
@@ -410,10 +410,10 @@ ensc_status build_job(Job& J, int precision, int reorder, int blk_e, int blk_n) 
         D.fel = df;
         D.xcomp = dx;
     } else {
-        std::vector<double> b(24 * es, 0.0), v(es, 0.0);
+        std::vector<double> b(9 * es, 0.0), v(es, 0.0);
         for (size_t i = 0; i < Ne; ++i) {
             const size_t eo = J.e_new2old[i];
-            for (int q = 0; q < 24; ++q) b[(size_t)q * es + i] = B0[24 * eo + q];
+            for (int q = 0; q < 9; ++q) b[(size_t)q * es + i] = G8[9 * eo + q];
             v[i] = V0[eo];
         }
         double *db, *dv, *ds, *dq, *dp, *df;
@@ -424,7 +424,7 @@ ensc_status build_job(Job& J, int precision, int reorder, int blk_e, int blk_n) 
         if ((st = upload(&dq, none, 12 * es, A))) return st;
         if ((st = upload(&dp, none, es, A))) return st;
         if ((st = upload(&df, none, 8 * es * 3, A))) return st;
-        D.B0 = db;
+        D.Gi = db;
         D.V0 = dv;
         D.sigma = ds;
         D.qia = dq;

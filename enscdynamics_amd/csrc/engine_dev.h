@@ -22,6 +22,22 @@ template <class T>
 __device__ __forceinline__ ENSC_GLB T& gst(T* base, uint32_t idx) {
     return *(ENSC_GLB T*)((ENSC_GLB char*)base + (uint32_t)(idx * (uint32_t)sizeof(T)));
 }
+// 16-byte pair of doubles: node records are read and written two values per
+// access (records are 80 bytes, so every even index is 16-byte aligned).
+typedef double dbl2 __attribute__((ext_vector_type(2)));
+struct D2 {
+    double a, b;
+};
+__device__ __forceinline__ D2 gld2(const double* base, uint32_t idx) {
+    const dbl2 v = *(const ENSC_GLB dbl2*)((const ENSC_GLB char*)base + (uint32_t)(idx * 8u));
+    return D2{v.x, v.y};
+}
+__device__ __forceinline__ void gst2(double* base, uint32_t idx, double a, double b) {
+    dbl2 v;
+    v.x = a;
+    v.y = b;
+    *(ENSC_GLB dbl2*)((ENSC_GLB char*)base + (uint32_t)(idx * 8u)) = v;
+}
 
 // What the element kernel did in the current step iteration ("tick").
 enum : int32_t { KIND_NONE = 0, KIND_STEP = 1, KIND_PREPASS = 2, KIND_INIT = 3 };
@@ -42,13 +58,14 @@ struct Ctl {
     long long nsteps;
     long long step_limit;  // no step starts once nsteps reaches it
     int32_t kind, have_lpre, init_pending, step_index;
-    uint32_t ticket_e, ticket_n, err, pad;
+    uint32_t err, pad;
 };
 
 struct JobDev {
     // element arrays, SoA with stride ne_s (precision-dependent element type)
     const uint32_t* conn;  // [8][ne_s]
-    const void* B0;        // [24][ne_s]  B0(k,j) at row 3k+j
+    const void* Gi;        // [9][ne_s]  Gi(i,j) = (dX/dxi)^-1(i,j) / 8 at row 3i+j;
+                           //   B0(k,i) = sum_j Gi(i,j) * sign(k,j) (FEValues at the centre)
     const void* V0;        // [ne_s]
     void* sigma;           // [6][ne_s]
     void* qia;             // [12][ne_s]

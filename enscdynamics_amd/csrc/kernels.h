@@ -5,20 +5,25 @@
 //   k_element : one element per lane.  Gathers node x/u/v, runs the element
 //               body (elem_body.inc), writes the element's 8 nodal forces,
 //               and reduces min edge / min stable dt / work terms per block.
-//               The last block of each job (atomic ticket) reduces the block
-//               partials in a fixed order and advances the job's time-step
-//               scalars exactly as update_minVertex_perMaterial_and_dt and
-//               the run-loop clamps do (exDyna3D.cpp:663-686, 90-97, 775-780).
+//   k_fin_element : one block per job.  Reduces the job's block partials in
+//               a fixed order and advances its time-step scalars exactly as
+//               update_minVertex_perMaterial_and_dt and the run-loop clamps
+//               do (exDyna3D.cpp:663-686, 90-97, 775-780).
 //   k_node    : one node per lane.  Sums the node's element forces in the
 //               canonical (corner, element) order, adds gravity, applies the
 //               BCs and reactions (2029-2145), and does the central-difference
 //               update a -> v -> u -> x (642-696).
+//   k_fin_node : one block per job: kinetic energy, external work, history.
+// Cross-block results travel through per-block partials and the launch
+// boundaries; no kernel waits on, or fences for, another block.
 // The step size of a tick depends on the min edge of the current
 // coordinates, which the element pass itself computes.  Only the overshoot
 // clamp near the end time (682-685) needs that value before the pass; a tick
 // that can hit it (decided from the previous pass's bound) runs a min-edge
 // pre-pass instead and the next tick does the step.
 #pragma once
+
+#include <type_traits>
 
 #include "engine_dev.h"
 #include "../../include/ensc_engine.h"
@@ -41,6 +46,60 @@ template <>
 struct Vec4<float> {
     typedef float4 T;
 };
+
+// Corner signs of the hex8 shape-function derivatives at the centre
+// (Element_HexN8: dN(d, k) = CS[k][d] / 8).
+__device__ constexpr double CS_[8][3] = {{-1, -1, -1}, {1, -1, -1}, {1, 1, -1}, {-1, 1, -1},
+                                         {-1, -1, 1},  {1, -1, 1},  {1, 1, 1},  {-1, 1, 1}};
+
+// Row k of the reference gradient, B0(k, i) = sum_j Ji(i,j) dN(j,k), from
+// Gi = Ji / 8: each product is +-Gi(i,j) exactly, summed in the same order
+// (j = 0, 1, 2) as the setup pass (fevalues.h:233-265), so the FP64 path
+// reproduces the stored B0 bit for bit.
+template <class R>
+__device__ __forceinline__ void bgrad(const R Gi[3][3], int k, R Bk[3]) {
+#pragma clang fp contract(off)
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        R t = CS_[k][0] > 0 ? Gi[i][0] : -Gi[i][0];
+        t += CS_[k][1] > 0 ? Gi[i][1] : -Gi[i][1];
+        t += CS_[k][2] > 0 ? Gi[i][2] : -Gi[i][2];
+        Bk[i] = t;
+    }
+}
+
+// An element's stored state, loaded at the top of the element pass (before
+// the block's node staging, so the two sets of loads overlap).
+template <class R>
+struct ElemState {
+    uint32_t nd[8];  // corner node ids (gathered path) or LDS slots (staged path)
+    R Gi[3][3];
+    R V0;
+    R s[6];
+    R Q[12];
+    R ep;
+};
+
+template <class R, bool STAGED>
+__device__ __forceinline__ void load_state(const JobDev& J, uint32_t e, bool full, ElemState<R>& S) {
+    const uint32_t es = J.ne_s;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) S.nd[k] = STAGED ? (uint32_t)gld(J.lconn, k * es + e) : gld(J.conn, k * es + e);
+    if (!full) return;
+    const R* Gp = static_cast<const R*>(J.Gi);
+    const R* Sp = static_cast<const R*>(J.sigma);
+    const R* Qp = static_cast<const R*>(J.qia);
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) S.Gi[i][j] = gld(Gp, (3 * i + j) * es + e);
+    S.V0 = gld(static_cast<const R*>(J.V0), e);
+#pragma unroll
+    for (int q = 0; q < 6; ++q) S.s[q] = gld(Sp, q * es + e);
+#pragma unroll
+    for (int q = 0; q < 12; ++q) S.Q[q] = gld(Qp, q * es + e);
+    S.ep = J.plastic ? gld(static_cast<const R*>(J.epsp), e) : R(0);
+}
 
 // Eigen 3.4's 3x3 determinant (Determinant.h:19-24, 56-65).
 template <class R>
@@ -199,80 +258,97 @@ __device__ __forceinline__ T wave_min(T v) {
 // Velocity BCs with an amplitude (extension) change the prescribed velocity
 // every step; the element pass applies the value at this step's time to the
 // corners it reads.  Out of line: jobs without amplitudes never call it.
-__device__ __noinline__ void bc_override(const JobDev& J, uint32_t g, double t, double* v) {
+// Values travel by value so the caller keeps them in registers.
+struct V3 {
+    double x, y, z;
+};
+__device__ __noinline__ V3 bc_override(const JobDev& J, uint32_t g, double t, V3 in) {
 #pragma clang fp contract(off)
     const uint8_t mk = gld(J.bcmask, g);
-    if (!mk) return;
+    if (!mk) return in;
+    double v[3] = {in.x, in.y, in.z};
     for (int d = 0; d < 3; ++d) {
         if (mk & (BC_VEL0 << d)) v[d] = gld(J.bcval, g * 3 + d) * amp_value(J, gld(J.bcamp, g * 3 + d), t);
         if (mk & (BC_SPC0 << d)) v[d] = 0.0;
     }
+    return V3{v[0], v[1], v[2]};
 }
 
 #define CORNER_FENCE
 // Element body instantiations: corner data gathered from HBM (G) or from
 // the block's node records staged in LDS (S).
-#define ELEM_NODES_SETUP                                                   \
-    const double* ND = J.node;                                             \
-    uint32_t nid[8];                                                       \
-    _Pragma("unroll") for (int k = 0; k < 8; ++k) nid[k] = gld(J.conn, k * es + e);
-#define NVAL(k, i) gld(ND, nid[k] * NODE_REC + (i))
-#define DYN_BC_OVERRIDE(k, v) \
-    if (J.dyn_bc && kind == KIND_STEP) bc_override(J, nid[k], time_n, v)
-__device__ __forceinline__ void element_fp64_G(const JobDev& J, uint32_t e, double dt, int32_t kind, double time_n,
-                                               double& Le, double& dts, double* en, uint32_t& errf,
-                                               const ENSC_LDS double* lds) {
+#define NVAL(k, i) gld((const double*)J.node, ES.nd[k] * NODE_REC + (i))
+#define DYN_BC_OVERRIDE(k, v)                                              \
+    if (J.dyn_bc && kind == KIND_STEP) {                                   \
+        const V3 o_ = bc_override(J, ES.nd[k], time_n, V3{v[0], v[1], v[2]}); \
+        v[0] = o_.x;                                                       \
+        v[1] = o_.y;                                                       \
+        v[2] = o_.z;                                                       \
+    }
+__device__ __forceinline__ void element_fp64_G(const JobDev& J, uint32_t e, const ElemState<double>& ES, double dt,
+                                               int32_t kind, double time_n, double& Le, double& dts, double* en,
+                                               uint32_t& errf, const ENSC_LDS double* lds) {
 #pragma clang fp contract(off)
     typedef double R;
     constexpr bool MIXED = false;
 #include "elem_body.inc"
 }
-__device__ __forceinline__ void element_mixed_G(const JobDev& J, uint32_t e, double dt, int32_t kind, double time_n,
-                                                double& Le, double& dts, double* en, uint32_t& errf,
-                                                const ENSC_LDS double* lds) {
+__device__ __forceinline__ void element_mixed_G(const JobDev& J, uint32_t e, const ElemState<float>& ES, double dt,
+                                                int32_t kind, double time_n, double& Le, double& dts, double* en,
+                                                uint32_t& errf, const ENSC_LDS double* lds) {
 #pragma clang fp contract(fast)
     typedef float R;
     constexpr bool MIXED = true;
 #include "elem_body.inc"
 }
-#undef ELEM_NODES_SETUP
 #undef NVAL
 #undef DYN_BC_OVERRIDE
-#define ELEM_NODES_SETUP                                                   \
-    uint32_t lid[8];                                                       \
-    _Pragma("unroll") for (int k = 0; k < 8; ++k) lid[k] = gld(J.lconn, k * es + e);
-#define NVAL(k, i) lds[lid[k] * 9 + (i)]
+#define NVAL(k, i) lds[ES.nd[k] * 9 + (i)]
 #define DYN_BC_OVERRIDE(k, v)  // applied once per node while staging
-__device__ __forceinline__ void element_fp64_S(const JobDev& J, uint32_t e, double dt, int32_t kind, double time_n,
-                                               double& Le, double& dts, double* en, uint32_t& errf,
-                                               const ENSC_LDS double* lds) {
+__device__ __forceinline__ void element_fp64_S(const JobDev& J, uint32_t e, const ElemState<double>& ES, double dt,
+                                               int32_t kind, double time_n, double& Le, double& dts, double* en,
+                                               uint32_t& errf, const ENSC_LDS double* lds) {
 #pragma clang fp contract(off)
     typedef double R;
     constexpr bool MIXED = false;
 #include "elem_body.inc"
 }
-__device__ __forceinline__ void element_mixed_S(const JobDev& J, uint32_t e, double dt, int32_t kind, double time_n,
-                                                double& Le, double& dts, double* en, uint32_t& errf,
-                                                const ENSC_LDS double* lds) {
+__device__ __forceinline__ void element_mixed_S(const JobDev& J, uint32_t e, const ElemState<float>& ES, double dt,
+                                                int32_t kind, double time_n, double& Le, double& dts, double* en,
+                                                uint32_t& errf, const ENSC_LDS double* lds) {
 #pragma clang fp contract(fast)
     typedef float R;
     constexpr bool MIXED = true;
 #include "elem_body.inc"
 }
-#undef ELEM_NODES_SETUP
 #undef NVAL
 #undef DYN_BC_OVERRIDE
 
-// Fixed-order reduction of a job's block partials by wave 0 of the job's
-// last block: lane l folds blocks l, l+64, ... in ascending order, then a
-// fixed xor-shuffle tree combines the lanes.  Same inputs, same result.
-template <int NV>
-__device__ inline void reduce_partials(const double* part, uint32_t nblk, const bool* is_min, double* out) {
-    const int t = threadIdx.x;
+// Fixed-order reduction of a job's block partials by the job's last block:
+// thread t folds blocks t, t+BLK, t+2*BLK, ... in ascending order (loads
+// issued four at a time), a fixed xor-shuffle tree combines each wave, and
+// thread 0 adds the waves in index order.  Same inputs, same result.
+// Every thread of the block must call it; the result is valid in thread 0.
+template <int NV, int BLK>
+__device__ inline void reduce_partials(const double* part, uint32_t nblk, const bool* is_min, double* out,
+                                       double (*s_tmp)[NV]) {
+    const uint32_t t = threadIdx.x;
     double acc[NV];
 #pragma unroll
     for (int q = 0; q < NV; ++q) acc[q] = is_min[q] ? __builtin_inf() : 0.0;
-    for (uint32_t b = t; b < nblk; b += 64) {
+    uint32_t b = t;
+    for (; b + 3 * BLK < nblk; b += 4 * BLK) {
+        double v[4][NV];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int q = 0; q < NV; ++q) v[u][q] = __builtin_nontemporal_load(part + (size_t)(b + u * BLK) * 8 + q);
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int q = 0; q < NV; ++q) acc[q] = is_min[q] ? (v[u][q] < acc[q] ? v[u][q] : acc[q]) : acc[q] + v[u][q];
+    }
+    for (; b < nblk; b += BLK) {
 #pragma unroll
         for (int q = 0; q < NV; ++q) {
             const double v = __builtin_nontemporal_load(part + (size_t)b * 8 + q);
@@ -280,11 +356,24 @@ __device__ inline void reduce_partials(const double* part, uint32_t nblk, const 
         }
     }
 #pragma unroll
-    for (int q = 0; q < NV; ++q) out[q] = is_min[q] ? wave_min(acc[q]) : wave_sum(acc[q]);
+    for (int q = 0; q < NV; ++q) acc[q] = is_min[q] ? wave_min(acc[q]) : wave_sum(acc[q]);
+    const int w = t >> 6;
+    if ((t & 63) == 0)
+#pragma unroll
+        for (int q = 0; q < NV; ++q) s_tmp[w][q] = acc[q];
+    __syncthreads();
+    if (t == 0) {
+#pragma unroll
+        for (int q = 0; q < NV; ++q) {
+            double r = s_tmp[0][q];
+            for (int x = 1; x < BLK / 64; ++x) r = is_min[q] ? (s_tmp[x][q] < r ? s_tmp[x][q] : r) : r + s_tmp[x][q];
+            out[q] = r;
+        }
+    }
 }
 
-template <bool MIXED, int BLK, bool STAGED>
-__global__ __launch_bounds__(BLK) void k_element(const JobDev* __restrict__ jobs, const uint32_t* __restrict__ blk2job) {
+template <bool MIXED, int BLK, bool STAGED, int WPE>
+__global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(WPE))) void k_element(const JobDev* __restrict__ jobs, const uint32_t* __restrict__ blk2job) {
 #pragma clang fp contract(off)
     extern __shared__ double s_nodes[];  // STAGED: [block's nodes][x0 x1 x2 v0 v1 v2 u0 u1 u2]
     const uint32_t b = blockIdx.x;
@@ -294,6 +383,10 @@ __global__ __launch_bounds__(BLK) void k_element(const JobDev* __restrict__ jobs
     if (dc.kind == KIND_NONE) return;
     const uint32_t lb = b - J.eblk0;
     const uint32_t e = lb * BLK + threadIdx.x;
+    const bool act = e < J.ne;
+    typedef typename std::conditional<MIXED, float, double>::type R;
+    ElemState<R> ES;
+    load_state<R, STAGED>(J, act ? e : J.ne - 1, dc.kind != KIND_PREPASS, ES);
     ENSC_LDS double* lds = (ENSC_LDS double*)s_nodes;
     if (STAGED) {
         // stage the block's node records once (x, v, u); velocity BCs with an
@@ -302,41 +395,43 @@ __global__ __launch_bounds__(BLK) void k_element(const JobDev* __restrict__ jobs
         for (uint32_t i = threadIdx.x; i < n1 - n0; i += BLK) {
             const uint32_t g = gld(J.blk_nodes, n0 + i);
             const uint32_t r0 = g * NODE_REC;
-            double v[3] = {gld(J.node, r0 + 3), gld(J.node, r0 + 4), gld(J.node, r0 + 5)};
-            if (J.dyn_bc && dc.kind == KIND_STEP) bc_override(J, g, dc.time_n, v);
+            const D2 p0 = gld2(J.node, r0), p1 = gld2(J.node, r0 + 2), p2 = gld2(J.node, r0 + 4),
+                     p3 = gld2(J.node, r0 + 6);
+            const double p8 = gld(J.node, r0 + 8);
+            V3 v{p1.b, p2.a, p2.b};
+            if (J.dyn_bc && dc.kind == KIND_STEP) v = bc_override(J, g, dc.time_n, v);
             ENSC_LDS double* d = lds + i * 9;
-            d[0] = gld(J.node, r0);
-            d[1] = gld(J.node, r0 + 1);
-            d[2] = gld(J.node, r0 + 2);
-            d[3] = v[0];
-            d[4] = v[1];
-            d[5] = v[2];
-            d[6] = gld(J.node, r0 + 6);
-            d[7] = gld(J.node, r0 + 7);
-            d[8] = gld(J.node, r0 + 8);
+            d[0] = p0.a;
+            d[1] = p0.b;
+            d[2] = p1.a;
+            d[3] = v.x;
+            d[4] = v.y;
+            d[5] = v.z;
+            d[6] = p3.a;
+            d[7] = p3.b;
+            d[8] = p8;
         }
         __syncthreads();
     }
     double Le = __builtin_inf(), dts = __builtin_inf();
     double en[3] = {0.0, 0.0, 0.0};
     uint32_t errf = 0;
-    if (e < J.ne) {
+    if (act) {
         if (STAGED) {
-            if (MIXED)
-                element_mixed_S(J, e, dc.dt_i_1, dc.kind, dc.time_n, Le, dts, en, errf, lds);
+            if constexpr (MIXED)
+                element_mixed_S(J, e, ES, dc.dt_i_1, dc.kind, dc.time_n, Le, dts, en, errf, lds);
             else
-                element_fp64_S(J, e, dc.dt_i_1, dc.kind, dc.time_n, Le, dts, en, errf, lds);
+                element_fp64_S(J, e, ES, dc.dt_i_1, dc.kind, dc.time_n, Le, dts, en, errf, lds);
         } else {
-            if (MIXED)
-                element_mixed_G(J, e, dc.dt_i_1, dc.kind, dc.time_n, Le, dts, en, errf, lds);
+            if constexpr (MIXED)
+                element_mixed_G(J, e, ES, dc.dt_i_1, dc.kind, dc.time_n, Le, dts, en, errf, lds);
             else
-                element_fp64_G(J, e, dc.dt_i_1, dc.kind, dc.time_n, Le, dts, en, errf, lds);
+                element_fp64_G(J, e, ES, dc.dt_i_1, dc.kind, dc.time_n, Le, dts, en, errf, lds);
         }
     }
     // block partials: wave shuffles, then LDS across the block's waves
     __shared__ double s_red[BLK / 64][5];
     __shared__ uint32_t s_err;
-    __shared__ int s_last;
     if (threadIdx.x == 0) s_err = 0;
     Le = wave_min(Le);
     dts = wave_min(dts);
@@ -370,16 +465,25 @@ __global__ __launch_bounds__(BLK) void k_element(const JobDev* __restrict__ jobs
         p[3] = r3;
         p[4] = r4;
         if (s_err) atomicOr(&ctl->err, s_err);
-        __threadfence();
-        const uint32_t t = atomicAdd(&ctl->ticket_e, 1u);
-        s_last = (t == J.nblk_e - 1);
     }
-    __syncthreads();
-    if (!s_last || threadIdx.x >= 64) return;
-    __threadfence();
+}
+
+// Per-job end of the element pass: one block per job in the job table
+// reduces the job's block partials (written by k_element; the launch
+// boundary orders them) and advances the job's time-step scalars exactly as
+// update_minVertex_perMaterial_and_dt and the run-loop clamps do
+// (exDyna3D.cpp:663-686, 90-97, 775-780).
+template <int BLK>
+__global__ __launch_bounds__(BLK) void k_fin_element(const JobDev* __restrict__ jobs) {
+#pragma clang fp contract(off)
+    const JobDev& J = jobs[blockIdx.x];
+    Ctl* ctl = J.ctl;
+    const Decision dc = decide(*ctl, J.factor, J.wos);
+    if (dc.kind == KIND_NONE) return;
+    __shared__ double s_red[BLK / 64][5];
     const bool is_min[5] = {true, true, false, false, false};
     double tot[5];
-    reduce_partials<5>(J.part_e, J.nblk_e, is_min, tot);
+    reduce_partials<5, BLK>(J.part_e, J.nblk_e, is_min, tot, s_red);
     if (threadIdx.x == 0) {
         Ctl& c = *ctl;
         const double Lmin = tot[0], dtsn = tot[1];
@@ -404,7 +508,7 @@ __global__ __launch_bounds__(BLK) void k_element(const JobDev* __restrict__ jobs
                 dt1 = dt;
             }
             if (c.time + dt1 > c.step_end) dt = dt1 = c.step_end - c.time;
-            if (dt1 != dc.dt_i_1) atomicOr(&c.err, (uint32_t)ENSC_EF_DT);  // cannot happen: same inputs as decide()
+            if (dt1 != dc.dt_i_1) c.err |= (uint32_t)ENSC_EF_DT;  // cannot happen: same inputs as decide()
             // computeSate (775-780)
             double t = c.time + dt1;
             if (t > c.total_time) t = c.total_time;
@@ -418,11 +522,9 @@ __global__ __launch_bounds__(BLK) void k_element(const JobDev* __restrict__ jobs
             c.hg += tot[3];
             c.vis += tot[4];
             c.grav_amp = (J.grav_on && J.grav_amp >= 0) ? deal_amp(J, J.grav_amp, t, dt) : 1.0;
-            if (!(dtsn > 0.0) || !(dt >= 0.0)) atomicOr(&c.err, (uint32_t)ENSC_EF_DT);
+            if (!(dtsn > 0.0) || !(dt >= 0.0)) c.err |= (uint32_t)ENSC_EF_DT;
             c.kind = KIND_STEP;
         }
-        c.ticket_e = 0;
-        __threadfence();
     }
 }
 
@@ -438,49 +540,65 @@ __global__ __launch_bounds__(BLK) void k_node(const JobDev* __restrict__ jobs, c
     const uint32_t n = lb * BLK + threadIdx.x;
     double ke = 0.0, pw = 0.0;
     if (n < J.nn) {
+        const uint32_t r0 = n * NODE_REC;
+        const D2 p0 = gld2(J.node, r0), p1 = gld2(J.node, r0 + 2), p2 = gld2(J.node, r0 + 4),
+                 p3 = gld2(J.node, r0 + 6), p4 = gld2(J.node, r0 + 8);
+        double x[3] = {p0.a, p0.b, p1.a};
+        double v[3] = {p1.b, p2.a, p2.b};
+        double u[3] = {p3.a, p3.b, p4.a};
+        const double im = p4.b;
+        // Element forces of the node in (corner, original element) order;
+        // slots are filled from the front, ELL_EMPTY after the last.  Each
+        // chunk of 8 slots issues all its loads before summing.
         double acc[3] = {0.0, 0.0, 0.0}, grp[3] = {0.0, 0.0, 0.0};
         int gk = -1;
-        const uint32_t es = J.ne_s;
-        for (int w = 0; w < J.ell_w; ++w) {
-            const uint32_t sl = gld(J.ell, w * J.nn_s + n);
-            if (sl == ELL_EMPTY) break;
-            const int k = (int)(sl & 7u);
-            const uint32_t idx = k * es + (sl >> 3);
-            double f0, f1, f2;
-            if (MIXED) {
-                const float* q = static_cast<const float*>(J.fel);
-                f0 = gld(q, 3 * idx);
-                f1 = gld(q, 3 * idx + 1);
-                f2 = gld(q, 3 * idx + 2);
-            } else {
-                const double* q = static_cast<const double*>(J.fel);
-                f0 = gld(q, 3 * idx);
-                f1 = gld(q, 3 * idx + 1);
-                f2 = gld(q, 3 * idx + 2);
-            }
-            if (k != gk) {
-                if (gk >= 0) {
-                    acc[0] += grp[0];
-                    acc[1] += grp[1];
-                    acc[2] += grp[2];
+        const uint32_t es = J.ne_s, W = (uint32_t)J.ell_w;
+        for (uint32_t w0 = 0; w0 < W; w0 += 8) {
+            uint32_t sl[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) sl[j] = (w0 + j < W) ? gld(J.ell, (w0 + j) * J.nn_s + n) : ELL_EMPTY;
+            double f[8][3];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                f[j][0] = f[j][1] = f[j][2] = 0.0;
+                if (sl[j] != ELL_EMPTY) {
+                    const uint32_t idx = (sl[j] & 7u) * es + (sl[j] >> 3);
+                    if (MIXED) {
+                        const float* q = static_cast<const float*>(J.fel);
+                        f[j][0] = gld(q, 3 * idx);
+                        f[j][1] = gld(q, 3 * idx + 1);
+                        f[j][2] = gld(q, 3 * idx + 2);
+                    } else {
+                        const double* q = static_cast<const double*>(J.fel);
+                        f[j][0] = gld(q, 3 * idx);
+                        f[j][1] = gld(q, 3 * idx + 1);
+                        f[j][2] = gld(q, 3 * idx + 2);
+                    }
                 }
-                grp[0] = grp[1] = grp[2] = 0.0;
-                gk = k;
             }
-            grp[0] += f0;
-            grp[1] += f1;
-            grp[2] += f2;
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                if (sl[j] == ELL_EMPTY) continue;
+                const int k = (int)(sl[j] & 7u);
+                if (k != gk) {
+                    if (gk >= 0) {
+                        acc[0] += grp[0];
+                        acc[1] += grp[1];
+                        acc[2] += grp[2];
+                    }
+                    grp[0] = grp[1] = grp[2] = 0.0;
+                    gk = k;
+                }
+                grp[0] += f[j][0];
+                grp[1] += f[j][1];
+                grp[2] += f[j][2];
+            }
         }
         if (gk >= 0) {
             acc[0] += grp[0];
             acc[1] += grp[1];
             acc[2] += grp[2];
         }
-        const uint32_t r0 = n * NODE_REC;
-        double x[3] = {gld(J.node, r0), gld(J.node, r0 + 1), gld(J.node, r0 + 2)};
-        double v[3] = {gld(J.node, r0 + 3), gld(J.node, r0 + 4), gld(J.node, r0 + 5)};
-        double u[3] = {gld(J.node, r0 + 6), gld(J.node, r0 + 7), gld(J.node, r0 + 8)};
-        const double im = gld(J.node, r0 + 9);
         double a[3] = {0.0 + acc[0], 0.0 + acc[1], 0.0 + acc[2]};
         if (J.grav_on) {  // apply_gravity (800-826)
             const double ag = ctl->grav_amp * J.grav_g;
@@ -537,18 +655,13 @@ __global__ __launch_bounds__(BLK) void k_node(const JobDev* __restrict__ jobs, c
             k2 += vd * vd;
         }
         ke = 0.5 * k2 / im;
-        gst(J.node, r0) = x[0];
-        gst(J.node, r0 + 1) = x[1];
-        gst(J.node, r0 + 2) = x[2];
-        gst(J.node, r0 + 3) = v[0];
-        gst(J.node, r0 + 4) = v[1];
-        gst(J.node, r0 + 5) = v[2];
-        gst(J.node, r0 + 6) = u[0];
-        gst(J.node, r0 + 7) = u[1];
+        gst2(J.node, r0, x[0], x[1]);
+        gst2(J.node, r0 + 2, x[2], v[0]);
+        gst2(J.node, r0 + 4, v[1], v[2]);
+        gst2(J.node, r0 + 6, u[0], u[1]);
         gst(J.node, r0 + 8) = u[2];
     }
     __shared__ double s_red[BLK / 64][2];
-    __shared__ int s_last;
     ke = wave_sum(ke);
     pw = wave_sum(pw);
     const int w = threadIdx.x >> 6;
@@ -566,16 +679,22 @@ __global__ __launch_bounds__(BLK) void k_node(const JobDev* __restrict__ jobs, c
         double* p = J.part_n + (size_t)lb * 8;
         p[0] = r0;
         p[1] = r1;
-        __threadfence();
-        const uint32_t t = atomicAdd(&ctl->ticket_n, 1u);
-        s_last = (t == J.nblk_n - 1);
     }
-    __syncthreads();
-    if (!s_last || threadIdx.x >= 64) return;
-    __threadfence();
+}
+
+// Per-job end of the nodal update: kinetic energy, external work and the
+// job's history row for the step (one block per job in the job table).
+template <int BLK>
+__global__ __launch_bounds__(BLK) void k_fin_node(const JobDev* __restrict__ jobs) {
+#pragma clang fp contract(off)
+    const JobDev& J = jobs[blockIdx.x];
+    Ctl* ctl = J.ctl;
+    if (ctl->kind != KIND_STEP) return;
+    const double dt_i = ctl->dt_i, dt_i_1 = ctl->dt_i_1;
+    __shared__ double s_red[BLK / 64][2];
     const bool is_min[2] = {false, false};
     double tot[2];
-    reduce_partials<2>(J.part_n, J.nblk_n, is_min, tot);
+    reduce_partials<2, BLK>(J.part_n, J.nblk_n, is_min, tot, s_red);
     if (threadIdx.x == 0) {
         Ctl& c = *ctl;
         c.ke = tot[0];
@@ -594,14 +713,12 @@ __global__ __launch_bounds__(BLK) void k_node(const JobDev* __restrict__ jobs, c
             h[ENSC_H_WEXT] = c.wext;
             for (int p = 0; p < J.n_probe; ++p) {
                 const double* r = J.node + (size_t)J.probe[p] * NODE_REC + 6;
-                h[ENSC_H_BASE + 3 * p + 0] = __builtin_nontemporal_load(r + 0);
-                h[ENSC_H_BASE + 3 * p + 1] = __builtin_nontemporal_load(r + 1);
-                h[ENSC_H_BASE + 3 * p + 2] = __builtin_nontemporal_load(r + 2);
+                h[ENSC_H_BASE + 3 * p + 0] = r[0];
+                h[ENSC_H_BASE + 3 * p + 1] = r[1];
+                h[ENSC_H_BASE + 3 * p + 2] = r[2];
             }
         }
         c.kind = KIND_NONE;
-        c.ticket_n = 0;
-        __threadfence();
     }
 }
 

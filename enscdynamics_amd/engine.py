@@ -31,7 +31,7 @@ EXPORTS = [
     "ensc_last_error", "ensc_device_count", "ensc_engine_create", "ensc_engine_destroy", "ensc_engine_init",
     "ensc_engine_set_step", "ensc_engine_advance", "ensc_engine_run", "ensc_engine_enqueue_steps",
     "ensc_engine_sync", "ensc_engine_download", "ensc_engine_scalars", "ensc_engine_history",
-    "ensc_engine_kernel_times", "ensc_engine_set_timing", "ensc_engine_device_bytes",
+    "ensc_engine_kernel_times", "ensc_engine_phase_times", "ensc_engine_set_timing", "ensc_engine_device_bytes",
 ]
 
 
@@ -50,7 +50,7 @@ class Options(C.Structure):
         ("probe_nodes", C.POINTER(C.c_uint32)),
         ("hist_cap", C.c_int64),
         ("staging", C.c_int32),
-        ("pad_", C.c_int32),
+        ("waves", C.c_int32),
     ]
 
 
@@ -81,6 +81,7 @@ def load_library(path: str = LIB_PATH):
         ("ensc_engine_scalars", [vp, vp]),
         ("ensc_engine_history", [vp, vp, C.c_int64, C.POINTER(C.c_int64)]),
         ("ensc_engine_kernel_times", [vp, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
+        ("ensc_engine_phase_times", [vp, C.c_void_p]),
         ("ensc_engine_set_timing", [vp, C.c_int32]),
     ]:
         f = getattr(L, name)
@@ -107,7 +108,7 @@ class Engine:
 
     def __init__(self, model: Model, precision: int = FP64, reorder: bool = False, block_elem: int = 0,
                  block_node: int = 0, probes: Sequence[int] = (), hist_cap: int = 0, device: int = 0,
-                 staging: bool = True):
+                 staging: bool = True, waves: int = 0):
         L = load_library()
         self.model = model
         self.precision = precision
@@ -124,6 +125,7 @@ class Engine:
         o.probe_nodes = pr.ctypes.data_as(C.POINTER(C.c_uint32)) if len(pr) else None
         o.hist_cap = hist_cap
         o.staging = 0 if staging else 2
+        o.waves = waves
         h = C.c_void_p()
         _check(L.ensc_engine_create(C.byref(self._desc), C.byref(o), C.byref(h)))
         self.h = h
@@ -178,6 +180,13 @@ class Engine:
         a, b, n = C.c_double(), C.c_double(), C.c_int64()
         _check(_lib.ensc_engine_kernel_times(self.h, C.byref(a), C.byref(b), C.byref(n)))
         return a.value, b.value, n.value
+
+    def phase_times(self) -> Tuple[float, float, float, float]:
+        """Mean ms per launch of the last timed enqueue: element pass, element
+        finalize, nodal update, node finalize."""
+        out = np.zeros(4)
+        _check(_lib.ensc_engine_phase_times(self.h, out.ctypes.data))
+        return tuple(float(x) for x in out)
 
     def get(self, name: str) -> np.ndarray:
         shapes = {"u": (self.Nn, 3), "v": (self.Nn, 3), "x": (self.Nn, 3), "rf": (self.Nn, 3),

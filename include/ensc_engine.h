@@ -58,7 +58,9 @@ typedef struct ensc_options {
     int64_t hist_cap;       /* rows of per-step history kept on device */
     int32_t staging;        /* 0: stage element-block nodes in LDS when they fit
                                (default), 2: always gather corners from HBM */
-    int32_t pad_;
+    int32_t waves;          /* element-kernel occupancy target, waves per SIMD
+                               (1, 2 or 3; 0 = default for the precision).
+                               Staged kernels only. */
 } ensc_options;
 
 /* Scalars of one job (ensc_engine_scalars). */
@@ -120,11 +122,14 @@ ensc_status ensc_engine_history(ensc_engine* eng, double* out, int64_t rows, int
  * HIP events on the engine's stream: mean ms per launch of each kernel and
  * the number of launches timed. */
 ensc_status ensc_engine_kernel_times(ensc_engine* eng, double* elem_ms, double* node_ms, int64_t* launches);
+/* Mean duration in ms of each launch of the last timed enqueue, in launch
+ * order: element pass, element finalize, nodal update, node finalize. */
+ensc_status ensc_engine_phase_times(ensc_engine* eng, double* ms4);
 /* Enable (1) / disable (0) per-kernel event timing for later enqueue calls. */
 ensc_status ensc_engine_set_timing(ensc_engine* eng, int32_t on);
 /* Device bytes held by the engine. */
 int64_t ensc_engine_device_bytes(ensc_engine* eng);
-/* Capture `ticks` step iterations (element + node launch each) as a
+/* Capture `ticks` step iterations (the four launches of each) as a
  * hipGraph and replay it for enqueued steps (0: plain launches).  The graph
  * is re-captured whenever the set of stepped jobs changes. */
 ensc_status ensc_engine_set_graph(ensc_engine* eng, int32_t ticks);
