@@ -26,7 +26,7 @@ using namespace ensc;
         if (e_ != hipSuccess) return fail(ENSC_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
-typedef void (*KernelFn)(const JobDev*, const uint32_t*);
+typedef void (*KernelFn)(const JobDev*, const uint2*);
 
 namespace {
 
@@ -161,6 +161,21 @@ ensc_status grow_ctl(ensc_engine* E, uint32_t need) {
     return write_ctl(E);
 }
 
+// Launched block h runs on XCD h % NXCD (the dispatcher deals workgroups to
+// the XCDs round robin).  Give each XCD a contiguous range of the logical
+// blocks, so blocks that share nodes (neighbours in the space-filling order)
+// meet in the same L2.  A permutation: results do not depend on it.
+constexpr uint32_t NXCD = 8;
+std::vector<uint2> xcd_order(const std::vector<uint2>& logical) {
+    const size_t n = logical.size(), q = n / NXCD, r = n % NXCD;
+    std::vector<uint2> out(n);
+    for (size_t h = 0; h < n; ++h) {
+        const size_t x = h % NXCD, i = h / NXCD;
+        out[h] = logical[x * q + std::min(x, r) + i];
+    }
+    return out;
+}
+
 // Rebuild the job table and block maps for the jobs that are not finished.
 ensc_status relayout(ensc_engine* E) {
     ensc_status st;
@@ -173,36 +188,42 @@ ensc_status relayout(ensc_engine* E) {
         const bool idle = J->inited && !c.init_pending && !(c.time < c.step_end) && !J->pending_step;
         if (!idle || J->keep_in_layout) E->live.push_back(s);
     }
-    std::vector<uint32_t> b2e, b2n;
+    // block maps: (job, block of the job) per launched block, in XCD order
+    std::vector<uint2> le, ln;
     std::vector<JobDev> tab;
+    std::vector<size_t> pe0, pn0;
     uint32_t lds_nodes = 0;
     for (uint32_t i = 0; i < E->live.size(); ++i) {
         Job& J = *E->slots[E->live[i]];
-        JobDev& D = J.dev;
-        D.eblk0 = (uint32_t)b2e.size();
-        D.nblk0 = (uint32_t)b2n.size();
-        for (uint32_t b = 0; b < D.nblk_e; ++b) b2e.push_back(i);
-        for (uint32_t b = 0; b < D.nblk_n; ++b) b2n.push_back(i);
+        const JobDev& D = J.dev;
+        pe0.push_back(le.size());
+        pn0.push_back(ln.size());
+        for (uint32_t b = 0; b < D.nblk_e; ++b) le.push_back(make_uint2(i, b));
+        for (uint32_t b = 0; b < D.nblk_n; ++b) ln.push_back(make_uint2(i, b));
         lds_nodes = std::max(lds_nodes, J.lds_nodes);
     }
+    const std::vector<uint2> b2e = xcd_order(le), b2n = xcd_order(ln);
     E->nblk_e = (uint32_t)b2e.size();
     E->nblk_n = (uint32_t)b2n.size();
     if (!E->jobs_buf.ensure(std::max<size_t>(1, E->live.size()) * sizeof(JobDev)) ||
-        !E->b2e_buf.ensure(std::max<size_t>(1, b2e.size()) * 4) || !E->b2n_buf.ensure(std::max<size_t>(1, b2n.size()) * 4) ||
-        !E->pe_buf.ensure(std::max<size_t>(1, b2e.size()) * 8 * sizeof(double)) ||
-        !E->pn_buf.ensure(std::max<size_t>(1, b2n.size()) * 8 * sizeof(double)))
+        !E->b2e_buf.ensure(std::max<size_t>(1, b2e.size()) * sizeof(uint2)) ||
+        !E->b2n_buf.ensure(std::max<size_t>(1, b2n.size()) * sizeof(uint2)) ||
+        !E->pe_buf.ensure(std::max<size_t>(1, b2e.size()) * 5 * sizeof(double)) ||
+        !E->pn_buf.ensure(std::max<size_t>(1, b2n.size()) * 2 * sizeof(double)))
         return fail(ENSC_ERR_MEM, "layout allocation failed");
     for (uint32_t i = 0; i < E->live.size(); ++i) {
         Job& J = *E->slots[E->live[i]];
         J.dev.ctl = E->d_ctl + E->live[i];
-        J.dev.part_e = static_cast<double*>(E->pe_buf.p) + (size_t)J.dev.eblk0 * 8;
-        J.dev.part_n = static_cast<double*>(E->pn_buf.p) + (size_t)J.dev.nblk0 * 8;
+        J.dev.part_e = static_cast<double*>(E->pe_buf.p) + pe0[i] * 5;
+        J.dev.part_n = static_cast<double*>(E->pn_buf.p) + pn0[i] * 2;
         tab.push_back(J.dev);
     }
     if (!tab.empty())
         HIPCHK(hipMemcpyAsync(E->jobs_buf.p, tab.data(), tab.size() * sizeof(JobDev), hipMemcpyHostToDevice, E->stream));
-    if (!b2e.empty()) HIPCHK(hipMemcpyAsync(E->b2e_buf.p, b2e.data(), b2e.size() * 4, hipMemcpyHostToDevice, E->stream));
-    if (!b2n.empty()) HIPCHK(hipMemcpyAsync(E->b2n_buf.p, b2n.data(), b2n.size() * 4, hipMemcpyHostToDevice, E->stream));
+    if (!b2e.empty())
+        HIPCHK(hipMemcpyAsync(E->b2e_buf.p, b2e.data(), b2e.size() * sizeof(uint2), hipMemcpyHostToDevice, E->stream));
+    if (!b2n.empty())
+        HIPCHK(hipMemcpyAsync(E->b2n_buf.p, b2n.data(), b2n.size() * sizeof(uint2), hipMemcpyHostToDevice, E->stream));
     HIPCHK(hipStreamSynchronize(E->stream));
     const bool mixed = E->precision == ENSC_MIXED;
     // LDS staging when every live job's largest block node list fits 64 KiB
@@ -226,12 +247,12 @@ void launch_phase(ensc_engine* E, int phase) {
     switch (phase) {
         case 0:
             hipLaunchKernelGGL(E->ke, dim3(E->nblk_e), dim3(E->blk_e), E->lds_bytes, E->stream, jobs,
-                               static_cast<const uint32_t*>(E->b2e_buf.p));
+                               static_cast<const uint2*>(E->b2e_buf.p));
             break;
         case 1: hipLaunchKernelGGL(k_fin_element<FIN_BLK>, dim3(nj), dim3(FIN_BLK), 0, E->stream, jobs); break;
         case 2:
             hipLaunchKernelGGL(E->kn, dim3(E->nblk_n), dim3(E->blk_n), 0, E->stream, jobs,
-                               static_cast<const uint32_t*>(E->b2n_buf.p));
+                               static_cast<const uint2*>(E->b2n_buf.p));
             break;
         default: hipLaunchKernelGGL(k_fin_node<FIN_BLK>, dim3(nj), dim3(FIN_BLK), 0, E->stream, jobs); break;
     }

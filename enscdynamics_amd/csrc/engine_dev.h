@@ -95,13 +95,13 @@ struct JobDev {
     double lambda, G, K, rho, wos, c_dr, c_cr, cvisl, cvisq, factor, yield0, hard;
     int32_t plastic, dyn_bc, ell_w, n_probe;
     uint32_t ne, nn, ne_s, nn_s;
-    uint32_t eblk0, nblk_e, nblk0, nblk_n;
+    uint32_t nblk_e, nblk_n;
     uint32_t probe[MAX_PROBE];
     double* hist;
     long long hist_cap;
     Ctl* ctl;
-    double* part_e;  // [nblk_e][8]: min L, min dtStable, internal, hourglass, viscous work
-    double* part_n;  // [nblk_n][2]: kinetic energy, external power
+    double* part_e;  // [5][nblk_e]: min L, min dtStable, internal, hourglass, viscous work
+    double* part_n;  // [2][nblk_n]: kinetic energy, external power
 };
 
 }  // namespace ensc

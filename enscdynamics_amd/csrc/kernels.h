@@ -324,10 +324,11 @@ __device__ __forceinline__ void element_mixed_S(const JobDev& J, uint32_t e, con
 #undef NVAL
 #undef DYN_BC_OVERRIDE
 
-// Fixed-order reduction of a job's block partials by the job's last block:
-// thread t folds blocks t, t+BLK, t+2*BLK, ... in ascending order (loads
-// issued four at a time), a fixed xor-shuffle tree combines each wave, and
-// thread 0 adds the waves in index order.  Same inputs, same result.
+// Fixed-order reduction of a job's block partials (value q of block b at
+// part[q * nblk + b]): thread t folds blocks t, t+BLK, t+2*BLK, ... in
+// ascending order (loads issued four at a time), a fixed xor-shuffle tree
+// combines each wave, and thread 0 adds the waves in index order.  Same
+// inputs, same result.
 // Every thread of the block must call it; the result is valid in thread 0.
 template <int NV, int BLK>
 __device__ inline void reduce_partials(const double* part, uint32_t nblk, const bool* is_min, double* out,
@@ -342,7 +343,7 @@ __device__ inline void reduce_partials(const double* part, uint32_t nblk, const 
 #pragma unroll
         for (int u = 0; u < 4; ++u)
 #pragma unroll
-            for (int q = 0; q < NV; ++q) v[u][q] = __builtin_nontemporal_load(part + (size_t)(b + u * BLK) * 8 + q);
+            for (int q = 0; q < NV; ++q) v[u][q] = part[(size_t)q * nblk + b + u * BLK];
 #pragma unroll
         for (int u = 0; u < 4; ++u)
 #pragma unroll
@@ -351,7 +352,7 @@ __device__ inline void reduce_partials(const double* part, uint32_t nblk, const 
     for (; b < nblk; b += BLK) {
 #pragma unroll
         for (int q = 0; q < NV; ++q) {
-            const double v = __builtin_nontemporal_load(part + (size_t)b * 8 + q);
+            const double v = part[(size_t)q * nblk + b];
             acc[q] = is_min[q] ? (v < acc[q] ? v : acc[q]) : acc[q] + v;
         }
     }
@@ -373,15 +374,15 @@ __device__ inline void reduce_partials(const double* part, uint32_t nblk, const 
 }
 
 template <bool MIXED, int BLK, bool STAGED, int WPE>
-__global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(WPE))) void k_element(const JobDev* __restrict__ jobs, const uint32_t* __restrict__ blk2job) {
+__global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(WPE))) void k_element(const JobDev* __restrict__ jobs, const uint2* __restrict__ blkmap) {
 #pragma clang fp contract(off)
     extern __shared__ double s_nodes[];  // STAGED: [block's nodes][x0 x1 x2 v0 v1 v2 u0 u1 u2]
-    const uint32_t b = blockIdx.x;
-    const JobDev& J = jobs[blk2job[b]];
+    const uint2 bm = blkmap[blockIdx.x];  // (job, block of the job)
+    const JobDev& J = jobs[bm.x];
     Ctl* ctl = J.ctl;
     const Decision dc = decide(*ctl, J.factor, J.wos);
     if (dc.kind == KIND_NONE) return;
-    const uint32_t lb = b - J.eblk0;
+    const uint32_t lb = bm.y;
     const uint32_t e = lb * BLK + threadIdx.x;
     const bool act = e < J.ne;
     typedef typename std::conditional<MIXED, float, double>::type R;
@@ -458,12 +459,13 @@ __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(WPE))) void
             r3 += s_red[q][3];
             r4 += s_red[q][4];
         }
-        double* p = J.part_e + (size_t)lb * 8;
+        double* p = J.part_e + lb;
+        const uint32_t nb = J.nblk_e;
         p[0] = r0;
-        p[1] = r1;
-        p[2] = r2;
-        p[3] = r3;
-        p[4] = r4;
+        p[nb] = r1;
+        p[2 * nb] = r2;
+        p[3 * nb] = r3;
+        p[4 * nb] = r4;
         if (s_err) atomicOr(&ctl->err, s_err);
     }
 }
@@ -529,14 +531,14 @@ __global__ __launch_bounds__(BLK) void k_fin_element(const JobDev* __restrict__ 
 }
 
 template <bool MIXED, int BLK>
-__global__ __launch_bounds__(BLK) void k_node(const JobDev* __restrict__ jobs, const uint32_t* __restrict__ blk2job) {
+__global__ __launch_bounds__(BLK) void k_node(const JobDev* __restrict__ jobs, const uint2* __restrict__ blkmap) {
 #pragma clang fp contract(off)
-    const uint32_t b = blockIdx.x;
-    const JobDev& J = jobs[blk2job[b]];
+    const uint2 bm = blkmap[blockIdx.x];  // (job, block of the job)
+    const JobDev& J = jobs[bm.x];
     Ctl* ctl = J.ctl;
     if (ctl->kind != KIND_STEP) return;
     const double dt_i = ctl->dt_i, dt_i_1 = ctl->dt_i_1, time = ctl->time;
-    const uint32_t lb = b - J.nblk0;
+    const uint32_t lb = bm.y;
     const uint32_t n = lb * BLK + threadIdx.x;
     double ke = 0.0, pw = 0.0;
     if (n < J.nn) {
@@ -676,9 +678,9 @@ __global__ __launch_bounds__(BLK) void k_node(const JobDev* __restrict__ jobs, c
             r0 += s_red[q][0];
             r1 += s_red[q][1];
         }
-        double* p = J.part_n + (size_t)lb * 8;
+        double* p = J.part_n + lb;
         p[0] = r0;
-        p[1] = r1;
+        p[J.nblk_n] = r1;
     }
 }
 

@@ -31,7 +31,7 @@ EXPORTS = [
     "ensc_last_error", "ensc_device_count", "ensc_engine_create", "ensc_engine_destroy", "ensc_engine_init",
     "ensc_engine_set_step", "ensc_engine_advance", "ensc_engine_run", "ensc_engine_enqueue_steps",
     "ensc_engine_sync", "ensc_engine_download", "ensc_engine_scalars", "ensc_engine_history",
-    "ensc_engine_kernel_times", "ensc_engine_phase_times", "ensc_engine_set_timing", "ensc_engine_device_bytes",
+    "ensc_engine_set_graph", "ensc_engine_kernel_times", "ensc_engine_phase_times", "ensc_engine_set_timing", "ensc_engine_device_bytes",
 ]
 
 
@@ -83,6 +83,7 @@ def load_library(path: str = LIB_PATH):
         ("ensc_engine_kernel_times", [vp, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
         ("ensc_engine_phase_times", [vp, C.c_void_p]),
         ("ensc_engine_set_timing", [vp, C.c_int32]),
+        ("ensc_engine_set_graph", [vp, C.c_int32]),
     ]:
         f = getattr(L, name)
         f.restype = C.c_int
@@ -172,6 +173,11 @@ class Engine:
 
     def sync(self) -> None:
         _check(_lib.ensc_engine_sync(self.h))
+
+    def set_graph(self, ticks: int) -> None:
+        """Replay enqueued steps as a hipGraph of `ticks` step iterations
+        (0: plain launches)."""
+        _check(_lib.ensc_engine_set_graph(self.h, int(ticks)))
 
     def set_timing(self, on: bool) -> None:
         _check(_lib.ensc_engine_set_timing(self.h, int(on)))
