@@ -1,16 +1,25 @@
 #!/usr/bin/env python3
 """Benchmark of the MI355X step engine.
 
-Default workload: the 1M-element plastic cube (100^3 C3D8R, J2 yield 250 MPa,
-hardening 1 GPa, z=0 fixed, z=1 pushed at 10 m/s), inputs resident in HBM.
-One "step" = one iteration of exDyna3D's step loop (element pass + nodal
-update + dt reduction) over the whole mesh.  The headline value is the
-mixed-precision element-steps/s; the FP64 engine is timed in the same run on
-the same steps, and the relative displacement error of mixed vs FP64 after
-those steps is reported beside it.  The CPU baseline is the FP64
-restatement (oracle/) timed on the host's cores on a bounded sample.
+Headline (configuration 3): the 1M-element plastic cube (100^3 C3D8R, J2
+yield 250 MPa, hardening 1 GPa, z=0 fixed, z=1 pushed at 10 m/s), inputs
+resident in HBM.  One "step" = one iteration of exDyna3D's step loop
+(element pass + nodal update + dt reduction) over the whole mesh.  The
+headline value is the mixed-precision element-steps/s; the FP64 engine is
+timed in the same run on the same steps, and the relative displacement error
+of mixed vs FP64 after those steps is reported beside it.
 
-    python bench.py [--gpus 1] [--steps 200] [--warmup 20] [--workload c3|c2|c1]
+Also in the same JSON line:
+  * batched  (configuration 4): 256 independent meshes in one engine, each
+    run to its own end time; solves/s and element-steps/s.
+  * serving  (configuration 5): continuous-batching server fed with
+    requests drawn from the config-4 generator (up to 5e5 elements, half
+    J2); capacity (all submitted at once), Poisson arrivals at 80% of that
+    capacity with p50/p99 latency, and the same jobs run one at a time.
+  * cpu_baseline: the FP64 restatement (oracle/) timed on the host's cores on
+    a bounded sample of configuration 3.
+
+    python bench.py [--gpus 1] [--steps 200] [--warmup 20] [--sections c3,c4,c5]
 """
 from __future__ import annotations
 
@@ -28,54 +37,202 @@ sys.path.insert(0, ROOT)
 
 from enscdynamics_amd import decks  # noqa: E402
 from enscdynamics_amd import engine as eng  # noqa: E402
+from enscdynamics_amd.batch import Batch, Server  # noqa: E402
 
 METRIC = "elem-steps/s mixed vs FP64 at ≤1e-4 rel disp err; batched solves/s + p99"
 HBM_PEAK_GBS = 8000.0
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_traffic_c3_cube_1M.json")
 
 
-def bytes_per_elem_step(precision: int, plastic: bool, r: float):
-    """Algorithmic bytes per element-step: compulsory once-per-step reads and
-    writes of persistent state at the stored precision.  Returns
-    (whole step, element kernel share)."""
-    if precision == eng.FP64:
-        elem = 520 + (16 if plastic else 0)
-        node = 152 * r
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def alg_bytes(precision: int, plastic: bool, r: float):
+    """Algorithmic bytes per element-step: the compulsory once-per-step read
+    and write of persistent state as this engine stores it (force exchange,
+    staging redundancy and block partials are overhead and not counted).
+    Returns (element pass, nodal update) per element."""
+    s = 4 if precision == eng.MIXED else 8
+    # 8 corner ids (16-bit block-local), Gi (9), V0, sigma r+w, QIA r+w, eps_p r+w
+    elem = 16 + 9 * s + s + 12 * s + 24 * s + (2 * s if plastic else 0)
+    # node values each element pass reads once: X, v, u
+    elem += 72 * r
+    if precision == eng.MIXED:
+        # v, u read+write, invM, compensation of u (3 x FP32) read+write
+        node = (48 + 48 + 8 + 24) * r
     else:
-        elem = 276 + (8 if plastic else 0)
-        node = 200 * r
-    return elem + node, elem + 72 * r
+        # x, v, u read+write, invM
+        node = (144 + 8) * r
+    return elem, node
 
 
-def workload(name: str):
-    if name == "c3":
-        return decks.config3(n=100, steps=3000)
-    if name == "c2":
-        return decks.config2()
-    if name == "c1":
-        return decks.config1()
-    raise SystemExit(f"unknown workload {name}")
+def c3_section(args, m):
+    r = m.n_nodes / m.n_elems
+    res = {}
+    for prec, name in ((eng.MIXED, "mixed"), (eng.FP64, "fp64")):
+        e = eng.Engine(m, prec, reorder=True, staging=not args.no_staging)
+        if args.graph:
+            e.set_graph(args.graph)
+        e.init()
+        e.set_current_step(1)
+        e.enqueue(args.warmup)
+        e.sync()
+        n0 = int(e.scalars()["nsteps"])
+        t0 = time.perf_counter()
+        e.enqueue(args.steps)
+        e.sync()
+        t1 = time.perf_counter()
+        n1 = int(e.scalars()["nsteps"])
+        assert n1 - n0 == args.steps, f"only {n1 - n0} of {args.steps} ticks were full steps"
+        # per-launch durations, events on the engine's stream around each launch
+        e.set_timing(True)
+        e.enqueue(args.kernel_steps)
+        e.sync()
+        ph = e.phase_times()
+        e.set_timing(False)
+        res[name] = dict(engine=e, secs=t1 - t0, phases=ph)
+        log(f"{name}: {args.steps} steps in {t1 - t0:.4f} s; per launch (ms): element {ph[0]:.4f}, "
+            f"element finalize {ph[1]:.4f}, node {ph[2]:.4f}, node finalize {ph[3]:.4f}")
+    u_mix = res["mixed"]["engine"].get("u")
+    u_64 = res["fp64"]["engine"].get("u")
+    rel_err = float(np.abs(u_mix - u_64).max() / max(np.abs(u_64).max(), 1e-300))
+    out = {}
+    for name, prec in (("mixed", eng.MIXED), ("fp64", eng.FP64)):
+        d = res[name]
+        be, bn = alg_bytes(prec, m.plastic, r)
+        ph = d["phases"]
+        out[name] = {
+            "value": m.n_elems * args.steps / d["secs"],
+            "ms_per_step": 1e3 * d["secs"] / args.steps,
+            "launch_ms": {"element": ph[0], "element_finalize": ph[1], "node": ph[2], "node_finalize": ph[3]},
+            "alg_bytes_per_elem": {"element": be, "node": bn},
+            "element_gbs": be * m.n_elems / (ph[0] * 1e-3) / 1e9,
+            "node_gbs": bn * m.n_elems / (ph[2] * 1e-3) / 1e9,
+            "step_gbs": (be + bn) * m.n_elems * args.steps / d["secs"] / 1e9,
+        }
+        d["engine"].close()
+    out["rel_disp_err_mixed_vs_fp64"] = rel_err
+    return out
 
 
-def time_engine(m, precision, warmup, steps, timed_kernel_steps, staging=True):
-    e = eng.Engine(m, precision, reorder=True, staging=staging)
-    e.init()
-    e.set_current_step(1)
-    e.enqueue(warmup)
-    e.sync()
-    n0 = int(e.scalars()["nsteps"])
+def traffic_entry(kernel_key: str):
+    """HBM bytes per launch of a kernel from the committed counter summary
+    (rocprofv3 FETCH_SIZE and WRITE_SIZE passes, scaled by the same counters
+    on k_copy_probe), or None when the summary is absent."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    k = t.get("kernels", {}).get(kernel_key)
+    return None if k is None else k.get("hbm_bytes_per_launch")
+
+
+def c4_section(args):
+    models = decks.config4(args.c4_jobs)
+    ne = np.array([m.n_elems for m in models], dtype=np.float64)
+    log(f"c4: {len(models)} meshes, {int(ne.sum())} elements")
     t0 = time.perf_counter()
-    e.enqueue(steps)
-    e.sync()
+    b = Batch(models, eng.MIXED, reorder=True, graph_ticks=args.graph)
     t1 = time.perf_counter()
-    n1 = int(e.scalars()["nsteps"])
-    assert n1 - n0 == steps, f"only {n1 - n0} of {steps} ticks were full steps"
-    e.set_timing(True)
-    e.enqueue(timed_kernel_steps)
-    e.sync()
-    elem_ms, node_ms, launches = e.kernel_times()
-    e.set_timing(False)
-    s = e.scalars()
-    return e, (t1 - t0), elem_ms, node_ms, s
+    b.run()
+    t2 = time.perf_counter()
+    steps = np.array([b.scalars(i)["nsteps"] for i in range(len(models))])
+    errs = [b.scalars(i)["err"] for i in range(len(models))]
+    assert all(x == 0.0 for x in errs), "a batched job raised an error flag"
+    b.close()
+    secs = t2 - t1
+    log(f"c4: built in {t1 - t0:.2f} s, ran in {secs:.3f} s")
+    return {"workload": f"config 4: {len(models)} hex-block meshes, Ne log-uniform [1e3, 5e4], "
+                        f"{int(ne.sum())} elements, own end times (2000-8000 CFL steps), elastic, mixed",
+            "jobs": len(models), "solves_per_s": len(models) / secs, "seconds": secs,
+            "elem_steps_per_s": float((ne * steps).sum() / secs), "build_seconds": t1 - t0,
+            "graph_ticks": args.graph}
+
+
+def _pump_until(srv, pending, t0, done_set, ticks):
+    """Submit arrivals whose time has come, pump, collect completions."""
+    now = time.perf_counter() - t0
+    while pending and pending[0][0] <= now:
+        ta, m, idx = pending.pop(0)
+        tk = srv.submit(m, keep=False, t_arrival=srv.clock() - (now - ta))
+        done_set[tk] = idx
+    return srv.pump(ticks)
+
+
+def c5_section(args):
+    models = decks.config5(args.c5_jobs)
+    ne = np.array([m.n_elems for m in models], dtype=np.float64)
+    log(f"c5: {len(models)} requests, {int(ne.sum())} elements, largest {int(ne.max())}")
+    budget = 64 << 30
+
+    def serve(arrivals):
+        srv = Server(eng.MIXED, budget_bytes=budget, max_live=args.c5_live, graph_ticks=args.graph)
+        pending = [(a, m, i) for i, (a, m) in enumerate(zip(arrivals, models))]
+        pending.sort(key=lambda x: x[0])
+        tick2idx = {}
+        finished = []
+        t0 = time.perf_counter()
+        guard = 0
+        while len(finished) < len(models):
+            q, live, _ = srv.status()
+            if not live and not q and pending:
+                wait = pending[0][0] - (time.perf_counter() - t0)
+                if wait > 0:
+                    time.sleep(wait)
+            finished += _pump_until(srv, pending, t0, tick2idx, args.c5_ticks)
+            guard += 1
+            assert guard < 10_000_000
+        t1 = time.perf_counter()
+        lat = []
+        for tk in finished:
+            s, _, times = srv.result(tk)
+            assert s["err"] == 0.0 and s["done"] == 1.0
+            lat.append(times[2] - times[0])
+        srv.close()
+        return t1 - t0, np.array(lat)
+
+    secs_sat, lat_sat = serve([0.0] * len(models))
+    cap = len(models) / secs_sat
+    log(f"c5: saturated {secs_sat:.3f} s ({cap:.2f} solves/s)")
+    rate = 0.8 * cap
+    rng = np.random.default_rng(55)
+    arr = np.cumsum(rng.exponential(1.0 / rate, size=len(models)))
+    arr -= arr[0]
+    secs_p, lat_p = serve(list(arr))
+    log(f"c5: Poisson {rate:.2f}/s run {secs_p:.3f} s, p50 {np.percentile(lat_p, 50):.3f} s, "
+        f"p99 {np.percentile(lat_p, 99):.3f} s")
+    # the same jobs one at a time, one engine each (build, run, results)
+    t0 = time.perf_counter()
+    lat_seq = []
+    for m in models:
+        a = time.perf_counter()
+        e = eng.Engine(m, eng.MIXED, reorder=True)
+        if args.graph:
+            e.set_graph(args.graph)
+        e.run()
+        s = e.scalars()
+        assert s["err"] == 0.0
+        e.close()
+        lat_seq.append(time.perf_counter() - a)
+    secs_seq = time.perf_counter() - t0
+    log(f"c5: one at a time {secs_seq:.3f} s")
+    return {"workload": f"config 5: {len(models)} requests from the config-4 generator, Ne log-uniform "
+                        f"[1e3, 5e5], half J2, mixed, admitted by HBM footprint (budget {budget >> 30} GiB, "
+                        f"<= {args.c5_live} live)",
+            "jobs": len(models),
+            "capacity_solves_per_s": cap,
+            "poisson_rate_per_s": rate,
+            "poisson_solves_per_s": len(models) / secs_p,
+            "latency_s": {"p50": float(np.percentile(lat_p, 50)), "p99": float(np.percentile(lat_p, 99)),
+                          "saturated_p50": float(np.percentile(lat_sat, 50)),
+                          "saturated_p99": float(np.percentile(lat_sat, 99))},
+            "one_at_a_time_solves_per_s": len(models) / secs_seq,
+            "one_at_a_time_latency_s": {"p50": float(np.percentile(lat_seq, 50)),
+                                        "p99": float(np.percentile(lat_seq, 99))},
+            "speedup_vs_one_at_a_time": cap / (len(models) / secs_seq),
+            "ticks_per_pump": args.c5_ticks}
 
 
 def cpu_baseline(m, steps=100, warm=10):
@@ -99,17 +256,78 @@ def cpu_baseline(m, steps=100, warm=10):
                       f"{o.threads} OpenMP threads on {cpu}"}
 
 
+KERNEL_KEYS = (("k_element<true", "k_element_mixed"), ("k_element<false", "k_element_fp64"),
+               ("k_node<true", "k_node_mixed"), ("k_node<false", "k_node_fp64"),
+               ("k_fin_element", "k_fin_element"), ("k_fin_node", "k_fin_node"), ("k_copy_probe", "k_copy_probe"))
+
+
+def _pmc_rows(d: str, counter: str):
+    import csv
+    import glob
+
+    rows = {}
+    files = glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True)
+    if not files:
+        raise SystemExit(f"no counter_collection.csv under {d}")
+    for fn in files:
+        with open(fn, newline="") as f:
+            for row in csv.DictReader(f):
+                if row.get("Counter_Name") != counter:
+                    continue
+                name = row.get("Kernel_Name", "")
+                for pat, key in KERNEL_KEYS:
+                    if pat in name:
+                        rows.setdefault(key, []).append(float(row["Counter_Value"]))
+                        break
+    return rows
+
+
+def pmc_summary(fetch_dir: str, write_dir: str, copy_bytes: int, command: str) -> dict:
+    """HBM bytes per launch from two counter passes (FETCH_SIZE, WRITE_SIZE,
+    KiB per dispatch), each scaled by what the same counter reads for
+    k_copy_probe, which moves copy_bytes in and copy_bytes out."""
+    fe, wr = _pmc_rows(fetch_dir, "FETCH_SIZE"), _pmc_rows(write_dir, "WRITE_SIZE")
+    cf = float(np.median(fe["k_copy_probe"]))
+    cw = float(np.median(wr["k_copy_probe"]))
+    ff, fw = copy_bytes / 1024.0 / cf, copy_bytes / 1024.0 / cw
+    out = {"command": command,
+           "copy_probe": {"bytes_each_way": copy_bytes, "fetch_kib": cf, "write_kib": cw,
+                          "fetch_factor": ff, "write_factor": fw},
+           "kernels": {}}
+    for _, key in KERNEL_KEYS:
+        if key == "k_copy_probe" or key not in fe or key not in wr:
+            continue
+        f_med, w_med = float(np.median(fe[key])), float(np.median(wr[key]))
+        out["kernels"][key] = {"dispatches": len(fe[key]), "fetch_kib_median": f_med, "write_kib_median": w_med,
+                               "hbm_bytes_per_launch": f_med * 1024.0 * ff + w_med * 1024.0 * fw}
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", default="c3")
-    ap.add_argument("--kernel-steps", type=int, default=50)
+    ap.add_argument("--kernel-steps", type=int, default=50, help="steps timed per launch with events")
+    ap.add_argument("--sections", default="c3,c4,c5", help="comma list of c3, c4, c5")
+    ap.add_argument("--graph", type=int, default=20, help="step iterations per captured hipGraph (0: none)")
+    ap.add_argument("--c4-jobs", type=int, default=256)
+    ap.add_argument("--c5-jobs", type=int, default=48)
+    ap.add_argument("--c5-live", type=int, default=64)
+    ap.add_argument("--c5-ticks", type=int, default=50)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=100)
     ap.add_argument("--no-staging", action="store_true", help="gather element corners from HBM instead of LDS")
+    ap.add_argument("--copy-bytes", type=int, default=1 << 30, help="size of the copy-bandwidth probe")
+    ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
+                    help="summarize two rocprofv3 counter passes (FETCH_SIZE, WRITE_SIZE) of this script "
+                         "into HBM bytes per launch (JSON on stdout) and exit")
+    ap.add_argument("--pmc-command", default="", help="command line recorded in the --pmc-summary output")
     args = ap.parse_args()
+    if args.pmc_summary:
+        print(json.dumps(pmc_summary(args.pmc_summary[0], args.pmc_summary[1], args.copy_bytes, args.pmc_command),
+                         indent=1))
+        return 0
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on one GPU only: every job of this engine fits one MI355X and the "
               "multi-GPU layout is design-only (see DESIGN.md); run with --gpus 1", file=sys.stderr)
@@ -117,65 +335,49 @@ def main() -> int:
     if eng.device_count() < 1:
         print("no HIP device visible", file=sys.stderr)
         return 1
-    m = workload(args.workload)
-    print(f"[bench] workload {m.name}: {m.n_elems} elements, {m.n_nodes} nodes", file=sys.stderr, flush=True)
-    r = m.n_nodes / m.n_elems
-    res = {}
-    for prec, name in ((eng.MIXED, "mixed"), (eng.FP64, "fp64")):
-        e, secs, kms, nms, s = time_engine(m, prec, args.warmup, args.steps, args.kernel_steps,
-                                           staging=not args.no_staging)
-        res[name] = dict(engine=e, secs=secs, elem_ms=kms, node_ms=nms, scalars=s)
-        print(f"[bench] {name}: {args.steps} steps in {secs:.4f} s; element kernel {kms:.4f} ms, "
-              f"node kernel {nms:.4f} ms", file=sys.stderr, flush=True)
-    u_mix = res["mixed"]["engine"].get("u")
-    u_64 = res["fp64"]["engine"].get("u")
-    rel_err = float(np.abs(u_mix - u_64).max() / max(np.abs(u_64).max(), 1e-300))
-    mixed = res["mixed"]
-    fp64 = res["fp64"]
-    ms_per_step = 1e3 * mixed["secs"] / args.steps
-    value = m.n_elems * args.steps / mixed["secs"]
-    v64 = m.n_elems * args.steps / fp64["secs"]
-    b_step, b_k1 = bytes_per_elem_step(eng.MIXED, m.plastic, r)
-    b_step64, b_k1_64 = bytes_per_elem_step(eng.FP64, m.plastic, r)
-    achieved = b_k1 * m.n_elems / (mixed["elem_ms"] * 1e-3) / 1e9
+    sections = set(args.sections.split(","))
+    copy_gbs = eng.copy_bandwidth(args.copy_bytes, 10)
+    log(f"device copy bandwidth (read+write): {copy_gbs:.1f} GB/s")
+    m = decks.config3(n=100, steps=3000)
+    log(f"workload {m.name}: {m.n_elems} elements, {m.n_nodes} nodes")
+    c3 = c3_section(args, m)
+    mixed, fp64 = c3["mixed"], c3["fp64"]
+    be, _ = mixed["alg_bytes_per_elem"]["element"], None
+    achieved = mixed["element_gbs"]
     out = {
         "metric": METRIC,
-        "value": value,
+        "value": mixed["value"],
         "unit": "elem-steps/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
+        "ms_per_step": mixed["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32+f64",
         "data": "synthetic",
-        "config": {"workload": f"{m.name}: {m.n_elems} C3D8R, {m.n_nodes} nodes, "
-                               f"{'J2 plastic' if m.plastic else 'elastic'}, mixed precision (FP32 element, "
-                               f"FP64 nodal/dt/energy)",
-                   "elements": m.n_elems, "nodes": m.n_nodes, "parallelism": "single GPU"},
+        "config": {"workload": f"config 3, {m.name}: {m.n_elems} C3D8R, {m.n_nodes} nodes, "
+                               f"{'J2 plastic' if m.plastic else 'elastic'}, mixed precision (FP32 element and "
+                               f"constitutive update, FP64 nodal state, dt and energies)",
+                   "elements": m.n_elems, "nodes": m.n_nodes, "graph_ticks": args.graph},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_element (mixed)", "bytes_per_elem": b_k1,
-                     "avg_launch_ms": mixed["elem_ms"]},
-        "fp64": {"value": v64, "ms_per_step": 1e3 * fp64["secs"] / args.steps, "elem_kernel_ms": fp64["elem_ms"],
-                 "node_kernel_ms": fp64["node_ms"],
-                 "elem_kernel_gbs": b_k1_64 * m.n_elems / (fp64["elem_ms"] * 1e-3) / 1e9},
-        "mixed_over_fp64": value / v64,
-        "rel_disp_err_mixed_vs_fp64": rel_err,
-        "step_roofline": {"mixed_gbs": b_step * value / 1e9, "fp64_gbs": b_step64 * v64 / 1e9,
-                          "mixed_frac": b_step * value / 1e9 / HBM_PEAK_GBS,
-                          "fp64_frac": b_step64 * v64 / 1e9 / HBM_PEAK_GBS,
-                          "bytes_per_elem_step": {"mixed": b_step, "fp64": b_step64}},
-        "node_kernel_ms": mixed["node_ms"],
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_entry("k_element_mixed"),
+                     "kernel": "k_element (mixed, staged, block assembly)",
+                     "alg_bytes_per_launch": be * m.n_elems, "avg_launch_ms": mixed["launch_ms"]["element"]},
+        "mixed": mixed,
+        "fp64": fp64,
+        "mixed_over_fp64": mixed["value"] / fp64["value"],
+        "rel_disp_err_mixed_vs_fp64": c3["rel_disp_err_mixed_vs_fp64"],
+        "copy_probe_gbs": copy_gbs,
     }
-    for k in ("engine",):
-        mixed.pop(k)
-        fp64.pop(k)
+    if "c4" in sections:
+        out["batched"] = c4_section(args)
+    if "c5" in sections:
+        out["serving"] = c5_section(args)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(m, args.cpu_steps)
-        print(f"[bench] cpu baseline: {out['cpu_baseline']['value']:.3e} elem-steps/s", file=sys.stderr, flush=True)
+        log(f"cpu baseline: {out['cpu_baseline']['value']:.3e} elem-steps/s")
     print(json.dumps(out), flush=True)
     return 0
 

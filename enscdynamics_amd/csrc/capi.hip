@@ -81,6 +81,21 @@ struct ensc_engine {
 
 namespace {
 
+// Plain device copy, 16 bytes per lane per iteration: the reference kernel
+// for HBM bandwidth and for scaling hardware byte counters.
+__global__ __launch_bounds__(256) void k_copy_probe(const double2* __restrict__ a, double2* __restrict__ b, size_t n) {
+    const size_t st = (size_t)gridDim.x * 256;
+    size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    for (; i + 3 * st < n; i += 4 * st) {
+        const double2 x0 = a[i], x1 = a[i + st], x2 = a[i + 2 * st], x3 = a[i + 3 * st];
+        b[i] = x0;
+        b[i + st] = x1;
+        b[i + 2 * st] = x2;
+        b[i + 3 * st] = x3;
+    }
+    for (; i < n; i += st) b[i] = a[i];
+}
+
 template <bool M, int W>
 KernelFn pick_staged(int blk) {
     switch (blk) {
@@ -211,11 +226,19 @@ ensc_status relayout(ensc_engine* E) {
         !E->pe_buf.ensure(std::max<size_t>(1, b2e.size()) * 5 * sizeof(double)) ||
         !E->pn_buf.ensure(std::max<size_t>(1, b2n.size()) * 2 * sizeof(double)))
         return fail(ENSC_ERR_MEM, "layout allocation failed");
+    const bool mixed = E->precision == ENSC_MIXED;
+    // LDS staging when every live job's largest block node list fits 64 KiB;
+    // block assembly (mixed) adds a 3-double accumulator per staged node
+    E->staged = E->staging_opt != 2 && (size_t)lds_nodes * 9 * sizeof(double) <= 65536;
+    bool any_asm = false;
+    const bool asm_fits = E->staged && mixed && (size_t)lds_nodes * 12 * sizeof(double) <= 65536;
     for (uint32_t i = 0; i < E->live.size(); ++i) {
         Job& J = *E->slots[E->live[i]];
         J.dev.ctl = E->d_ctl + E->live[i];
         J.dev.part_e = static_cast<double*>(E->pe_buf.p) + pe0[i] * 5;
         J.dev.part_n = static_cast<double*>(E->pn_buf.p) + pn0[i] * 2;
+        J.dev.block_asm = (asm_fits && J.asm_capable) ? 1 : 0;
+        any_asm |= J.dev.block_asm != 0;
         tab.push_back(J.dev);
     }
     if (!tab.empty())
@@ -225,10 +248,7 @@ ensc_status relayout(ensc_engine* E) {
     if (!b2n.empty())
         HIPCHK(hipMemcpyAsync(E->b2n_buf.p, b2n.data(), b2n.size() * sizeof(uint2), hipMemcpyHostToDevice, E->stream));
     HIPCHK(hipStreamSynchronize(E->stream));
-    const bool mixed = E->precision == ENSC_MIXED;
-    // LDS staging when every live job's largest block node list fits 64 KiB
-    E->staged = E->staging_opt != 2 && (size_t)lds_nodes * 9 * sizeof(double) <= 65536;
-    E->lds_bytes = E->staged ? (size_t)std::max<uint32_t>(lds_nodes, 1) * 9 * sizeof(double) : 0;
+    E->lds_bytes = E->staged ? (size_t)std::max<uint32_t>(lds_nodes, 1) * (any_asm ? 12 : 9) * sizeof(double) : 0;
     E->ke = mixed ? pick_e<true>(E->blk_e, E->staged, E->waves) : pick_e<false>(E->blk_e, E->staged, E->waves);
     E->kn = mixed ? pick_n<true>(E->blk_n) : pick_n<false>(E->blk_n);
     E->dirty = false;
@@ -236,10 +256,11 @@ ensc_status relayout(ensc_engine* E) {
     return ENSC_OK;
 }
 
-// One tick = four launches: element pass, per-job element finalize, nodal
-// update, per-job node finalize.  `ev` (timing only) gets an event after
-// each launch.
-constexpr int FIN_BLK = 256;
+// One tick = element pass, per-job element finalize (which also closes the
+// previous tick's nodal update), nodal update.  A run of ticks ends with one
+// per-job node finalize.  With timing on, every tick launches all four
+// phases so each is timed on its own.
+constexpr int FIN_BLK = 1024;
 void launch_phase(ensc_engine* E, int phase) {
     const JobDev* jobs = static_cast<const JobDev*>(E->jobs_buf.p);
     const uint32_t nj = (uint32_t)E->live.size();
@@ -257,8 +278,10 @@ void launch_phase(ensc_engine* E, int phase) {
         default: hipLaunchKernelGGL(k_fin_node<FIN_BLK>, dim3(nj), dim3(FIN_BLK), 0, E->stream, jobs); break;
     }
 }
-void launch_tick(ensc_engine* E) {
-    for (int p = 0; p < 4; ++p) launch_phase(E, p);
+void launch_ticks(ensc_engine* E, int64_t n) {
+    for (int64_t i = 0; i < n; ++i)
+        for (int p = 0; p < 3; ++p) launch_phase(E, p);
+    if (n > 0) launch_phase(E, 3);
 }
 
 ensc_status build_graph(ensc_engine* E) {
@@ -271,7 +294,7 @@ ensc_status build_graph(ensc_engine* E) {
         E->graph = nullptr;
     }
     HIPCHK(hipStreamBeginCapture(E->stream, hipStreamCaptureModeThreadLocal));
-    for (int i = 0; i < E->graph_ticks; ++i) launch_tick(E);
+    launch_ticks(E, E->graph_ticks);
     HIPCHK(hipStreamEndCapture(E->stream, &E->graph));
     HIPCHK(hipGraphInstantiate(&E->graph_exec, E->graph, nullptr, nullptr, 0));
     E->graph_valid = true;
@@ -301,7 +324,7 @@ ensc_status enqueue_ticks(ensc_engine* E, int64_t n) {
             if (!E->graph_valid && (st = build_graph(E))) return st;
             for (; i + E->graph_ticks <= n; i += E->graph_ticks) HIPCHK(hipGraphLaunch(E->graph_exec, E->stream));
         }
-        for (; i < n; ++i) launch_tick(E);
+        launch_ticks(E, n - i);
         E->pending_ticks = 0;
     }
     HIPCHK(hipGetLastError());
@@ -341,10 +364,13 @@ ensc_status engine_new(const ensc_options* opts, std::unique_ptr<ensc_engine>& E
     E->reorder = o.reorder;
     E->staging_opt = o.staging;
     auto okblk = [](int b) { return b == 64 || b == 128 || b == 256; };
-    E->blk_e = okblk(o.block_elem) ? o.block_elem : 256;
+    // defaults from block-size / occupancy sweeps on MI355X (configuration 3):
+    // FP64 128 threads at one wave per SIMD (the body needs > 256 registers),
+    // mixed 256 threads at two
+    E->blk_e = okblk(o.block_elem) ? o.block_elem : (o.precision == ENSC_FP64 ? 128 : 256);
     E->blk_n = okblk(o.block_node) ? o.block_node : 256;
     if (o.waves < 0 || o.waves > 3) return fail(ENSC_ERR_ARG, "waves must be 0 (default), 1, 2 or 3");
-    E->waves = o.waves ? o.waves : 2;
+    E->waves = o.waves ? o.waves : (o.precision == ENSC_FP64 ? 1 : 2);
     HIPCHK(hipSetDevice(E->device));
     HIPCHK(hipStreamCreateWithFlags(&E->stream, hipStreamNonBlocking));
     return ENSC_OK;
@@ -553,12 +579,21 @@ ensc_status download(ensc_engine* E, Job& J, const std::string& f, double* out, 
     const bool mixed = E->precision == ENSC_MIXED;
     if (f == "u" || f == "v" || f == "x") {
         if (count < (int64_t)(3 * Nn)) return fail(ENSC_ERR_ARG, "buffer too small");
-        std::vector<double> rec(Nn * NODE_REC);
-        HIPCHK(hipMemcpy(rec.data(), J.dev.node, rec.size() * sizeof(double), hipMemcpyDeviceToHost));
-        const int off = f == "x" ? 0 : (f == "v" ? 3 : 6);
+        std::vector<double> pos(Nn * POS_REC), vu(Nn * VU_REC);
+        HIPCHK(hipMemcpy(pos.data(), J.dev.pos, pos.size() * sizeof(double), hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(vu.data(), J.dev.vu, vu.size() * sizeof(double), hipMemcpyDeviceToHost));
         for (size_t i = 0; i < Nn; ++i) {
             const size_t o = J.n_new2old[i];
-            for (int d = 0; d < 3; ++d) out[3 * o + d] = rec[i * NODE_REC + off + d];
+            for (int d = 0; d < 3; ++d) {
+                double x;
+                if (f == "v")
+                    x = vu[i * VU_REC + d];
+                else if (f == "u")
+                    x = vu[i * VU_REC + 3 + d];
+                else  // the mixed engine keeps the reference position: x = X + u
+                    x = mixed ? pos[i * POS_REC + d] + vu[i * VU_REC + 3 + d] : pos[i * POS_REC + d];
+                out[3 * o + d] = x;
+            }
         }
         return ENSC_OK;
     }
@@ -696,6 +731,44 @@ int32_t ensc_device_count(void) {
     int n = 0;
     if (hipGetDeviceCount(&n) != hipSuccess) return 0;
     return n;
+}
+
+ensc_status ensc_copy_bandwidth(int32_t device, int64_t bytes, int32_t reps, double* gbs) {
+    if (bytes < 16 || reps < 1 || !gbs) return fail(ENSC_ERR_ARG, "bytes >= 16, reps >= 1 and gbs required");
+    if (ensc_device_count() <= 0) return fail(ENSC_ERR_NODEV, "no HIP device visible");
+    HIPCHK(hipSetDevice(device));
+    const size_t n = (size_t)bytes / 16;
+    void *a = nullptr, *b = nullptr;
+    HIPCHK(hipMalloc(&a, n * 16));
+    if (hipMalloc(&b, n * 16) != hipSuccess) {
+        (void)hipFree(a);
+        return fail(ENSC_ERR_MEM, "copy buffer allocation failed");
+    }
+    hipStream_t s = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    ensc_status st = ENSC_OK;
+    float ms = 0.f;
+    if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
+        hipEventCreate(&e1) != hipSuccess || hipMemsetAsync(a, 0, n * 16, s) != hipSuccess) {
+        st = fail(ENSC_ERR_HIP, "copy probe set-up failed");
+    } else {
+        const unsigned grid = 256 * 8 * 4;
+        hipLaunchKernelGGL(k_copy_probe, dim3(grid), dim3(256), 0, s, (const double2*)a, (double2*)b, n);
+        (void)hipEventRecord(e0, s);
+        for (int r = 0; r < reps; ++r)
+            hipLaunchKernelGGL(k_copy_probe, dim3(grid), dim3(256), 0, s, (const double2*)a, (double2*)b, n);
+        (void)hipEventRecord(e1, s);
+        if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
+            st = fail(ENSC_ERR_HIP, "copy probe failed");
+        else
+            *gbs = 2.0 * (double)(n * 16) * reps / (ms * 1e-3) / 1e9;
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (s) (void)hipStreamDestroy(s);
+    (void)hipFree(a);
+    (void)hipFree(b);
+    return st;
 }
 
 ensc_status ensc_engine_create(const ensc_model_desc* model, const ensc_options* opts, ensc_engine** out) {

@@ -361,15 +361,47 @@ ensc_status build_job(Job& J, int precision, int reorder, int blk_e, int blk_n) 
             J.lds_nodes = std::max<uint32_t>(J.lds_nodes, (uint32_t)tmp.size());
         }
     }
+    // Block assembly (mixed engine) needs every element block to be free of
+    // corner conflicts: no two of its elements share a node at the same
+    // corner index.  Hex meshes of one orientation always are.
+    J.asm_capable = mixed;
+    {
+        const size_t nb = blk_ptr.size() - 1;
+        std::vector<uint32_t> stamp(J.lds_nodes + 1, 0);
+        uint32_t tag = 0;
+        for (size_t b = 0; b < nb && J.asm_capable; ++b) {
+            const size_t e0 = b * blk_e, e1 = std::min(Ne, e0 + blk_e);
+            for (int k = 0; k < 8 && J.asm_capable; ++k) {
+                ++tag;
+                for (size_t e = e0; e < e1; ++e) {
+                    const uint16_t l = lconn[(size_t)k * es + e];
+                    if (stamp[l] == tag) {
+                        J.asm_capable = false;
+                        break;
+                    }
+                    stamp[l] = tag;
+                }
+            }
+        }
+    }
+    // each node's positions in blk_nodes (one per element block holding it),
+    // ascending: the order the nodal update adds the block partials
+    std::vector<uint32_t> pos_ptr(Nn + 1, 0), pos_list(blk_nodes.size());
+    for (uint32_t g : blk_nodes) pos_ptr[g + 1]++;
+    for (size_t i = 0; i < Nn; ++i) pos_ptr[i + 1] += pos_ptr[i];
+    {
+        std::vector<uint32_t> fill(pos_ptr.begin(), pos_ptr.end() - 1);
+        for (size_t p = 0; p < blk_nodes.size(); ++p) pos_list[fill[blk_nodes[p]]++] = (uint32_t)p;
+    }
     // node records
-    std::vector<double> rec((size_t)Nn * NODE_REC, 0.0);
+    std::vector<double> pos((size_t)Nn * POS_REC, 0.0), vu((size_t)Nn * VU_REC, 0.0);
     for (size_t i = 0; i < Nn; ++i) {
         const size_t o = J.n_new2old[i];
         for (int d = 0; d < 3; ++d) {
-            rec[i * NODE_REC + d] = M.x0[3 * o + d];
-            rec[i * NODE_REC + 3 + d] = M.v_init[3 * o + d];
+            pos[i * POS_REC + d] = M.x0[3 * o + d];
+            vu[i * VU_REC + d] = M.v_init[3 * o + d];
         }
-        rec[i * NODE_REC + 9] = invM[o];
+        pos[i * POS_REC + 3] = invM[o];
     }
     // initial kinetic energy
     double ke0 = 0.0;
@@ -408,7 +440,15 @@ ensc_status build_job(Job& J, int precision, int reorder, int blk_e, int blk_n) 
         D.qia = dq;
         D.epsp = dp;
         D.fel = df;
-        D.xcomp = dx;
+        D.ucomp = dx;
+        float* dpf;
+        uint32_t *dpp, *dpl;
+        if ((st = upload(&dpf, none, blk_nodes.size() * 3, A))) return st;
+        if ((st = upload(&dpp, pos_ptr, pos_ptr.size(), A))) return st;
+        if ((st = upload(&dpl, pos_list, pos_list.size(), A))) return st;
+        D.part_f = dpf;
+        D.pos_ptr = dpp;
+        D.pos_list = dpl;
     } else {
         std::vector<double> b(9 * es, 0.0), v(es, 0.0);
         for (size_t i = 0; i < Ne; ++i) {
@@ -430,11 +470,13 @@ ensc_status build_job(Job& J, int precision, int reorder, int blk_e, int blk_n) 
         D.qia = dq;
         D.epsp = dp;
         D.fel = df;
-        D.xcomp = nullptr;
+        D.ucomp = nullptr;
     }
-    double* d_node;
-    if ((st = upload(&d_node, rec, rec.size(), A))) return st;
-    D.node = d_node;
+    double *d_pos, *d_vu;
+    if ((st = upload(&d_pos, pos, pos.size(), A))) return st;
+    if ((st = upload(&d_vu, vu, vu.size(), A))) return st;
+    D.pos = d_pos;
+    D.vu = d_vu;
     std::vector<double> none_d;
     if ((st = upload(&D.rf, none_d, 3 * Nn, A))) return st;
     std::vector<uint8_t> none_u8;

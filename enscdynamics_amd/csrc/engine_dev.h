@@ -45,7 +45,15 @@ enum : int32_t { KIND_NONE = 0, KIND_STEP = 1, KIND_PREPASS = 2, KIND_INIT = 3 }
 // BC mask bits per node: bit d = velocity BC on dof d, bit 3+d = SPC on dof d.
 constexpr uint8_t BC_VEL0 = 1u, BC_SPC0 = 8u;
 constexpr uint32_t ELL_EMPTY = 0xFFFFFFFFu;
-constexpr int NODE_REC = 10;  // x0 x1 x2 v0 v1 v2 u0 u1 u2 invM (doubles)
+// Node state, two arrays of 16-byte-aligned records:
+//   pos [nn][4] = [X0 X1 X2 invM]   (32 bytes)
+//   vu  [nn][6] = [v0 v1 v2 u0 u1 u2] (48 bytes)
+// X is the current position in the FP64 engine (accumulated like the
+// reference's vertices) and the reference position in the mixed engine,
+// where the current position is X + u and pos is never rewritten.  Keeping
+// v and u apart from X makes the nodal update's stores contiguous.
+constexpr int POS_REC = 4, VU_REC = 6;
+constexpr int NODE_BYTES = 8 * (POS_REC + VU_REC);
 constexpr int MAX_PROBE = 8;
 
 // Per-job scalar state, resident on the device.  The time-step scalars are
@@ -70,14 +78,23 @@ struct JobDev {
     void* sigma;           // [6][ne_s]
     void* qia;             // [12][ne_s]
     void* epsp;            // [ne_s]
-    void* fel;             // [8][ne_s][3] element nodal forces (corner-major, packed xyz)
+    void* fel;             // [8][ne_s][3] element nodal forces (corner-major, packed xyz);
+                           //   gathered assembly only
     // LDS staging of each element block's nodes (null when not staged)
     const uint16_t* lconn;     // [8][ne_s] corner -> index in the block's node list
     const uint32_t* blk_ptr;   // [nblk_e + 1] offsets into blk_nodes
     const uint32_t* blk_nodes; // node ids of each element block, ascending
+    // block assembly (mixed engine): each element block sums its elements'
+    // corner forces per block node in LDS and writes one partial per
+    // (block, node) at the node's position in blk_nodes; the nodal update
+    // adds a node's partials in ascending position order.
+    float* part_f;             // [n_pos][3] partial nodal forces
+    const uint32_t* pos_ptr;   // [nn + 1] offsets into pos_list
+    const uint32_t* pos_list;  // positions in blk_nodes of each node, ascending
     // node arrays
-    double* node;          // [nn][NODE_REC]
-    float* xcomp;          // [nn][4] compensation of x (mixed) or null
+    double* pos;           // [nn][POS_REC]  X, invM
+    double* vu;            // [nn][VU_REC]   v, u
+    float* ucomp;          // [nn][4] compensation of the u accumulation (mixed) or null
     double* rf;            // [nn][3]
     const uint8_t* bcmask; // [nn]
     const double* bcval;   // [nn][3]
@@ -94,6 +111,7 @@ struct JobDev {
     // material and parameters (Material_elastic.cpp:9-14; exDyna3D.h:179-188)
     double lambda, G, K, rho, wos, c_dr, c_cr, cvisl, cvisq, factor, yield0, hard;
     int32_t plastic, dyn_bc, ell_w, n_probe;
+    int32_t block_asm, pad_asm;  // 1: block assembly (staged mixed engine, conflict-free blocks)
     uint32_t ne, nn, ne_s, nn_s;
     uint32_t nblk_e, nblk_n;
     uint32_t probe[MAX_PROBE];

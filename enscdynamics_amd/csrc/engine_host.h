@@ -66,6 +66,7 @@ struct Job {
     size_t ne_s = 0, nn_s = 0;
     uint32_t ell_w = 0;
     uint32_t lds_nodes = 0;  // largest distinct-node count of an element block
+    bool asm_capable = false; // mixed engine, every element block free of corner conflicts
     double ke0 = 0.0;
     // setCurrentStep state (exDyna3D.h:215-220)
     uint32_t cur_step = 0;

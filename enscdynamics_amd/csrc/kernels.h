@@ -13,7 +13,9 @@
 //               canonical (corner, element) order, adds gravity, applies the
 //               BCs and reactions (2029-2145), and does the central-difference
 //               update a -> v -> u -> x (642-696).
-//   k_fin_node : one block per job: kinetic energy, external work, history.
+//   k_fin_node : one block per job: kinetic energy, external work, history
+//               (also done at the start of the next k_fin_element, so only
+//               the last step iteration of a run launches it).
 // Cross-block results travel through per-block partials and the launch
 // boundaries; no kernel waits on, or fences for, another block.
 // The step size of a tick depends on the min edge of the current
@@ -276,8 +278,19 @@ __device__ __noinline__ V3 bc_override(const JobDev& J, uint32_t g, double t, V3
 
 #define CORNER_FENCE
 // Element body instantiations: corner data gathered from HBM (G) or from
-// the block's node records staged in LDS (S).
-#define NVAL(k, i) gld((const double*)J.node, ES.nd[k] * NODE_REC + (i))
+// the block's node records staged in LDS (S).  NVAL(k, i) is value i of
+// corner k's x0 x1 x2 v0 v1 v2 u0 u1 u2, with x the current position (in
+// the mixed engine the record holds the reference position, so x = X + u).
+
+#define ELEM_ARGS                                                                                          \
+    const JobDev &J, uint32_t e, const ElemState<R> &ES, double dt, int32_t kind, double time_n, bool act, \
+        bool asmode, double &Le, double &dts, double *en, uint32_t &errf, const ENSC_LDS double *lds,     \
+        ENSC_LDS double *acc
+#define NVAL(k, i)                                                                                      \
+    ((i) < 3 ? (MIXED ? gld((const double*)J.pos, ES.nd[k] * POS_REC + (i)) +                           \
+                            gld((const double*)J.vu, ES.nd[k] * VU_REC + 3 + (i))                           \
+                      : gld((const double*)J.pos, ES.nd[k] * POS_REC + (i)))                              \
+             : gld((const double*)J.vu, ES.nd[k] * VU_REC + (i)-3))
 #define DYN_BC_OVERRIDE(k, v)                                              \
     if (J.dyn_bc && kind == KIND_STEP) {                                   \
         const V3 o_ = bc_override(J, ES.nd[k], time_n, V3{v[0], v[1], v[2]}); \
@@ -285,44 +298,41 @@ __device__ __noinline__ V3 bc_override(const JobDev& J, uint32_t g, double t, V3
         v[1] = o_.y;                                                       \
         v[2] = o_.z;                                                       \
     }
-__device__ __forceinline__ void element_fp64_G(const JobDev& J, uint32_t e, const ElemState<double>& ES, double dt,
-                                               int32_t kind, double time_n, double& Le, double& dts, double* en,
-                                               uint32_t& errf, const ENSC_LDS double* lds) {
+template <class R>
+__device__ __forceinline__ void element_fp64_G(ELEM_ARGS) {
 #pragma clang fp contract(off)
-    typedef double R;
-    constexpr bool MIXED = false;
+    constexpr bool MIXED = false, ASM_CAP = false;
 #include "elem_body.inc"
 }
-__device__ __forceinline__ void element_mixed_G(const JobDev& J, uint32_t e, const ElemState<float>& ES, double dt,
-                                                int32_t kind, double time_n, double& Le, double& dts, double* en,
-                                                uint32_t& errf, const ENSC_LDS double* lds) {
+template <class R>
+__device__ __forceinline__ void element_mixed_G(ELEM_ARGS) {
 #pragma clang fp contract(fast)
-    typedef float R;
-    constexpr bool MIXED = true;
+    constexpr bool MIXED = true, ASM_CAP = false;
 #include "elem_body.inc"
 }
 #undef NVAL
 #undef DYN_BC_OVERRIDE
-#define NVAL(k, i) lds[ES.nd[k] * 9 + (i)]
+
+// Staged record in LDS: [x0 x1 x2 v0 v1 v2 u0 u1 u2] as copied from the node
+// record (x0..2 = the record's X).
+#define NVAL(k, i) \
+    ((MIXED && (i) < 3) ? lds[ES.nd[k] * 9 + (i)] + lds[ES.nd[k] * 9 + 6 + (i)] : lds[ES.nd[k] * 9 + (i)])
 #define DYN_BC_OVERRIDE(k, v)  // applied once per node while staging
-__device__ __forceinline__ void element_fp64_S(const JobDev& J, uint32_t e, const ElemState<double>& ES, double dt,
-                                               int32_t kind, double time_n, double& Le, double& dts, double* en,
-                                               uint32_t& errf, const ENSC_LDS double* lds) {
+template <class R>
+__device__ __forceinline__ void element_fp64_S(ELEM_ARGS) {
 #pragma clang fp contract(off)
-    typedef double R;
-    constexpr bool MIXED = false;
+    constexpr bool MIXED = false, ASM_CAP = false;
 #include "elem_body.inc"
 }
-__device__ __forceinline__ void element_mixed_S(const JobDev& J, uint32_t e, const ElemState<float>& ES, double dt,
-                                                int32_t kind, double time_n, double& Le, double& dts, double* en,
-                                                uint32_t& errf, const ENSC_LDS double* lds) {
+template <class R>
+__device__ __forceinline__ void element_mixed_S(ELEM_ARGS) {
 #pragma clang fp contract(fast)
-    typedef float R;
-    constexpr bool MIXED = true;
+    constexpr bool MIXED = true, ASM_CAP = true;
 #include "elem_body.inc"
 }
 #undef NVAL
 #undef DYN_BC_OVERRIDE
+#undef ELEM_ARGS
 
 // Fixed-order reduction of a job's block partials (value q of block b at
 // part[q * nblk + b]): thread t folds blocks t, t+BLK, t+2*BLK, ... in
@@ -389,17 +399,23 @@ __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(WPE))) void
     ElemState<R> ES;
     load_state<R, STAGED>(J, act ? e : J.ne - 1, dc.kind != KIND_PREPASS, ES);
     ENSC_LDS double* lds = (ENSC_LDS double*)s_nodes;
+    // block assembly: staged mixed jobs whose blocks are conflict-free, on
+    // ticks that compute forces (uniform over the block)
+    const bool asmode = MIXED && STAGED && J.block_asm && dc.kind != KIND_PREPASS;
+    uint32_t n0 = 0, nbn = 0;
+    ENSC_LDS double* acc = lds;
     if (STAGED) {
-        // stage the block's node records once (x, v, u); velocity BCs with an
+        // stage the block's node records once (X, v, u); velocity BCs with an
         // amplitude are applied here, once per node
-        const uint32_t n0 = gld(J.blk_ptr, lb), n1 = gld(J.blk_ptr, lb + 1);
-        for (uint32_t i = threadIdx.x; i < n1 - n0; i += BLK) {
+        n0 = gld(J.blk_ptr, lb);
+        nbn = gld(J.blk_ptr, lb + 1) - n0;
+        acc = lds + nbn * 9;
+        for (uint32_t i = threadIdx.x; i < nbn; i += BLK) {
             const uint32_t g = gld(J.blk_nodes, n0 + i);
-            const uint32_t r0 = g * NODE_REC;
-            const D2 p0 = gld2(J.node, r0), p1 = gld2(J.node, r0 + 2), p2 = gld2(J.node, r0 + 4),
-                     p3 = gld2(J.node, r0 + 6);
-            const double p8 = gld(J.node, r0 + 8);
-            V3 v{p1.b, p2.a, p2.b};
+            const D2 p0 = gld2(J.pos, g * POS_REC), p1 = gld2(J.pos, g * POS_REC + 2);
+            const D2 p2 = gld2(J.vu, g * VU_REC), p3 = gld2(J.vu, g * VU_REC + 2), p4 = gld2(J.vu, g * VU_REC + 4);
+            // [X0 X1 | X2 im] [v0 v1 | v2 u0 | u1 u2]
+            V3 v{p2.a, p2.b, p3.a};
             if (J.dyn_bc && dc.kind == KIND_STEP) v = bc_override(J, g, dc.time_n, v);
             ENSC_LDS double* d = lds + i * 9;
             d[0] = p0.a;
@@ -408,26 +424,48 @@ __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(WPE))) void
             d[3] = v.x;
             d[4] = v.y;
             d[5] = v.z;
-            d[6] = p3.a;
-            d[7] = p3.b;
-            d[8] = p8;
+            d[6] = p3.b;
+            d[7] = p4.a;
+            d[8] = p4.b;
+            if (asmode) {
+                acc[3 * i] = 0.0;
+                acc[3 * i + 1] = 0.0;
+                acc[3 * i + 2] = 0.0;
+            }
         }
         __syncthreads();
     }
     double Le = __builtin_inf(), dts = __builtin_inf();
     double en[3] = {0.0, 0.0, 0.0};
     uint32_t errf = 0;
-    if (act) {
-        if (STAGED) {
+    if (asmode) {
+        // every lane runs the body (padding lanes on the last element) so the
+        // whole block meets the assembly barriers; padding results are dropped
+        if constexpr (MIXED && STAGED)
+            element_mixed_S<R>(J, act ? e : J.ne - 1, ES, dc.dt_i_1, dc.kind, dc.time_n, act, true, Le, dts, en, errf,
+                               lds, acc);
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < nbn; i += BLK) {
+            float* pf = J.part_f + (size_t)(n0 + i) * 3;
+            gst(pf, 0u) = (float)acc[3 * i];
+            gst(pf, 1u) = (float)acc[3 * i + 1];
+            gst(pf, 2u) = (float)acc[3 * i + 2];
+        }
+        if (!act) {
+            Le = dts = __builtin_inf();
+            en[0] = en[1] = en[2] = 0.0;
+        }
+    } else if (act) {
+        if constexpr (STAGED) {
             if constexpr (MIXED)
-                element_mixed_S(J, e, ES, dc.dt_i_1, dc.kind, dc.time_n, Le, dts, en, errf, lds);
+                element_mixed_S<R>(J, e, ES, dc.dt_i_1, dc.kind, dc.time_n, true, false, Le, dts, en, errf, lds, acc);
             else
-                element_fp64_S(J, e, ES, dc.dt_i_1, dc.kind, dc.time_n, Le, dts, en, errf, lds);
+                element_fp64_S<R>(J, e, ES, dc.dt_i_1, dc.kind, dc.time_n, true, false, Le, dts, en, errf, lds, acc);
         } else {
             if constexpr (MIXED)
-                element_mixed_G(J, e, ES, dc.dt_i_1, dc.kind, dc.time_n, Le, dts, en, errf, lds);
+                element_mixed_G<R>(J, e, ES, dc.dt_i_1, dc.kind, dc.time_n, true, false, Le, dts, en, errf, lds, acc);
             else
-                element_fp64_G(J, e, ES, dc.dt_i_1, dc.kind, dc.time_n, Le, dts, en, errf, lds);
+                element_fp64_G<R>(J, e, ES, dc.dt_i_1, dc.kind, dc.time_n, true, false, Le, dts, en, errf, lds, acc);
         }
     }
     // block partials: wave shuffles, then LDS across the block's waves
@@ -471,7 +509,8 @@ __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(WPE))) void
 }
 
 // Per-job end of the element pass: one block per job in the job table
-// reduces the job's block partials (written by k_element; the launch
+// closes the previous nodal update if still open (node_phase_end), reduces
+// the job's block partials (written by k_element; the launch
 // boundary orders them) and advances the job's time-step scalars exactly as
 // update_minVertex_perMaterial_and_dt and the run-loop clamps do
 // (exDyna3D.cpp:663-686, 90-97, 775-780).
@@ -480,6 +519,12 @@ __global__ __launch_bounds__(BLK) void k_fin_element(const JobDev* __restrict__ 
 #pragma clang fp contract(off)
     const JobDev& J = jobs[blockIdx.x];
     Ctl* ctl = J.ctl;
+    {
+        // the previous step iteration's nodal update, if still open
+        __shared__ double s_red_n[BLK / 64][2];
+        node_phase_end<BLK>(J, s_red_n);
+        __syncthreads();
+    }
     const Decision dc = decide(*ctl, J.factor, J.wos);
     if (dc.kind == KIND_NONE) return;
     __shared__ double s_red[BLK / 64][5];
@@ -542,17 +587,49 @@ __global__ __launch_bounds__(BLK) void k_node(const JobDev* __restrict__ jobs, c
     const uint32_t n = lb * BLK + threadIdx.x;
     double ke = 0.0, pw = 0.0;
     if (n < J.nn) {
-        const uint32_t r0 = n * NODE_REC;
-        const D2 p0 = gld2(J.node, r0), p1 = gld2(J.node, r0 + 2), p2 = gld2(J.node, r0 + 4),
-                 p3 = gld2(J.node, r0 + 6), p4 = gld2(J.node, r0 + 8);
-        double x[3] = {p0.a, p0.b, p1.a};
-        double v[3] = {p1.b, p2.a, p2.b};
-        double u[3] = {p3.a, p3.b, p4.a};
-        const double im = p4.b;
+        // [X0 X1 | X2 im] [v0 v1 | v2 u0 | u1 u2]; the mixed engine never
+        // rewrites X (the reference position)
+        const D2 p1 = gld2(J.pos, n * POS_REC + 2);
+        const D2 p2 = gld2(J.vu, n * VU_REC), p3 = gld2(J.vu, n * VU_REC + 2), p4 = gld2(J.vu, n * VU_REC + 4);
+        double x[3] = {0.0, 0.0, p1.a};
+        if (!MIXED) {
+            const D2 p0 = gld2(J.pos, n * POS_REC);
+            x[0] = p0.a;
+            x[1] = p0.b;
+        }
+        double v[3] = {p2.a, p2.b, p3.a};
+        double u[3] = {p3.b, p4.a, p4.b};
+        const double im = p1.b;
+        double acc[3] = {0.0, 0.0, 0.0};
+        if (MIXED && J.block_asm) {
+            // block assembly: the node's partials, one per element block that
+            // holds it, in ascending block order
+            const uint32_t q0 = gld(J.pos_ptr, n), q1 = gld(J.pos_ptr, n + 1);
+            for (uint32_t q = q0; q < q1; q += 4) {
+                float f[4][3];
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    f[j][0] = f[j][1] = f[j][2] = 0.f;
+                    if (q + j < q1) {
+                        const uint32_t pos = gld(J.pos_list, q + j);
+                        f[j][0] = gld(J.part_f, 3 * pos);
+                        f[j][1] = gld(J.part_f, 3 * pos + 1);
+                        f[j][2] = gld(J.part_f, 3 * pos + 2);
+                    }
+                }
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    if (q + j < q1) {
+                        acc[0] += (double)f[j][0];
+                        acc[1] += (double)f[j][1];
+                        acc[2] += (double)f[j][2];
+                    }
+            }
+        } else {
         // Element forces of the node in (corner, original element) order;
         // slots are filled from the front, ELL_EMPTY after the last.  Each
         // chunk of 8 slots issues all its loads before summing.
-        double acc[3] = {0.0, 0.0, 0.0}, grp[3] = {0.0, 0.0, 0.0};
+        double grp[3] = {0.0, 0.0, 0.0};
         int gk = -1;
         const uint32_t es = J.ne_s, W = (uint32_t)J.ell_w;
         for (uint32_t w0 = 0; w0 < W; w0 += 8) {
@@ -601,6 +678,7 @@ __global__ __launch_bounds__(BLK) void k_node(const JobDev* __restrict__ jobs, c
             acc[1] += grp[1];
             acc[2] += grp[2];
         }
+        }
         double a[3] = {0.0 + acc[0], 0.0 + acc[1], 0.0 + acc[2]};
         if (J.grav_on) {  // apply_gravity (800-826)
             const double ag = ctl->grav_amp * J.grav_g;
@@ -640,28 +718,31 @@ __global__ __launch_bounds__(BLK) void k_node(const JobDev* __restrict__ jobs, c
             double vd = v[d];
             if (mk & (BC_VEL0 << d)) vd = vbc[d];
             if (mk & (BC_SPC0 << d)) vd = 0.0;
-            vd += hs * ad / 2.0;          // update_velocity (646-649)
-            u[d] += dt_i * vd;            // update_displacement (651-654)
-            const double dx = dt_i * vd;  // move_mesh (688-696)
+            vd += hs * ad / 2.0;  // update_velocity (646-649)
+            const double dx = dt_i * vd;
             if (MIXED) {
-                // compensated (Kahan) position accumulation
-                const float cf = gld(J.xcomp, n * 4 + d);
+                // update_displacement (651-654), compensated (Kahan); the
+                // position is X + u, so move_mesh has nothing to store
+                const float cf = gld(J.ucomp, n * 4 + d);
                 const double y = dx - (double)cf;
-                const double t = x[d] + y;
-                gst(J.xcomp, n * 4 + d) = (float)((t - x[d]) - y);
-                x[d] = t;
+                const double t = u[d] + y;
+                gst(J.ucomp, n * 4 + d) = (float)((t - u[d]) - y);
+                u[d] = t;
             } else {
-                x[d] += dx;
+                u[d] += dx;  // update_displacement (651-654)
+                x[d] += dx;  // move_mesh (688-696)
             }
             v[d] = vd;
             k2 += vd * vd;
         }
         ke = 0.5 * k2 / im;
-        gst2(J.node, r0, x[0], x[1]);
-        gst2(J.node, r0 + 2, x[2], v[0]);
-        gst2(J.node, r0 + 4, v[1], v[2]);
-        gst2(J.node, r0 + 6, u[0], u[1]);
-        gst(J.node, r0 + 8) = u[2];
+        if (!MIXED) {
+            gst2(J.pos, n * POS_REC, x[0], x[1]);
+            gst2(J.pos, n * POS_REC + 2, x[2], im);
+        }
+        gst2(J.vu, n * VU_REC, v[0], v[1]);
+        gst2(J.vu, n * VU_REC + 2, v[2], u[0]);
+        gst2(J.vu, n * VU_REC + 4, u[1], u[2]);
     }
     __shared__ double s_red[BLK / 64][2];
     ke = wave_sum(ke);
@@ -684,16 +765,16 @@ __global__ __launch_bounds__(BLK) void k_node(const JobDev* __restrict__ jobs, c
     }
 }
 
-// Per-job end of the nodal update: kinetic energy, external work and the
-// job's history row for the step (one block per job in the job table).
+// End of a job's nodal update: kinetic energy, external work and the
+// job's history row for the step.  Runs when the job's scalars say a nodal
+// update is pending (kind == STEP) and marks it done; the same test makes a
+// second call a no-op.  Every thread of the block must call it.
 template <int BLK>
-__global__ __launch_bounds__(BLK) void k_fin_node(const JobDev* __restrict__ jobs) {
+__device__ inline void node_phase_end(const JobDev& J, double (*s_red)[2]) {
 #pragma clang fp contract(off)
-    const JobDev& J = jobs[blockIdx.x];
     Ctl* ctl = J.ctl;
     if (ctl->kind != KIND_STEP) return;
     const double dt_i = ctl->dt_i, dt_i_1 = ctl->dt_i_1;
-    __shared__ double s_red[BLK / 64][2];
     const bool is_min[2] = {false, false};
     double tot[2];
     reduce_partials<2, BLK>(J.part_n, J.nblk_n, is_min, tot, s_red);
@@ -714,7 +795,7 @@ __global__ __launch_bounds__(BLK) void k_fin_node(const JobDev* __restrict__ job
             h[ENSC_H_VIS] = c.vis;
             h[ENSC_H_WEXT] = c.wext;
             for (int p = 0; p < J.n_probe; ++p) {
-                const double* r = J.node + (size_t)J.probe[p] * NODE_REC + 6;
+                const double* r = J.vu + (size_t)J.probe[p] * VU_REC + 3;
                 h[ENSC_H_BASE + 3 * p + 0] = r[0];
                 h[ENSC_H_BASE + 3 * p + 1] = r[1];
                 h[ENSC_H_BASE + 3 * p + 2] = r[2];
@@ -722,6 +803,15 @@ __global__ __launch_bounds__(BLK) void k_fin_node(const JobDev* __restrict__ job
         }
         c.kind = KIND_NONE;
     }
+}
+
+// Per-job node finalize (one block per job in the job table).  The element
+// finalize of the next step iteration does the same work first, so a run
+// of step iterations needs this launch only after its last one.
+template <int BLK>
+__global__ __launch_bounds__(BLK) void k_fin_node(const JobDev* __restrict__ jobs) {
+    __shared__ double s_red[BLK / 64][2];
+    node_phase_end<BLK>(jobs[blockIdx.x], s_red);
 }
 
 // setCurrentStep's apply_boundary_condition_vec (exDyna3D.cpp:1911, 379-490)
@@ -736,10 +826,10 @@ __global__ void k_apply_bc(JobDev* __restrict__ jobs, uint32_t job, const uint32
     if (i < n) {
         const uint32_t nd = nodes[i];
         const uint8_t mk = J.bcmask[nd];
-        double* rec = J.node + (size_t)nd * NODE_REC;
+        double* rec = J.vu + (size_t)nd * VU_REC;
         for (int d = 0; d < 3; ++d) {
-            if (mk & (BC_VEL0 << d)) rec[3 + d] = J.bcval[(size_t)nd * 3 + d] * amp_value(J, J.bcamp[(size_t)nd * 3 + d], time);
-            if (mk & (BC_SPC0 << d)) rec[3 + d] = 0.0;
+            if (mk & (BC_VEL0 << d)) rec[d] = J.bcval[(size_t)nd * 3 + d] * amp_value(J, J.bcamp[(size_t)nd * 3 + d], time);
+            if (mk & (BC_SPC0 << d)) rec[d] = 0.0;
         }
     }
     if (set_end && i == 0) J.ctl->step_end = time + period;

@@ -149,8 +149,13 @@ namespace {
 int64_t footprint(const ModelCopy& c, int precision) {
     const int64_t es = (int64_t)round_up(c.Ne, 256), nn = c.Nn;
     const int64_t r = precision == ENSC_MIXED ? 4 : 8;
-    const int64_t elem = 32 + 16 + 24 * r + r + 6 * r + 12 * r + r + 24 * r;
-    const int64_t node = NODE_REC * 8 + 24 + 1 + 24 + 6 + 8 * 4 + (precision == ENSC_MIXED ? 16 : 0) + 16;
+    // conn, lconn, Gi, V0, sigma, QIA, plastic strain, corner forces
+    const int64_t elem = 32 + 16 + 9 * r + r + 6 * r + 12 * r + r + 24 * r;
+    // record, reactions, BC mask/values/amplitudes, 8-wide gather table,
+    // staging lists (< 2 positions per node); mixed: u compensation and the
+    // block-assembly partials and position lists
+    const int64_t node = NODE_BYTES + 24 + 1 + 24 + 6 + 8 * 4 + 8 +
+                         (precision == ENSC_MIXED ? 16 + 4 + 2 * (4 + 12) : 0) + 16;
     return es * elem + nn * node + 64 * 1024;
 }
 

@@ -210,6 +210,14 @@ def config4(n_jobs: int = 256, seed: int = 4, k_lo: int = 2000, k_hi: int = 8000
     return [random_job(rng, ne_lo, ne_hi, 0.0, k_lo=k_lo, k_hi=k_hi, name=f"c4j{i}") for i in range(n_jobs)]
 
 
+def config5(n_jobs: int = 1000, seed: int = 5, k_lo: int = 2000, k_hi: int = 8000,
+            ne_lo: float = 1e3, ne_hi: float = 5e5) -> List[Model]:
+    """Serving stream requests: the config-4 generator with sizes up to
+    5e5 elements, elastic or J2 plastic with probability 1/2 each."""
+    rng = np.random.default_rng(seed)
+    return [random_job(rng, ne_lo, ne_hi, 0.5, k_lo=k_lo, k_hi=k_hi, name=f"c5j{i}") for i in range(n_jobs)]
+
+
 # --------------------------------------------------------------------------
 # .inp writer
 # --------------------------------------------------------------------------

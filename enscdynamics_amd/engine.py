@@ -31,7 +31,7 @@ EXPORTS = [
     "ensc_last_error", "ensc_device_count", "ensc_engine_create", "ensc_engine_destroy", "ensc_engine_init",
     "ensc_engine_set_step", "ensc_engine_advance", "ensc_engine_run", "ensc_engine_enqueue_steps",
     "ensc_engine_sync", "ensc_engine_download", "ensc_engine_scalars", "ensc_engine_history",
-    "ensc_engine_set_graph", "ensc_engine_kernel_times", "ensc_engine_phase_times", "ensc_engine_set_timing", "ensc_engine_device_bytes",
+    "ensc_engine_set_graph", "ensc_copy_bandwidth", "ensc_engine_kernel_times", "ensc_engine_phase_times", "ensc_engine_set_timing", "ensc_engine_device_bytes",
 ]
 
 
@@ -90,6 +90,8 @@ def load_library(path: str = LIB_PATH):
         f.argtypes = args
     L.ensc_engine_device_bytes.restype = C.c_int64
     L.ensc_engine_device_bytes.argtypes = [vp]
+    L.ensc_copy_bandwidth.restype = C.c_int
+    L.ensc_copy_bandwidth.argtypes = [C.c_int32, C.c_int64, C.c_int32, C.POINTER(C.c_double)]
     _lib = L
     return L
 
@@ -102,6 +104,14 @@ def _check(rc: int) -> None:
 
 def device_count() -> int:
     return int(load_library().ensc_device_count())
+
+
+def copy_bandwidth(nbytes: int = 1 << 30, reps: int = 10, device: int = 0) -> float:
+    """Read+write GB/s of a plain device copy of `nbytes` (k_copy_probe)."""
+    L = load_library()
+    g = C.c_double()
+    _check(L.ensc_copy_bandwidth(device, int(nbytes), int(reps), C.byref(g)))
+    return g.value
 
 
 class Engine:

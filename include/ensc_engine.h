@@ -81,6 +81,11 @@ typedef struct ensc_engine ensc_engine;
 const char* ensc_last_error(void);
 /* Number of visible HIP devices (0 when none). */
 int32_t ensc_device_count(void);
+/* Measurement utility (no counterpart in the reference): times `reps`
+ * launches of a plain device-to-device copy of `bytes` bytes on `device`
+ * and returns the read+write bandwidth in GB/s.  Its kernel, k_copy_probe,
+ * is the known-size reference for scaling hardware byte counters. */
+ensc_status ensc_copy_bandwidth(int32_t device, int64_t bytes, int32_t reps, double* gbs);
 
 /* Lifecycle.  create() copies the model; the caller keeps its arrays.
  * Replaces the reference's DataIn population + init_data + compute_invMassMatrix

@@ -152,6 +152,44 @@ def test_determinism_two_runs():
     assert r[0]["s"]["ie"] == r[1]["s"]["ie"]
 
 
+def test_mixed_renumbering_within_tolerance():
+    # the mixed engine adds each node's force partials per element block, so
+    # a different numbering (different blocks) changes rounding only
+    m = decks.config3(n=10, steps=300)
+    a = eng.Engine(m, eng.MIXED, reorder=False).run()
+    b = eng.Engine(m, eng.MIXED, reorder=True).run()
+    for f in ("u", "sigma"):
+        err = rel(a.get(f), b.get(f))
+        print(f"[mixed renumbering] {f}: {err:.2e}", flush=True)
+        assert err <= 1e-5, f
+    assert a.scalars()["nsteps"] == b.scalars()["nsteps"]
+
+
+def rotated_corners(m, every=2):
+    """Same mesh with every `every`-th element's corners renumbered by a
+    quarter turn about the element's third axis (still right-handed).  Blocks
+    then hold elements that share a node at the same corner index, which
+    rules out block assembly for the job."""
+    q = m.copy() if hasattr(m, "copy") else m
+    c = np.array(q.conn, copy=True)
+    rot = [1, 2, 3, 0, 5, 6, 7, 4]
+    c[::every] = c[::every][:, rot]
+    q.conn = c
+    return q
+
+
+def test_mixed_gathered_assembly_fallback_vs_fp64():
+    m = rotated_corners(decks.config3(n=8, steps=400))
+    a = eng.Engine(m, eng.FP64, reorder=True).run()
+    b = eng.Engine(m, eng.MIXED, reorder=True).run()
+    err = rel(b.get("u"), a.get("u"))
+    print(f"[mixed gathered fallback] u vs fp64: {err:.2e}", flush=True)
+    assert err <= TOL_MIXED
+    o = Oracle(m, 0).init()
+    o.run(0, [], 0)
+    assert rel(a.get("u"), o.get("u"), 1e-12) <= TOL_FP64
+
+
 def test_mixed_vs_fp64_c1():
     m = decks.config1()
     tip = m.tip_node
