@@ -26,7 +26,7 @@ using namespace ensc;
         if (e_ != hipSuccess) return fail(ENSC_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
-typedef void (*KernelFn)(const JobDev*, const uint2*);
+typedef void (*KernelFn)(const JobDev*, const uint4*);
 
 namespace {
 
@@ -84,16 +84,7 @@ namespace {
 // Plain device copy, 16 bytes per lane per iteration: the reference kernel
 // for HBM bandwidth and for scaling hardware byte counters.
 __global__ __launch_bounds__(256) void k_copy_probe(const double2* __restrict__ a, double2* __restrict__ b, size_t n) {
-    const size_t st = (size_t)gridDim.x * 256;
-    size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
-    for (; i + 3 * st < n; i += 4 * st) {
-        const double2 x0 = a[i], x1 = a[i + st], x2 = a[i + 2 * st], x3 = a[i + 3 * st];
-        b[i] = x0;
-        b[i + st] = x1;
-        b[i + 2 * st] = x2;
-        b[i + 3 * st] = x3;
-    }
-    for (; i < n; i += st) b[i] = a[i];
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) b[i] = a[i];
 }
 
 template <bool M, int W>
@@ -181,9 +172,9 @@ ensc_status grow_ctl(ensc_engine* E, uint32_t need) {
 // blocks, so blocks that share nodes (neighbours in the space-filling order)
 // meet in the same L2.  A permutation: results do not depend on it.
 constexpr uint32_t NXCD = 8;
-std::vector<uint2> xcd_order(const std::vector<uint2>& logical) {
+std::vector<uint4> xcd_order(const std::vector<uint4>& logical) {
     const size_t n = logical.size(), q = n / NXCD, r = n % NXCD;
-    std::vector<uint2> out(n);
+    std::vector<uint4> out(n);
     for (size_t h = 0; h < n; ++h) {
         const size_t x = h % NXCD, i = h / NXCD;
         out[h] = logical[x * q + std::min(x, r) + i];
@@ -204,7 +195,7 @@ ensc_status relayout(ensc_engine* E) {
         if (!idle || J->keep_in_layout) E->live.push_back(s);
     }
     // block maps: (job, block of the job) per launched block, in XCD order
-    std::vector<uint2> le, ln;
+    std::vector<uint4> le, ln;
     std::vector<JobDev> tab;
     std::vector<size_t> pe0, pn0;
     uint32_t lds_nodes = 0;
@@ -213,16 +204,17 @@ ensc_status relayout(ensc_engine* E) {
         const JobDev& D = J.dev;
         pe0.push_back(le.size());
         pn0.push_back(ln.size());
-        for (uint32_t b = 0; b < D.nblk_e; ++b) le.push_back(make_uint2(i, b));
-        for (uint32_t b = 0; b < D.nblk_n; ++b) ln.push_back(make_uint2(i, b));
+        for (uint32_t b = 0; b < D.nblk_e; ++b)
+            le.push_back(make_uint4(i, b, J.blk_ptr_h[b], J.blk_ptr_h[b + 1] - J.blk_ptr_h[b]));
+        for (uint32_t b = 0; b < D.nblk_n; ++b) ln.push_back(make_uint4(i, b, 0, 0));
         lds_nodes = std::max(lds_nodes, J.lds_nodes);
     }
-    const std::vector<uint2> b2e = xcd_order(le), b2n = xcd_order(ln);
+    const std::vector<uint4> b2e = xcd_order(le), b2n = xcd_order(ln);
     E->nblk_e = (uint32_t)b2e.size();
     E->nblk_n = (uint32_t)b2n.size();
     if (!E->jobs_buf.ensure(std::max<size_t>(1, E->live.size()) * sizeof(JobDev)) ||
-        !E->b2e_buf.ensure(std::max<size_t>(1, b2e.size()) * sizeof(uint2)) ||
-        !E->b2n_buf.ensure(std::max<size_t>(1, b2n.size()) * sizeof(uint2)) ||
+        !E->b2e_buf.ensure(std::max<size_t>(1, b2e.size()) * sizeof(uint4)) ||
+        !E->b2n_buf.ensure(std::max<size_t>(1, b2n.size()) * sizeof(uint4)) ||
         !E->pe_buf.ensure(std::max<size_t>(1, b2e.size()) * 5 * sizeof(double)) ||
         !E->pn_buf.ensure(std::max<size_t>(1, b2n.size()) * 2 * sizeof(double)))
         return fail(ENSC_ERR_MEM, "layout allocation failed");
@@ -244,9 +236,9 @@ ensc_status relayout(ensc_engine* E) {
     if (!tab.empty())
         HIPCHK(hipMemcpyAsync(E->jobs_buf.p, tab.data(), tab.size() * sizeof(JobDev), hipMemcpyHostToDevice, E->stream));
     if (!b2e.empty())
-        HIPCHK(hipMemcpyAsync(E->b2e_buf.p, b2e.data(), b2e.size() * sizeof(uint2), hipMemcpyHostToDevice, E->stream));
+        HIPCHK(hipMemcpyAsync(E->b2e_buf.p, b2e.data(), b2e.size() * sizeof(uint4), hipMemcpyHostToDevice, E->stream));
     if (!b2n.empty())
-        HIPCHK(hipMemcpyAsync(E->b2n_buf.p, b2n.data(), b2n.size() * sizeof(uint2), hipMemcpyHostToDevice, E->stream));
+        HIPCHK(hipMemcpyAsync(E->b2n_buf.p, b2n.data(), b2n.size() * sizeof(uint4), hipMemcpyHostToDevice, E->stream));
     HIPCHK(hipStreamSynchronize(E->stream));
     E->lds_bytes = E->staged ? (size_t)std::max<uint32_t>(lds_nodes, 1) * (any_asm ? 12 : 9) * sizeof(double) : 0;
     E->ke = mixed ? pick_e<true>(E->blk_e, E->staged, E->waves) : pick_e<false>(E->blk_e, E->staged, E->waves);
@@ -268,12 +260,12 @@ void launch_phase(ensc_engine* E, int phase) {
     switch (phase) {
         case 0:
             hipLaunchKernelGGL(E->ke, dim3(E->nblk_e), dim3(E->blk_e), E->lds_bytes, E->stream, jobs,
-                               static_cast<const uint2*>(E->b2e_buf.p));
+                               static_cast<const uint4*>(E->b2e_buf.p));
             break;
         case 1: hipLaunchKernelGGL(k_fin_element<FIN_BLK>, dim3(nj), dim3(FIN_BLK), 0, E->stream, jobs); break;
         case 2:
             hipLaunchKernelGGL(E->kn, dim3(E->nblk_n), dim3(E->blk_n), 0, E->stream, jobs,
-                               static_cast<const uint2*>(E->b2n_buf.p));
+                               static_cast<const uint4*>(E->b2n_buf.p));
             break;
         default: hipLaunchKernelGGL(k_fin_node<FIN_BLK>, dim3(nj), dim3(FIN_BLK), 0, E->stream, jobs); break;
     }
@@ -376,12 +368,9 @@ ensc_status engine_new(const ensc_options* opts, std::unique_ptr<ensc_engine>& E
     return ENSC_OK;
 }
 
-// Build one job into a free slot.  The engine must be idle.
-ensc_status add_job(ensc_engine* E, const ensc_model_desc* model, const ensc_options* opts, uint32_t* slot_out) {
-    std::unique_ptr<Job> J(new Job());
-    std::string err;
-    if (!copy_model(model, J->model, err)) return fail(ENSC_ERR_ARG, err);
-    ensc_status st = build_job(*J, E->precision, E->reorder, E->blk_e, E->blk_n);
+// Upload a prepared job into a free slot.  The engine must be idle.
+ensc_status add_prepared(ensc_engine* E, std::unique_ptr<Job> J, const ensc_options* opts, uint32_t* slot_out) {
+    ensc_status st = upload_job(*J);
     if (st) return st;
     if (opts && opts->n_probe) {
         if (opts->n_probe > MAX_PROBE) return fail(ENSC_ERR_ARG, "at most 8 probe nodes");
@@ -419,6 +408,16 @@ ensc_status add_job(ensc_engine* E, const ensc_model_desc* model, const ensc_opt
     if ((st = write_ctl(E))) return st;
     *slot_out = slot;
     return ENSC_OK;
+}
+
+// Build one job into a free slot.  The engine must be idle.
+ensc_status add_job(ensc_engine* E, const ensc_model_desc* model, const ensc_options* opts, uint32_t* slot_out) {
+    std::unique_ptr<Job> J(new Job());
+    std::string err;
+    if (!copy_model(model, J->model, err)) return fail(ENSC_ERR_ARG, err);
+    ensc_status st = prepare_job(*J, E->precision, E->reorder, E->blk_e, E->blk_n);
+    if (st) return st;
+    return add_prepared(E, std::move(J), opts, slot_out);
 }
 
 // setCurrentStep for one job: host inheritance, BC arrays, device BC
@@ -650,6 +649,21 @@ ensc_status engine_add_job(ensc_engine* E, const ensc_model_desc* m, uint32_t* s
     ensc_status st = sync(E);
     if (st) return st;
     return add_job(E, m, nullptr, slot);
+}
+// Host preparation of a job for this engine's settings; touches no engine
+// state, so a worker thread may run it while the engine steps.
+ensc_status engine_prepare_job(const ensc_engine* E, ModelCopy&& model, std::unique_ptr<Job>& out) {
+    std::unique_ptr<Job> J(new Job());
+    J->model = std::move(model);
+    ensc_status st = prepare_job(*J, E->precision, E->reorder, E->blk_e, E->blk_n);
+    if (st) return st;
+    out = std::move(J);
+    return ENSC_OK;
+}
+ensc_status engine_add_prepared(ensc_engine* E, std::unique_ptr<Job> J, uint32_t* slot) {
+    ensc_status st = sync(E);
+    if (st) return st;
+    return add_prepared(E, std::move(J), nullptr, slot);
 }
 // Stream-ordered start of a newly added job: its init pass runs in the next
 // tick together with every other live job; its step 1 is switched in after.

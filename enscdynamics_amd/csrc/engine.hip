@@ -287,9 +287,10 @@ DeviceArena::~DeviceArena() {
     for (void* p : ptrs) (void)hipFree(p);
 }
 
-// Builds one job's device state.  The job's ctl/partials pointers are filled
-// in by the engine (they live in engine-wide arrays).
-ensc_status build_job(Job& J, int precision, int reorder, int blk_e, int blk_n) {
+// Host half of building a job: setup arithmetic, numbering and every table
+// the device needs, into J.img.  Touches nothing but J (no HIP calls), so
+// it may run on any thread.
+ensc_status prepare_job(Job& J, int precision, int reorder, int blk_e, int blk_n) {
     const ModelCopy& M = J.model;
     const size_t Ne = M.Ne, Nn = M.Nn;
     const bool mixed = precision == ENSC_MIXED;
@@ -411,23 +412,65 @@ ensc_status build_job(Job& J, int precision, int reorder, int blk_e, int blk_n) 
         ke0 += 0.5 * k2 / invM[n];
     }
     J.ke0 = ke0;
+    JobImage& I = *(J.img = std::unique_ptr<JobImage>(new JobImage()));
+    I.precision = precision;
+    I.blk_e = blk_e;
+    I.blk_n = blk_n;
+    I.W = W;
+    I.es = es;
+    I.ns = ns;
+    if (mixed) {
+        I.gi32.assign(9 * es, 0.f);
+        I.v032.assign(es, 0.f);
+        for (size_t i = 0; i < Ne; ++i) {
+            const size_t eo = J.e_new2old[i];
+            for (int q = 0; q < 9; ++q) I.gi32[(size_t)q * es + i] = (float)G8[9 * eo + q];
+            I.v032[i] = (float)V0[eo];
+        }
+    } else {
+        I.gi64.assign(9 * es, 0.0);
+        I.v064.assign(es, 0.0);
+        for (size_t i = 0; i < Ne; ++i) {
+            const size_t eo = J.e_new2old[i];
+            for (int q = 0; q < 9; ++q) I.gi64[(size_t)q * es + i] = G8[9 * eo + q];
+            I.v064[i] = V0[eo];
+        }
+    }
+    I.conn_d = std::move(conn_d);
+    I.ell = std::move(ell);
+    I.lconn = std::move(lconn);
+    I.blk_ptr = std::move(blk_ptr);
+    I.blk_nodes = std::move(blk_nodes);
+    I.pos_ptr = std::move(pos_ptr);
+    I.pos_list = std::move(pos_list);
+    I.pos = std::move(pos);
+    I.vu = std::move(vu);
+    J.blk_ptr_h = I.blk_ptr;
+    return ENSC_OK;
+}
+
+// Device half: allocations and uploads from J.img, then the image is freed.
+// The job's ctl/partials pointers are filled in by the engine (they live in
+// engine-wide arrays).
+ensc_status upload_job(Job& J) {
+    if (!J.img) return fail(ENSC_ERR_STATE, "job has no prepared image");
+    const JobImage& I = *J.img;
+    const ModelCopy& M = J.model;
+    const size_t Ne = M.Ne, Nn = M.Nn, es = I.es, ns = I.ns;
+    const bool mixed = I.precision == ENSC_MIXED;
+    const int blk_e = I.blk_e, blk_n = I.blk_n;
+    const uint32_t W = I.W;
     DeviceArena& A = J.arena;
     JobDev& D = J.dev;
     std::memset(&D, 0, sizeof(D));
     ensc_status st;
     uint32_t* d_conn;
-    if ((st = upload(&d_conn, conn_d, conn_d.size(), A))) return st;
+    if ((st = upload(&d_conn, I.conn_d, I.conn_d.size(), A))) return st;
     D.conn = d_conn;
     if (mixed) {
-        std::vector<float> b(9 * es, 0.f), v(es, 0.f);
-        for (size_t i = 0; i < Ne; ++i) {
-            const size_t eo = J.e_new2old[i];
-            for (int q = 0; q < 9; ++q) b[(size_t)q * es + i] = (float)G8[9 * eo + q];
-            v[i] = (float)V0[eo];
-        }
         float *db, *dv, *ds, *dq, *dp, *df, *dx;
-        if ((st = upload(&db, b, b.size(), A))) return st;
-        if ((st = upload(&dv, v, v.size(), A))) return st;
+        if ((st = upload(&db, I.gi32, I.gi32.size(), A))) return st;
+        if ((st = upload(&dv, I.v032, I.v032.size(), A))) return st;
         std::vector<float> none;
         if ((st = upload(&ds, none, 6 * es, A))) return st;
         if ((st = upload(&dq, none, 12 * es, A))) return st;
@@ -443,22 +486,16 @@ ensc_status build_job(Job& J, int precision, int reorder, int blk_e, int blk_n) 
         D.ucomp = dx;
         float* dpf;
         uint32_t *dpp, *dpl;
-        if ((st = upload(&dpf, none, blk_nodes.size() * 3, A))) return st;
-        if ((st = upload(&dpp, pos_ptr, pos_ptr.size(), A))) return st;
-        if ((st = upload(&dpl, pos_list, pos_list.size(), A))) return st;
+        if ((st = upload(&dpf, none, I.blk_nodes.size() * 3, A))) return st;
+        if ((st = upload(&dpp, I.pos_ptr, I.pos_ptr.size(), A))) return st;
+        if ((st = upload(&dpl, I.pos_list, I.pos_list.size(), A))) return st;
         D.part_f = dpf;
         D.pos_ptr = dpp;
         D.pos_list = dpl;
     } else {
-        std::vector<double> b(9 * es, 0.0), v(es, 0.0);
-        for (size_t i = 0; i < Ne; ++i) {
-            const size_t eo = J.e_new2old[i];
-            for (int q = 0; q < 9; ++q) b[(size_t)q * es + i] = G8[9 * eo + q];
-            v[i] = V0[eo];
-        }
         double *db, *dv, *ds, *dq, *dp, *df;
-        if ((st = upload(&db, b, b.size(), A))) return st;
-        if ((st = upload(&dv, v, v.size(), A))) return st;
+        if ((st = upload(&db, I.gi64, I.gi64.size(), A))) return st;
+        if ((st = upload(&dv, I.v064, I.v064.size(), A))) return st;
         std::vector<double> none;
         if ((st = upload(&ds, none, 6 * es, A))) return st;
         if ((st = upload(&dq, none, 12 * es, A))) return st;
@@ -473,8 +510,8 @@ ensc_status build_job(Job& J, int precision, int reorder, int blk_e, int blk_n) 
         D.ucomp = nullptr;
     }
     double *d_pos, *d_vu;
-    if ((st = upload(&d_pos, pos, pos.size(), A))) return st;
-    if ((st = upload(&d_vu, vu, vu.size(), A))) return st;
+    if ((st = upload(&d_pos, I.pos, I.pos.size(), A))) return st;
+    if ((st = upload(&d_vu, I.vu, I.vu.size(), A))) return st;
     D.pos = d_pos;
     D.vu = d_vu;
     std::vector<double> none_d;
@@ -494,13 +531,13 @@ ensc_status build_job(Job& J, int precision, int reorder, int blk_e, int blk_n) 
     D.bcamp = d_bcamp;
     J.d_bcamp = d_bcamp;
     uint32_t* d_ell;
-    if ((st = upload(&d_ell, ell, ell.size(), A))) return st;
+    if ((st = upload(&d_ell, I.ell, I.ell.size(), A))) return st;
     D.ell = d_ell;
     uint16_t* d_lconn;
     uint32_t *d_bptr, *d_bnodes;
-    if ((st = upload(&d_lconn, lconn, lconn.size(), A))) return st;
-    if ((st = upload(&d_bptr, blk_ptr, blk_ptr.size(), A))) return st;
-    if ((st = upload(&d_bnodes, blk_nodes, blk_nodes.size(), A))) return st;
+    if ((st = upload(&d_lconn, I.lconn, I.lconn.size(), A))) return st;
+    if ((st = upload(&d_bptr, I.blk_ptr, I.blk_ptr.size(), A))) return st;
+    if ((st = upload(&d_bnodes, I.blk_nodes, I.blk_nodes.size(), A))) return st;
     D.lconn = d_lconn;
     D.blk_ptr = d_bptr;
     D.blk_nodes = d_bnodes;
@@ -545,7 +582,13 @@ ensc_status build_job(Job& J, int precision, int reorder, int blk_e, int blk_n) 
         J.grav[s].g = M.steps[s].grav_value;
         for (int d = 0; d < 3; ++d) J.grav[s].dir[d] = M.steps[s].grav_dir[d];
     }
+    J.img.reset();
     return ENSC_OK;
+}
+
+ensc_status build_job(Job& J, int precision, int reorder, int blk_e, int blk_n) {
+    ensc_status st = prepare_job(J, precision, reorder, blk_e, blk_n);
+    return st ? st : upload_job(J);
 }
 
 // setCurrentStep's BC inheritance (exDyna3D.cpp:1760-1908), on the job's

@@ -58,6 +58,17 @@ struct DeviceArena {
     ~DeviceArena();
 };
 
+// Host-side tables of a job, ready to upload (prepare_job -> upload_job).
+struct JobImage {
+    int precision = 0, blk_e = 256, blk_n = 256;
+    uint32_t W = 0;
+    size_t es = 0, ns = 0;
+    std::vector<uint32_t> conn_d, ell, blk_ptr, blk_nodes, pos_ptr, pos_list;
+    std::vector<uint16_t> lconn;
+    std::vector<double> gi64, v064, pos, vu;
+    std::vector<float> gi32, v032;
+};
+
 struct Job {
     ModelCopy model;
     JobDev dev{};
@@ -67,6 +78,8 @@ struct Job {
     uint32_t ell_w = 0;
     uint32_t lds_nodes = 0;  // largest distinct-node count of an element block
     bool asm_capable = false; // mixed engine, every element block free of corner conflicts
+    std::vector<uint32_t> blk_ptr_h;  // host copy of the staging list offsets (block maps)
+    std::unique_ptr<JobImage> img;    // between prepare_job and upload_job
     double ke0 = 0.0;
     // setCurrentStep state (exDyna3D.h:215-220)
     uint32_t cur_step = 0;
@@ -88,6 +101,8 @@ struct Job {
 };
 
 bool copy_model(const ensc_model_desc* m, ModelCopy& c, std::string& err);
+ensc_status prepare_job(Job& J, int precision, int reorder, int blk_e, int blk_n);
+ensc_status upload_job(Job& J);
 ensc_status build_job(Job& J, int precision, int reorder, int blk_e, int blk_n);
 bool job_set_step(Job& J, uint32_t si);
 void job_bc_arrays(const Job& J, std::vector<uint8_t>& mask, std::vector<double>& val, std::vector<int16_t>& amp,

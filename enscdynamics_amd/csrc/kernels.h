@@ -70,6 +70,55 @@ __device__ __forceinline__ void bgrad(const R Gi[3][3], int k, R Bk[3]) {
     }
 }
 
+// Walsh sign patterns of the 8 hex corners.  Pattern index s has bit 0 for
+// xi, bit 1 for eta, bit 2 for zeta; its value at corner k is the product of
+// the corner's signs CS_[k] over the axes in s.
+enum : int { W_0 = 0, W_X = 1, W_Y = 2, W_XY = 3, W_Z = 4, W_XZ = 5, W_YZ = 6, W_XYZ = 7 };
+// corner k -> binary index (xi bit + 2 eta bit + 4 zeta bit, bit set = +1)
+__device__ constexpr int CORNER_BIN_[8] = {0, 1, 3, 2, 4, 5, 7, 6};
+
+// w[s] = sum_k pattern_s(k) x[k]: a 3-stage butterfly (24 additions).
+template <class R>
+__device__ __forceinline__ void walsh8(const R x[8], R w[8]) {
+    R a[8], b[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) a[CORNER_BIN_[k]] = x[k];
+#pragma unroll
+    for (int h = 1; h < 8; h <<= 1) {
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            if (i & h) continue;
+            b[i] = a[i | h] + a[i];      // axis not in the pattern
+            b[i | h] = a[i | h] - a[i];  // axis in the pattern: (+1 side) - (-1 side)
+        }
+#pragma unroll
+        for (int i = 0; i < 8; ++i) a[i] = b[i];
+    }
+#pragma unroll
+    for (int i = 0; i < 8; ++i) w[i] = a[i];
+}
+
+// f[k] = sum_s c[s] pattern_s(k): the expansion back to the corners.
+template <class R>
+__device__ __forceinline__ void walsh8_expand(const R c[8], R f[8]) {
+    R a[8], b[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) a[i] = c[i];
+#pragma unroll
+    for (int h = 1; h < 8; h <<= 1) {
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            if (i & h) continue;
+            b[i] = a[i] - a[i | h];      // -1 side of the axis
+            b[i | h] = a[i] + a[i | h];  // +1 side
+        }
+#pragma unroll
+        for (int i = 0; i < 8; ++i) a[i] = b[i];
+    }
+#pragma unroll
+    for (int k = 0; k < 8; ++k) f[k] = a[CORNER_BIN_[k]];
+}
+
 // An element's stored state, loaded at the top of the element pass (before
 // the block's node staging, so the two sets of loads overlap).
 template <class R>
@@ -264,6 +313,13 @@ __device__ __forceinline__ T wave_min(T v) {
 struct V3 {
     double x, y, z;
 };
+__device__ __noinline__ double bc_override_c(const JobDev& J, uint32_t g, double t, int d, double v) {
+#pragma clang fp contract(off)
+    const uint8_t mk = gld(J.bcmask, g);
+    if (mk & (BC_VEL0 << d)) v = gld(J.bcval, g * 3 + d) * amp_value(J, gld(J.bcamp, g * 3 + d), t);
+    if (mk & (BC_SPC0 << d)) v = 0.0;
+    return v;
+}
 __device__ __noinline__ V3 bc_override(const JobDev& J, uint32_t g, double t, V3 in) {
 #pragma clang fp contract(off)
     const uint8_t mk = gld(J.bcmask, g);
@@ -298,6 +354,8 @@ __device__ __noinline__ V3 bc_override(const JobDev& J, uint32_t g, double t, V3
         v[1] = o_.y;                                                       \
         v[2] = o_.z;                                                       \
     }
+#define CORNER_V(k, i) \
+    ((J.dyn_bc && kind == KIND_STEP) ? bc_override_c(J, ES.nd[k], time_n, (i), NVAL(k, 3 + (i))) : NVAL(k, 3 + (i)))
 template <class R>
 __device__ __forceinline__ void element_fp64_G(ELEM_ARGS) {
 #pragma clang fp contract(off)
@@ -308,16 +366,18 @@ template <class R>
 __device__ __forceinline__ void element_mixed_G(ELEM_ARGS) {
 #pragma clang fp contract(fast)
     constexpr bool MIXED = true, ASM_CAP = false;
-#include "elem_body.inc"
+#include "elem_body_mixed.inc"
 }
 #undef NVAL
 #undef DYN_BC_OVERRIDE
+#undef CORNER_V
 
 // Staged record in LDS: [x0 x1 x2 v0 v1 v2 u0 u1 u2] as copied from the node
 // record (x0..2 = the record's X).
 #define NVAL(k, i) \
     ((MIXED && (i) < 3) ? lds[ES.nd[k] * 9 + (i)] + lds[ES.nd[k] * 9 + 6 + (i)] : lds[ES.nd[k] * 9 + (i)])
 #define DYN_BC_OVERRIDE(k, v)  // applied once per node while staging
+#define CORNER_V(k, i) NVAL(k, 3 + (i))
 template <class R>
 __device__ __forceinline__ void element_fp64_S(ELEM_ARGS) {
 #pragma clang fp contract(off)
@@ -328,10 +388,11 @@ template <class R>
 __device__ __forceinline__ void element_mixed_S(ELEM_ARGS) {
 #pragma clang fp contract(fast)
     constexpr bool MIXED = true, ASM_CAP = true;
-#include "elem_body.inc"
+#include "elem_body_mixed.inc"
 }
 #undef NVAL
 #undef DYN_BC_OVERRIDE
+#undef CORNER_V
 #undef ELEM_ARGS
 
 // Fixed-order reduction of a job's block partials (value q of block b at
@@ -384,32 +445,29 @@ __device__ inline void reduce_partials(const double* part, uint32_t nblk, const 
 }
 
 template <bool MIXED, int BLK, bool STAGED, int WPE>
-__global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(WPE))) void k_element(const JobDev* __restrict__ jobs, const uint2* __restrict__ blkmap) {
+__global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(WPE))) void k_element(const JobDev* __restrict__ jobs, const uint4* __restrict__ blkmap) {
 #pragma clang fp contract(off)
     extern __shared__ double s_nodes[];  // STAGED: [block's nodes][x0 x1 x2 v0 v1 v2 u0 u1 u2]
-    const uint2 bm = blkmap[blockIdx.x];  // (job, block of the job)
+    // (job, block of the job, first position of the block's node list, its length)
+    const uint4 bm = blkmap[blockIdx.x];
     const JobDev& J = jobs[bm.x];
     Ctl* ctl = J.ctl;
-    const Decision dc = decide(*ctl, J.factor, J.wos);
-    if (dc.kind == KIND_NONE) return;
-    const uint32_t lb = bm.y;
+    const uint32_t lb = bm.y, n0 = bm.z, nbn = bm.w;
     const uint32_t e = lb * BLK + threadIdx.x;
     const bool act = e < J.ne;
     typedef typename std::conditional<MIXED, float, double>::type R;
+    const Decision dc = decide(*ctl, J.factor, J.wos);
+    if (dc.kind == KIND_NONE) return;
     ElemState<R> ES;
     load_state<R, STAGED>(J, act ? e : J.ne - 1, dc.kind != KIND_PREPASS, ES);
     ENSC_LDS double* lds = (ENSC_LDS double*)s_nodes;
     // block assembly: staged mixed jobs whose blocks are conflict-free, on
     // ticks that compute forces (uniform over the block)
     const bool asmode = MIXED && STAGED && J.block_asm && dc.kind != KIND_PREPASS;
-    uint32_t n0 = 0, nbn = 0;
-    ENSC_LDS double* acc = lds;
+    ENSC_LDS double* acc = lds + nbn * 9;
     if (STAGED) {
         // stage the block's node records once (X, v, u); velocity BCs with an
         // amplitude are applied here, once per node
-        n0 = gld(J.blk_ptr, lb);
-        nbn = gld(J.blk_ptr, lb + 1) - n0;
-        acc = lds + nbn * 9;
         for (uint32_t i = threadIdx.x; i < nbn; i += BLK) {
             const uint32_t g = gld(J.blk_nodes, n0 + i);
             const D2 p0 = gld2(J.pos, g * POS_REC), p1 = gld2(J.pos, g * POS_REC + 2);
@@ -576,9 +634,9 @@ __global__ __launch_bounds__(BLK) void k_fin_element(const JobDev* __restrict__ 
 }
 
 template <bool MIXED, int BLK>
-__global__ __launch_bounds__(BLK) void k_node(const JobDev* __restrict__ jobs, const uint2* __restrict__ blkmap) {
+__global__ __launch_bounds__(BLK) void k_node(const JobDev* __restrict__ jobs, const uint4* __restrict__ blkmap) {
 #pragma clang fp contract(off)
-    const uint2 bm = blkmap[blockIdx.x];  // (job, block of the job)
+    const uint4 bm = blkmap[blockIdx.x];  // (job, block of the job, -, -)
     const JobDev& J = jobs[bm.x];
     Ctl* ctl = J.ctl;
     if (ctl->kind != KIND_STEP) return;

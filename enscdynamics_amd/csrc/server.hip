@@ -1,28 +1,35 @@
 // Continuous-batching solve server (include/ensc_engine.h, ensc_server_*).
 //
-// Requests queue on the host.  Each pump admits queued jobs in arrival order
-// while their device footprint fits the budget, starts them (their init
-// pass runs inside the next step iteration of the live batch, then step 1 is
-// switched in), advances every live job together for a number of step
-// iterations, and retires the jobs that reached their end time: their
-// results are copied to the host and their device memory is released.  A
-// job's arithmetic never involves another job (per-job reductions in fixed
-// order), so its result is the same alone or in any mix.
+// Requests queue on the host.  Worker threads prepare queued jobs (setup
+// arithmetic, renumbering, gather and staging tables: prepare_job) while the
+// GPU steps the live ones.  Each pump admits prepared jobs in arrival order
+// while their device footprint fits the budget (upload_job), starts them
+// (their init pass runs inside the next step iteration of the live batch,
+// then step 1 is switched in), advances every live job together for a
+// number of step iterations, and retires the jobs that reached their end
+// time: their results are copied to the host and their device memory is
+// released.  A job's arithmetic never involves another job (per-job
+// reductions in fixed order), so its result is the same alone or in any mix.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
+#include <condition_variable>
 #include <cstring>
 #include <deque>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "engine_host.h"
 
 namespace ensc {
 ensc_status engine_new_public(const ensc_options* o, ensc_engine** out);
-ensc_status engine_add_job(ensc_engine* E, const ensc_model_desc* m, uint32_t* slot);
+ensc_status engine_prepare_job(const ensc_engine* E, ModelCopy&& model, std::unique_ptr<Job>& out);
+ensc_status engine_add_prepared(ensc_engine* E, std::unique_ptr<Job> J, uint32_t* slot);
 ensc_status engine_start_jobs(ensc_engine* E, const std::vector<uint32_t>& slots);
 ensc_status engine_ticks(ensc_engine* E, int64_t n);
 ensc_status engine_poll(ensc_engine* E, std::vector<uint32_t>& finished);
@@ -37,84 +44,18 @@ using namespace ensc;
 
 namespace {
 
-// A deep copy of a submitted model, with its own ensc_model_desc view.
-struct OwnedModel {
-    ModelCopy c;
-    std::vector<std::vector<ensc_bc_entry>> spc, vel;
-    std::vector<ensc_step_def> steps;
-    std::vector<ensc_amplitude> amps;
-    ensc_model_desc d{};
-    void bind() {
-        spc.assign(c.steps.size(), {});
-        vel.assign(c.steps.size(), {});
-        steps.assign(c.steps.size(), ensc_step_def{});
-        for (size_t s = 0; s < c.steps.size(); ++s) {
-            const StepCopy& S = c.steps[s];
-            auto mk = [](const BcCopy& b) {
-                ensc_bc_entry e{};
-                e.nodes = b.nodes.data();
-                e.n_nodes = (uint32_t)b.nodes.size();
-                e.d0 = b.d0;
-                e.d1 = b.d1;
-                e.cond = b.cond;
-                e.value = b.value;
-                e.amp = b.amp;
-                return e;
-            };
-            for (const auto& b : S.spc) spc[s].push_back(mk(b));
-            for (const auto& b : S.vel) vel[s].push_back(mk(b));
-            ensc_step_def& t = steps[s];
-            t.period = S.period;
-            t.reset_spc = S.reset_spc;
-            t.reset_vel = S.reset_vel;
-            t.reset_dload = S.reset_dload;
-            t.grav_on = S.grav_on;
-            t.grav_value = S.grav_value;
-            t.grav_amp = S.grav_amp;
-            for (int q = 0; q < 3; ++q) t.grav_dir[q] = S.grav_dir[q];
-            t.n_spc = (uint32_t)spc[s].size();
-            t.spc = spc[s].data();
-            t.n_vel = (uint32_t)vel[s].size();
-            t.vel = vel[s].data();
-        }
-        const size_t na = c.amp_off.empty() ? 0 : c.amp_off.size() - 1;
-        amps.assign(na, ensc_amplitude{});
-        for (size_t i = 0; i < na; ++i) {
-            amps[i].t = c.amp_t.data() + c.amp_off[i];
-            amps[i].a = c.amp_a.data() + c.amp_off[i];
-            amps[i].n = (uint32_t)(c.amp_off[i + 1] - c.amp_off[i]);
-        }
-        d = ensc_model_desc{};
-        d.n_nodes = c.Nn;
-        d.n_elems = c.Ne;
-        d.x0 = c.x0.data();
-        d.conn = c.conn.data();
-        d.E = c.E;
-        d.nu = c.nu;
-        d.rho = c.rho;
-        d.plastic = c.plastic;
-        d.yield0 = c.yield0;
-        d.hard = c.hard;
-        d.c_dr = c.c_dr;
-        d.c_cr = c.c_cr;
-        d.factor = c.factor;
-        d.cvisl = c.cvisl;
-        d.cvisq = c.cvisq;
-        d.total_time = c.total_time;
-        d.v_init = c.v_init.data();
-        d.n_steps = (uint32_t)steps.size();
-        d.steps = steps.data();
-        d.n_amps = (uint32_t)amps.size();
-        d.amps = amps.empty() ? nullptr : amps.data();
-    }
-};
+enum PrepState : int { PREP_QUEUED = 0, PREP_RUNNING = 1, PREP_READY = 2, PREP_FAILED = 3 };
 
 struct Request {
     uint64_t ticket = 0;
-    std::unique_ptr<OwnedModel> model;
+    ModelCopy model;           // moved into the job by the worker
+    std::unique_ptr<Job> job;  // prepared host image
     int64_t footprint = 0;
     int32_t keep = 0;
     double t_submit = 0.0;
+    uint32_t n_nodes = 0;
+    int state = PREP_QUEUED;  // guarded by ensc_server::mu
+    std::string err;
 };
 
 struct Record {
@@ -124,6 +65,8 @@ struct Record {
     uint32_t n_nodes = 0;
     int32_t keep = 0;
     bool finished = false;
+    bool failed = false;
+    std::string err;
 };
 
 }  // namespace
@@ -135,11 +78,40 @@ struct ensc_server {
     int32_t max_live = 0;
     std::chrono::steady_clock::time_point t0;
     uint64_t next_ticket = 1;
-    std::deque<Request> queue;
+    std::deque<std::shared_ptr<Request>> queue;  // admission order
     std::map<uint32_t, std::pair<uint64_t, int64_t>> live;  // slot -> (ticket, bytes)
     std::map<uint64_t, Record> rec;
+    // preparation workers
+    std::mutex mu;
+    std::condition_variable cv;
+    std::deque<std::shared_ptr<Request>> todo;  // preparation order (= arrival order)
+    std::vector<std::thread> workers;
+    bool stop = false;
     double now() const {
         return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    }
+    void work() {
+        for (;;) {
+            std::shared_ptr<Request> r;
+            {
+                std::unique_lock<std::mutex> lk(mu);
+                cv.wait(lk, [&] { return stop || !todo.empty(); });
+                if (stop) return;
+                r = todo.front();
+                todo.pop_front();
+                r->state = PREP_RUNNING;
+            }
+            std::unique_ptr<Job> J;
+            const ensc_status st = engine_prepare_job(eng, std::move(r->model), J);
+            const std::string err = st ? std::string(ensc_last_error()) : std::string();
+            {
+                std::lock_guard<std::mutex> lk(mu);
+                r->job = std::move(J);
+                r->err = err;
+                r->state = st ? PREP_FAILED : PREP_READY;
+            }
+            cv.notify_all();
+        }
     }
 };
 
@@ -174,12 +146,22 @@ ensc_status ensc_server_create(const ensc_options* opts, int64_t hbm_budget_byte
     S->budget = hbm_budget_bytes > 0 ? hbm_budget_bytes : (int64_t)64 << 30;
     S->max_live = max_live_jobs > 0 ? max_live_jobs : 1024;
     S->t0 = std::chrono::steady_clock::now();
+    const unsigned hc = std::max(1u, std::thread::hardware_concurrency());
+    const unsigned nw = std::min(4u, hc);
+    ensc_server* raw = S.get();
+    for (unsigned i = 0; i < nw; ++i) S->workers.emplace_back([raw] { raw->work(); });
     *out = S.release();
     return ENSC_OK;
 }
 
 ensc_status ensc_server_destroy(ensc_server* S) {
     if (!S) return ENSC_OK;
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        S->stop = true;
+    }
+    S->cv.notify_all();
+    for (auto& t : S->workers) t.join();
     ensc_engine_destroy(S->eng);
     delete S;
     return ENSC_OK;
@@ -193,21 +175,26 @@ ensc_status ensc_server_set_graph(ensc_server* S, int32_t ticks) {
 ensc_status ensc_server_submit(ensc_server* S, const ensc_model_desc* model, int32_t keep_result, double t_arrival,
                                uint64_t* ticket) {
     if (!S || !ticket) return fail(ENSC_ERR_ARG, "null argument");
-    Request r;
-    r.model.reset(new OwnedModel());
+    std::shared_ptr<Request> r(new Request());
     std::string err;
-    if (!copy_model(model, r.model->c, err)) return fail(ENSC_ERR_ARG, err);
-    r.model->bind();
-    r.footprint = footprint(r.model->c, S->precision);
-    if (r.footprint > S->budget) return fail(ENSC_ERR_MEM, "job larger than the server's device budget");
-    r.ticket = S->next_ticket++;
-    r.keep = keep_result;
-    r.t_submit = t_arrival >= 0.0 ? t_arrival : S->now();
-    Record& rc = S->rec[r.ticket];
-    rc.t_submit = r.t_submit;
+    if (!copy_model(model, r->model, err)) return fail(ENSC_ERR_ARG, err);
+    r->footprint = footprint(r->model, S->precision);
+    if (r->footprint > S->budget) return fail(ENSC_ERR_MEM, "job larger than the server's device budget");
+    r->ticket = S->next_ticket++;
+    r->keep = keep_result;
+    r->n_nodes = r->model.Nn;
+    r->t_submit = t_arrival >= 0.0 ? t_arrival : S->now();
+    Record& rc = S->rec[r->ticket];
+    rc.t_submit = r->t_submit;
     rc.keep = keep_result;
-    *ticket = r.ticket;
-    S->queue.push_back(std::move(r));
+    rc.n_nodes = r->n_nodes;
+    *ticket = r->ticket;
+    S->queue.push_back(r);
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        S->todo.push_back(r);
+    }
+    S->cv.notify_all();
     return ENSC_OK;
 }
 
@@ -215,19 +202,40 @@ ensc_status ensc_server_pump(ensc_server* S, int32_t ticks, uint64_t* done, uint
     if (!S || !n_done) return fail(ENSC_ERR_ARG, "null argument");
     *n_done = 0;
     ensc_status st;
-    // 1. admission by footprint, arrival order
+    auto report = [&](uint64_t t) {
+        if (done && *n_done < cap) done[*n_done] = t;
+        ++*n_done;
+    };
+    // 1. admission of prepared jobs by footprint, arrival order; with nothing
+    //    on the device, wait for the next job's preparation
     std::vector<uint32_t> started;
-    while (!S->queue.empty() && (int32_t)S->live.size() < S->max_live &&
-           S->used + S->queue.front().footprint <= S->budget) {
-        Request r = std::move(S->queue.front());
+    while (!S->queue.empty() && (int32_t)S->live.size() < S->max_live) {
+        const std::shared_ptr<Request> r = S->queue.front();
+        int state;
+        {
+            std::unique_lock<std::mutex> lk(S->mu);
+            if (r->state < PREP_READY && S->live.empty() && started.empty())
+                S->cv.wait(lk, [&] { return r->state >= PREP_READY; });
+            state = r->state;
+        }
+        if (state < PREP_READY) break;
+        if (state == PREP_FAILED) {
+            Record& rc = S->rec[r->ticket];
+            rc.failed = rc.finished = true;
+            rc.err = r->err;
+            rc.t_admit = rc.t_finish = S->now();
+            S->queue.pop_front();
+            report(r->ticket);
+            continue;
+        }
+        if (S->used + r->footprint > S->budget) break;
         S->queue.pop_front();
         uint32_t slot = 0;
-        if ((st = engine_add_job(S->eng, &r.model->d, &slot))) return st;
+        if ((st = engine_add_prepared(S->eng, std::move(r->job), &slot))) return st;
         const int64_t bytes = engine_job_bytes(S->eng, slot);
         S->used += bytes;
-        S->live[slot] = {r.ticket, bytes};
-        S->rec[r.ticket].t_admit = S->now();
-        S->rec[r.ticket].n_nodes = r.model->c.Nn;
+        S->live[slot] = {r->ticket, bytes};
+        S->rec[r->ticket].t_admit = S->now();
         started.push_back(slot);
     }
     // 2. new jobs: init pass inside the next step iteration, then step 1
@@ -252,8 +260,7 @@ ensc_status ensc_server_pump(ensc_server* S, int32_t ticks, uint64_t* done, uint
         rc.finished = true;
         S->used -= it->second.second;
         S->live.erase(it);
-        if (done && *n_done < cap) done[*n_done] = t;
-        ++*n_done;
+        report(t);
     }
     return ENSC_OK;
 }
@@ -270,6 +277,7 @@ ensc_status ensc_server_result(ensc_server* S, uint64_t ticket, double* scalars,
         times[2] = rc.t_finish;
     }
     if (!rc.finished) return fail(ENSC_ERR_STATE, "job not finished");
+    if (rc.failed) return fail(ENSC_ERR_ARG, "job could not be set up: " + rc.err);
     if (scalars) std::memcpy(scalars, rc.scalars, sizeof(rc.scalars));
     if (u) {
         if (!rc.keep) return fail(ENSC_ERR_STATE, "result fields were not kept for this ticket");

@@ -152,12 +152,17 @@ ensc_status ensc_batch_job_scalars(ensc_engine* eng, uint32_t job, double* out /
 ensc_status ensc_batch_job_download(ensc_engine* eng, uint32_t job, const char* field, double* out, int64_t count);
 
 /* ---- continuous-batching solve server ----
- * No counterpart in the reference.  Requests queue on the host; each pump
- * admits queued jobs in arrival order while their device footprint fits
- * hbm_budget_bytes (and at most max_live_jobs are live), runs their init
- * pass inside the next step iteration of the live batch, advances every
- * live job together for `ticks` step iterations, and retires jobs that
- * reached their end time (results to the host, device memory freed).
+ * No counterpart in the reference.  Requests queue on the host and worker
+ * threads prepare them (setup arithmetic, renumbering, gather and staging
+ * tables) while the GPU steps.  Each pump admits prepared jobs in arrival
+ * order while their device footprint fits hbm_budget_bytes (and at most
+ * max_live_jobs are live), runs their init pass inside the next step
+ * iteration of the live batch, advances every live job together for
+ * `ticks` step iterations, and retires jobs that reached their end time
+ * (results to the host, device memory freed).  With nothing on the device,
+ * a pump waits for the next job's preparation.  A job whose set-up fails
+ * (e.g. an inverted element) is retired at once and ensc_server_result
+ * returns ENSC_ERR_ARG with the reason.
  * Times are seconds on the server's steady clock (ensc_server_clock). */
 typedef struct ensc_server ensc_server;
 ensc_status ensc_server_create(const ensc_options* opts, int64_t hbm_budget_bytes, int32_t max_live_jobs,

@@ -87,3 +87,24 @@ def test_server_alone_equals_in_mix_and_latency():
         a = eng.Engine(m, eng.MIXED, reorder=True).run()
         assert np.array_equal(u, a.get("u")), t
         assert s["nsteps"] == a.scalars()["nsteps"]
+
+
+def test_server_reports_a_job_that_cannot_be_set_up():
+    jobs = small_jobs(3, seed=31)
+    bad = jobs[1]
+    bad.conn = np.array(bad.conn, copy=True)
+    bad.conn[0] = bad.conn[0][[4, 5, 6, 7, 0, 1, 2, 3]]  # inverted element
+    srv = Server(eng.MIXED, budget_bytes=64 << 20, max_live=4, graph_ticks=8)
+    tickets = [srv.submit(m, keep=False) for m in jobs]
+    done = []
+    guard = 0
+    while len(done) < len(jobs):
+        done += srv.pump(64)
+        guard += 1
+        assert guard < 10000
+    assert sorted(done) == sorted(tickets)
+    with pytest.raises(eng.EnscError, match="set up"):
+        srv.result(tickets[1])
+    for t in (tickets[0], tickets[2]):
+        s, _, _ = srv.result(t)
+        assert s["done"] == 1.0 and s["err"] == 0.0
