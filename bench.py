@@ -330,7 +330,7 @@ def main() -> int:
         return 0
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on one GPU only: every job of this engine fits one MI355X and the "
-              "multi-GPU layout is design-only (see DESIGN.md); run with --gpus 1", file=sys.stderr)
+              "multi-GPU layout is design-only; run with --gpus 1", file=sys.stderr)
         return 2
     if eng.device_count() < 1:
         print("no HIP device visible", file=sys.stderr)

@@ -5,7 +5,7 @@
  * (Explicit3D/include/exDyna3D.h:36-222).  The reference exposes a C++
  * class with public run()/init()/setCurrentStep() and protected virtual
  * computeSate()/compute_a() hooks; a binding that replaces its step loop
- * calls these entry points instead (see INTEGRATION.md for the C++ shim).
+ * calls these entry points instead.
  *
  * One ensc_engine holds one or more jobs (meshes) resident in device
  * memory.  All calls are host-thread-safe per engine only when serialized by

@@ -6,7 +6,8 @@
 // restatement and the reader can be checked against the real reference.
 //
 // The reference writes VTU files through VTK, which is not available here,
-// so DataOut (Explicit3D/include/dataOut.h:44-97) is given empty bodies.
+// so DataOut (Explicit3D/include/dataOut.h:44-97) is given empty bodies
+// (dataout_stub.cpp).
 // Stepping is observed by subclassing exDyna3D and overriding the virtual
 // computeSate() (Explicit3D/include/exDyna3D.h:75): the base step runs
 // unchanged, then the state is recorded.
@@ -26,46 +27,12 @@
 #include <stdexcept>
 #include <unistd.h>
 
-namespace EnSC {
-DataOut::DataOut(const Types::VerticesAll<3>& v, const std::vector<Element_HexN8>& e, Types::Real t)
-    : opVerticesAll(v), opTriaHex(e), time(t) {}
-void DataOut::add_node_data(DataOutType::DataType, const Eigen::Matrix<Types::Real, Eigen::Dynamic, 1>*, std::string) {}
-void DataOut::add_ele_data(DataOutType::DataType, const Eigen::Matrix<Types::Real, Eigen::Dynamic, 1>*, std::string) {}
-void DataOut::write_vtu(const std::string&) {}
-void DataOut::write_vtu_virtualParticles(const unsigned int&, const FSI_share_data&) const {}
-}  // namespace EnSC
+#include "npy.h"
+
+using npy::npy_f64;
+using npy::npy_u32;
 
 namespace {
-
-// Minimal .npy (v1.0) writer: little-endian, C order.
-void write_npy(const std::string& path, const char* descr, size_t elsize,
-               const std::vector<size_t>& shape, const void* data) {
-    std::string shp = "(";
-    size_t n = 1;
-    for (size_t i = 0; i < shape.size(); ++i) {
-        shp += std::to_string(shape[i]);
-        shp += (shape.size() == 1) ? "," : (i + 1 < shape.size() ? ", " : "");
-        n *= shape[i];
-    }
-    shp += ")";
-    std::string hdr = std::string("{'descr': '") + descr + "', 'fortran_order': False, 'shape': " + shp + ", }";
-    size_t total = 10 + hdr.size() + 1;
-    size_t pad = (64 - total % 64) % 64;
-    hdr += std::string(pad, ' ');
-    hdr += '\n';
-    FILE* f = std::fopen(path.c_str(), "wb");
-    if (!f) throw std::runtime_error("cannot write " + path);
-    const unsigned char magic[8] = {0x93, 'N', 'U', 'M', 'P', 'Y', 1, 0};
-    std::fwrite(magic, 1, 8, f);
-    uint16_t hl = (uint16_t)hdr.size();
-    std::fwrite(&hl, 2, 1, f);
-    std::fwrite(hdr.data(), 1, hdr.size(), f);
-    if (n) std::fwrite(data, elsize, n, f);
-    std::fclose(f);
-}
-
-void npy_f64(const std::string& p, const std::vector<size_t>& s, const double* d) { write_npy(p, "<f8", 8, s, d); }
-void npy_u32(const std::string& p, const std::vector<size_t>& s, const uint32_t* d) { write_npy(p, "<u4", 4, s, d); }
 
 std::string jstr(const std::string& s) {
     std::string o = "\"";
