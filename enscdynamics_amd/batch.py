@@ -62,7 +62,7 @@ def _options(precision: int, reorder: bool, block_elem: int, staging: bool, wave
     o.n_probe = 0
     o.probe_nodes = None
     o.hist_cap = 0
-    o.staging = 0 if staging else 2
+    o.staging = (0 if staging else 2) if isinstance(staging, bool) else int(staging)
     o.waves = waves
     return o
 

@@ -53,6 +53,7 @@ struct DevBuf {
 
 struct ensc_engine {
     int device = 0, precision = ENSC_FP64, blk_e = 256, blk_n = 256, reorder = 0, staging_opt = 0, waves = 2;
+    bool pipe_opt = true;  // staging option 0: pipelined element pass where it applies
     hipStream_t stream = nullptr;
     std::vector<std::unique_ptr<Job>> slots;  // job id = slot index; null = free
     // per-slot scalars (device array + pinned host mirror), never moved while jobs live
@@ -66,6 +67,9 @@ struct ensc_engine {
     bool dirty = true;
     KernelFn ke = nullptr, kn = nullptr;
     bool staged = false;
+    uint32_t nblk_pipe = 0;  // element blocks of the pipelined jobs (first in the map); nblk_e: the rest
+    uint32_t n_asm = 0;      // live jobs using block assembly
+    uint32_t pipe_grid = 0;  // the pipelined kernel's block count (2 per CU)
     size_t lds_bytes = 0;
     // timing
     bool timing = false;
@@ -194,36 +198,55 @@ ensc_status relayout(ensc_engine* E) {
         const bool idle = J->inited && !c.init_pending && !(c.time < c.step_end) && !J->pending_step;
         if (!idle || J->keep_in_layout) E->live.push_back(s);
     }
-    // block maps: (job, block of the job) per launched block, in XCD order
-    std::vector<uint4> le, ln;
+    const bool mixed = E->precision == ENSC_MIXED;
+    uint32_t lds_nodes = 0;
+    for (uint32_t s : E->live) lds_nodes = std::max(lds_nodes, E->slots[s]->lds_nodes);
+    // LDS staging when every live job's largest block node list fits 64 KiB;
+    // block assembly (mixed) adds a 3-float accumulator per staged node
+    E->staged = E->staging_opt != 2 && (size_t)lds_nodes * 9 * sizeof(double) <= 65536;
+    const bool asm_fits = E->staged && mixed && (size_t)lds_nodes * (9 * sizeof(double) + 12) <= 65536;
+    // The pipelined element pass takes the jobs that use block assembly,
+    // have no amplitude velocity BCs and whose tiles fit its buffers; the
+    // rest run k_element.  The choice depends on the job alone, so a job's
+    // arithmetic is the same in any mix.
+    auto pipe_job = [&](const Job& J) {
+        return E->pipe_opt && asm_fits && J.asm_capable && !J.dev.dyn_bc && E->blk_e == PIPE_BLK &&
+               J.lds_nodes <= (uint32_t)PIPE_CAP;
+    };
+    // block maps: (job, block of the job, staged range) per launched block,
+    // in XCD order; the pipelined jobs' element blocks first
+    std::vector<uint4> lp, lr, ln;
     std::vector<JobDev> tab;
     std::vector<size_t> pe0, pn0;
-    uint32_t lds_nodes = 0;
+    size_t ne_blocks = 0;
     for (uint32_t i = 0; i < E->live.size(); ++i) {
         Job& J = *E->slots[E->live[i]];
         const JobDev& D = J.dev;
-        pe0.push_back(le.size());
+        pe0.push_back(ne_blocks);
         pn0.push_back(ln.size());
+        std::vector<uint4>& le = pipe_job(J) ? lp : lr;
         for (uint32_t b = 0; b < D.nblk_e; ++b)
             le.push_back(make_uint4(i, b, J.blk_ptr_h[b], J.blk_ptr_h[b + 1] - J.blk_ptr_h[b]));
+        ne_blocks += D.nblk_e;
         for (uint32_t b = 0; b < D.nblk_n; ++b) ln.push_back(make_uint4(i, b, 0, 0));
-        lds_nodes = std::max(lds_nodes, J.lds_nodes);
     }
-    const std::vector<uint4> b2e = xcd_order(le), b2n = xcd_order(ln);
-    E->nblk_e = (uint32_t)b2e.size();
+    std::vector<uint4> b2e = xcd_order(lp);
+    {
+        const std::vector<uint4> r = xcd_order(lr);
+        b2e.insert(b2e.end(), r.begin(), r.end());
+    }
+    const std::vector<uint4> b2n = xcd_order(ln);
+    E->nblk_pipe = (uint32_t)lp.size();
+    E->nblk_e = (uint32_t)lr.size();
     E->nblk_n = (uint32_t)b2n.size();
     if (!E->jobs_buf.ensure(std::max<size_t>(1, E->live.size()) * sizeof(JobDev)) ||
         !E->b2e_buf.ensure(std::max<size_t>(1, b2e.size()) * sizeof(uint4)) ||
         !E->b2n_buf.ensure(std::max<size_t>(1, b2n.size()) * sizeof(uint4)) ||
-        !E->pe_buf.ensure(std::max<size_t>(1, b2e.size()) * 5 * sizeof(double)) ||
+        !E->pe_buf.ensure(std::max<size_t>(1, ne_blocks) * 5 * sizeof(double)) ||
         !E->pn_buf.ensure(std::max<size_t>(1, b2n.size()) * 2 * sizeof(double)))
         return fail(ENSC_ERR_MEM, "layout allocation failed");
-    const bool mixed = E->precision == ENSC_MIXED;
-    // LDS staging when every live job's largest block node list fits 64 KiB;
-    // block assembly (mixed) adds a 3-double accumulator per staged node
-    E->staged = E->staging_opt != 2 && (size_t)lds_nodes * 9 * sizeof(double) <= 65536;
     bool any_asm = false;
-    const bool asm_fits = E->staged && mixed && (size_t)lds_nodes * 12 * sizeof(double) <= 65536;
+    E->n_asm = 0;
     for (uint32_t i = 0; i < E->live.size(); ++i) {
         Job& J = *E->slots[E->live[i]];
         J.dev.ctl = E->d_ctl + E->live[i];
@@ -231,6 +254,7 @@ ensc_status relayout(ensc_engine* E) {
         J.dev.part_n = static_cast<double*>(E->pn_buf.p) + pn0[i] * 2;
         J.dev.block_asm = (asm_fits && J.asm_capable) ? 1 : 0;
         any_asm |= J.dev.block_asm != 0;
+        E->n_asm += J.dev.block_asm != 0;
         tab.push_back(J.dev);
     }
     if (!tab.empty())
@@ -240,7 +264,7 @@ ensc_status relayout(ensc_engine* E) {
     if (!b2n.empty())
         HIPCHK(hipMemcpyAsync(E->b2n_buf.p, b2n.data(), b2n.size() * sizeof(uint4), hipMemcpyHostToDevice, E->stream));
     HIPCHK(hipStreamSynchronize(E->stream));
-    E->lds_bytes = E->staged ? (size_t)std::max<uint32_t>(lds_nodes, 1) * (any_asm ? 12 : 9) * sizeof(double) : 0;
+    E->lds_bytes = E->staged ? (size_t)std::max<uint32_t>(lds_nodes, 1) * (9 * sizeof(double) + (any_asm ? 12 : 0)) : 0;
     E->ke = mixed ? pick_e<true>(E->blk_e, E->staged, E->waves) : pick_e<false>(E->blk_e, E->staged, E->waves);
     E->kn = mixed ? pick_n<true>(E->blk_n) : pick_n<false>(E->blk_n);
     E->dirty = false;
@@ -259,8 +283,12 @@ void launch_phase(ensc_engine* E, int phase) {
     if (!nj) return;
     switch (phase) {
         case 0:
-            hipLaunchKernelGGL(E->ke, dim3(E->nblk_e), dim3(E->blk_e), E->lds_bytes, E->stream, jobs,
-                               static_cast<const uint4*>(E->b2e_buf.p));
+            if (E->nblk_pipe)
+                hipLaunchKernelGGL(k_element_pipe<2>, dim3(std::min(E->nblk_pipe, E->pipe_grid)), dim3(PIPE_BLK), 0,
+                                   E->stream, jobs, static_cast<const uint4*>(E->b2e_buf.p), E->nblk_pipe);
+            if (E->nblk_e)
+                hipLaunchKernelGGL(E->ke, dim3(E->nblk_e), dim3(E->blk_e), E->lds_bytes, E->stream, jobs,
+                                   static_cast<const uint4*>(E->b2e_buf.p) + E->nblk_pipe);
             break;
         case 1: hipLaunchKernelGGL(k_fin_element<FIN_BLK>, dim3(nj), dim3(FIN_BLK), 0, E->stream, jobs); break;
         case 2:
@@ -365,6 +393,10 @@ ensc_status engine_new(const ensc_options* opts, std::unique_ptr<ensc_engine>& E
     E->waves = o.waves ? o.waves : (o.precision == ENSC_FP64 ? 1 : 2);
     HIPCHK(hipSetDevice(E->device));
     HIPCHK(hipStreamCreateWithFlags(&E->stream, hipStreamNonBlocking));
+    int ncu = 0;
+    HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, E->device));
+    E->pipe_grid = (uint32_t)std::max(8, (ncu * 2 + 7) / 8 * 8);  // two per CU, a multiple of the XCD count
+    E->pipe_opt = o.staging == 0;
     return ENSC_OK;
 }
 
@@ -931,6 +963,19 @@ ensc_status ensc_engine_phase_times(ensc_engine* E, double* ms4) {
     ensc_status st = sync(E);
     if (st) return st;
     for (int p = 0; p < 4; ++p) ms4[p] = E->phase_ms[p];
+    return ENSC_OK;
+}
+
+ensc_status ensc_engine_layout(ensc_engine* E, int64_t* out) {
+    if (!E || !out) return fail(ENSC_ERR_ARG, "null argument");
+    ensc_status st = sync(E);
+    if (st) return st;
+    out[ENSC_L_STAGED] = E->staged ? 1 : 0;
+    out[ENSC_L_PIPE_BLOCKS] = E->nblk_pipe;
+    out[ENSC_L_ELEM_BLOCKS] = E->nblk_e;
+    out[ENSC_L_NODE_BLOCKS] = E->nblk_n;
+    out[ENSC_L_LDS_BYTES] = (int64_t)E->lds_bytes;
+    out[ENSC_L_ASM_JOBS] = E->n_asm;
     return ENSC_OK;
 }
 

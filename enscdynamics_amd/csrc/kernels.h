@@ -290,6 +290,21 @@ __device__ inline Decision decide(const Ctl& c, double factor, double wos) {
     return d;
 }
 
+// Block barrier for LDS traffic only.  __syncthreads() (and a workgroup
+// fence on LDS, which also covers global->LDS copies) makes every wave wait
+// for all its outstanding global loads, stores and copies; the element
+// kernels exchange data between waves through LDS written by ordinary LDS
+// stores, so they only need those to complete: s_waitcnt lgkmcnt(0)
+// (immediate 0xC07F: vmcnt and expcnt left at their maximum), then the
+// barrier.  The signal fences keep the compiler from moving memory
+// accesses across it.
+__device__ __forceinline__ void lds_sync() {
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __builtin_amdgcn_s_waitcnt(0xC07F);
+    __builtin_amdgcn_s_barrier();
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
+
 template <class T>
 __device__ __forceinline__ T wave_sum(T v) {
 #pragma unroll
@@ -338,10 +353,12 @@ __device__ __noinline__ V3 bc_override(const JobDev& J, uint32_t g, double t, V3
 // corner k's x0 x1 x2 v0 v1 v2 u0 u1 u2, with x the current position (in
 // the mixed engine the record holds the reference position, so x = X + u).
 
+#define ACC_W 3                  // block assembly: 3 floats per staged node
+#define ASM_BEGIN ((void)0)      // the accumulators start zeroed
 #define ELEM_ARGS                                                                                          \
     const JobDev &J, uint32_t e, const ElemState<R> &ES, double dt, int32_t kind, double time_n, bool act, \
         bool asmode, double &Le, double &dts, double *en, uint32_t &errf, const ENSC_LDS double *lds,     \
-        ENSC_LDS double *acc
+        ENSC_LDS float *acc
 #define NVAL(k, i)                                                                                      \
     ((i) < 3 ? (MIXED ? gld((const double*)J.pos, ES.nd[k] * POS_REC + (i)) +                           \
                             gld((const double*)J.vu, ES.nd[k] * VU_REC + 3 + (i))                           \
@@ -394,39 +411,44 @@ __device__ __forceinline__ void element_mixed_S(ELEM_ARGS) {
 #undef DYN_BC_OVERRIDE
 #undef CORNER_V
 #undef ELEM_ARGS
+#undef ACC_W
+#undef ASM_BEGIN
 
 // Fixed-order reduction of a job's block partials (value q of block b at
-// part[q * nblk + b]): thread t folds blocks t, t+BLK, t+2*BLK, ... in
-// ascending order (loads issued four at a time), a fixed xor-shuffle tree
-// combines each wave, and thread 0 adds the waves in index order.  Same
-// inputs, same result.
-// Every thread of the block must call it; the result is valid in thread 0.
-template <int NV, int BLK>
-__device__ inline void reduce_partials(const double* part, uint32_t nblk, const bool* is_min, double* out,
-                                       double (*s_tmp)[NV]) {
+// part[q * nblk + b]).  fold_partials: thread t folds blocks t, t+BLK,
+// t+2*BLK, ... in ascending order, the node partials (values 0-1, when
+// `node`) and the element partials (values 2-6, when `elem`) in the same
+// loop, loads issued four blocks at a time.  combine_partials: a fixed
+// xor-shuffle tree per wave, then thread 0 adds the waves in index order.
+// Same inputs, same result.  Every thread of the block must call both.
+template <int BLK>
+__device__ inline void fold_partials(const JobDev& J, bool node, bool elem, const bool* is_min, double* acc) {
     const uint32_t t = threadIdx.x;
-    double acc[NV];
 #pragma unroll
-    for (int q = 0; q < NV; ++q) acc[q] = is_min[q] ? __builtin_inf() : 0.0;
-    uint32_t b = t;
-    for (; b + 3 * BLK < nblk; b += 4 * BLK) {
-        double v[4][NV];
+    for (int q = 0; q < 7; ++q) acc[q] = is_min[q] ? __builtin_inf() : 0.0;
+    const uint32_t nn = node ? J.nblk_n : 0u, ne = elem ? J.nblk_e : 0u;
+    const uint32_t nmax = nn > ne ? nn : ne;
+    for (uint32_t b0 = t; b0 < nmax; b0 += 4 * BLK) {
+        double v[4][7];
 #pragma unroll
-        for (int u = 0; u < 4; ++u)
+        for (int u = 0; u < 4; ++u) {
+            const uint32_t b = b0 + u * BLK;
 #pragma unroll
-            for (int q = 0; q < NV; ++q) v[u][q] = part[(size_t)q * nblk + b + u * BLK];
+            for (int q = 0; q < 2; ++q) v[u][q] = (b < nn) ? J.part_n[(size_t)q * nn + b] : 0.0;
 #pragma unroll
-        for (int u = 0; u < 4; ++u)
-#pragma unroll
-            for (int q = 0; q < NV; ++q) acc[q] = is_min[q] ? (v[u][q] < acc[q] ? v[u][q] : acc[q]) : acc[q] + v[u][q];
-    }
-    for (; b < nblk; b += BLK) {
-#pragma unroll
-        for (int q = 0; q < NV; ++q) {
-            const double v = part[(size_t)q * nblk + b];
-            acc[q] = is_min[q] ? (v < acc[q] ? v : acc[q]) : acc[q] + v;
+            for (int q = 2; q < 7; ++q)
+                v[u][q] = (b < ne) ? J.part_e[(size_t)(q - 2) * ne + b] : (is_min[q] ? __builtin_inf() : 0.0);
         }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int q = 0; q < 7; ++q) acc[q] = is_min[q] ? (v[u][q] < acc[q] ? v[u][q] : acc[q]) : acc[q] + v[u][q];
     }
+}
+
+template <int NV, int BLK>
+__device__ inline void combine_partials(double* acc, const bool* is_min, double (*s_tmp)[NV], double* out) {
+    const uint32_t t = threadIdx.x;
 #pragma unroll
     for (int q = 0; q < NV; ++q) acc[q] = is_min[q] ? wave_min(acc[q]) : wave_sum(acc[q]);
     const int w = t >> 6;
@@ -441,6 +463,81 @@ __device__ inline void reduce_partials(const double* part, uint32_t nblk, const 
             for (int x = 1; x < BLK / 64; ++x) r = is_min[q] ? (s_tmp[x][q] < r ? s_tmp[x][q] : r) : r + s_tmp[x][q];
             out[q] = r;
         }
+    }
+}
+
+// End of a job's nodal update (thread 0): kinetic energy, external work and
+// the job's history row for the step; marks the update closed.
+__device__ inline void node_phase_finish(const JobDev& J, double ke, double pw) {
+#pragma clang fp contract(off)
+    Ctl& c = *J.ctl;
+    c.ke = ke;
+    c.wext += pw * (0.5 * (c.dt_i + c.dt_i_1));
+    const long long row = c.nsteps - 1;
+    if (J.hist && row >= 0 && row < J.hist_cap) {
+        double* h = J.hist + row * (ENSC_H_BASE + 3 * J.n_probe);
+        h[ENSC_H_TIME] = c.time;
+        h[ENSC_H_DT_I] = c.dt_i;
+        h[ENSC_H_DT_I_1] = c.dt_i_1;
+        h[ENSC_H_DTS_MIN] = c.dts_min;
+        h[ENSC_H_KE] = c.ke;
+        h[ENSC_H_IE] = c.ie;
+        h[ENSC_H_HG] = c.hg;
+        h[ENSC_H_VIS] = c.vis;
+        h[ENSC_H_WEXT] = c.wext;
+        for (int p = 0; p < J.n_probe; ++p) {
+            const double* r = J.vu + (size_t)J.probe[p] * VU_REC + 3;
+            h[ENSC_H_BASE + 3 * p + 0] = r[0];
+            h[ENSC_H_BASE + 3 * p + 1] = r[1];
+            h[ENSC_H_BASE + 3 * p + 2] = r[2];
+        }
+    }
+    c.kind = KIND_NONE;
+}
+
+// End of a job's element pass (thread 0): tot = min edge, min dtStable,
+// stress, hourglass and viscous work of the pass.
+__device__ inline void elem_phase_finish(const JobDev& J, const Decision& dc, const double* tot) {
+#pragma clang fp contract(off)
+    Ctl& c = *J.ctl;
+    const double Lmin = tot[0], dtsn = tot[1];
+    if (dc.kind == KIND_PREPASS) {
+        c.lmin_pre = Lmin;
+        c.have_lpre = 1;
+        c.kind = KIND_PREPASS;
+    } else if (dc.kind == KIND_INIT) {
+        c.dts_min = dtsn;
+        c.init_pending = 0;
+        c.kind = KIND_INIT;
+    } else {
+        // update_minVertex_perMaterial_and_dt (663-686) + run-loop clamp (90-97)
+        double dt1 = c.dt_i;
+        const double time_prev = c.time + dt1;
+        double dt = Lmin / J.wos;
+        const double tmp = c.dts_min;
+        dt = dt < tmp ? dt : tmp;
+        dt *= J.factor;
+        if ((c.total_time - time_prev) < dt && (c.total_time - time_prev) > 0.0) {
+            dt = c.total_time - time_prev;
+            dt1 = dt;
+        }
+        if (c.time + dt1 > c.step_end) dt = dt1 = c.step_end - c.time;
+        if (dt1 != dc.dt_i_1) c.err |= (uint32_t)ENSC_EF_DT;  // cannot happen: same inputs as decide()
+        // computeSate (775-780)
+        double t = c.time + dt1;
+        if (t > c.total_time) t = c.total_time;
+        c.time = t;
+        c.dt_i = dt;
+        c.dt_i_1 = dt1;
+        c.dts_min = dtsn;
+        c.nsteps += 1;
+        c.have_lpre = 0;
+        c.ie += tot[2];
+        c.hg += tot[3];
+        c.vis += tot[4];
+        c.grav_amp = (J.grav_on && J.grav_amp >= 0) ? deal_amp(J, J.grav_amp, t, dt) : 1.0;
+        if (!(dtsn > 0.0) || !(dt >= 0.0)) c.err |= (uint32_t)ENSC_EF_DT;
+        c.kind = KIND_STEP;
     }
 }
 
@@ -464,7 +561,7 @@ __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(WPE))) void
     // block assembly: staged mixed jobs whose blocks are conflict-free, on
     // ticks that compute forces (uniform over the block)
     const bool asmode = MIXED && STAGED && J.block_asm && dc.kind != KIND_PREPASS;
-    ENSC_LDS double* acc = lds + nbn * 9;
+    ENSC_LDS float* acc = (ENSC_LDS float*)(lds + nbn * 9);  // block assembly: FP32 sum per block node
     if (STAGED) {
         // stage the block's node records once (X, v, u); velocity BCs with an
         // amplitude are applied here, once per node
@@ -486,12 +583,12 @@ __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(WPE))) void
             d[7] = p4.a;
             d[8] = p4.b;
             if (asmode) {
-                acc[3 * i] = 0.0;
-                acc[3 * i + 1] = 0.0;
-                acc[3 * i + 2] = 0.0;
+                acc[3 * i] = 0.f;
+                acc[3 * i + 1] = 0.f;
+                acc[3 * i + 2] = 0.f;
             }
         }
-        __syncthreads();
+        lds_sync();
     }
     double Le = __builtin_inf(), dts = __builtin_inf();
     double en[3] = {0.0, 0.0, 0.0};
@@ -502,12 +599,12 @@ __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(WPE))) void
         if constexpr (MIXED && STAGED)
             element_mixed_S<R>(J, act ? e : J.ne - 1, ES, dc.dt_i_1, dc.kind, dc.time_n, act, true, Le, dts, en, errf,
                                lds, acc);
-        __syncthreads();
+        lds_sync();
         for (uint32_t i = threadIdx.x; i < nbn; i += BLK) {
             float* pf = J.part_f + (size_t)(n0 + i) * 3;
-            gst(pf, 0u) = (float)acc[3 * i];
-            gst(pf, 1u) = (float)acc[3 * i + 1];
-            gst(pf, 2u) = (float)acc[3 * i + 2];
+            gst(pf, 0u) = acc[3 * i];
+            gst(pf, 1u) = acc[3 * i + 1];
+            gst(pf, 2u) = acc[3 * i + 2];
         }
         if (!act) {
             Le = dts = __builtin_inf();
@@ -535,7 +632,7 @@ __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(WPE))) void
     en[0] = wave_sum(en[0]);
     en[1] = wave_sum(en[1]);
     en[2] = wave_sum(en[2]);
-    __syncthreads();
+    lds_sync();
     if (errf) atomicOr(&s_err, errf);
     const int w = threadIdx.x >> 6;
     if ((threadIdx.x & 63) == 0) {
@@ -545,7 +642,7 @@ __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(WPE))) void
         s_red[w][3] = en[1];
         s_red[w][4] = en[2];
     }
-    __syncthreads();
+    lds_sync();
     if (threadIdx.x == 0) {
         double r0 = s_red[0][0], r1 = s_red[0][1], r2 = s_red[0][2], r3 = s_red[0][3], r4 = s_red[0][4];
         for (int q = 1; q < BLK / 64; ++q) {
@@ -566,70 +663,241 @@ __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(WPE))) void
     }
 }
 
-// Per-job end of the element pass: one block per job in the job table
-// closes the previous nodal update if still open (node_phase_end), reduces
-// the job's block partials (written by k_element; the launch
-// boundary orders them) and advances the job's time-step scalars exactly as
-// update_minVertex_perMaterial_and_dt and the run-loop clamps do
-// (exDyna3D.cpp:663-686, 90-97, 775-780).
+// Per-job end of the element pass: one block per job in the job table.  It
+// closes the previous nodal update if still open (node_phase_finish), and
+// reduces the job's element block partials (written by k_element; the
+// launch boundary orders them) to advance the job's time-step scalars
+// exactly as update_minVertex_perMaterial_and_dt and the run-loop clamps do
+// (exDyna3D.cpp:663-686, 90-97, 775-780).  Both sets of partials are folded
+// in one pass so their loads overlap.
 template <int BLK>
 __global__ __launch_bounds__(BLK) void k_fin_element(const JobDev* __restrict__ jobs) {
 #pragma clang fp contract(off)
     const JobDev& J = jobs[blockIdx.x];
     Ctl* ctl = J.ctl;
-    {
-        // the previous step iteration's nodal update, if still open
-        __shared__ double s_red_n[BLK / 64][2];
-        node_phase_end<BLK>(J, s_red_n);
-        __syncthreads();
-    }
+    const bool node_open = ctl->kind == KIND_STEP;
     const Decision dc = decide(*ctl, J.factor, J.wos);
-    if (dc.kind == KIND_NONE) return;
-    __shared__ double s_red[BLK / 64][5];
-    const bool is_min[5] = {true, true, false, false, false};
-    double tot[5];
-    reduce_partials<5, BLK>(J.part_e, J.nblk_e, is_min, tot, s_red);
+    const bool elem = dc.kind != KIND_NONE;
+    if (!node_open && !elem) return;
+    // values 0-1: node partials (sums); 2-6: element partials (2 minima, 3 sums)
+    const bool is_min[7] = {false, false, true, true, false, false, false};
+    double acc[7];
+    fold_partials<BLK>(J, node_open, elem, is_min, acc);
+    __shared__ double s_red[BLK / 64][7];
+    double tot[7];
+    combine_partials<7, BLK>(acc, is_min, s_red, tot);
     if (threadIdx.x == 0) {
-        Ctl& c = *ctl;
-        const double Lmin = tot[0], dtsn = tot[1];
-        if (dc.kind == KIND_PREPASS) {
-            c.lmin_pre = Lmin;
-            c.have_lpre = 1;
-            c.kind = KIND_PREPASS;
-        } else if (dc.kind == KIND_INIT) {
-            c.dts_min = dtsn;
-            c.init_pending = 0;
-            c.kind = KIND_INIT;
-        } else {
-            // update_minVertex_perMaterial_and_dt (663-686) + run-loop clamp (90-97)
-            double dt1 = c.dt_i;
-            const double time_prev = c.time + dt1;
-            double dt = Lmin / J.wos;
-            const double tmp = c.dts_min;
-            dt = dt < tmp ? dt : tmp;
-            dt *= J.factor;
-            if ((c.total_time - time_prev) < dt && (c.total_time - time_prev) > 0.0) {
-                dt = c.total_time - time_prev;
-                dt1 = dt;
-            }
-            if (c.time + dt1 > c.step_end) dt = dt1 = c.step_end - c.time;
-            if (dt1 != dc.dt_i_1) c.err |= (uint32_t)ENSC_EF_DT;  // cannot happen: same inputs as decide()
-            // computeSate (775-780)
-            double t = c.time + dt1;
-            if (t > c.total_time) t = c.total_time;
-            c.time = t;
-            c.dt_i = dt;
-            c.dt_i_1 = dt1;
-            c.dts_min = dtsn;
-            c.nsteps += 1;
-            c.have_lpre = 0;
-            c.ie += tot[2];
-            c.hg += tot[3];
-            c.vis += tot[4];
-            c.grav_amp = (J.grav_on && J.grav_amp >= 0) ? deal_amp(J, J.grav_amp, t, dt) : 1.0;
-            if (!(dtsn > 0.0) || !(dt >= 0.0)) c.err |= (uint32_t)ENSC_EF_DT;
-            c.kind = KIND_STEP;
+        if (node_open) node_phase_finish(J, tot[0], tot[1]);
+        if (elem) elem_phase_finish(J, dc, tot + 2);
+    }
+}
+// ---------------------------------------------------------------------------
+// Pipelined element pass (mixed engine, LDS staging, block assembly).
+//
+// A persistent grid of blocks walks the element blocks ("tiles") of the
+// block map: block b takes tiles b, b + grid, b + 2*grid, ... (so a block
+// stays on the XCD whose contiguous range of tiles it serves).  The next
+// tile's node records travel HBM -> LDS with global_load_lds (no registers
+// involved) into the second of two staging buffers while the current tile
+// is computed from the first; the only exposed latency per tile is one
+// round trip for its element state and the next tile's node ids.  The body
+// is element_mixed_S's (same operations, same assembly order); the two
+// instantiations may contract multiply-adds differently, so a job is given
+// one of the two passes by its own properties and keeps it in any batch.
+// Jobs with amplitude velocity BCs (applied to staged velocities) use
+// k_element.
+//
+// LDS per block is exactly the two buffers, 2 x 5 x 16 B x PIPE_CAP = 80 KiB,
+// so two blocks fit a CU's 160 KiB: once every corner value of the tile has
+// been read, the tile's force accumulators (4 floats per node) take the
+// [X0 X1] row of the current buffer and the block reduction takes its
+// [X2 1/m] row.
+constexpr int PIPE_BLK = 256;
+constexpr int PIPE_CAP = 512;  // staged nodes per tile; larger tiles use k_element
+struct alignas(16) Pair {
+    double a, b;
+};
+typedef ENSC_LDS Pair (*PipeBuf)[PIPE_CAP];  // [5][PIPE_CAP]: [X0 X1] [X2 1/m] [v0 v1] [v2 u0] [u1 u2]
+
+// Staging of a tile's node records into a buffer: each thread owns slots
+// (wave's first slot) + lane + 256*j, j < 2 (PIPE_CAP == 512).  pipe_ids
+// loads the node ids of those slots; pipe_copy then issues five 16-byte
+// global->LDS copies per slot, which complete in the background.
+__device__ __forceinline__ void pipe_ids(const JobDev& J, uint32_t n0, uint32_t nbn, uint32_t g[2]) {
+    const uint32_t s0 = threadIdx.x;
+#pragma unroll
+    for (int j = 0; j < 2; ++j) g[j] = (s0 + j * PIPE_BLK < nbn) ? gld(J.blk_nodes, n0 + s0 + j * PIPE_BLK) : 0u;
+}
+__device__ __forceinline__ void pipe_copy(const JobDev& J, uint32_t nbn, const uint32_t g[2], PipeBuf buf) {
+    const uint32_t lane = threadIdx.x & 63u, w0 = threadIdx.x & ~63u;
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+        const uint32_t s0 = w0 + j * PIPE_BLK;  // the wave's first slot: the copies land at lane * 16 from it
+        if (s0 + lane < nbn) {
+            const ENSC_GLB char* pp = (const ENSC_GLB char*)J.pos + g[j] * (uint32_t)(8 * POS_REC);
+            const ENSC_GLB char* pv = (const ENSC_GLB char*)J.vu + g[j] * (uint32_t)(8 * VU_REC);
+            __builtin_amdgcn_global_load_lds((const ENSC_GLB void*)pp, (ENSC_LDS void*)&buf[0][s0], 16, 0, 0);
+            __builtin_amdgcn_global_load_lds((const ENSC_GLB void*)(pp + 16), (ENSC_LDS void*)&buf[1][s0], 16, 0, 0);
+            __builtin_amdgcn_global_load_lds((const ENSC_GLB void*)pv, (ENSC_LDS void*)&buf[2][s0], 16, 0, 0);
+            __builtin_amdgcn_global_load_lds((const ENSC_GLB void*)(pv + 16), (ENSC_LDS void*)&buf[3][s0], 16, 0, 0);
+            __builtin_amdgcn_global_load_lds((const ENSC_GLB void*)(pv + 32), (ENSC_LDS void*)&buf[4][s0], 16, 0, 0);
         }
+    }
+}
+// Accumulators of the current tile in its [X0 X1] row, 4 floats per node,
+// zeroed after every wave has read its corner values.
+__device__ __forceinline__ ENSC_LDS float* pipe_acc(PipeBuf buf) { return (ENSC_LDS float*)&buf[0][0]; }
+__device__ __forceinline__ void pipe_acc_clear(ENSC_LDS float* acc, uint32_t nbn) {
+    lds_sync();
+    for (uint32_t i = threadIdx.x; i < nbn; i += PIPE_BLK) {
+        acc[4 * i] = 0.f;
+        acc[4 * i + 1] = 0.f;
+        acc[4 * i + 2] = 0.f;
+    }
+}
+
+#define PX(n, i) ((i) == 0 ? buf[0][n].a : ((i) == 1 ? buf[0][n].b : buf[1][n].a))
+#define PV(n, i) ((i) == 0 ? buf[2][n].a : ((i) == 1 ? buf[2][n].b : buf[3][n].a))
+#define PU(n, i) ((i) == 0 ? buf[3][n].b : ((i) == 1 ? buf[4][n].a : buf[4][n].b))
+#define NVAL(k, i) \
+    ((i) < 3 ? PX(ES.nd[k], (i)) + PU(ES.nd[k], (i)) : ((i) < 6 ? PV(ES.nd[k], (i)-3) : PU(ES.nd[k], (i)-6)))
+#define CORNER_V(k, i) NVAL(k, 3 + (i))
+#define ACC_W 4
+#define ASM_BEGIN pipe_acc_clear(acc, nbn)
+__device__ __forceinline__ void element_mixed_P(const JobDev& J, uint32_t e, const ElemState<float>& ES, double dt,
+                                                int32_t kind, double time_n, bool act, bool asmode, double& Le,
+                                                double& dts, double* en, uint32_t& errf, const PipeBuf buf,
+                                                uint32_t nbn) {
+#pragma clang fp contract(fast)
+    typedef float R;
+    constexpr bool MIXED = true, ASM_CAP = true;
+    ENSC_LDS float* acc = pipe_acc(buf);
+#include "elem_body_mixed.inc"
+}
+#undef NVAL
+#undef CORNER_V
+#undef PX
+#undef PV
+#undef PU
+#undef ACC_W
+#undef ASM_BEGIN
+
+// One tile: compute it from `cur` while the next tile is copied into `nxt`.
+// The one wait of a tile, at its start, covers this tile's element state,
+// the next tile's node ids, this tile's copies (issued during the previous
+// tile) and the previous tile's stores; the barrier after it also means
+// every wave is done with `nxt` (the previous tile's buffer).  Returns false
+// after the block's last tile.
+__device__ __forceinline__ bool pipe_tile(const JobDev* __restrict__ jobs, const uint4* __restrict__ blkmap,
+                                          uint32_t ntiles, uint32_t& t, PipeBuf cur, PipeBuf nxt) {
+#pragma clang fp contract(off)
+    const uint4 bm = blkmap[t];  // (job, block of the job, first staged position, staged count)
+    const JobDev& J = jobs[bm.x];
+    Ctl* ctl = J.ctl;
+    const uint32_t lb = bm.y, nbn = bm.w;
+    const uint32_t e = lb * PIPE_BLK + threadIdx.x;
+    const bool act = e < J.ne;
+    const Decision dc = decide(*ctl, J.factor, J.wos);
+    // (issued on every path, so the wait below covers them on every path)
+    ElemState<float> ES;
+    load_state<float, true>(J, act ? e : J.ne - 1, true, ES);
+    const uint32_t tn = t + gridDim.x;
+    const bool more = tn < ntiles;
+    const uint4 bn = blkmap[more ? tn : t];
+    uint32_t g[2];
+    pipe_ids(jobs[bn.x], bn.z, more ? bn.w : 0u, g);
+    __builtin_amdgcn_s_waitcnt(0);
+    lds_sync();
+    if (more) pipe_copy(jobs[bn.x], bn.w, g, nxt);
+    if (dc.kind != KIND_NONE) {
+        const bool asmode = J.block_asm && dc.kind != KIND_PREPASS;
+        double Le = __builtin_inf(), dts = __builtin_inf();
+        double en[3] = {0.0, 0.0, 0.0};
+        uint32_t errf = 0;
+        if (asmode || act)
+            element_mixed_P(J, act ? e : J.ne - 1, ES, dc.dt_i_1, dc.kind, dc.time_n, act, asmode, Le, dts, en, errf,
+                            cur, nbn);
+        if (asmode) {
+            const ENSC_LDS float* acc = pipe_acc(cur);
+            lds_sync();
+            for (uint32_t i = threadIdx.x; i < nbn; i += PIPE_BLK) {
+                float* pf = J.part_f + (size_t)(bm.z + i) * 3;
+                gst(pf, 0u) = acc[4 * i];
+                gst(pf, 1u) = acc[4 * i + 1];
+                gst(pf, 2u) = acc[4 * i + 2];
+            }
+        }
+        if (!act) {
+            Le = dts = __builtin_inf();
+            en[0] = en[1] = en[2] = 0.0;
+        }
+        // block partials: wave shuffles, then the [X2 1/m] row of `cur`
+        // (free once every wave is past the barrier below) across the waves
+        Le = wave_min(Le);
+        dts = wave_min(dts);
+        en[0] = wave_sum(en[0]);
+        en[1] = wave_sum(en[1]);
+        en[2] = wave_sum(en[2]);
+        uint32_t werr = 0;
+#pragma unroll
+        for (int b = 0; b < 3; ++b)
+            if (__ballot((errf >> b) & 1u)) werr |= 1u << b;
+        lds_sync();
+        ENSC_LDS double(*s_red)[6] = (ENSC_LDS double(*)[6]) & cur[1][0];
+        const int w = threadIdx.x >> 6;
+        if ((threadIdx.x & 63) == 0) {
+            s_red[w][0] = Le;
+            s_red[w][1] = dts;
+            s_red[w][2] = en[0];
+            s_red[w][3] = en[1];
+            s_red[w][4] = en[2];
+            s_red[w][5] = (double)werr;
+        }
+        lds_sync();
+        if (threadIdx.x == 0) {
+            double r0 = s_red[0][0], r1 = s_red[0][1], r2 = s_red[0][2], r3 = s_red[0][3], r4 = s_red[0][4];
+            uint32_t er = (uint32_t)s_red[0][5];
+            for (int q = 1; q < PIPE_BLK / 64; ++q) {
+                r0 = s_red[q][0] < r0 ? s_red[q][0] : r0;
+                r1 = s_red[q][1] < r1 ? s_red[q][1] : r1;
+                r2 += s_red[q][2];
+                r3 += s_red[q][3];
+                r4 += s_red[q][4];
+                er |= (uint32_t)s_red[q][5];
+            }
+            double* p = J.part_e + lb;
+            const uint32_t nb = J.nblk_e;
+            p[0] = r0;
+            p[nb] = r1;
+            p[2 * nb] = r2;
+            p[3 * nb] = r3;
+            p[4 * nb] = r4;
+            if (er) atomicOr(&ctl->err, er);
+        }
+    }
+    t = tn;
+    return more;
+}
+
+template <int WPE>
+__global__ __launch_bounds__(PIPE_BLK) __attribute__((amdgpu_waves_per_eu(WPE))) void k_element_pipe(
+    const JobDev* __restrict__ jobs, const uint4* __restrict__ blkmap, uint32_t ntiles) {
+    __shared__ Pair s_bufA[5][PIPE_CAP];
+    __shared__ Pair s_bufB[5][PIPE_CAP];
+    const PipeBuf bufA = (PipeBuf)s_bufA;
+    const PipeBuf bufB = (PipeBuf)s_bufB;
+    uint32_t t = blockIdx.x;
+    if (t >= ntiles) return;
+    {
+        const uint4 bm = blkmap[t];
+        uint32_t g[2];
+        pipe_ids(jobs[bm.x], bm.z, bm.w, g);
+        pipe_copy(jobs[bm.x], bm.w, g, bufA);
+    }
+    // two tiles per trip so each buffer is a distinct LDS object in every
+    // access (the compiler then keeps copies into one buffer in flight while
+    // the other is read)
+    while (pipe_tile(jobs, blkmap, ntiles, t, bufA, bufB) && pipe_tile(jobs, blkmap, ntiles, t, bufB, bufA)) {
     }
 }
 
@@ -823,53 +1091,21 @@ __global__ __launch_bounds__(BLK) void k_node(const JobDev* __restrict__ jobs, c
     }
 }
 
-// End of a job's nodal update: kinetic energy, external work and the
-// job's history row for the step.  Runs when the job's scalars say a nodal
-// update is pending (kind == STEP) and marks it done; the same test makes a
-// second call a no-op.  Every thread of the block must call it.
-template <int BLK>
-__device__ inline void node_phase_end(const JobDev& J, double (*s_red)[2]) {
-#pragma clang fp contract(off)
-    Ctl* ctl = J.ctl;
-    if (ctl->kind != KIND_STEP) return;
-    const double dt_i = ctl->dt_i, dt_i_1 = ctl->dt_i_1;
-    const bool is_min[2] = {false, false};
-    double tot[2];
-    reduce_partials<2, BLK>(J.part_n, J.nblk_n, is_min, tot, s_red);
-    if (threadIdx.x == 0) {
-        Ctl& c = *ctl;
-        c.ke = tot[0];
-        c.wext += tot[1] * (0.5 * (dt_i + dt_i_1));
-        const long long row = c.nsteps - 1;
-        if (J.hist && row >= 0 && row < J.hist_cap) {
-            double* h = J.hist + row * (ENSC_H_BASE + 3 * J.n_probe);
-            h[ENSC_H_TIME] = c.time;
-            h[ENSC_H_DT_I] = c.dt_i;
-            h[ENSC_H_DT_I_1] = c.dt_i_1;
-            h[ENSC_H_DTS_MIN] = c.dts_min;
-            h[ENSC_H_KE] = c.ke;
-            h[ENSC_H_IE] = c.ie;
-            h[ENSC_H_HG] = c.hg;
-            h[ENSC_H_VIS] = c.vis;
-            h[ENSC_H_WEXT] = c.wext;
-            for (int p = 0; p < J.n_probe; ++p) {
-                const double* r = J.vu + (size_t)J.probe[p] * VU_REC + 3;
-                h[ENSC_H_BASE + 3 * p + 0] = r[0];
-                h[ENSC_H_BASE + 3 * p + 1] = r[1];
-                h[ENSC_H_BASE + 3 * p + 2] = r[2];
-            }
-        }
-        c.kind = KIND_NONE;
-    }
-}
-
-// Per-job node finalize (one block per job in the job table).  The element
-// finalize of the next step iteration does the same work first, so a run
-// of step iterations needs this launch only after its last one.
+// Per-job node finalize (one block per job in the job table): closes the
+// last nodal update of a run of step iterations.  The element finalize of
+// the next iteration does the same work first, so only the last iteration
+// of a run launches it.
 template <int BLK>
 __global__ __launch_bounds__(BLK) void k_fin_node(const JobDev* __restrict__ jobs) {
-    __shared__ double s_red[BLK / 64][2];
-    node_phase_end<BLK>(jobs[blockIdx.x], s_red);
+    const JobDev& J = jobs[blockIdx.x];
+    if (J.ctl->kind != KIND_STEP) return;
+    const bool is_min[7] = {false, false, true, true, false, false, false};
+    double acc[7];
+    fold_partials<BLK>(J, true, false, is_min, acc);
+    __shared__ double s_red[BLK / 64][7];
+    double tot[7];
+    combine_partials<7, BLK>(acc, is_min, s_red, tot);
+    if (threadIdx.x == 0) node_phase_finish(J, tot[0], tot[1]);
 }
 
 // setCurrentStep's apply_boundary_condition_vec (exDyna3D.cpp:1911, 379-490)

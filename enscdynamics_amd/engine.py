@@ -31,7 +31,7 @@ EXPORTS = [
     "ensc_last_error", "ensc_device_count", "ensc_engine_create", "ensc_engine_destroy", "ensc_engine_init",
     "ensc_engine_set_step", "ensc_engine_advance", "ensc_engine_run", "ensc_engine_enqueue_steps",
     "ensc_engine_sync", "ensc_engine_download", "ensc_engine_scalars", "ensc_engine_history",
-    "ensc_engine_set_graph", "ensc_copy_bandwidth", "ensc_engine_kernel_times", "ensc_engine_phase_times", "ensc_engine_set_timing", "ensc_engine_device_bytes",
+    "ensc_engine_set_graph", "ensc_copy_bandwidth", "ensc_engine_kernel_times", "ensc_engine_phase_times", "ensc_engine_layout", "ensc_engine_set_timing", "ensc_engine_device_bytes",
 ]
 
 
@@ -82,6 +82,7 @@ def load_library(path: str = LIB_PATH):
         ("ensc_engine_history", [vp, vp, C.c_int64, C.POINTER(C.c_int64)]),
         ("ensc_engine_kernel_times", [vp, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
         ("ensc_engine_phase_times", [vp, C.c_void_p]),
+        ("ensc_engine_layout", [vp, C.c_void_p]),
         ("ensc_engine_set_timing", [vp, C.c_int32]),
         ("ensc_engine_set_graph", [vp, C.c_int32]),
     ]:
@@ -135,7 +136,7 @@ class Engine:
         o.n_probe = len(pr)
         o.probe_nodes = pr.ctypes.data_as(C.POINTER(C.c_uint32)) if len(pr) else None
         o.hist_cap = hist_cap
-        o.staging = 0 if staging else 2
+        o.staging = (0 if staging else 2) if isinstance(staging, bool) else int(staging)
         o.waves = waves
         h = C.c_void_p()
         _check(L.ensc_engine_create(C.byref(self._desc), C.byref(o), C.byref(h)))
@@ -203,6 +204,13 @@ class Engine:
         out = np.zeros(4)
         _check(_lib.ensc_engine_phase_times(self.h, out.ctypes.data))
         return tuple(float(x) for x in out)
+
+    def layout(self) -> Dict[str, int]:
+        """Launch layout of the last step iteration (ensc_engine_layout)."""
+        out = np.zeros(6, dtype=np.int64)
+        _check(_lib.ensc_engine_layout(self.h, out.ctypes.data))
+        keys = ("staged", "pipe_blocks", "elem_blocks", "node_blocks", "lds_bytes", "asm_jobs")
+        return {k: int(v) for k, v in zip(keys, out)}
 
     def get(self, name: str) -> np.ndarray:
         shapes = {"u": (self.Nn, 3), "v": (self.Nn, 3), "x": (self.Nn, 3), "rf": (self.Nn, 3),

@@ -56,8 +56,12 @@ typedef struct ensc_options {
     uint32_t n_probe;       /* nodes whose displacement is recorded per step */
     const uint32_t* probe_nodes;
     int64_t hist_cap;       /* rows of per-step history kept on device */
-    int32_t staging;        /* 0: stage element-block nodes in LDS when they fit
-                               (default), 2: always gather corners from HBM */
+    int32_t staging;        /* 0: stage element-block nodes in LDS when they
+                               fit; mixed jobs with block assembly and at
+                               most 512 nodes per element block run the
+                               persistent pipelined element pass (default);
+                               1: staged, one block per element block;
+                               2: gather corners from HBM */
     int32_t waves;          /* element-kernel occupancy target, waves per SIMD
                                (1, 2 or 3; 0 = default for the precision).
                                Staged kernels only. */
@@ -130,6 +134,15 @@ ensc_status ensc_engine_kernel_times(ensc_engine* eng, double* elem_ms, double* 
 /* Mean duration in ms of each launch of the last timed enqueue, in launch
  * order: element pass, element finalize, nodal update, node finalize. */
 ensc_status ensc_engine_phase_times(ensc_engine* eng, double* ms4);
+/* Launch layout of the last step iteration (ENSC_L_COUNT values): whether
+ * element blocks stage their nodes in LDS, the element blocks run by the
+ * pipelined pass and by the one-block-per-tile pass, the node blocks, the
+ * dynamic LDS bytes per element block and the jobs using block assembly. */
+enum {
+    ENSC_L_STAGED = 0, ENSC_L_PIPE_BLOCKS, ENSC_L_ELEM_BLOCKS, ENSC_L_NODE_BLOCKS, ENSC_L_LDS_BYTES,
+    ENSC_L_ASM_JOBS, ENSC_L_COUNT
+};
+ensc_status ensc_engine_layout(ensc_engine* eng, int64_t* out /* ENSC_L_COUNT */);
 /* Enable (1) / disable (0) per-kernel event timing for later enqueue calls. */
 ensc_status ensc_engine_set_timing(ensc_engine* eng, int32_t on);
 /* Device bytes held by the engine. */

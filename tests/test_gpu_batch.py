@@ -23,15 +23,21 @@ def small_jobs(n, seed=7, plastic_p=0.0):
     return [decks.random_job(rng, 300, 4000, plastic_p, k_lo=150, k_hi=400, name=f"t{i}") for i in range(n)]
 
 
-@pytest.mark.parametrize("precision", [eng.FP64, eng.MIXED])
-def test_batch_alone_equals_in_mix(precision):
+@pytest.mark.parametrize("precision,staging", [(eng.FP64, 0), (eng.MIXED, 0), (eng.MIXED, 1)])
+def test_batch_alone_equals_in_mix(precision, staging):
+    # staging 0: jobs whose element blocks fit the pipelined pass run it,
+    # the others the one-block-per-tile pass, decided per job; staging 1:
+    # every job the one-block-per-tile pass
     jobs = small_jobs(10, seed=3, plastic_p=0.5)
-    b = Batch(jobs, precision, reorder=True).run()
+    # a ramped velocity BC (amplitude) keeps this job on the one-block-per-
+    # tile pass, so a mixed batch at staging 0 runs both passes per step
+    jobs.append(decks.config2(nx=12, ny=6, nz=6, steps=300))
+    b = Batch(jobs, precision, reorder=True, staging=staging).run()
     for i, m in enumerate(jobs):
         s = b.scalars(i)
         assert s["done"] == 1.0 and s["err"] == 0.0
         assert s["time"] == pytest.approx(m.total_time, rel=1e-12)
-        a = eng.Engine(m, precision, reorder=True).run()
+        a = eng.Engine(m, precision, reorder=True, staging=staging).run()
         for f in ("u", "v", "sigma", "qia"):
             assert np.array_equal(b.get(i, f), a.get(f)), (i, f)
         sa = a.scalars()

@@ -221,6 +221,36 @@ def test_mixed_vs_fp64_plastic_cube():
     assert e_err <= 0.01 * sa["wext"]
 
 
+def test_mixed_pipelined_pass_vs_fp64_plastic_cube():
+    # the persistent element pass that copies the next tile's nodes into
+    # LDS while the current tile computes (default for such jobs)
+    m = decks.config3(n=20, steps=800)
+    a = eng.Engine(m, eng.FP64).run()
+    b = eng.Engine(m, eng.MIXED, reorder=True)
+    b.init()
+    b.set_current_step(1)
+    b.advance(5)
+    lay = b.layout()
+    assert lay["staged"] == 1 and lay["asm_jobs"] == 1
+    assert lay["pipe_blocks"] == (m.n_elems + 255) // 256 and lay["elem_blocks"] == 0, lay
+    b.advance(0)
+    sb = b.scalars()
+    assert sb["done"] == 1.0 and sb["err"] == 0.0
+    err_u = rel(b.get("u"), a.get("u"))
+    sa = a.scalars()
+    e_err = abs((sb["ke"] + sb["ie"] + sb["hg"] + sb["vis"]) - (sa["ke"] + sa["ie"] + sa["hg"] + sa["vis"]))
+    print(f"[c3s mixed pipelined vs fp64] u {err_u:.2e} energy {e_err / sa['wext']:.2e} of W_ext", flush=True)
+    assert err_u <= TOL_MIXED
+    assert e_err <= 0.01 * sa["wext"]
+    # staging option 1 on the same job: same layout without the pipeline
+    c = eng.Engine(m, eng.MIXED, reorder=True, staging=1)
+    c.init()
+    c.set_current_step(1)
+    c.advance(5)
+    lay = c.layout()
+    assert lay["pipe_blocks"] == 0 and lay["elem_blocks"] == (m.n_elems + 255) // 256, lay
+
+
 def test_full_size_cube_fp64_vs_oracle_few_steps():
     # configuration 3 at its full 1M-element size: a few steps against the
     # oracle (size-independent check of the same arithmetic at scale)
