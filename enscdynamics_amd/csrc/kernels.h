@@ -355,6 +355,7 @@ __device__ __noinline__ V3 bc_override(const JobDev& J, uint32_t g, double t, V3
 
 #define ACC_W 3                  // block assembly: 3 floats per staged node
 #define ASM_BEGIN ((void)0)      // the accumulators start zeroed
+#define PREFETCH_HOOK ((void)0)
 #define ELEM_ARGS                                                                                          \
     const JobDev &J, uint32_t e, const ElemState<R> &ES, double dt, int32_t kind, double time_n, bool act, \
         bool asmode, double &Le, double &dts, double *en, uint32_t &errf, const ENSC_LDS double *lds,     \
@@ -413,6 +414,7 @@ __device__ __forceinline__ void element_mixed_S(ELEM_ARGS) {
 #undef ELEM_ARGS
 #undef ACC_W
 #undef ASM_BEGIN
+#undef PREFETCH_HOOK
 
 // Fixed-order reduction of a job's block partials (value q of block b at
 // part[q * nblk + b]).  fold_partials: thread t folds blocks t, t+BLK,
@@ -756,6 +758,24 @@ __device__ __forceinline__ void pipe_acc_clear(ENSC_LDS float* acc, uint32_t nbn
     }
 }
 
+// What a tile prefetches for the next one while its own corner forces and
+// assembly run: the next tile's element state (into the registers the
+// current state has just freed) and the node ids of the tile after that.
+// Indices are clamped to valid elements and a missing tile reads nothing,
+// so the loads are issued on every path.
+struct PipeNext {
+    const JobDev* J;   // the next tile's job
+    uint32_t e;        // this thread's element of the next tile (clamped)
+    const JobDev* J2;  // the job of the tile after
+    uint32_t n0, nbn;  // its staging range (nbn = 0: no such tile)
+    bool done;
+};
+__device__ __forceinline__ void pipe_prefetch(PipeNext& p, ElemState<float>& ESn, uint32_t gn[2]) {
+    load_state<float, true>(*p.J, p.e, true, ESn);
+    pipe_ids(*p.J2, p.n0, p.nbn, gn);
+    p.done = true;
+}
+
 #define PX(n, i) ((i) == 0 ? buf[0][n].a : ((i) == 1 ? buf[0][n].b : buf[1][n].a))
 #define PV(n, i) ((i) == 0 ? buf[2][n].a : ((i) == 1 ? buf[2][n].b : buf[3][n].a))
 #define PU(n, i) ((i) == 0 ? buf[3][n].b : ((i) == 1 ? buf[4][n].a : buf[4][n].b))
@@ -764,10 +784,11 @@ __device__ __forceinline__ void pipe_acc_clear(ENSC_LDS float* acc, uint32_t nbn
 #define CORNER_V(k, i) NVAL(k, 3 + (i))
 #define ACC_W 4
 #define ASM_BEGIN pipe_acc_clear(acc, nbn)
+#define PREFETCH_HOOK pipe_prefetch(pn, ESn, gn)
 __device__ __forceinline__ void element_mixed_P(const JobDev& J, uint32_t e, const ElemState<float>& ES, double dt,
                                                 int32_t kind, double time_n, bool act, bool asmode, double& Le,
                                                 double& dts, double* en, uint32_t& errf, const PipeBuf buf,
-                                                uint32_t nbn) {
+                                                uint32_t nbn, PipeNext& pn, ElemState<float>& ESn, uint32_t gn[2]) {
 #pragma clang fp contract(fast)
     typedef float R;
     constexpr bool MIXED = true, ASM_CAP = true;
@@ -781,15 +802,20 @@ __device__ __forceinline__ void element_mixed_P(const JobDev& J, uint32_t e, con
 #undef PU
 #undef ACC_W
 #undef ASM_BEGIN
+#undef PREFETCH_HOOK
 
-// One tile: compute it from `cur` while the next tile is copied into `nxt`.
-// The one wait of a tile, at its start, covers this tile's element state,
-// the next tile's node ids, this tile's copies (issued during the previous
-// tile) and the previous tile's stores; the barrier after it also means
-// every wave is done with `nxt` (the previous tile's buffer).  Returns false
-// after the block's last tile.
+// One tile: compute it from `cur` (its node records) and ES (its element
+// state) while the next tile's records are copied into `nxt`; g holds the
+// next tile's node ids.  Fills ESn and gn for the next tile.  The one wait
+// of a tile, at its start, covers this tile's copies (issued during the
+// previous tile), ES and g (prefetched by the previous tile) and the
+// previous tile's stores; the barrier after it also means every wave is
+// done with `nxt` (the previous tile's buffer).  Returns false after the
+// block's last tile.
 __device__ __forceinline__ bool pipe_tile(const JobDev* __restrict__ jobs, const uint4* __restrict__ blkmap,
-                                          uint32_t ntiles, uint32_t& t, PipeBuf cur, PipeBuf nxt) {
+                                          uint32_t ntiles, uint32_t& t, PipeBuf cur, PipeBuf nxt,
+                                          const ElemState<float>& ES, const uint32_t g[2], ElemState<float>& ESn,
+                                          uint32_t gn[2]) {
 #pragma clang fp contract(off)
     const uint4 bm = blkmap[t];  // (job, block of the job, first staged position, staged count)
     const JobDev& J = jobs[bm.x];
@@ -798,14 +824,20 @@ __device__ __forceinline__ bool pipe_tile(const JobDev* __restrict__ jobs, const
     const uint32_t e = lb * PIPE_BLK + threadIdx.x;
     const bool act = e < J.ne;
     const Decision dc = decide(*ctl, J.factor, J.wos);
-    // (issued on every path, so the wait below covers them on every path)
-    ElemState<float> ES;
-    load_state<float, true>(J, act ? e : J.ne - 1, true, ES);
-    const uint32_t tn = t + gridDim.x;
-    const bool more = tn < ntiles;
+    const uint32_t tn = t + gridDim.x, tnn = tn + gridDim.x;
+    const bool more = tn < ntiles, more2 = tnn < ntiles;
     const uint4 bn = blkmap[more ? tn : t];
-    uint32_t g[2];
-    pipe_ids(jobs[bn.x], bn.z, more ? bn.w : 0u, g);
+    const uint4 bnn = blkmap[more2 ? tnn : t];
+    PipeNext pn;
+    pn.J = &jobs[bn.x];
+    {
+        const uint32_t en = bn.y * PIPE_BLK + threadIdx.x;
+        pn.e = en < pn.J->ne ? en : pn.J->ne - 1;
+    }
+    pn.J2 = &jobs[bnn.x];
+    pn.n0 = bnn.z;
+    pn.nbn = more2 ? bnn.w : 0u;
+    pn.done = false;
     __builtin_amdgcn_s_waitcnt(0);
     lds_sync();
     if (more) pipe_copy(jobs[bn.x], bn.w, g, nxt);
@@ -816,7 +848,8 @@ __device__ __forceinline__ bool pipe_tile(const JobDev* __restrict__ jobs, const
         uint32_t errf = 0;
         if (asmode || act)
             element_mixed_P(J, act ? e : J.ne - 1, ES, dc.dt_i_1, dc.kind, dc.time_n, act, asmode, Le, dts, en, errf,
-                            cur, nbn);
+                            cur, nbn, pn, ESn, gn);
+        if (!pn.done) pipe_prefetch(pn, ESn, gn);
         if (asmode) {
             const ENSC_LDS float* acc = pipe_acc(cur);
             lds_sync();
@@ -874,6 +907,8 @@ __device__ __forceinline__ bool pipe_tile(const JobDev* __restrict__ jobs, const
             p[4 * nb] = r4;
             if (er) atomicOr(&ctl->err, er);
         }
+    } else {
+        pipe_prefetch(pn, ESn, gn);
     }
     t = tn;
     return more;
@@ -888,16 +923,25 @@ __global__ __launch_bounds__(PIPE_BLK) __attribute__((amdgpu_waves_per_eu(WPE)))
     const PipeBuf bufB = (PipeBuf)s_bufB;
     uint32_t t = blockIdx.x;
     if (t >= ntiles) return;
+    ElemState<float> esA, esB;
+    uint32_t gA[2], gB[2];
     {
+        // the first tile's copies and state, the second tile's node ids
         const uint4 bm = blkmap[t];
-        uint32_t g[2];
-        pipe_ids(jobs[bm.x], bm.z, bm.w, g);
-        pipe_copy(jobs[bm.x], bm.w, g, bufA);
+        const JobDev& J = jobs[bm.x];
+        pipe_ids(J, bm.z, bm.w, gA);
+        pipe_copy(J, bm.w, gA, bufA);
+        const uint32_t e = bm.y * PIPE_BLK + threadIdx.x;
+        load_state<float, true>(J, e < J.ne ? e : J.ne - 1, true, esA);
+        const uint32_t tn = t + gridDim.x;
+        const uint4 bn = blkmap[tn < ntiles ? tn : t];
+        pipe_ids(jobs[bn.x], bn.z, tn < ntiles ? bn.w : 0u, gB);
     }
     // two tiles per trip so each buffer is a distinct LDS object in every
     // access (the compiler then keeps copies into one buffer in flight while
     // the other is read)
-    while (pipe_tile(jobs, blkmap, ntiles, t, bufA, bufB) && pipe_tile(jobs, blkmap, ntiles, t, bufB, bufA)) {
+    while (pipe_tile(jobs, blkmap, ntiles, t, bufA, bufB, esA, gB, esB, gA) &&
+           pipe_tile(jobs, blkmap, ntiles, t, bufB, bufA, esB, gA, esA, gB)) {
     }
 }
 
