@@ -65,7 +65,8 @@ struct ensc_engine {
     DevBuf jobs_buf, b2e_buf, b2n_buf, pe_buf, pn_buf;
     uint32_t nblk_e = 0, nblk_n = 0;
     bool dirty = true;
-    KernelFn ke = nullptr, kn = nullptr;
+    KernelFn ke = nullptr, kn = nullptr, kn_asm = nullptr;
+    uint32_t nblk_n_asm = 0;  // node blocks of block-assembly jobs (first in the map); nblk_n: the rest
     bool staged = false;
     uint32_t nblk_pipe = 0;  // element blocks of the pipelined jobs (first in the map); nblk_e: the rest
     uint32_t n_asm = 0;      // live jobs using block assembly
@@ -117,12 +118,14 @@ KernelFn pick_e(int blk, bool staged, int waves) {
         default: return pick_staged<M, 2>(blk);
     }
 }
-template <bool M>
+// Node kernel for (precision, block size, force source): ASM sums the block
+// partials of block-assembly jobs, otherwise the per-corner forces.
+template <bool M, bool ASM>
 KernelFn pick_n(int blk) {
     switch (blk) {
-        case 64: return k_node<M, 64>;
-        case 128: return k_node<M, 128>;
-        default: return k_node<M, 256>;
+        case 64: return k_node<M, 64, ASM>;
+        case 128: return k_node<M, 128, ASM>;
+        default: return k_node<M, 256, ASM>;
     }
 }
 
@@ -215,35 +218,43 @@ ensc_status relayout(ensc_engine* E) {
     };
     // block maps: (job, block of the job, staged range) per launched block,
     // in XCD order; the pipelined jobs' element blocks first
-    std::vector<uint4> lp, lr, ln;
+    std::vector<uint4> lp, lr, lna, ln;
     std::vector<JobDev> tab;
     std::vector<size_t> pe0, pn0;
-    size_t ne_blocks = 0;
+    size_t ne_blocks = 0, n_blocks = 0;
     for (uint32_t i = 0; i < E->live.size(); ++i) {
         Job& J = *E->slots[E->live[i]];
         const JobDev& D = J.dev;
         pe0.push_back(ne_blocks);
-        pn0.push_back(ln.size());
+        pn0.push_back(n_blocks);
         std::vector<uint4>& le = pipe_job(J) ? lp : lr;
         for (uint32_t b = 0; b < D.nblk_e; ++b)
             le.push_back(make_uint4(i, b, J.blk_ptr_h[b], J.blk_ptr_h[b + 1] - J.blk_ptr_h[b]));
         ne_blocks += D.nblk_e;
-        for (uint32_t b = 0; b < D.nblk_n; ++b) ln.push_back(make_uint4(i, b, 0, 0));
+        // node blocks of block-assembly jobs first, then the others
+        std::vector<uint4>& lnj = (asm_fits && J.asm_capable) ? lna : ln;
+        for (uint32_t b = 0; b < D.nblk_n; ++b) lnj.push_back(make_uint4(i, b, 0, 0));
+        n_blocks += D.nblk_n;
     }
     std::vector<uint4> b2e = xcd_order(lp);
     {
         const std::vector<uint4> r = xcd_order(lr);
         b2e.insert(b2e.end(), r.begin(), r.end());
     }
-    const std::vector<uint4> b2n = xcd_order(ln);
+    std::vector<uint4> b2n = xcd_order(lna);
+    {
+        const std::vector<uint4> r = xcd_order(ln);
+        b2n.insert(b2n.end(), r.begin(), r.end());
+    }
     E->nblk_pipe = (uint32_t)lp.size();
     E->nblk_e = (uint32_t)lr.size();
-    E->nblk_n = (uint32_t)b2n.size();
+    E->nblk_n_asm = (uint32_t)lna.size();
+    E->nblk_n = (uint32_t)ln.size();
     if (!E->jobs_buf.ensure(std::max<size_t>(1, E->live.size()) * sizeof(JobDev)) ||
         !E->b2e_buf.ensure(std::max<size_t>(1, b2e.size()) * sizeof(uint4)) ||
         !E->b2n_buf.ensure(std::max<size_t>(1, b2n.size()) * sizeof(uint4)) ||
         !E->pe_buf.ensure(std::max<size_t>(1, ne_blocks) * 5 * sizeof(double)) ||
-        !E->pn_buf.ensure(std::max<size_t>(1, b2n.size()) * 2 * sizeof(double)))
+        !E->pn_buf.ensure(std::max<size_t>(1, n_blocks) * 2 * sizeof(double)))
         return fail(ENSC_ERR_MEM, "layout allocation failed");
     bool any_asm = false;
     E->n_asm = 0;
@@ -266,7 +277,8 @@ ensc_status relayout(ensc_engine* E) {
     HIPCHK(hipStreamSynchronize(E->stream));
     E->lds_bytes = E->staged ? (size_t)std::max<uint32_t>(lds_nodes, 1) * (9 * sizeof(double) + (any_asm ? 12 : 0)) : 0;
     E->ke = mixed ? pick_e<true>(E->blk_e, E->staged, E->waves) : pick_e<false>(E->blk_e, E->staged, E->waves);
-    E->kn = mixed ? pick_n<true>(E->blk_n) : pick_n<false>(E->blk_n);
+    E->kn = mixed ? pick_n<true, false>(E->blk_n) : pick_n<false, false>(E->blk_n);
+    E->kn_asm = pick_n<true, true>(E->blk_n);
     E->dirty = false;
     E->graph_valid = false;
     return ENSC_OK;
@@ -292,8 +304,12 @@ void launch_phase(ensc_engine* E, int phase) {
             break;
         case 1: hipLaunchKernelGGL(k_fin_element<FIN_BLK>, dim3(nj), dim3(FIN_BLK), 0, E->stream, jobs); break;
         case 2:
-            hipLaunchKernelGGL(E->kn, dim3(E->nblk_n), dim3(E->blk_n), 0, E->stream, jobs,
-                               static_cast<const uint4*>(E->b2n_buf.p));
+            if (E->nblk_n_asm)
+                hipLaunchKernelGGL(E->kn_asm, dim3(E->nblk_n_asm), dim3(E->blk_n), 0, E->stream, jobs,
+                                   static_cast<const uint4*>(E->b2n_buf.p));
+            if (E->nblk_n)
+                hipLaunchKernelGGL(E->kn, dim3(E->nblk_n), dim3(E->blk_n), 0, E->stream, jobs,
+                                   static_cast<const uint4*>(E->b2n_buf.p) + E->nblk_n_asm);
             break;
         default: hipLaunchKernelGGL(k_fin_node<FIN_BLK>, dim3(nj), dim3(FIN_BLK), 0, E->stream, jobs); break;
     }
@@ -550,20 +566,20 @@ ensc_status do_advance(ensc_engine* E, int64_t max_steps, int32_t* done) {
     size_t n_live_blocks = 0;
     for (;;) {
         bool all = true;
-        size_t done_blocks = 0;
         for (uint32_t s = 0; s < E->slots.size(); ++s) {
             Job* J = E->slots[s].get();
-            if (!J) continue;
-            const bool d = job_done(E->h_ctl[s]);
-            all = all && d;
-            if (d) done_blocks += J->dev.nblk_e;
+            if (J) all = all && job_done(E->h_ctl[s]);
         }
         if (all) break;
-        // drop finished jobs from the launch once they hold a quarter of the blocks
+        // drop finished jobs from the launch once they hold a quarter of its
+        // element blocks
+        size_t done_blocks = 0;
+        for (uint32_t s : E->live)
+            if (job_done(E->h_ctl[s])) done_blocks += E->slots[s]->dev.nblk_e;
         if (n_live_blocks && done_blocks * 4 > n_live_blocks) E->dirty = true;
         if (E->dirty) {
             if ((st = relayout(E))) return st;
-            n_live_blocks = E->nblk_e;
+            n_live_blocks = (size_t)E->nblk_pipe + E->nblk_e;
         }
         const bool t = E->timing;
         E->timing = false;
@@ -973,7 +989,7 @@ ensc_status ensc_engine_layout(ensc_engine* E, int64_t* out) {
     out[ENSC_L_STAGED] = E->staged ? 1 : 0;
     out[ENSC_L_PIPE_BLOCKS] = E->nblk_pipe;
     out[ENSC_L_ELEM_BLOCKS] = E->nblk_e;
-    out[ENSC_L_NODE_BLOCKS] = E->nblk_n;
+    out[ENSC_L_NODE_BLOCKS] = E->nblk_n_asm + E->nblk_n;
     out[ENSC_L_LDS_BYTES] = (int64_t)E->lds_bytes;
     out[ENSC_L_ASM_JOBS] = E->n_asm;
     return ENSC_OK;

@@ -945,8 +945,11 @@ __global__ __launch_bounds__(PIPE_BLK) __attribute__((amdgpu_waves_per_eu(WPE)))
     }
 }
 
-template <bool MIXED, int BLK>
-__global__ __launch_bounds__(BLK) void k_node(const JobDev* __restrict__ jobs, const uint4* __restrict__ blkmap) {
+// ASM: the launch covers block-assembly jobs only (mixed engine) and sums
+// their block partials; otherwise the per-corner forces through the ELL.
+template <bool MIXED, int BLK, bool ASM>
+__global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(ASM ? 8 : 1))) void k_node(
+    const JobDev* __restrict__ jobs, const uint4* __restrict__ blkmap) {
 #pragma clang fp contract(off)
     const uint4 bm = blkmap[blockIdx.x];  // (job, block of the job, -, -)
     const JobDev& J = jobs[bm.x];
@@ -971,7 +974,7 @@ __global__ __launch_bounds__(BLK) void k_node(const JobDev* __restrict__ jobs, c
         double u[3] = {p3.b, p4.a, p4.b};
         const double im = p1.b;
         double acc[3] = {0.0, 0.0, 0.0};
-        if (MIXED && J.block_asm) {
+        if (ASM) {
             // block assembly: the node's partials, one per element block that
             // holds it, in ascending block order
             const uint32_t q0 = gld(J.pos_ptr, n), q1 = gld(J.pos_ptr, n + 1);
