@@ -65,10 +65,12 @@ struct ensc_engine {
     DevBuf jobs_buf, b2e_buf, b2n_buf, pe_buf, pn_buf;
     uint32_t nblk_e = 0, nblk_n = 0;
     bool dirty = true;
-    KernelFn ke = nullptr, kn = nullptr, kn_asm = nullptr;
+    KernelFn ke = nullptr, ke_g = nullptr, kn = nullptr, kn_asm = nullptr;
+    uint32_t nblk_g = 0;      // element blocks of jobs that gather their corners from HBM (last in the map)
     uint32_t nblk_n_asm = 0;  // node blocks of block-assembly jobs (first in the map); nblk_n: the rest
     bool staged = false;
-    uint32_t nblk_pipe = 0;  // element blocks of the pipelined jobs (first in the map); nblk_e: the rest
+    uint32_t nblk_pipe = 0;  // element blocks of the pipelined jobs (first in the map); nblk_e: staged
+                             // blocks of k_element (next)
     uint32_t n_asm = 0;      // live jobs using block assembly
     uint32_t pipe_grid = 0;  // the pipelined kernel's block count (2 per CU)
     size_t lds_bytes = 0;
@@ -202,43 +204,55 @@ ensc_status relayout(ensc_engine* E) {
         if (!idle || J->keep_in_layout) E->live.push_back(s);
     }
     const bool mixed = E->precision == ENSC_MIXED;
-    uint32_t lds_nodes = 0;
-    for (uint32_t s : E->live) lds_nodes = std::max(lds_nodes, E->slots[s]->lds_nodes);
-    // LDS staging when every live job's largest block node list fits 64 KiB;
-    // block assembly (mixed) adds a 3-float accumulator per staged node
-    E->staged = E->staging_opt != 2 && (size_t)lds_nodes * 9 * sizeof(double) <= 65536;
-    const bool asm_fits = E->staged && mixed && (size_t)lds_nodes * (9 * sizeof(double) + 12) <= 65536;
-    // The pipelined element pass takes the jobs that use block assembly,
-    // have no amplitude velocity BCs and whose tiles fit its buffers; the
-    // rest run k_element.  The choice depends on the job alone, so a job's
-    // arithmetic is the same in any mix.
-    auto pipe_job = [&](const Job& J) {
-        return E->pipe_opt && asm_fits && J.asm_capable && !J.dev.dyn_bc && E->blk_e == PIPE_BLK &&
-               J.lds_nodes <= (uint32_t)PIPE_CAP;
+    // Every choice below depends on the job alone (and the engine options),
+    // so a job runs the same kernels, with the same arithmetic, alone and in
+    // any mix:
+    //  - staged: its largest block node list fits 64 KiB of LDS (72 B per
+    //    node; the mixed engine adds the block assembly accumulators, 12 B);
+    //  - block assembly (mixed): staged and free of corner conflicts;
+    //  - pipelined pass: block assembly, no amplitude velocity BCs, tiles of
+    //    PIPE_BLK elements and at most PIPE_CAP nodes.
+    const size_t lds_per_node = 9 * sizeof(double) + (mixed ? 12 : 0);
+    auto job_staged = [&](const Job& J) {
+        return E->staging_opt != 2 && (size_t)J.lds_nodes * lds_per_node <= 65536;
+    };
+    auto job_asm = [&](const Job& J) { return mixed && J.asm_capable && job_staged(J); };
+    auto job_pipe = [&](const Job& J) {
+        return E->pipe_opt && job_asm(J) && !J.dev.dyn_bc && E->blk_e == PIPE_BLK && J.lds_nodes <= (uint32_t)PIPE_CAP;
     };
     // block maps: (job, block of the job, staged range) per launched block,
-    // in XCD order; the pipelined jobs' element blocks first
-    std::vector<uint4> lp, lr, lna, ln;
+    // in XCD order; element blocks grouped pipelined | staged | gathered,
+    // node blocks block-assembly | per-corner forces
+    std::vector<uint4> lp, ls, lg, lna, ln;
     std::vector<JobDev> tab;
     std::vector<size_t> pe0, pn0;
     size_t ne_blocks = 0, n_blocks = 0;
+    uint32_t lds_nodes = 0;
+    bool any_asm = false;
+    E->n_asm = 0;
     for (uint32_t i = 0; i < E->live.size(); ++i) {
         Job& J = *E->slots[E->live[i]];
         const JobDev& D = J.dev;
         pe0.push_back(ne_blocks);
         pn0.push_back(n_blocks);
-        std::vector<uint4>& le = pipe_job(J) ? lp : lr;
+        const bool as = job_asm(J);
+        std::vector<uint4>& le = job_pipe(J) ? lp : (job_staged(J) ? ls : lg);
+        if (&le == &ls) {
+            lds_nodes = std::max(lds_nodes, J.lds_nodes);
+            any_asm |= as;
+        }
         for (uint32_t b = 0; b < D.nblk_e; ++b)
             le.push_back(make_uint4(i, b, J.blk_ptr_h[b], J.blk_ptr_h[b + 1] - J.blk_ptr_h[b]));
         ne_blocks += D.nblk_e;
-        // node blocks of block-assembly jobs first, then the others
-        std::vector<uint4>& lnj = (asm_fits && J.asm_capable) ? lna : ln;
+        std::vector<uint4>& lnj = as ? lna : ln;
         for (uint32_t b = 0; b < D.nblk_n; ++b) lnj.push_back(make_uint4(i, b, 0, 0));
         n_blocks += D.nblk_n;
+        J.dev.block_asm = as ? 1 : 0;
+        E->n_asm += as;
     }
-    std::vector<uint4> b2e = xcd_order(lp);
-    {
-        const std::vector<uint4> r = xcd_order(lr);
+    std::vector<uint4> b2e;
+    for (const std::vector<uint4>* l : {&lp, &ls, &lg}) {
+        const std::vector<uint4> r = xcd_order(*l);
         b2e.insert(b2e.end(), r.begin(), r.end());
     }
     std::vector<uint4> b2n = xcd_order(lna);
@@ -247,7 +261,8 @@ ensc_status relayout(ensc_engine* E) {
         b2n.insert(b2n.end(), r.begin(), r.end());
     }
     E->nblk_pipe = (uint32_t)lp.size();
-    E->nblk_e = (uint32_t)lr.size();
+    E->nblk_e = (uint32_t)ls.size();
+    E->nblk_g = (uint32_t)lg.size();
     E->nblk_n_asm = (uint32_t)lna.size();
     E->nblk_n = (uint32_t)ln.size();
     if (!E->jobs_buf.ensure(std::max<size_t>(1, E->live.size()) * sizeof(JobDev)) ||
@@ -256,16 +271,11 @@ ensc_status relayout(ensc_engine* E) {
         !E->pe_buf.ensure(std::max<size_t>(1, ne_blocks) * 5 * sizeof(double)) ||
         !E->pn_buf.ensure(std::max<size_t>(1, n_blocks) * 2 * sizeof(double)))
         return fail(ENSC_ERR_MEM, "layout allocation failed");
-    bool any_asm = false;
-    E->n_asm = 0;
     for (uint32_t i = 0; i < E->live.size(); ++i) {
         Job& J = *E->slots[E->live[i]];
         J.dev.ctl = E->d_ctl + E->live[i];
         J.dev.part_e = static_cast<double*>(E->pe_buf.p) + pe0[i] * 5;
         J.dev.part_n = static_cast<double*>(E->pn_buf.p) + pn0[i] * 2;
-        J.dev.block_asm = (asm_fits && J.asm_capable) ? 1 : 0;
-        any_asm |= J.dev.block_asm != 0;
-        E->n_asm += J.dev.block_asm != 0;
         tab.push_back(J.dev);
     }
     if (!tab.empty())
@@ -275,8 +285,10 @@ ensc_status relayout(ensc_engine* E) {
     if (!b2n.empty())
         HIPCHK(hipMemcpyAsync(E->b2n_buf.p, b2n.data(), b2n.size() * sizeof(uint4), hipMemcpyHostToDevice, E->stream));
     HIPCHK(hipStreamSynchronize(E->stream));
-    E->lds_bytes = E->staged ? (size_t)std::max<uint32_t>(lds_nodes, 1) * (9 * sizeof(double) + (any_asm ? 12 : 0)) : 0;
-    E->ke = mixed ? pick_e<true>(E->blk_e, E->staged, E->waves) : pick_e<false>(E->blk_e, E->staged, E->waves);
+    E->staged = E->nblk_pipe + E->nblk_e > 0;
+    E->lds_bytes = (size_t)std::max<uint32_t>(lds_nodes, 1) * (any_asm ? lds_per_node : 9 * sizeof(double));
+    E->ke = mixed ? pick_e<true>(E->blk_e, true, E->waves) : pick_e<false>(E->blk_e, true, E->waves);
+    E->ke_g = mixed ? pick_e<true>(E->blk_e, false, E->waves) : pick_e<false>(E->blk_e, false, E->waves);
     E->kn = mixed ? pick_n<true, false>(E->blk_n) : pick_n<false, false>(E->blk_n);
     E->kn_asm = pick_n<true, true>(E->blk_n);
     E->dirty = false;
@@ -301,6 +313,9 @@ void launch_phase(ensc_engine* E, int phase) {
             if (E->nblk_e)
                 hipLaunchKernelGGL(E->ke, dim3(E->nblk_e), dim3(E->blk_e), E->lds_bytes, E->stream, jobs,
                                    static_cast<const uint4*>(E->b2e_buf.p) + E->nblk_pipe);
+            if (E->nblk_g)
+                hipLaunchKernelGGL(E->ke_g, dim3(E->nblk_g), dim3(E->blk_e), 0, E->stream, jobs,
+                                   static_cast<const uint4*>(E->b2e_buf.p) + E->nblk_pipe + E->nblk_e);
             break;
         case 1: hipLaunchKernelGGL(k_fin_element<FIN_BLK>, dim3(nj), dim3(FIN_BLK), 0, E->stream, jobs); break;
         case 2:
@@ -579,7 +594,7 @@ ensc_status do_advance(ensc_engine* E, int64_t max_steps, int32_t* done) {
         if (n_live_blocks && done_blocks * 4 > n_live_blocks) E->dirty = true;
         if (E->dirty) {
             if ((st = relayout(E))) return st;
-            n_live_blocks = (size_t)E->nblk_pipe + E->nblk_e;
+            n_live_blocks = (size_t)E->nblk_pipe + E->nblk_e + E->nblk_g;
         }
         const bool t = E->timing;
         E->timing = false;
@@ -988,7 +1003,7 @@ ensc_status ensc_engine_layout(ensc_engine* E, int64_t* out) {
     if (st) return st;
     out[ENSC_L_STAGED] = E->staged ? 1 : 0;
     out[ENSC_L_PIPE_BLOCKS] = E->nblk_pipe;
-    out[ENSC_L_ELEM_BLOCKS] = E->nblk_e;
+    out[ENSC_L_ELEM_BLOCKS] = E->nblk_e + E->nblk_g;
     out[ENSC_L_NODE_BLOCKS] = E->nblk_n_asm + E->nblk_n;
     out[ENSC_L_LDS_BYTES] = (int64_t)E->lds_bytes;
     out[ENSC_L_ASM_JOBS] = E->n_asm;

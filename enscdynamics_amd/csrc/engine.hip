@@ -419,15 +419,9 @@ ensc_status prepare_job(Job& J, int precision, int reorder, int blk_e, int blk_n
     I.W = W;
     I.es = es;
     I.ns = ns;
-    if (mixed) {
-        I.gi32.assign(9 * es, 0.f);
-        I.v032.assign(es, 0.f);
-        for (size_t i = 0; i < Ne; ++i) {
-            const size_t eo = J.e_new2old[i];
-            for (int q = 0; q < 9; ++q) I.gi32[(size_t)q * es + i] = (float)G8[9 * eo + q];
-            I.v032[i] = (float)V0[eo];
-        }
-    } else {
+    // the mixed engine rebuilds Gi and V0 from the reference positions in
+    // its element body; only the FP64 engine stores them
+    if (!mixed) {
         I.gi64.assign(9 * es, 0.0);
         I.v064.assign(es, 0.0);
         for (size_t i = 0; i < Ne; ++i) {
@@ -468,17 +462,15 @@ ensc_status upload_job(Job& J) {
     if ((st = upload(&d_conn, I.conn_d, I.conn_d.size(), A))) return st;
     D.conn = d_conn;
     if (mixed) {
-        float *db, *dv, *ds, *dq, *dp, *df, *dx;
-        if ((st = upload(&db, I.gi32, I.gi32.size(), A))) return st;
-        if ((st = upload(&dv, I.v032, I.v032.size(), A))) return st;
+        float *ds, *dq, *dp, *df, *dx;
         std::vector<float> none;
         if ((st = upload(&ds, none, 6 * es, A))) return st;
         if ((st = upload(&dq, none, 12 * es, A))) return st;
         if ((st = upload(&dp, none, es, A))) return st;
         if ((st = upload(&df, none, 8 * es * 3, A))) return st;
         if ((st = upload(&dx, none, Nn * 4, A))) return st;
-        D.Gi = db;
-        D.V0 = dv;
+        D.Gi = nullptr;
+        D.V0 = nullptr;
         D.sigma = ds;
         D.qia = dq;
         D.epsp = dp;

@@ -65,8 +65,7 @@ struct JobImage {
     size_t es = 0, ns = 0;
     std::vector<uint32_t> conn_d, ell, blk_ptr, blk_nodes, pos_ptr, pos_list;
     std::vector<uint16_t> lconn;
-    std::vector<double> gi64, v064, pos, vu;
-    std::vector<float> gi32, v032;
+    std::vector<double> gi64, v064, pos, vu;  // gi64, v064: FP64 engine
 };
 
 struct Job {

@@ -124,7 +124,7 @@ __device__ __forceinline__ void walsh8_expand(const R c[8], R f[8]) {
 template <class R>
 struct ElemState {
     uint32_t nd[8];  // corner node ids (gathered path) or LDS slots (staged path)
-    R Gi[3][3];
+    R Gi[3][3];      // FP64 engine only (the mixed body rebuilds Gi and V0)
     R V0;
     R s[6];
     R Q[12];
@@ -137,14 +137,17 @@ __device__ __forceinline__ void load_state(const JobDev& J, uint32_t e, bool ful
 #pragma unroll
     for (int k = 0; k < 8; ++k) S.nd[k] = STAGED ? (uint32_t)gld(J.lconn, k * es + e) : gld(J.conn, k * es + e);
     if (!full) return;
-    const R* Gp = static_cast<const R*>(J.Gi);
     const R* Sp = static_cast<const R*>(J.sigma);
     const R* Qp = static_cast<const R*>(J.qia);
+    if (!std::is_same<R, float>::value) {
+        // the mixed body rebuilds Gi and V0 from the reference positions
+        const R* Gp = static_cast<const R*>(J.Gi);
 #pragma unroll
-    for (int i = 0; i < 3; ++i)
+        for (int i = 0; i < 3; ++i)
 #pragma unroll
-        for (int j = 0; j < 3; ++j) S.Gi[i][j] = gld(Gp, (3 * i + j) * es + e);
-    S.V0 = gld(static_cast<const R*>(J.V0), e);
+            for (int j = 0; j < 3; ++j) S.Gi[i][j] = gld(Gp, (3 * i + j) * es + e);
+        S.V0 = gld(static_cast<const R*>(J.V0), e);
+    }
 #pragma unroll
     for (int q = 0; q < 6; ++q) S.s[q] = gld(Sp, q * es + e);
 #pragma unroll

@@ -120,9 +120,11 @@ namespace {
 // Device bytes a job will hold (matches build_job's allocations).
 int64_t footprint(const ModelCopy& c, int precision) {
     const int64_t es = (int64_t)round_up(c.Ne, 256), nn = c.Nn;
-    const int64_t r = precision == ENSC_MIXED ? 4 : 8;
-    // conn, lconn, Gi, V0, sigma, QIA, plastic strain, corner forces
-    const int64_t elem = 32 + 16 + 9 * r + r + 6 * r + 12 * r + r + 24 * r;
+    const bool mixed = precision == ENSC_MIXED;
+    const int64_t r = mixed ? 4 : 8;
+    // conn, lconn, Gi and V0 (FP64 engine), sigma, QIA, plastic strain,
+    // corner forces
+    const int64_t elem = 32 + 16 + (mixed ? 0 : 10 * r) + 6 * r + 12 * r + r + 24 * r;
     // record, reactions, BC mask/values/amplitudes, 8-wide gather table,
     // staging lists (< 2 positions per node); mixed: u compensation and the
     // block-assembly partials and position lists

@@ -46,6 +46,36 @@ def test_batch_alone_equals_in_mix(precision, staging):
     print(f"[batch] {len(jobs)} jobs equal alone and in the mix", flush=True)
 
 
+def test_mixed_batch_of_staged_and_gathered_jobs_alone_equals_in_mix():
+    # without renumbering, a job with shuffled elements has element blocks
+    # whose node lists overflow the LDS budget and gathers its corners from
+    # HBM, while the others stage theirs: each job's kernels and arithmetic
+    # are chosen from the job alone
+    import dataclasses
+    jobs = small_jobs(4, seed=11, plastic_p=0.5)
+    rng = np.random.default_rng(12)
+    m = decks.random_job(rng, 3000, 4000, 0.0, k_lo=150, k_hi=300, name="shuffled")
+    m = dataclasses.replace(m, conn=np.ascontiguousarray(m.conn[rng.permutation(m.n_elems)]))
+    jobs.append(m)
+    b = Batch(jobs, eng.MIXED, reorder=False)
+    b.run()
+    staged = []
+    for i, mi in enumerate(jobs):
+        a = eng.Engine(mi, eng.MIXED, reorder=False)
+        a.init()
+        a.set_current_step(1)
+        a.advance(3)
+        lay = a.layout()
+        print(f"[mix] job {i}: {lay}", flush=True)
+        staged.append(lay["staged"])
+        a.advance(0)
+        s = b.scalars(i)
+        assert s["done"] == 1.0 and s["err"] == 0.0
+        for f in ("u", "v", "sigma", "qia"):
+            assert np.array_equal(b.get(i, f), a.get(f)), (i, f)
+    assert staged[-1] == 0 and any(staged[:-1]), staged
+
+
 def test_batch_fp64_matches_oracle():
     jobs = small_jobs(4, seed=5, plastic_p=0.5)
     b = Batch(jobs, eng.FP64).run()
