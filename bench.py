@@ -53,9 +53,11 @@ def alg_bytes(precision: int, plastic: bool, r: float):
     and write of persistent state as this engine stores it (force exchange,
     staging redundancy and block partials are overhead and not counted).
     Returns (element pass, nodal update) per element."""
-    s = 4 if precision == eng.MIXED else 8
-    # 8 corner ids (16-bit block-local), Gi (9), V0, sigma r+w, QIA r+w, eps_p r+w
-    elem = 16 + 9 * s + s + 12 * s + 24 * s + (2 * s if plastic else 0)
+    mixed = precision == eng.MIXED
+    s = 4 if mixed else 8
+    # 8 corner ids (16-bit block-local), Gi (9) and V0 (FP64 engine; the
+    # mixed body rebuilds them from X), sigma r+w, QIA r+w, eps_p r+w
+    elem = 16 + (0 if mixed else 10 * s) + 12 * s + 24 * s + (2 * s if plastic else 0)
     # node values each element pass reads once: X, v, u
     elem += 72 * r
     if precision == eng.MIXED:
@@ -91,7 +93,7 @@ def c3_section(args, m):
         e.sync()
         ph = e.phase_times()
         e.set_timing(False)
-        res[name] = dict(engine=e, secs=t1 - t0, phases=ph)
+        res[name] = dict(engine=e, secs=t1 - t0, phases=ph, layout=e.layout())
         log(f"{name}: {args.steps} steps in {t1 - t0:.4f} s; per launch (ms): element {ph[0]:.4f}, "
             f"element finalize {ph[1]:.4f}, node {ph[2]:.4f}, node finalize {ph[3]:.4f}")
     u_mix = res["mixed"]["engine"].get("u")
@@ -110,6 +112,7 @@ def c3_section(args, m):
             "element_gbs": be * m.n_elems / (ph[0] * 1e-3) / 1e9,
             "node_gbs": bn * m.n_elems / (ph[2] * 1e-3) / 1e9,
             "step_gbs": (be + bn) * m.n_elems * args.steps / d["secs"] / 1e9,
+            "layout": d["layout"],
         }
         d["engine"].close()
     out["rel_disp_err_mixed_vs_fp64"] = rel_err
@@ -256,7 +259,8 @@ def cpu_baseline(m, steps=100, warm=10):
                       f"{o.threads} OpenMP threads on {cpu}"}
 
 
-KERNEL_KEYS = (("k_element<true", "k_element_mixed"), ("k_element<false", "k_element_fp64"),
+KERNEL_KEYS = (("k_element_pipe", "k_element_pipe_mixed"), ("k_element<true", "k_element_mixed"),
+               ("k_element<false", "k_element_fp64"), ("k_node<true, 256, true>", "k_node_mixed_partials"),
                ("k_node<true", "k_node_mixed"), ("k_node<false", "k_node_fp64"),
                ("k_fin_element", "k_fin_element"), ("k_fin_node", "k_fin_node"), ("k_copy_probe", "k_copy_probe"))
 
@@ -344,6 +348,10 @@ def main() -> int:
     mixed, fp64 = c3["mixed"], c3["fp64"]
     be, _ = mixed["alg_bytes_per_elem"]["element"], None
     achieved = mixed["element_gbs"]
+    piped = mixed["layout"]["pipe_blocks"] > 0
+    ekey = "k_element_pipe_mixed" if piped else "k_element_mixed"
+    ename = ("k_element_pipe (mixed, pipelined LDS staging, block assembly)" if piped
+             else "k_element (mixed, staged, block assembly)")
     out = {
         "metric": METRIC,
         "value": mixed["value"],
@@ -362,8 +370,8 @@ def main() -> int:
                                f"constitutive update, FP64 nodal state, dt and energies)",
                    "elements": m.n_elems, "nodes": m.n_nodes, "graph_ticks": args.graph},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_entry("k_element_mixed"),
-                     "kernel": "k_element (mixed, staged, block assembly)",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_entry(ekey),
+                     "kernel": ename,
                      "alg_bytes_per_launch": be * m.n_elems, "avg_launch_ms": mixed["launch_ms"]["element"]},
         "mixed": mixed,
         "fp64": fp64,

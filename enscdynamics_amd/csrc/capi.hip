@@ -16,6 +16,7 @@
 #include <vector>
 
 #include "engine_host.h"
+#include "../../include/ensc_vtu.h"
 #include "kernels.h"
 
 using namespace ensc;
@@ -74,6 +75,10 @@ struct ensc_engine {
     uint32_t n_asm = 0;      // live jobs using block assembly
     uint32_t pipe_grid = 0;  // the pipelined kernel's block count (2 per CU)
     size_t lds_bytes = 0;
+    // output snapshots of a single-job engine: files go to out_dir (empty:
+    // the caller drains them), numbered from out_count
+    std::string out_dir;
+    int64_t out_count = 0;
     // timing
     bool timing = false;
     std::vector<hipEvent_t> evs;
@@ -568,6 +573,135 @@ ensc_status do_set_step(ensc_engine* E, uint32_t si) {
 
 // Run ticks until every job has reached its step end (or max_steps more
 // steps, or an error flag).  Finished jobs drop out of the layout.
+// ---- output snapshots (single-job engines) ----
+bool output_engine(const ensc_engine* E) { return E->slots.size() == 1 && E->slots[0] && E->h_ctl[0].out_on; }
+
+// The reference's rule for the output interval when none is given
+// (calculate_time_interval, exDyna3D.cpp:1947-1977).
+double reference_interval(double total) {
+    double ti;
+    if (total <= 0.0) {
+        ti = 1e-2;
+    } else {
+        ti = 1.0 / 50.0;
+        const int frames = static_cast<int>(total / ti) + 1;
+        if (frames < 10)
+            ti = total / 10.0;
+        else if (frames > 1000)
+            ti = total / 1000.0;
+    }
+    if (ti > total && total > 0.0) ti = total;
+    return ti;
+}
+
+// Fields of the oldest undrained snapshot of job 0 into host buffers in the
+// caller's numbering.  node: [Nn][12] (x, a, v, u); post: [Ne][13] (k_post).
+ensc_status snapshot_fields(ensc_engine* E, std::vector<double>* node, std::vector<double>* post, double* time) {
+    Job& J = *E->slots[0];
+    const Ctl& c = E->h_ctl[0];
+    if (!c.out_on || c.snap_head - c.snap_tail <= 0) return fail(ENSC_ERR_STATE, "no output snapshot is pending");
+    const int32_t slot = c.snap_tail % c.snap_cap;
+    const size_t Nn = J.model.Nn, Ne = J.model.Ne;
+    if (time) HIPCHK(hipMemcpy(time, J.dev.snap_time + slot, sizeof(double), hipMemcpyDeviceToHost));
+    if (node) {
+        std::vector<double> t(Nn * 12);
+        HIPCHK(hipMemcpy(t.data(), J.dev.snap_node + (size_t)slot * Nn * 12, t.size() * sizeof(double),
+                         hipMemcpyDeviceToHost));
+        node->resize(Nn * 12);
+        for (size_t i = 0; i < Nn; ++i)
+            std::memcpy(node->data() + 12 * (size_t)J.n_new2old[i], t.data() + 12 * i, 12 * sizeof(double));
+    }
+    if (post) {
+        if (!J.d_self) {
+            J.d_self = static_cast<JobDev*>(J.arena.alloc(sizeof(JobDev)));
+            if (!J.d_self) return fail(ENSC_ERR_MEM, "job table allocation failed");
+        }
+        J.self_copy = J.dev;
+        HIPCHK(hipMemcpyAsync(J.d_self, &J.self_copy, sizeof(JobDev), hipMemcpyHostToDevice, E->stream));
+        const uint32_t blocks = (uint32_t)((Ne + 255) / 256);
+        if (E->precision == ENSC_MIXED)
+            hipLaunchKernelGGL(k_post<true>, dim3(blocks), dim3(256), 0, E->stream, J.d_self, slot, J.d_post);
+        else
+            hipLaunchKernelGGL(k_post<false>, dim3(blocks), dim3(256), 0, E->stream, J.d_self, slot, J.d_post);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(E->stream));
+        std::vector<double> t(Ne * 13);
+        HIPCHK(hipMemcpy(t.data(), J.d_post, t.size() * sizeof(double), hipMemcpyDeviceToHost));
+        post->resize(Ne * 13);
+        for (size_t i = 0; i < Ne; ++i)
+            std::memcpy(post->data() + 13 * (size_t)J.e_new2old[i], t.data() + 13 * i, 13 * sizeof(double));
+    }
+    return ENSC_OK;
+}
+
+// output_results' file for the oldest snapshot: the point arrays a, force,
+// v, u, mass, the cell arrays vonMises, Stress, Green-Lagrange Strain
+// (exDyna3D.cpp:228-309), the snapshot's positions and time.
+ensc_status write_snapshot(ensc_engine* E, const char* path) {
+    Job& J = *E->slots[0];
+    std::vector<double> node, post;
+    double time = 0.0;
+    ensc_status st;
+    if ((st = snapshot_fields(E, &node, &post, &time))) return st;
+    const size_t Nn = J.model.Nn, Ne = J.model.Ne;
+    std::vector<double> x(3 * Nn), a(3 * Nn), v(3 * Nn), u(3 * Nn), f(3 * Nn, 0.0), m(Nn);
+    std::vector<double> pos(Nn * POS_REC);
+    HIPCHK(hipMemcpy(pos.data(), J.dev.pos, pos.size() * sizeof(double), hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < Nn; ++i)
+        for (int d = 0; d < 3; ++d) {
+            x[3 * i + d] = node[12 * i + d];
+            a[3 * i + d] = node[12 * i + 3 + d];
+            v[3 * i + d] = node[12 * i + 6 + d];
+            u[3 * i + d] = node[12 * i + 9 + d];
+        }
+    for (size_t i = 0; i < Nn; ++i) {
+        const double im = pos[i * POS_REC + 3];
+        m[J.n_new2old[i]] = im != 0.0 ? 1.0 / im : 0.0;
+    }
+    std::vector<double> vm(Ne), sg(6 * Ne), gl(6 * Ne);
+    for (size_t e = 0; e < Ne; ++e) {
+        vm[e] = post[13 * e];
+        for (int q = 0; q < 6; ++q) {
+            sg[6 * e + q] = post[13 * e + 1 + q];
+            gl[6 * e + q] = post[13 * e + 7 + q];
+        }
+    }
+    // solution_f is never written by the reference (zero, exDyna3D.cpp:179)
+    const ensc_vtu_array pd[5] = {{"a", 3, 0, a.data()},
+                                  {"force", 3, 0, f.data()},
+                                  {"v", 3, 0, v.data()},
+                                  {"u", 3, 0, u.data()},
+                                  {"mass", 1, 0, m.data()}};
+    const ensc_vtu_array cd[3] = {
+        {"vonMises", 1, 0, vm.data()}, {"Stress", 6, 0, sg.data()}, {"Green-Lagrange Strain", 6, 0, gl.data()}};
+    return ensc_vtu_write(path, (uint32_t)Nn, x.data(), (uint32_t)Ne, J.model.conn.data(), time, pd, 5, cd, 3);
+}
+
+ensc_status pop_snapshot(ensc_engine* E) {
+    Ctl& c = E->h_ctl[0];
+    if (!c.out_on || c.snap_head - c.snap_tail <= 0) return fail(ENSC_ERR_STATE, "no output snapshot is pending");
+    c.snap_tail += 1;
+    return write_ctl(E);
+}
+
+// With an output directory: write every pending snapshot as
+// <dir>/solution_<n>.vtu and drop it.  Without one: *full tells whether the
+// ring is full (the job then waits for the caller).
+ensc_status drain_outputs(ensc_engine* E, bool* full) {
+    ensc_status st;
+    Ctl& c = E->h_ctl[0];
+    if (!E->out_dir.empty()) {
+        while (c.snap_head - c.snap_tail > 0) {
+            const std::string path = E->out_dir + "/solution_" + std::to_string(E->out_count) + ".vtu";
+            if ((st = write_snapshot(E, path.c_str()))) return st;
+            ++E->out_count;
+            if ((st = pop_snapshot(E))) return st;
+        }
+    }
+    *full = c.snap_head - c.snap_tail >= c.snap_cap;
+    return ENSC_OK;
+}
+
 ensc_status do_advance(ensc_engine* E, int64_t max_steps, int32_t* done) {
     ensc_status st;
     if ((st = sync(E)) || (st = read_ctl(E))) return st;
@@ -596,14 +730,30 @@ ensc_status do_advance(ensc_engine* E, int64_t max_steps, int32_t* done) {
             if ((st = relayout(E))) return st;
             n_live_blocks = (size_t)E->nblk_pipe + E->nblk_e + E->nblk_g;
         }
+        int64_t n = chunk;
+        if (output_engine(E)) {
+            // output snapshots: drain the ring, and size the chunk so the free
+            // slots last (about t_int / dt_i steps per output)
+            bool full = false;
+            if ((st = drain_outputs(E, &full))) return st;
+            if (full) break;  // the caller drains (no output directory)
+            const Ctl& c = E->h_ctl[0];
+            const double per = c.dt_i > 0.0 ? c.t_int / c.dt_i : 1.0;
+            const double fit = (double)(c.snap_cap - (c.snap_head - c.snap_tail)) * std::max(per, 1.0);
+            n = std::max<int64_t>(1, std::min<int64_t>(n, (int64_t)fit));
+        }
         const bool t = E->timing;
         E->timing = false;
-        st = enqueue_ticks(E, chunk);
+        st = enqueue_ticks(E, n);
         E->timing = t;
         if (st) return st;
         if ((st = read_ctl(E))) return st;
         chunk = std::min<int64_t>(chunk * 2, std::max<int64_t>(256, E->graph_ticks));
         if (E->graph_ticks > 0) chunk = (chunk / E->graph_ticks) * E->graph_ticks;
+    }
+    if (output_engine(E)) {
+        bool full = false;
+        if ((st = drain_outputs(E, &full))) return st;
     }
     bool step_end = true, errs = false;
     for (uint32_t s = 0; s < E->slots.size(); ++s) {
@@ -1008,6 +1158,108 @@ ensc_status ensc_engine_layout(ensc_engine* E, int64_t* out) {
     out[ENSC_L_LDS_BYTES] = (int64_t)E->lds_bytes;
     out[ENSC_L_ASM_JOBS] = E->n_asm;
     return ENSC_OK;
+}
+
+ensc_status ensc_engine_set_output(ensc_engine* E, double interval, int32_t slots, const char* dir) {
+    if (!E) return fail(ENSC_ERR_ARG, "null engine");
+    if (E->slots.size() != 1 || !E->slots[0]) return fail(ENSC_ERR_STATE, "output snapshots need a single-job engine");
+    if (slots < 0 || slots > 64) return fail(ENSC_ERR_ARG, "slots must be 0..64");
+    ensc_status st;
+    if ((st = sync(E)) || (st = read_ctl(E))) return st;
+    Job& J = *E->slots[0];
+    Ctl& c = E->h_ctl[0];
+    if (slots == 0) {
+        c.out_on = 0;
+        c.snap_now = 0;
+        return write_ctl(E);
+    }
+    if ((uint32_t)slots != J.snap_slots) {
+        const size_t Nn = J.model.Nn, Ne = J.model.Ne;
+        J.dev.snap_node = static_cast<double*>(J.arena.alloc((size_t)slots * Nn * 12 * sizeof(double)));
+        J.dev.snap_elem = static_cast<double*>(J.arena.alloc((size_t)slots * Ne * 6 * sizeof(double)));
+        J.dev.snap_time = static_cast<double*>(J.arena.alloc((size_t)slots * sizeof(double)));
+        if (!J.d_post) J.d_post = static_cast<double*>(J.arena.alloc(Ne * 13 * sizeof(double)));
+        if (!J.dev.snap_node || !J.dev.snap_elem || !J.dev.snap_time || !J.d_post)
+            return fail(ENSC_ERR_MEM, "output snapshot allocation failed");
+        J.snap_slots = (uint32_t)slots;
+    }
+    c.out_on = 1;
+    c.t_int = interval > 0.0 ? interval : reference_interval(c.total_time);
+    c.t_out = c.time;  // the next step is written (the reference starts at time 0)
+    c.snap_head = c.snap_tail = 0;
+    c.snap_cap = slots;
+    c.snap_now = 0;
+    E->out_dir = dir ? dir : "";
+    E->out_count = 0;
+    E->dirty = true;  // the job table carries the ring's pointers
+    return write_ctl(E);
+}
+
+ensc_status ensc_engine_output_pending(ensc_engine* E, int32_t* n, double* time) {
+    if (!E || !n) return fail(ENSC_ERR_ARG, "null argument");
+    ensc_status st;
+    if ((st = sync(E)) || (st = read_ctl(E))) return st;
+    *n = output_engine(E) ? E->h_ctl[0].snap_head - E->h_ctl[0].snap_tail : 0;
+    if (time) {
+        *time = 0.0;
+        if (*n > 0) return snapshot_fields(E, nullptr, nullptr, time);
+    }
+    return ENSC_OK;
+}
+
+ensc_status ensc_engine_output_field(ensc_engine* E, const char* field, double* out, int64_t count) {
+    if (!E || !field || !out) return fail(ENSC_ERR_ARG, "null argument");
+    ensc_status st;
+    if ((st = sync(E)) || (st = read_ctl(E))) return st;
+    if (!output_engine(E)) return fail(ENSC_ERR_STATE, "output snapshots are off");
+    const Job& J = *E->slots[0];
+    const size_t Nn = J.model.Nn, Ne = J.model.Ne;
+    const std::string f(field);
+    static const char* node_f[4] = {"x", "a", "v", "u"};
+    for (int q = 0; q < 4; ++q)
+        if (f == node_f[q]) {
+            if (count < (int64_t)(3 * Nn)) return fail(ENSC_ERR_ARG, "buffer too small");
+            std::vector<double> node;
+            if ((st = snapshot_fields(E, &node, nullptr, nullptr))) return st;
+            for (size_t i = 0; i < Nn; ++i)
+                for (int d = 0; d < 3; ++d) out[3 * i + d] = node[12 * i + 3 * q + d];
+            return ENSC_OK;
+        }
+    int off = -1, nc = 0;
+    if (f == "vonMises") {
+        off = 0;
+        nc = 1;
+    } else if (f == "Stress") {
+        off = 1;
+        nc = 6;
+    } else if (f == "Green-Lagrange Strain") {
+        off = 7;
+        nc = 6;
+    } else {
+        return fail(ENSC_ERR_ARG, "unknown output field '" + f + "'");
+    }
+    if (count < (int64_t)(nc * Ne)) return fail(ENSC_ERR_ARG, "buffer too small");
+    std::vector<double> post;
+    if ((st = snapshot_fields(E, nullptr, &post, nullptr))) return st;
+    for (size_t e = 0; e < Ne; ++e)
+        for (int q = 0; q < nc; ++q) out[(size_t)nc * e + q] = post[13 * e + off + q];
+    return ENSC_OK;
+}
+
+ensc_status ensc_engine_output_write(ensc_engine* E, const char* path) {
+    if (!E || !path) return fail(ENSC_ERR_ARG, "null argument");
+    ensc_status st;
+    if ((st = sync(E)) || (st = read_ctl(E))) return st;
+    if (!output_engine(E)) return fail(ENSC_ERR_STATE, "output snapshots are off");
+    return write_snapshot(E, path);
+}
+
+ensc_status ensc_engine_output_pop(ensc_engine* E) {
+    if (!E) return fail(ENSC_ERR_ARG, "null engine");
+    ensc_status st;
+    if ((st = sync(E)) || (st = read_ctl(E))) return st;
+    if (!output_engine(E)) return fail(ENSC_ERR_STATE, "output snapshots are off");
+    return pop_snapshot(E);
 }
 
 ensc_status ensc_engine_set_timing(ensc_engine* E, int32_t on) {

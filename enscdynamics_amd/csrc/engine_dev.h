@@ -67,6 +67,12 @@ struct Ctl {
     long long step_limit;  // no step starts once nsteps reaches it
     int32_t kind, have_lpre, init_pending, step_index;
     uint32_t err, pad;
+    // output snapshots (exDyna3D::output_results, exDyna3D.cpp:207-219):
+    // next output time and interval; a ring of snap_cap slots of which
+    // snap_head have been filled and snap_tail drained by the host; snap_now
+    // is 1 + the slot this tick's nodal update fills (0: none)
+    double t_out, t_int;
+    int32_t out_on, snap_now, snap_head, snap_tail, snap_cap, pad2;
 };
 
 struct JobDev {
@@ -117,6 +123,12 @@ struct JobDev {
     uint32_t probe[MAX_PROBE];
     double* hist;
     long long hist_cap;
+    // output snapshot ring (null when output is off): per slot the nodes'
+    // x, a, v, u as output_results sees them (after compute_a, before the
+    // velocity update), the elements' stress, and the time
+    double* snap_node;  // [slots][nn][12]
+    double* snap_elem;  // [slots][ne][6]
+    double* snap_time;  // [slots]
     Ctl* ctl;
     double* part_e;  // [5][nblk_e]: min L, min dtStable, internal, hourglass, viscous work
     double* part_n;  // [2][nblk_n]: kinetic energy, external power

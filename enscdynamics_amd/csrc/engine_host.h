@@ -96,6 +96,8 @@ struct Job {
     bool pending_step = false;    // started by the serving loop, step 1 not switched in yet
     bool keep_in_layout = false;  // single-job engines keep their job in every launch
     JobDev* d_self = nullptr;     // device copy of this job's table entry (BC kernel)
+    uint32_t snap_slots = 0;      // output snapshot ring allocated (slots)
+    double* d_post = nullptr;     // [ne][13] output fields of one snapshot (k_post)
     JobDev self_copy{};
 };
 

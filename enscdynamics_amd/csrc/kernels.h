@@ -268,6 +268,8 @@ __device__ inline Decision decide(const Ctl& c, double factor, double wos) {
         return d;
     }
     if (!(c.time < c.step_end) || c.nsteps >= c.step_limit) return d;
+    // every output slot holds an undrained snapshot: wait for the host
+    if (c.out_on && c.snap_head - c.snap_tail >= c.snap_cap) return d;
     double dt1 = c.dt_i;
     const double rem = c.total_time - (c.time + dt1);
     // dt_i <= dtStable_min * factor always, so the overshoot clamp of
@@ -506,6 +508,7 @@ __device__ inline void elem_phase_finish(const JobDev& J, const Decision& dc, co
 #pragma clang fp contract(off)
     Ctl& c = *J.ctl;
     const double Lmin = tot[0], dtsn = tot[1];
+    c.snap_now = 0;
     if (dc.kind == KIND_PREPASS) {
         c.lmin_pre = Lmin;
         c.have_lpre = 1;
@@ -542,6 +545,15 @@ __device__ inline void elem_phase_finish(const JobDev& J, const Decision& dc, co
         c.vis += tot[4];
         c.grav_amp = (J.grav_on && J.grav_amp >= 0) ? deal_amp(J, J.grav_amp, t, dt) : 1.0;
         if (!(dtsn > 0.0) || !(dt >= 0.0)) c.err |= (uint32_t)ENSC_EF_DT;
+        // output_results (207-219): this step's state is written when time
+        // reaches the next output time or the total time
+        if (c.out_on && (t >= c.t_out || t >= c.total_time)) {
+            const int32_t slot = c.snap_head % c.snap_cap;
+            c.snap_now = slot + 1;
+            c.snap_head += 1;
+            c.t_out += c.t_int;
+            J.snap_time[slot] = t;
+        }
         c.kind = KIND_STEP;
     }
 }
@@ -959,9 +971,19 @@ __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(ASM ? 8 : 1
     Ctl* ctl = J.ctl;
     if (ctl->kind != KIND_STEP) return;
     const double dt_i = ctl->dt_i, dt_i_1 = ctl->dt_i_1, time = ctl->time;
+    const int32_t snap = ctl->snap_now - 1;  // output slot of this step, or -1
     const uint32_t lb = bm.y;
     const uint32_t n = lb * BLK + threadIdx.x;
     double ke = 0.0, pw = 0.0;
+    if (snap >= 0 && n < J.nn) {
+        // output_results' element stress (this step's, already updated)
+        typedef typename std::conditional<MIXED, float, double>::type R_;
+        double* se = J.snap_elem + (size_t)snap * J.ne * 6;
+        const R_* sg = static_cast<const R_*>(J.sigma);
+        for (uint32_t e = n; e < J.ne; e += J.nn)
+#pragma unroll
+            for (int q = 0; q < 6; ++q) se[(size_t)e * 6 + q] = (double)sg[(size_t)q * J.ne_s + e];
+    }
     if (n < J.nn) {
         // [X0 X1 | X2 im] [v0 v1 | v2 u0 | u1 u2]; the mixed engine never
         // rewrites X (the reference position)
@@ -1088,12 +1110,33 @@ __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(ASM ? 8 : 1
         }
         const double hs = dt_i + dt_i_1;
         double k2 = 0.0;
+        double* sn = snap >= 0 ? J.snap_node + ((size_t)snap * J.nn + n) * 12 : nullptr;
+        if (sn) {
+            // output_results' node state: x and u before this step's update
+            double xs[3] = {x[0], x[1], x[2]};
+            if (MIXED) {
+                const D2 p0 = gld2(J.pos, n * POS_REC);
+                xs[0] = p0.a + u[0];
+                xs[1] = p0.b + u[1];
+                xs[2] = p1.a + u[2];
+            }
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                sn[d] = xs[d];
+                sn[9 + d] = u[d];
+            }
+        }
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
             const double ad = a[d] * im;  // compute_a (642-644)
             double vd = v[d];
             if (mk & (BC_VEL0 << d)) vd = vbc[d];
             if (mk & (BC_SPC0 << d)) vd = 0.0;
+            if (sn) {
+                // a after compute_a; v after the velocity BCs
+                sn[3 + d] = ad;
+                sn[6 + d] = vd;
+            }
             vd += hs * ad / 2.0;  // update_velocity (646-649)
             const double dx = dt_i * vd;
             if (MIXED) {
@@ -1177,6 +1220,98 @@ __global__ void k_apply_bc(JobDev* __restrict__ jobs, uint32_t job, const uint32
         }
     }
     if (set_end && i == 0) J.ctl->step_end = time + period;
+}
+
+// Fields output_results writes for a captured snapshot (exDyna3D.cpp:
+// 228-309, computeVonMises 1111-1133), one element per thread, FP64:
+//   out[0]     von Mises stress of the element's stress;
+//   out[1..6]  the stress in the order handed to VTK: xx yy zz xy yz xz;
+//   out[7..12] Green-Lagrange strain 0.5 (F^T F - I), same order, with
+//              F = I + u B0 from the snapshot's displacements (the F the
+//              step's element pass formed, exDyna3D.cpp:864).
+// The FP64 engine rebuilds B0 from its stored Gi in the element body's
+// order; the mixed engine, which stores no Gi, forms Gi in FP64 from the
+// reference positions (Gi = (XL^-1)^T, XL[j][m] = sum_k X_k[j] c_k[m]).
+template <bool MIXED>
+__global__ void k_post(const JobDev* __restrict__ job, int32_t slot, double* __restrict__ out) {
+#pragma clang fp contract(off)
+    const JobDev& J = *job;
+    const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= J.ne) return;
+    const double* sn = J.snap_node + (size_t)slot * J.nn * 12;
+    const double* se = J.snap_elem + ((size_t)slot * J.ne + e) * 6;
+    uint32_t nd[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) nd[k] = J.conn[(size_t)k * J.ne_s + e];
+    double G[3][3];
+    if (MIXED) {
+        double XL[3][3];
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+#pragma unroll
+            for (int m = 0; m < 3; ++m) {
+                double t = 0.0;
+#pragma unroll
+                for (int k = 0; k < 8; ++k) {
+                    const double X = J.pos[(size_t)nd[k] * POS_REC + j];
+                    t = CS_[k][m] > 0 ? t + X : t - X;
+                }
+                XL[j][m] = t;
+            }
+        double XLi[3][3];
+        inverse3<double>(XL, XLi);
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) G[i][j] = XLi[j][i];
+    } else {
+        const double* Gp = static_cast<const double*>(J.Gi);
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) G[i][j] = Gp[(size_t)(3 * i + j) * J.ne_s + e];
+    }
+    double F[3][3];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        double Bk[3];
+        bgrad<double>(G, k, Bk);
+        const double* uk = sn + (size_t)nd[k] * 12 + 9;
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) F[i][j] = (k == 0) ? uk[i] * Bk[j] : F[i][j] + uk[i] * Bk[j];
+    }
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) F[i][j] = (i == j ? 1.0 : 0.0) + F[i][j];
+    double E[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            const double ftf = F[0][i] * F[0][j] + F[1][i] * F[1][j] + F[2][i] * F[2][j];
+            E[i][j] = 0.5 * (ftf - (i == j ? 1.0 : 0.0));
+        }
+    const double t1 = (se[0] - se[1]) * (se[0] - se[1]);
+    const double t2 = (se[1] - se[2]) * (se[1] - se[2]);
+    const double t3 = (se[2] - se[0]) * (se[2] - se[0]);
+    const double t4 = 6.0 * (se[3] * se[3] + se[4] * se[4] + se[5] * se[5]);
+    double* o = out + (size_t)e * 13;
+    o[0] = sqrt((t1 + t2 + t3 + t4) / 2.0);
+    o[1] = se[0];
+    o[2] = se[1];
+    o[3] = se[2];
+    o[4] = se[3];
+    o[5] = se[5];
+    o[6] = se[4];
+    o[7] = E[0][0];
+    o[8] = E[1][1];
+    o[9] = E[2][2];
+    o[10] = E[0][1];
+    o[11] = E[1][2];
+    o[12] = E[0][2];
 }
 
 }  // namespace ensc

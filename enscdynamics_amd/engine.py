@@ -31,7 +31,9 @@ EXPORTS = [
     "ensc_last_error", "ensc_device_count", "ensc_engine_create", "ensc_engine_destroy", "ensc_engine_init",
     "ensc_engine_set_step", "ensc_engine_advance", "ensc_engine_run", "ensc_engine_enqueue_steps",
     "ensc_engine_sync", "ensc_engine_download", "ensc_engine_scalars", "ensc_engine_history",
-    "ensc_engine_set_graph", "ensc_copy_bandwidth", "ensc_engine_kernel_times", "ensc_engine_phase_times", "ensc_engine_layout", "ensc_engine_set_timing", "ensc_engine_device_bytes",
+    "ensc_engine_set_graph", "ensc_copy_bandwidth", "ensc_engine_kernel_times", "ensc_engine_phase_times", "ensc_engine_layout", "ensc_engine_set_timing",
+    "ensc_engine_set_output", "ensc_engine_output_pending", "ensc_engine_output_field", "ensc_engine_output_write",
+    "ensc_engine_output_pop", "ensc_vtu_write", "ensc_engine_device_bytes",
 ]
 
 
@@ -83,6 +85,11 @@ def load_library(path: str = LIB_PATH):
         ("ensc_engine_kernel_times", [vp, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
         ("ensc_engine_phase_times", [vp, C.c_void_p]),
         ("ensc_engine_layout", [vp, C.c_void_p]),
+        ("ensc_engine_set_output", [vp, C.c_double, C.c_int32, C.c_char_p]),
+        ("ensc_engine_output_pending", [vp, C.POINTER(C.c_int32), C.POINTER(C.c_double)]),
+        ("ensc_engine_output_field", [vp, C.c_char_p, C.c_void_p, C.c_int64]),
+        ("ensc_engine_output_write", [vp, C.c_char_p]),
+        ("ensc_engine_output_pop", [vp]),
         ("ensc_engine_set_timing", [vp, C.c_int32]),
         ("ensc_engine_set_graph", [vp, C.c_int32]),
     ]:
@@ -211,6 +218,32 @@ class Engine:
         _check(_lib.ensc_engine_layout(self.h, out.ctypes.data))
         keys = ("staged", "pipe_blocks", "elem_blocks", "node_blocks", "lds_bytes", "asm_jobs")
         return {k: int(v) for k, v in zip(keys, out)}
+
+    # --- output snapshots (exDyna3D::output_results) ---
+    def set_output(self, interval: float = 0.0, slots: int = 4, directory: Optional[str] = None) -> None:
+        """Capture output_results' state at every output time (interval <= 0:
+        the reference's rule) into `slots` device slots; with a directory,
+        advance/run write <directory>/solution_<n>.vtu as they go."""
+        _check(_lib.ensc_engine_set_output(self.h, float(interval), int(slots),
+                                           directory.encode() if directory else None))
+
+    def output_pending(self) -> Tuple[int, float]:
+        n, t = C.c_int32(), C.c_double()
+        _check(_lib.ensc_engine_output_pending(self.h, C.byref(n), C.byref(t)))
+        return int(n.value), float(t.value)
+
+    def output_field(self, name: str) -> np.ndarray:
+        shapes = {"x": (self.Nn, 3), "a": (self.Nn, 3), "v": (self.Nn, 3), "u": (self.Nn, 3),
+                  "vonMises": (self.Ne,), "Stress": (self.Ne, 6), "Green-Lagrange Strain": (self.Ne, 6)}
+        out = np.zeros(shapes[name])
+        _check(_lib.ensc_engine_output_field(self.h, name.encode(), out.ctypes.data, out.size))
+        return out
+
+    def output_write(self, path: str) -> None:
+        _check(_lib.ensc_engine_output_write(self.h, path.encode()))
+
+    def output_pop(self) -> None:
+        _check(_lib.ensc_engine_output_pop(self.h))
 
     def get(self, name: str) -> np.ndarray:
         shapes = {"u": (self.Nn, 3), "v": (self.Nn, 3), "x": (self.Nn, 3), "rf": (self.Nn, 3),

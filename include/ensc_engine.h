@@ -143,6 +143,33 @@ enum {
     ENSC_L_ASM_JOBS, ENSC_L_COUNT
 };
 ensc_status ensc_engine_layout(ensc_engine* eng, int64_t* out /* ENSC_L_COUNT */);
+/* ---- output snapshots: exDyna3D::output_results (exDyna3D.cpp:207-336) ----
+ * Single-job engines.  At every step whose time reaches the next output time
+ * (the first step after enabling, then every `interval`) or the total time,
+ * the device captures what output_results writes: node positions, a (after
+ * compute_a), v (after the velocity BCs) and u, all before the step's
+ * velocity update, and the element stress.  Snapshots go to a ring of
+ * `slots` device buffers (1..64; 0 turns output off).  interval <= 0 takes
+ * the reference's rule (calculate_time_interval, exDyna3D.cpp:1947-1977).
+ * With `dir` set, ensc_engine_advance / ensc_engine_run write each snapshot
+ * as <dir>/solution_<n>.vtu (n = 0, 1, ...; include/ensc_vtu.h) as they
+ * go, like the reference's ResultData/solution_<n>.vtu.  With dir NULL the
+ * caller drains the ring (output_pending / output_field / output_write /
+ * output_pop); a job waits while every slot is full, and advance returns
+ * with *done = 0 so the caller can drain. */
+ensc_status ensc_engine_set_output(ensc_engine* eng, double interval, int32_t slots, const char* dir);
+/* Undrained snapshots and the time of the oldest (time may be NULL). */
+ensc_status ensc_engine_output_pending(ensc_engine* eng, int32_t* n, double* time);
+/* A field of the oldest snapshot, caller's numbering: "x", "a", "v", "u"
+ * [n_nodes][3]; "vonMises" [n_elems]; "Stress", "Green-Lagrange Strain"
+ * [n_elems][6] in the order the reference hands to VTK (xx yy zz xy yz xz).
+ * Green-Lagrange strain is 0.5 (F^T F - I) with the step's F = I + u B0. */
+ensc_status ensc_engine_output_field(ensc_engine* eng, const char* field, double* out, int64_t count);
+/* Write the oldest snapshot as a .vtu file with the reference's arrays. */
+ensc_status ensc_engine_output_write(ensc_engine* eng, const char* path);
+/* Drop the oldest snapshot. */
+ensc_status ensc_engine_output_pop(ensc_engine* eng);
+
 /* Enable (1) / disable (0) per-kernel event timing for later enqueue calls. */
 ensc_status ensc_engine_set_timing(ensc_engine* eng, int32_t on);
 /* Device bytes held by the engine. */

@@ -48,6 +48,14 @@ double oracle_time_steps(void* h, int64_t n_steps);
 int64_t oracle_get(void* h, const char* field, double* out);
 void oracle_scalars(void* h, double* out /* OS_COUNT */);
 int oracle_threads(void* h);
+/* output_results (exDyna3D.cpp:207-336): record a snapshot at every output
+ * time (interval <= 0: calculate_time_interval's rule, 1947-1977). */
+void oracle_set_output(void* h, double interval);
+int64_t oracle_snapshots(void* h);
+/* Field of snapshot k: "time" [1]; "x","a","v","u" [Nn][3]; "vonMises" [Ne];
+ * "Stress", "Green-Lagrange Strain" [Ne][6] (xx yy zz xy yz xz).  Returns
+ * the count written or -1. */
+int64_t oracle_snapshot(void* h, int64_t k, const char* field, double* out);
 
 #ifdef __cplusplus
 }

@@ -47,6 +47,11 @@ def lib():
         L.oracle_scalars.argtypes = [C.c_void_p, C.c_void_p]
         L.oracle_threads.restype = C.c_int
         L.oracle_threads.argtypes = [C.c_void_p]
+        L.oracle_set_output.argtypes = [C.c_void_p, C.c_double]
+        L.oracle_snapshots.restype = C.c_int64
+        L.oracle_snapshots.argtypes = [C.c_void_p]
+        L.oracle_snapshot.restype = C.c_int64
+        L.oracle_snapshot.argtypes = [C.c_void_p, C.c_int64, C.c_char_p, C.c_void_p]
         _lib = L
     return _lib
 
@@ -90,6 +95,22 @@ class Oracle:
         n = lib().oracle_get(self.h, name.encode(), out.ctypes.data)
         if n < 0:
             raise KeyError(name)
+        return out
+
+    def set_output(self, interval: float = 0.0) -> "Oracle":
+        """Record output_results snapshots (interval <= 0: the reference's rule)."""
+        lib().oracle_set_output(self.h, float(interval))
+        return self
+
+    def snapshots(self) -> int:
+        return int(lib().oracle_snapshots(self.h))
+
+    def snapshot(self, k: int, name: str) -> np.ndarray:
+        shapes = {"time": (1,), "x": (self.Nn, 3), "a": (self.Nn, 3), "v": (self.Nn, 3), "u": (self.Nn, 3),
+                  "vonMises": (self.Ne,), "Stress": (self.Ne, 6), "Green-Lagrange Strain": (self.Ne, 6)}
+        out = np.zeros(shapes[name])
+        if lib().oracle_snapshot(self.h, int(k), name.encode(), out.ctypes.data) < 0:
+            raise KeyError((k, name))
         return out
 
     def scalars(self) -> Dict[str, float]:

@@ -64,6 +64,7 @@ struct Sim {
     std::vector<double> epsp;          // [Ne]
     std::vector<double> dtStable, Length;
     std::vector<double> fel;           // [Ne][8][3] per-element nodal forces
+    std::vector<double> Fel;           // [Ne][3][3] F of the last element pass (all_F, exDyna3D.h:153)
     std::vector<uint32_t> csr_ptr, csr_slot;  // node -> (8*e + k), sorted by (k, e)
     double time = 0, totalTime = 0, dt_i = 0, dt_i_1 = 0;
     size_t currentStepIndex = 0;
@@ -197,6 +198,7 @@ struct Sim {
         dtStable.assign(Ne, 0.0);
         Length.assign(Ne, 0.0);
         fel.assign(24 * Ne, 0.0);
+        Fel.assign(9 * Ne, 0.0);
         BT.assign(24 * Ne, 0.0);
         V0.assign(Ne, 0.0);
         std::vector<double> mass(Nn, 0.0);
@@ -301,6 +303,8 @@ struct Sim {
                 for (int k = 1; k < 8; ++k) s += ue[i][k] * B[3 * k + j];
                 F[i][j] = (i == j ? 1.0 : 0.0) + s;
             }
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) Fel[9 * e + 3 * i + j] = F[i][j];
         const double J = det3(F);
         double Fi[3][3];
         inverse3(F, Fi);
@@ -676,6 +680,66 @@ struct Sim {
     }
     double last_lmin = 0.0;
 
+    // output_results (207-336), called by computeSate after compute_a: at
+    // an output time (or the total time) it records what the reference hands
+    // to DataOut: node positions, a (= the summed force times invM,
+    // compute_a 642-644), v and u, and per element the von Mises stress
+    // (computeVonMises 1111-1133), the stress in VTK order and the
+    // Green-Lagrange strain 0.5 (F^T F - I) of the pass's F (300-318).
+    struct Snapshot {
+        double time;
+        std::vector<double> node;  // [Nn][12]: x, a, v, u
+        std::vector<double> post;  // [Ne][13]: von Mises, stress (6), GL strain (6)
+    };
+    bool out_on = false;
+    double time_output = 0.0, time_interval = 0.0;
+    std::vector<Snapshot> snaps;
+    void output_results() {
+        if (!out_on) return;
+        if (!(time >= time_output) && !(time >= totalTime)) return;
+        time_output += time_interval;
+        Snapshot S;
+        S.time = time;
+        S.node.resize(12 * Nn);
+        for (size_t n = 0; n < Nn; ++n)
+            for (int d = 0; d < 3; ++d) {
+                S.node[12 * n + d] = x[3 * n + d];
+                S.node[12 * n + 3 + d] = a[3 * n + d] * invM[n];
+                S.node[12 * n + 6 + d] = v[3 * n + d];
+                S.node[12 * n + 9 + d] = u[3 * n + d];
+            }
+        S.post.resize(13 * Ne);
+        for (size_t e = 0; e < Ne; ++e) {
+            const double* sg = &sigma[6 * e];
+            const double t1 = (sg[0] - sg[1]) * (sg[0] - sg[1]);
+            const double t2 = (sg[1] - sg[2]) * (sg[1] - sg[2]);
+            const double t3 = (sg[2] - sg[0]) * (sg[2] - sg[0]);
+            const double t4 = 6.0 * (sg[3] * sg[3] + sg[4] * sg[4] + sg[5] * sg[5]);
+            double* o = &S.post[13 * e];
+            o[0] = std::sqrt((t1 + t2 + t3 + t4) / 2.0);
+            o[1] = sg[0];
+            o[2] = sg[1];
+            o[3] = sg[2];
+            o[4] = sg[3];
+            o[5] = sg[5];  // yz
+            o[6] = sg[4];  // xz
+            const double* F = &Fel[9 * e];
+            double E[3][3];
+            for (int i = 0; i < 3; ++i)
+                for (int j = 0; j < 3; ++j) {
+                    const double ftf = F[0 + i] * F[0 + j] + F[3 + i] * F[3 + j] + F[6 + i] * F[6 + j];
+                    E[i][j] = 0.5 * (ftf - (i == j ? 1.0 : 0.0));
+                }
+            o[7] = E[0][0];
+            o[8] = E[1][1];
+            o[9] = E[2][2];
+            o[10] = E[0][1];
+            o[11] = E[1][2];
+            o[12] = E[0][2];
+        }
+        snaps.push_back(std::move(S));
+    }
+
     // computeSate (774-793).
     void computeSate() {
         time += dt_i_1;
@@ -684,6 +748,7 @@ struct Sim {
         add_inForce_to_rhs(dt_i_1);
         apply_gravity();
         apply_bc_a(0.5 * (dt_i + dt_i_1));
+        output_results();
         nodal_update();
         ++nsteps;
     }
@@ -809,5 +874,64 @@ void oracle_scalars(void* h, double* o) {
 }
 
 int oracle_threads(void* h) { return static_cast<Sim*>(h)->nthreads; }
+
+void oracle_set_output(void* h, double interval) {
+    Sim* s = static_cast<Sim*>(h);
+    s->out_on = true;
+    s->time_output = 0.0;  // init (exDyna3D.cpp:107)
+    double ti = interval;
+    if (!(ti > 0.0)) {
+        // calculate_time_interval (1947-1977)
+        const double T = s->totalTime;
+        if (T <= 0.0) {
+            ti = 1e-2;
+        } else {
+            ti = 1.0 / 50.0;
+            const int frames = static_cast<int>(T / ti) + 1;
+            if (frames < 10)
+                ti = T / 10.0;
+            else if (frames > 1000)
+                ti = T / 1000.0;
+        }
+        if (ti > T && T > 0.0) ti = T;
+    }
+    s->time_interval = ti;
+    s->snaps.clear();
+}
+
+int64_t oracle_snapshots(void* h) { return (int64_t)static_cast<Sim*>(h)->snaps.size(); }
+
+int64_t oracle_snapshot(void* h, int64_t k, const char* f, double* out) {
+    Sim* s = static_cast<Sim*>(h);
+    if (k < 0 || k >= (int64_t)s->snaps.size()) return -1;
+    const Sim::Snapshot& S = s->snaps[(size_t)k];
+    if (!std::strcmp(f, "time")) {
+        out[0] = S.time;
+        return 1;
+    }
+    static const char* node_f[4] = {"x", "a", "v", "u"};
+    for (int q = 0; q < 4; ++q)
+        if (!std::strcmp(f, node_f[q])) {
+            for (size_t n = 0; n < s->Nn; ++n)
+                for (int d = 0; d < 3; ++d) out[3 * n + d] = S.node[12 * n + 3 * q + d];
+            return (int64_t)(3 * s->Nn);
+        }
+    int off = -1, nc = 0;
+    if (!std::strcmp(f, "vonMises")) {
+        off = 0;
+        nc = 1;
+    } else if (!std::strcmp(f, "Stress")) {
+        off = 1;
+        nc = 6;
+    } else if (!std::strcmp(f, "Green-Lagrange Strain")) {
+        off = 7;
+        nc = 6;
+    } else {
+        return -1;
+    }
+    for (size_t e = 0; e < s->Ne; ++e)
+        for (int q = 0; q < nc; ++q) out[(size_t)nc * e + q] = S.post[13 * e + off + q];
+    return (int64_t)(nc * s->Ne);
+}
 
 }  // extern "C"

@@ -1,0 +1,100 @@
+"""Output snapshots on the GPU (exDyna3D::output_results): the state the
+reference writes at each output time, captured on the device, the
+post-processed fields and the .vtu files, against the CPU oracle."""
+import os
+
+import numpy as np
+import pytest
+
+from enscdynamics_amd import decks
+from enscdynamics_amd import engine as eng
+from enscdynamics_amd.vtu import read_vtu
+from oracle.oracle import Oracle
+
+pytestmark = pytest.mark.gpu
+
+NODE_F = ("x", "a", "v", "u")
+ELEM_F = ("vonMises", "Stress", "Green-Lagrange Strain")
+
+
+@pytest.fixture(scope="module")
+def c1():
+    m = decks.config1()
+    o = Oracle(m, 0).set_output(0.0).init()
+    o.run(0, [], 0)
+    return m, o
+
+
+def drain(e):
+    """Advance a job with caller-drained output slots; collect every snapshot."""
+    snaps = []
+    e.init()
+    for si in range(1, len(e.model.steps)):
+        e.set_current_step(si)
+        while True:
+            done = e.advance(0)
+            n, t = e.output_pending()
+            while n:
+                snaps.append({"time": t, **{f: e.output_field(f) for f in NODE_F + ELEM_F}})
+                e.output_pop()
+                n, t = e.output_pending()
+            if done:
+                break
+    return snaps
+
+
+def test_fp64_snapshots_equal_the_oracle(c1):
+    m, o = c1
+    e = eng.Engine(m, eng.FP64, reorder=True)
+    e.set_output(0.0, slots=2)
+    snaps = drain(e)
+    assert len(snaps) == o.snapshots()
+    worst = 0.0
+    for k, s in enumerate(snaps):
+        assert s["time"] == o.snapshot(k, "time")[0], k
+        for f in NODE_F + ELEM_F:
+            ref = o.snapshot(k, f)
+            d = np.abs(s[f] - ref).max() / max(np.abs(ref).max(), 1e-300)
+            worst = max(worst, d)
+            assert d <= 1e-12, (k, f, d)
+    print(f"[output fp64] {len(snaps)} snapshots, max rel diff {worst:.2e}", flush=True)
+
+
+def test_vtu_files_written_during_the_run(c1, tmp_path):
+    # one slot: the job waits for every file to be written before going on
+    m, o = c1
+    e = eng.Engine(m, eng.FP64)
+    e.set_output(0.0, slots=1, directory=str(tmp_path))
+    e.run()
+    files = sorted(os.listdir(tmp_path), key=lambda s: int(s.split("_")[1].split(".")[0]))
+    assert files == [f"solution_{k}.vtu" for k in range(o.snapshots())]
+    for k in (0, len(files) // 2, len(files) - 1):
+        r = read_vtu(str(tmp_path / files[k]))
+        assert r["time"] == np.float32(o.snapshot(k, "time")[0])
+        assert r["n_points"] == m.n_nodes and r["n_cells"] == m.n_elems
+        assert np.array_equal(r["connectivity"].reshape(-1, 8), m.conn.astype(np.int64))
+        assert np.allclose(r["points"], o.snapshot(k, "x").astype(np.float32), rtol=1e-7, atol=0)
+        assert list(r["point_data"]) == ["a", "force", "v", "u", "mass"]
+        assert list(r["cell_data"]) == list(ELEM_F)
+        for f in ("a", "v", "u"):
+            assert np.allclose(r["point_data"][f], o.snapshot(k, f).astype(np.float32), rtol=1e-6, atol=0), f
+        assert not np.any(r["point_data"]["force"])
+        assert np.allclose(r["point_data"]["mass"], (1.0 / o.get("invM")).astype(np.float32), rtol=1e-6)
+        for f in ELEM_F:
+            ref = o.snapshot(k, f).astype(np.float32).reshape(r["cell_data"][f].shape)
+            assert np.allclose(r["cell_data"][f], ref, rtol=1e-5, atol=1e-6 * np.abs(ref).max()), f
+    print(f"[output vtu] {len(files)} files", flush=True)
+
+
+def test_mixed_snapshots_within_tolerance(c1):
+    m, o = c1
+    e = eng.Engine(m, eng.MIXED, reorder=True)
+    e.set_output(0.0, slots=4)
+    snaps = drain(e)
+    assert len(snaps) == o.snapshots()
+    k = len(snaps) - 1
+    for f in ("x", "u", "Stress", "vonMises", "Green-Lagrange Strain"):
+        ref = o.snapshot(k, f)
+        d = np.abs(snaps[k][f] - ref).max() / np.abs(ref).max()
+        print(f"[output mixed] {f}: {d:.2e}", flush=True)
+        assert d <= 1e-4 if f in ("x", "u") else d <= 1e-3, (f, d)
