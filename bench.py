@@ -19,7 +19,7 @@ Also in the same JSON line:
   * cpu_baseline: the FP64 restatement (oracle/) timed on the host's cores on
     a bounded sample of configuration 3.
 
-    python bench.py [--gpus 1] [--steps 200] [--warmup 20] [--sections c3,c4,c5]
+    python bench.py [--gpus 1] [--steps 200] [--warmup 20] [--sections c3,out,c4,c5]
 """
 from __future__ import annotations
 
@@ -117,6 +117,54 @@ def c3_section(args, m):
         d["engine"].close()
     out["rel_disp_err_mixed_vs_fp64"] = rel_err
     return out
+
+
+def out_section(args, m, ms_per_step):
+    """Cost of output_results on the device engine (mixed, configuration 3):
+    snapshot capture inside the step, post-processing (von Mises, stress in
+    VTK order, Green-Lagrange strain) with the copy to the host, and writing
+    one .vtu file."""
+    import tempfile
+
+    res = {}
+    e = eng.Engine(m, eng.MIXED, reorder=True)
+    if args.graph:
+        e.set_graph(args.graph)
+    e.init()
+    e.set_current_step(1)
+    e.enqueue(args.warmup)
+    e.sync()
+    dt = e.scalars()["dt_i"]
+    e.set_output(10.5 * dt, slots=4)  # a snapshot every 10-11 steps
+    e.enqueue(2)  # the layout and graph pick up the output ring; one snapshot
+    e.sync()
+    while e.output_pending()[0]:
+        e.output_pop()
+    n = 40
+    t0 = time.perf_counter()
+    e.enqueue(n)
+    e.sync()
+    t_out = time.perf_counter() - t0
+    k, _ = e.output_pending()
+    res["steps"] = n
+    res["snapshots"] = k
+    res["ms_per_step_with_output"] = 1e3 * t_out / n
+    res["capture_ms_per_snapshot"] = (1e3 * t_out - n * ms_per_step) / max(k, 1)
+    t0 = time.perf_counter()
+    for f in ("vonMises", "Stress", "Green-Lagrange Strain", "x", "a", "v", "u"):
+        e.output_field(f)
+    res["fields_to_host_ms"] = 1e3 * (time.perf_counter() - t0)
+    with tempfile.TemporaryDirectory() as d:
+        p = os.path.join(d, "solution_0.vtu")
+        t0 = time.perf_counter()
+        e.output_write(p)
+        res["vtu_write_ms"] = 1e3 * (time.perf_counter() - t0)
+        res["vtu_bytes"] = os.path.getsize(p)
+    e.close()
+    log(f"out: {k} snapshots in {n} steps, capture {res['capture_ms_per_snapshot']:.3f} ms each, "
+        f"fields to host {res['fields_to_host_ms']:.1f} ms, .vtu {res['vtu_bytes'] / 1e6:.0f} MB in "
+        f"{res['vtu_write_ms']:.0f} ms")
+    return res
 
 
 def traffic_entry(kernel_key: str):
@@ -313,7 +361,7 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--kernel-steps", type=int, default=50, help="steps timed per launch with events")
-    ap.add_argument("--sections", default="c3,c4,c5", help="comma list of c3, c4, c5")
+    ap.add_argument("--sections", default="c3,out,c4,c5", help="comma list of c3, out, c4, c5")
     ap.add_argument("--graph", type=int, default=20, help="step iterations per captured hipGraph (0: none)")
     ap.add_argument("--c4-jobs", type=int, default=256)
     ap.add_argument("--c5-jobs", type=int, default=48)
@@ -379,6 +427,8 @@ def main() -> int:
         "rel_disp_err_mixed_vs_fp64": c3["rel_disp_err_mixed_vs_fp64"],
         "copy_probe_gbs": copy_gbs,
     }
+    if "out" in sections:
+        out["output"] = out_section(args, m, mixed["ms_per_step"])
     if "c4" in sections:
         out["batched"] = c4_section(args)
     if "c5" in sections:

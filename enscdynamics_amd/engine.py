@@ -122,6 +122,17 @@ def copy_bandwidth(nbytes: int = 1 << 30, reps: int = 10, device: int = 0) -> fl
     return g.value
 
 
+def write_history_csv(path: str, rows: np.ndarray, probes: Sequence[int]) -> int:
+    """CSV of history rows laid out as ensc_engine_history returns them."""
+    cols = list(H_NAMES) + [f"u{c}_node{p}" for p in probes for c in "xyz"]
+    rows = np.asarray(rows, dtype=np.float64).reshape(-1, len(cols))
+    with open(path, "w") as f:
+        f.write(",".join(["step"] + cols) + "\n")
+        for i, r in enumerate(rows):
+            f.write(",".join([str(i + 1)] + [repr(float(v)) for v in r]) + "\n")
+    return rows.shape[0]
+
+
 class Engine:
     """One job resident on the GPU, driven like exDyna3D."""
 
@@ -264,6 +275,12 @@ class Engine:
         got = C.c_int64()
         _check(_lib.ensc_engine_history(self.h, out.ctypes.data if rows else None, rows, C.byref(got)))
         return out[: got.value]
+
+    def write_history_csv(self, path: str) -> int:
+        """Per-step history as CSV (one row per step: time, dt_i, dt_i_1,
+        min dtStable, kinetic, internal, hourglass and viscous energy,
+        external work, then u of each probe node); returns the rows."""
+        return write_history_csv(path, self.history(), [int(p) for p in self._probes])
 
     def device_bytes(self) -> int:
         return int(_lib.ensc_engine_device_bytes(self.h))

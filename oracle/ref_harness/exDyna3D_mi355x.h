@@ -10,6 +10,8 @@
 // one that is built and tested.
 #pragma once
 
+#include <sys/stat.h>
+
 #include <cstdint>
 #include <stdexcept>
 #include <string>
@@ -23,7 +25,10 @@ namespace EnSC {
 
 class exDyna3D_MI355X : public exDyna3D {
 public:
-    explicit exDyna3D_MI355X(int precision = ENSC_FP64) : precision_(precision) {}
+    // write_results: output_results' files (ResultData/solution_<n>.vtu,
+    // exDyna3D.cpp:207-336) from device snapshots, as the reference writes them
+    explicit exDyna3D_MI355X(int precision = ENSC_FP64, bool write_results = true)
+        : precision_(precision), write_results_(write_results) {}
 
     // exDyna3D::run (exDyna3D.cpp:54-102) on the GPU.
     void run() override {
@@ -137,6 +142,13 @@ public:
         ensc_engine* eng = nullptr;
         check(ensc_engine_create(&m, &o, &eng));
         try {
+            if (write_results_) {
+                // the output interval as run() sets it (exDyna3D.cpp:65,
+                // 1947-1977); the files go where DataOut::write_vtu puts them
+                calculate_time_interval();
+                ::mkdir("ResultData", 0755);
+                check(ensc_engine_set_output(eng, time_interval, 4, "ResultData"));
+            }
             check(ensc_engine_run(eng));
             // --- results back into the reference's members ---
             std::vector<double> buf(3 * Nn);
@@ -170,6 +182,7 @@ public:
 
 protected:
     int precision_;
+    bool write_results_;
     long steps_run_ = 0;
     static void check(ensc_status s) {
         if (s != ENSC_OK) throw std::runtime_error(std::string("EnSC-MI355X: ") + ensc_last_error());

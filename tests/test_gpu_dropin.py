@@ -66,6 +66,31 @@ def test_reference_reader_with_gpu_step_loop_mixed(case, tmp_path):
     assert err <= 1e-4
 
 
+def test_dropin_writes_the_reference_result_files(tmp_path):
+    # exDyna3D_MI355X::run also writes output_results' files,
+    # ResultData/solution_<n>.vtu, from device snapshots: as many as the
+    # reference's cadence gives, each holding the reference's state of that
+    # step (the oracle's output snapshots), in Float32 as VTK stores it
+    from enscdynamics_amd.vtu import read_vtu
+    from oracle.oracle import Oracle
+    case = "ref_test_step"
+    d, u, s, x, steps = run_dropin(case, "fp64", tmp_path)
+    o = Oracle(G.model_from_dump(d), 0).set_output(0.0).init()
+    o.run(0, [], 0)
+    files = os.listdir(tmp_path / "ResultData")
+    print(f"[dropin output] {case}: {len(files)} files, oracle {o.snapshots()}", flush=True)
+    assert sorted(files) == sorted(f"solution_{k}.vtu" for k in range(o.snapshots()))
+    for k in (0, o.snapshots() // 2, o.snapshots() - 1):
+        r = read_vtu(str(tmp_path / "ResultData" / f"solution_{k}.vtu"))
+        assert r["time"] == pytest.approx(o.snapshot(k, "time")[0], rel=1e-7)
+        for f in ("u", "v", "a"):
+            ref = o.snapshot(k, f)
+            assert np.allclose(r["point_data"][f], ref, rtol=1e-6, atol=1e-6 * max(np.abs(ref).max(), 1e-30)), f
+        for f in ("vonMises", "Stress", "Green-Lagrange Strain"):
+            ref = o.snapshot(k, f).reshape(r["cell_data"][f].shape)
+            assert np.allclose(r["cell_data"][f], ref, rtol=1e-5, atol=1e-6 * max(np.abs(ref).max(), 1e-30)), f
+
+
 def test_mixed_long_tiny_strain_run_stays_near_fp64(tmp_path):
     # Job-6: 40,004 steps of ~1e-7 strain.  FP32 element arithmetic and
     # state (sigma, QIA) carry rounding that this long, lightly damped run

@@ -118,3 +118,14 @@ def test_oracle_output_is_the_state_before_the_update(c1_outputs):
     assert np.array_equal(o.snapshot(k, "u") + sc["dt_i"] * v_end, o.get("u"))
     sg = o.get("sigma")
     assert np.array_equal(o.snapshot(k, "Stress"), sg[:, [0, 1, 2, 3, 5, 4]])
+
+
+def test_history_csv_layout(tmp_path):
+    rows = np.arange(2 * (len(eng.H_NAMES) + 6), dtype=np.float64).reshape(2, -1) * 1e-3
+    p = str(tmp_path / "h.csv")
+    assert eng.write_history_csv(p, rows, [7, 9]) == 2
+    lines = open(p).read().splitlines()
+    assert lines[0].split(",")[:3] == ["step", "time", "dt_i"]
+    assert lines[0].split(",")[-3:] == ["ux_node9", "uy_node9", "uz_node9"]
+    back = np.array([[float(v) for v in ln.split(",")[1:]] for ln in lines[1:]])
+    assert np.array_equal(back, rows)
