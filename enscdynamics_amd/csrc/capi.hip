@@ -305,7 +305,7 @@ ensc_status relayout(ensc_engine* E) {
 // previous tick's nodal update), nodal update.  A run of ticks ends with one
 // per-job node finalize.  With timing on, every tick launches all four
 // phases so each is timed on its own.
-constexpr int FIN_BLK = 1024;
+constexpr int FIN_BLK = 512;
 void launch_phase(ensc_engine* E, int phase) {
     const JobDev* jobs = static_cast<const JobDev*>(E->jobs_buf.p);
     const uint32_t nj = (uint32_t)E->live.size();

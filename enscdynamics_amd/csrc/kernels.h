@@ -425,9 +425,10 @@ __device__ __forceinline__ void element_mixed_S(ELEM_ARGS) {
 // part[q * nblk + b]).  fold_partials: thread t folds blocks t, t+BLK,
 // t+2*BLK, ... in ascending order, the node partials (values 0-1, when
 // `node`) and the element partials (values 2-6, when `elem`) in the same
-// loop, loads issued four blocks at a time.  combine_partials: a fixed
+// loop, loads issued FOLD_U blocks at a time.  combine_partials: a fixed
 // xor-shuffle tree per wave, then thread 0 adds the waves in index order.
 // Same inputs, same result.  Every thread of the block must call both.
+constexpr int FOLD_U = 4;  // blocks per thread whose loads are issued together
 template <int BLK>
 __device__ inline void fold_partials(const JobDev& J, bool node, bool elem, const bool* is_min, double* acc) {
     const uint32_t t = threadIdx.x;
@@ -435,10 +436,10 @@ __device__ inline void fold_partials(const JobDev& J, bool node, bool elem, cons
     for (int q = 0; q < 7; ++q) acc[q] = is_min[q] ? __builtin_inf() : 0.0;
     const uint32_t nn = node ? J.nblk_n : 0u, ne = elem ? J.nblk_e : 0u;
     const uint32_t nmax = nn > ne ? nn : ne;
-    for (uint32_t b0 = t; b0 < nmax; b0 += 4 * BLK) {
-        double v[4][7];
+    for (uint32_t b0 = t; b0 < nmax; b0 += FOLD_U * BLK) {
+        double v[FOLD_U][7];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < FOLD_U; ++u) {
             const uint32_t b = b0 + u * BLK;
 #pragma unroll
             for (int q = 0; q < 2; ++q) v[u][q] = (b < nn) ? J.part_n[(size_t)q * nn + b] : 0.0;
@@ -447,7 +448,7 @@ __device__ inline void fold_partials(const JobDev& J, bool node, bool elem, cons
                 v[u][q] = (b < ne) ? J.part_e[(size_t)(q - 2) * ne + b] : (is_min[q] ? __builtin_inf() : 0.0);
         }
 #pragma unroll
-        for (int u = 0; u < 4; ++u)
+        for (int u = 0; u < FOLD_U; ++u)
 #pragma unroll
             for (int q = 0; q < 7; ++q) acc[q] = is_min[q] ? (v[u][q] < acc[q] ? v[u][q] : acc[q]) : acc[q] + v[u][q];
     }
@@ -475,7 +476,7 @@ __device__ inline void combine_partials(double* acc, const bool* is_min, double 
 
 // End of a job's nodal update (thread 0): kinetic energy, external work and
 // the job's history row for the step; marks the update closed.
-__device__ inline void node_phase_finish(const JobDev& J, double ke, double pw) {
+__device__ __forceinline__ void node_phase_finish(const JobDev& J, double ke, double pw) {
 #pragma clang fp contract(off)
     Ctl& c = *J.ctl;
     c.ke = ke;
@@ -504,7 +505,7 @@ __device__ inline void node_phase_finish(const JobDev& J, double ke, double pw) 
 
 // End of a job's element pass (thread 0): tot = min edge, min dtStable,
 // stress, hourglass and viscous work of the pass.
-__device__ inline void elem_phase_finish(const JobDev& J, const Decision& dc, const double* tot) {
+__device__ __forceinline__ void elem_phase_finish(const JobDev& J, const Decision& dc, const double* tot) {
 #pragma clang fp contract(off)
     Ctl& c = *J.ctl;
     const double Lmin = tot[0], dtsn = tot[1];
