@@ -16,8 +16,11 @@ Also in the same JSON line:
     requests drawn from the config-4 generator (up to 5e5 elements, half
     J2); capacity (all submitted at once), Poisson arrivals at 80% of that
     capacity with p50/p99 latency, and the same jobs run one at a time.
-  * cpu_baseline: the FP64 restatement (oracle/) timed on the host's cores on
-    a bounded sample of configuration 3.
+  * cpu_baseline: the reference solver's own step loop (oracle/_ref/ref_timer,
+    built from the reference sources) timed on the host's cores on a bounded
+    sample of configuration 3, with the FP64 restatement (oracle/) beside it
+    as cpu_restatement; the restatement alone where the reference build is
+    absent.
 
     python bench.py [--gpus 1] [--steps 200] [--warmup 20] [--sections c3,out,c4,c5]
 """
@@ -27,6 +30,7 @@ import argparse
 import json
 import os
 import platform
+import subprocess
 import sys
 import time
 
@@ -286,25 +290,68 @@ def c5_section(args):
             "ticks_per_pump": args.c5_ticks}
 
 
-def cpu_baseline(m, steps=100, warm=10):
+REF_TIMER = os.path.join(ROOT, "oracle", "_ref", "ref_timer")
+
+
+def _cpu_model() -> str:
+    cpu = platform.processor() or platform.machine()
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return cpu
+
+
+def cpu_port(m, steps=100, warm=10):
+    """The FP64 restatement (oracle/restate.cpp) on every host core."""
     from oracle.oracle import Oracle  # CPU baseline leg only
 
     threads = os.cpu_count() or 1
     o = Oracle(m, threads).init()
     o.time_steps(warm)
     secs = o.time_steps(steps)
-    cpu = platform.processor() or platform.machine()
-    try:
-        with open("/proc/cpuinfo") as f:
-            for line in f:
-                if line.startswith("model name"):
-                    cpu = line.split(":", 1)[1].strip()
-                    break
-    except OSError:
-        pass
     return {"value": m.n_elems * steps / secs, "unit": "elem-steps/s", "cores": o.threads, "kind": "port",
             "sample": f"{m.name}: {warm} warm-up + {steps} timed steps of the FP64 restatement, "
-                      f"{o.threads} OpenMP threads on {cpu}"}
+                      f"{o.threads} OpenMP threads on {_cpu_model()}"}
+
+
+def cpu_reference(m, steps=20, warm=2):
+    """The reference solver itself (oracle/_ref/ref_timer, built from its
+    sources): its reader parses the deck of the same mesh, init() runs, and
+    `steps` iterations of its step loop are timed after `warm` (dt update,
+    computeSate with output_results, its VTK writer stubbed)."""
+    import tempfile
+
+    with tempfile.TemporaryDirectory() as d:
+        decks.write_inp(m, os.path.join(d, "deck.inp"))
+        r = subprocess.run([REF_TIMER, d, "deck.inp", str(warm), str(steps), repr(m.c_dr), repr(m.c_cr),
+                            repr(m.factor)], capture_output=True, text=True, timeout=600)
+    if r.returncode != 0:
+        raise RuntimeError(f"ref_timer failed: {r.stderr[-500:]}")
+    f = r.stdout.split()
+    kv = {f[i]: f[i + 1] for i in range(len(f) - 1) if f[i] in ("elements", "steps", "seconds", "threads")}
+    ne, n, secs, thr = int(kv["elements"]), int(kv["steps"]), float(kv["seconds"]), int(kv["threads"])
+    return {"value": ne * n / secs, "unit": "elem-steps/s", "cores": thr, "kind": "reference",
+            "sample": f"{m.name}: the reference's own reader, init and step loop (oracle/_ref/ref_timer), "
+                      f"{warm} warm-up + {n} timed steps ({secs:.2f} s), {thr} OpenMP threads on {_cpu_model()}"}
+
+
+def cpu_baseline(m, args):
+    """The reference's CPU step loop when its build travelled with the tree,
+    with the restatement beside it; else the restatement alone."""
+    port = cpu_port(m, args.cpu_steps)
+    log(f"cpu restatement: {port['value']:.3e} elem-steps/s")
+    if os.path.exists(REF_TIMER):
+        try:
+            ref = cpu_reference(m, args.cpu_ref_steps)
+            log(f"cpu reference: {ref['value']:.3e} elem-steps/s")
+            return ref, port
+        except (RuntimeError, OSError, subprocess.SubprocessError) as ex:
+            log(f"cpu reference not timed: {ex}")
+    return port, None
 
 
 KERNEL_KEYS = (("k_element_pipe", "k_element_pipe_mixed"), ("k_element<true", "k_element_mixed"),
@@ -368,7 +415,8 @@ def main() -> int:
     ap.add_argument("--c5-live", type=int, default=64)
     ap.add_argument("--c5-ticks", type=int, default=50)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=100)
+    ap.add_argument("--cpu-steps", type=int, default=100, help="timed steps of the CPU restatement")
+    ap.add_argument("--cpu-ref-steps", type=int, default=20, help="timed steps of the reference solver")
     ap.add_argument("--no-staging", action="store_true", help="gather element corners from HBM instead of LDS")
     ap.add_argument("--copy-bytes", type=int, default=1 << 30, help="size of the copy-bandwidth probe")
     ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
@@ -434,7 +482,10 @@ def main() -> int:
     if "c5" in sections:
         out["serving"] = c5_section(args)
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(m, args.cpu_steps)
+        base, port = cpu_baseline(m, args)
+        out["cpu_baseline"] = base
+        if port is not None:
+            out["cpu_restatement"] = port
         log(f"cpu baseline: {out['cpu_baseline']['value']:.3e} elem-steps/s")
     print(json.dumps(out), flush=True)
     return 0
