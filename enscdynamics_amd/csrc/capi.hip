@@ -61,20 +61,21 @@ struct ensc_engine {
     Ctl* d_ctl = nullptr;
     Ctl* h_ctl = nullptr;
     uint32_t ctl_cap = 0;
-    // layout of the jobs currently stepped
+    // layout of the jobs currently stepped (relayout): the element block
+    // map holds the pipelined jobs' blocks, then the staged k_element
+    // blocks, then the gathered ones; the node block map the block-assembly
+    // jobs' blocks, then the others
     std::vector<uint32_t> live;
     DevBuf jobs_buf, b2e_buf, b2n_buf, pe_buf, pn_buf;
-    uint32_t nblk_e = 0, nblk_n = 0;
+    uint32_t nblk_pipe = 0, nblk_e = 0, nblk_g = 0;  // element blocks: pipelined, staged, gathered
+    uint32_t nblk_n_asm = 0, nblk_n = 0;             // node blocks: block partials, per-corner forces
     bool dirty = true;
-    KernelFn ke = nullptr, ke_g = nullptr, kn = nullptr, kn_asm = nullptr;
-    uint32_t nblk_g = 0;      // element blocks of jobs that gather their corners from HBM (last in the map)
-    uint32_t nblk_n_asm = 0;  // node blocks of block-assembly jobs (first in the map); nblk_n: the rest
-    bool staged = false;
-    uint32_t nblk_pipe = 0;  // element blocks of the pipelined jobs (first in the map); nblk_e: staged
-                             // blocks of k_element (next)
-    uint32_t n_asm = 0;      // live jobs using block assembly
-    uint32_t pipe_grid = 0;  // the pipelined kernel's block count (2 per CU)
-    size_t lds_bytes = 0;
+    KernelFn ke = nullptr, ke_g = nullptr;   // k_element staged / gathered
+    KernelFn kn = nullptr, kn_asm = nullptr;  // k_node per-corner / block partials
+    bool staged = false;      // some live job stages its nodes in LDS
+    uint32_t n_asm = 0;       // live jobs using block assembly
+    uint32_t pipe_grid = 0;   // the pipelined kernel's block count (2 per CU)
+    size_t lds_bytes = 0;     // dynamic LDS of the staged k_element
     // output snapshots of a single-job engine: files go to out_dir (empty:
     // the caller drains them), numbered from out_count
     std::string out_dir;

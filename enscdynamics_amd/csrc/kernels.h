@@ -1,21 +1,33 @@
 // HIP kernels of the step engine (gfx950, wave64).
 //
 // One step iteration of the reference loop (exDyna3D.cpp:86-100) is one
-// "tick" = two launches over every live job:
-//   k_element : one element per lane.  Gathers node x/u/v, runs the element
-//               body (elem_body.inc), writes the element's 8 nodal forces,
-//               and reduces min edge / min stable dt / work terms per block.
-//   k_fin_element : one block per job.  Reduces the job's block partials in
-//               a fixed order and advances its time-step scalars exactly as
+// "tick" = three phases over every live job:
+//   element pass : one element per lane.  Stages the element block's nodes
+//               (X or x, v, u) in LDS (or gathers them from HBM), runs the
+//               element body (elem_body.inc in FP64, elem_body_mixed.inc in
+//               mixed precision), and writes either the element's 8 nodal
+//               forces or, with block assembly, one partial force per
+//               (element block, node); reduces min edge / min stable dt /
+//               work terms per block.  k_element takes one element block
+//               per launched block; k_element_pipe is a persistent,
+//               software-pipelined version for block-assembly jobs.
+//   k_fin_element : one block per job.  Closes the previous nodal update
+//               (kinetic energy, external work, history row), reduces the
+//               element pass's block partials in a fixed order and advances
+//               the job's time-step scalars exactly as
 //               update_minVertex_perMaterial_and_dt and the run-loop clamps
-//               do (exDyna3D.cpp:663-686, 90-97, 775-780).
-//   k_node    : one node per lane.  Sums the node's element forces in the
-//               canonical (corner, element) order, adds gravity, applies the
-//               BCs and reactions (2029-2145), and does the central-difference
-//               update a -> v -> u -> x (642-696).
-//   k_fin_node : one block per job: kinetic energy, external work, history
-//               (also done at the start of the next k_fin_element, so only
-//               the last step iteration of a run launches it).
+//               do (exDyna3D.cpp:663-686, 90-97, 775-780); decides the
+//               output snapshot of the tick (output_results, 207-219).
+//   k_node    : one node per lane.  Sums the node's forces (per-corner
+//               forces in the canonical (corner, element) order, or the
+//               block partials in block order), adds gravity, applies the
+//               BCs and reactions (2029-2145), captures the output snapshot
+//               when one is due, and does the central-difference update
+//               a -> v -> u -> x (642-696).
+//   k_fin_node : one block per job: closes the last nodal update of a run
+//               of ticks (otherwise the next k_fin_element does).
+// Off the step path: k_apply_bc (a step switch's velocity BCs) and k_post
+// (the fields of an output snapshot).
 // Cross-block results travel through per-block partials and the launch
 // boundaries; no kernel waits on, or fences for, another block.
 // The step size of a tick depends on the min edge of the current
