@@ -944,6 +944,10 @@ int64_t engine_job_bytes(ensc_engine* E, uint32_t s) {
     Job* J = job_at(E, s);
     return J ? (int64_t)J->arena.total : 0;
 }
+uint32_t engine_job_blocks(ensc_engine* E, uint32_t s) {
+    Job* J = job_at(E, s);
+    return J ? J->dev.nblk_e : 0;
+}
 ensc_status engine_set_graph(ensc_engine* E, int32_t ticks) {
     E->graph_ticks = std::max(0, ticks);
     E->graph_valid = false;

@@ -794,14 +794,22 @@ __device__ __forceinline__ void pipe_acc_clear(ENSC_LDS float* acc, uint32_t nbn
 struct PipeNext {
     const JobDev* J;   // the next tile's job
     uint32_t e;        // this thread's element of the next tile (clamped)
+    bool on;           // that job steps in this tick (else nothing is loaded for it)
     const JobDev* J2;  // the job of the tile after
-    uint32_t n0, nbn;  // its staging range (nbn = 0: no such tile)
+    uint32_t n0, nbn;  // its staging range (nbn = 0: no such tile, or its job does not step)
     bool done;
 };
 __device__ __forceinline__ void pipe_prefetch(PipeNext& p, ElemState<float>& ESn, uint32_t gn[2]) {
-    load_state<float, true>(*p.J, p.e, true, ESn);
+    if (p.on) load_state<float, true>(*p.J, p.e, true, ESn);
     pipe_ids(*p.J2, p.n0, p.nbn, gn);
     p.done = true;
+}
+// Whether a job computes anything in this tick.  Its scalars do not change
+// during the element pass, so a tile can ask this of the tiles after it:
+// the tiles of finished or waiting jobs (still in a batch's launch until the
+// layout is rebuilt) then cost neither copies nor state loads.
+__device__ __forceinline__ bool pipe_steps(const JobDev& J) {
+    return decide(*J.ctl, J.factor, J.wos).kind != KIND_NONE;
 }
 
 #define PX(n, i) ((i) == 0 ? buf[0][n].a : ((i) == 1 ? buf[0][n].b : buf[1][n].a))
@@ -862,13 +870,14 @@ __device__ __forceinline__ bool pipe_tile(const JobDev* __restrict__ jobs, const
         const uint32_t en = bn.y * PIPE_BLK + threadIdx.x;
         pn.e = en < pn.J->ne ? en : pn.J->ne - 1;
     }
+    pn.on = more && pipe_steps(*pn.J);
     pn.J2 = &jobs[bnn.x];
     pn.n0 = bnn.z;
-    pn.nbn = more2 ? bnn.w : 0u;
+    pn.nbn = (more2 && pipe_steps(*pn.J2)) ? bnn.w : 0u;
     pn.done = false;
     __builtin_amdgcn_s_waitcnt(0);
     lds_sync();
-    if (more) pipe_copy(jobs[bn.x], bn.w, g, nxt);
+    if (pn.on) pipe_copy(jobs[bn.x], bn.w, g, nxt);
     if (dc.kind != KIND_NONE) {
         const bool asmode = J.block_asm && dc.kind != KIND_PREPASS;
         double Le = __builtin_inf(), dts = __builtin_inf();
@@ -957,13 +966,15 @@ __global__ __launch_bounds__(PIPE_BLK) __attribute__((amdgpu_waves_per_eu(WPE)))
         // the first tile's copies and state, the second tile's node ids
         const uint4 bm = blkmap[t];
         const JobDev& J = jobs[bm.x];
-        pipe_ids(J, bm.z, bm.w, gA);
-        pipe_copy(J, bm.w, gA, bufA);
-        const uint32_t e = bm.y * PIPE_BLK + threadIdx.x;
-        load_state<float, true>(J, e < J.ne ? e : J.ne - 1, true, esA);
+        if (pipe_steps(J)) {
+            pipe_ids(J, bm.z, bm.w, gA);
+            pipe_copy(J, bm.w, gA, bufA);
+            const uint32_t e = bm.y * PIPE_BLK + threadIdx.x;
+            load_state<float, true>(J, e < J.ne ? e : J.ne - 1, true, esA);
+        }
         const uint32_t tn = t + gridDim.x;
         const uint4 bn = blkmap[tn < ntiles ? tn : t];
-        pipe_ids(jobs[bn.x], bn.z, tn < ntiles ? bn.w : 0u, gB);
+        pipe_ids(jobs[bn.x], bn.z, (tn < ntiles && pipe_steps(jobs[bn.x])) ? bn.w : 0u, gB);
     }
     // two tiles per trip so each buffer is a distinct LDS object in every
     // access (the compiler then keeps copies into one buffer in flight while

@@ -7,8 +7,8 @@
 // (their init pass runs inside the next step iteration of the live batch,
 // then step 1 is switched in), advances every live job together for a
 // number of step iterations, and retires the jobs that reached their end
-// time: their results are copied to the host and their device memory is
-// released.  A job's arithmetic never involves another job (per-job
+// time: their results are copied to the host, and their device memory is
+// released in groups (pump, step 4).  A job's arithmetic never involves another job (per-job
 // reductions in fixed order), so its result is the same alone or in any mix.
 #include <hip/hip_runtime.h>
 
@@ -37,6 +37,7 @@ ensc_status engine_job_scalars(ensc_engine* E, uint32_t s, double* out);
 ensc_status engine_job_download(ensc_engine* E, uint32_t s, const char* f, double* out, int64_t count);
 ensc_status engine_remove_job(ensc_engine* E, uint32_t s);
 int64_t engine_job_bytes(ensc_engine* E, uint32_t s);
+uint32_t engine_job_blocks(ensc_engine* E, uint32_t s);
 ensc_status engine_set_graph(ensc_engine* E, int32_t ticks);
 }  // namespace ensc
 
@@ -80,6 +81,9 @@ struct ensc_server {
     uint64_t next_ticket = 1;
     std::deque<std::shared_ptr<Request>> queue;  // admission order
     std::map<uint32_t, std::pair<uint64_t, int64_t>> live;  // slot -> (ticket, bytes)
+    // finished jobs whose results are out but which are still in the engine
+    // (slot, bytes): removed together, so the launch layout changes rarely
+    std::vector<std::pair<uint32_t, int64_t>> retired;
     std::map<uint64_t, Record> rec;
     // preparation workers
     std::mutex mu;
@@ -208,6 +212,15 @@ ensc_status ensc_server_pump(ensc_server* S, int32_t ticks, uint64_t* done, uint
         if (done && *n_done < cap) done[*n_done] = t;
         ++*n_done;
     };
+    auto remove_retired = [&]() -> ensc_status {
+        for (const auto& z : S->retired) {
+            const ensc_status s2 = engine_remove_job(S->eng, z.first);
+            if (s2) return s2;
+            S->used -= z.second;
+        }
+        S->retired.clear();
+        return ENSC_OK;
+    };
     // 1. admission of prepared jobs by footprint, arrival order; with nothing
     //    on the device, wait for the next job's preparation
     std::vector<uint32_t> started;
@@ -230,6 +243,7 @@ ensc_status ensc_server_pump(ensc_server* S, int32_t ticks, uint64_t* done, uint
             report(r->ticket);
             continue;
         }
+        if (S->used + r->footprint > S->budget && !S->retired.empty() && (st = remove_retired())) return st;
         if (S->used + r->footprint > S->budget) break;
         S->queue.pop_front();
         uint32_t slot = 0;
@@ -244,7 +258,12 @@ ensc_status ensc_server_pump(ensc_server* S, int32_t ticks, uint64_t* done, uint
     if (!started.empty() && (st = engine_start_jobs(S->eng, started))) return st;
     // 3. advance every live job together
     if (!S->live.empty() && ticks > 0 && (st = engine_ticks(S->eng, ticks))) return st;
-    // 4. retire finished jobs: results to the host, device memory released
+    // 4. retire finished jobs: results to the host.  Their device memory is
+    //    released once the retired jobs hold a quarter of the element blocks
+    //    still in the launch, when nothing live is left, or when an
+    //    admission needs the room (a finished job's blocks exit at once, so
+    //    keeping it in the layout costs little, while every layout change
+    //    rebuilds the block maps and the step graph).
     std::vector<uint32_t> fin;
     if ((st = engine_poll(S->eng, fin))) return st;
     for (uint32_t slot : fin) {
@@ -257,12 +276,17 @@ ensc_status ensc_server_pump(ensc_server* S, int32_t ticks, uint64_t* done, uint
             rc.u.assign(3 * (size_t)rc.n_nodes, 0.0);
             if ((st = engine_job_download(S->eng, slot, "u", rc.u.data(), (int64_t)rc.u.size()))) return st;
         }
-        if ((st = engine_remove_job(S->eng, slot))) return st;
         rc.t_finish = S->now();
         rc.finished = true;
-        S->used -= it->second.second;
+        S->retired.emplace_back(slot, it->second.second);
         S->live.erase(it);
         report(t);
+    }
+    if (!S->retired.empty()) {
+        uint64_t zb = 0, lb = 0;
+        for (const auto& z : S->retired) zb += engine_job_blocks(S->eng, z.first);
+        for (const auto& l : S->live) lb += engine_job_blocks(S->eng, l.first);
+        if (S->live.empty() || zb * 4 > zb + lb) return remove_retired();
     }
     return ENSC_OK;
 }

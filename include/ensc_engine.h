@@ -199,7 +199,9 @@ ensc_status ensc_batch_job_download(ensc_engine* eng, uint32_t job, const char* 
  * max_live_jobs are live), runs their init pass inside the next step
  * iteration of the live batch, advances every live job together for
  * `ticks` step iterations, and retires jobs that reached their end time
- * (results to the host, device memory freed).  With nothing on the device,
+ * (results to the host at once; their device memory is freed in groups,
+ * when retired jobs hold a quarter of the launch, when nothing is live or
+ * when an admission needs the room).  With nothing on the device,
  * a pump waits for the next job's preparation.  A job whose set-up fails
  * (e.g. an inverted element) is retired at once and ensc_server_result
  * returns ENSC_ERR_ARG with the reason.
@@ -217,6 +219,7 @@ ensc_status ensc_server_submit(ensc_server* srv, const ensc_model_desc* model, i
 ensc_status ensc_server_pump(ensc_server* srv, int32_t ticks, uint64_t* done, uint32_t cap, uint32_t* n_done);
 ensc_status ensc_server_result(ensc_server* srv, uint64_t ticket, double* scalars /* ENSC_S_COUNT */, double* u,
                                int64_t u_count, double* times /* submit, admit, finish */);
+/* bytes_used: device bytes of live jobs and of retired jobs not yet freed. */
 ensc_status ensc_server_status(ensc_server* srv, uint32_t* queued, uint32_t* live, int64_t* bytes_used);
 double ensc_server_clock(ensc_server* srv);
 
