@@ -111,7 +111,9 @@ static bool setup_geometry(const std::vector<double>& x0, const std::vector<uint
     return true;
 }
 
-// Space-filling (Morton) element order from element centroids; nodes are
+// Space-filling (Morton) element order from element centroids, quantized
+// with one scale for all three axes (the longest extent), so that runs of
+// consecutive elements are compact in space for any aspect ratio; nodes are
 // then numbered in first-touch order of the reordered elements.
 static void morton_order(const std::vector<double>& x0, const std::vector<uint32_t>& conn, size_t Ne, size_t Nn,
                          std::vector<uint32_t>& e_new2old, std::vector<uint32_t>& n_new2old) {
@@ -130,6 +132,7 @@ static void morton_order(const std::vector<double>& x0, const std::vector<uint32
         v = (v | v << 2) & 0x1249249249249249ULL;
         return v;
     };
+    const double span = std::max(hi[0] - lo[0], std::max(hi[1] - lo[1], hi[2] - lo[2]));
     std::vector<std::pair<uint64_t, uint32_t>> key(Ne);
     for (size_t e = 0; e < Ne; ++e) {
         double c[3] = {0, 0, 0};
@@ -137,7 +140,6 @@ static void morton_order(const std::vector<double>& x0, const std::vector<uint32
             for (int d = 0; d < 3; ++d) c[d] += x0[3 * (size_t)conn[8 * e + k] + d];
         uint64_t q[3];
         for (int d = 0; d < 3; ++d) {
-            const double span = hi[d] - lo[d];
             const double t = span > 0 ? (c[d] / 8.0 - lo[d]) / span : 0.0;
             q[d] = (uint64_t)std::min(2097151.0, std::max(0.0, t * 2097151.0));
         }
