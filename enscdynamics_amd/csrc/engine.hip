@@ -486,6 +486,11 @@ ensc_status upload_job(Job& J) {
         D.part_f = dpf;
         D.pos_ptr = dpp;
         D.pos_list = dpl;
+        std::vector<double> im(Nn);
+        for (size_t i = 0; i < Nn; ++i) im[i] = I.pos[i * POS_REC + 3];
+        double* dim;
+        if ((st = upload(&dim, im, im.size(), A))) return st;
+        D.inv_m = dim;
     } else {
         double *db, *dv, *ds, *dq, *dp, *df;
         if ((st = upload(&db, I.gi64, I.gi64.size(), A))) return st;

@@ -794,22 +794,14 @@ __device__ __forceinline__ void pipe_acc_clear(ENSC_LDS float* acc, uint32_t nbn
 struct PipeNext {
     const JobDev* J;   // the next tile's job
     uint32_t e;        // this thread's element of the next tile (clamped)
-    bool on;           // that job steps in this tick (else nothing is loaded for it)
     const JobDev* J2;  // the job of the tile after
-    uint32_t n0, nbn;  // its staging range (nbn = 0: no such tile, or its job does not step)
+    uint32_t n0, nbn;  // its staging range (nbn = 0: no such tile)
     bool done;
 };
 __device__ __forceinline__ void pipe_prefetch(PipeNext& p, ElemState<float>& ESn, uint32_t gn[2]) {
-    if (p.on) load_state<float, true>(*p.J, p.e, true, ESn);
+    load_state<float, true>(*p.J, p.e, true, ESn);
     pipe_ids(*p.J2, p.n0, p.nbn, gn);
     p.done = true;
-}
-// Whether a job computes anything in this tick.  Its scalars do not change
-// during the element pass, so a tile can ask this of the tiles after it:
-// the tiles of finished or waiting jobs (still in a batch's launch until the
-// layout is rebuilt) then cost neither copies nor state loads.
-__device__ __forceinline__ bool pipe_steps(const JobDev& J) {
-    return decide(*J.ctl, J.factor, J.wos).kind != KIND_NONE;
 }
 
 #define PX(n, i) ((i) == 0 ? buf[0][n].a : ((i) == 1 ? buf[0][n].b : buf[1][n].a))
@@ -870,14 +862,13 @@ __device__ __forceinline__ bool pipe_tile(const JobDev* __restrict__ jobs, const
         const uint32_t en = bn.y * PIPE_BLK + threadIdx.x;
         pn.e = en < pn.J->ne ? en : pn.J->ne - 1;
     }
-    pn.on = more && pipe_steps(*pn.J);
     pn.J2 = &jobs[bnn.x];
     pn.n0 = bnn.z;
-    pn.nbn = (more2 && pipe_steps(*pn.J2)) ? bnn.w : 0u;
+    pn.nbn = more2 ? bnn.w : 0u;
     pn.done = false;
     __builtin_amdgcn_s_waitcnt(0);
     lds_sync();
-    if (pn.on) pipe_copy(jobs[bn.x], bn.w, g, nxt);
+    if (more) pipe_copy(jobs[bn.x], bn.w, g, nxt);
     if (dc.kind != KIND_NONE) {
         const bool asmode = J.block_asm && dc.kind != KIND_PREPASS;
         double Le = __builtin_inf(), dts = __builtin_inf();
@@ -966,15 +957,13 @@ __global__ __launch_bounds__(PIPE_BLK) __attribute__((amdgpu_waves_per_eu(WPE)))
         // the first tile's copies and state, the second tile's node ids
         const uint4 bm = blkmap[t];
         const JobDev& J = jobs[bm.x];
-        if (pipe_steps(J)) {
-            pipe_ids(J, bm.z, bm.w, gA);
-            pipe_copy(J, bm.w, gA, bufA);
-            const uint32_t e = bm.y * PIPE_BLK + threadIdx.x;
-            load_state<float, true>(J, e < J.ne ? e : J.ne - 1, true, esA);
-        }
+        pipe_ids(J, bm.z, bm.w, gA);
+        pipe_copy(J, bm.w, gA, bufA);
+        const uint32_t e = bm.y * PIPE_BLK + threadIdx.x;
+        load_state<float, true>(J, e < J.ne ? e : J.ne - 1, true, esA);
         const uint32_t tn = t + gridDim.x;
         const uint4 bn = blkmap[tn < ntiles ? tn : t];
-        pipe_ids(jobs[bn.x], bn.z, (tn < ntiles && pipe_steps(jobs[bn.x])) ? bn.w : 0u, gB);
+        pipe_ids(jobs[bn.x], bn.z, tn < ntiles ? bn.w : 0u, gB);
     }
     // two tiles per trip so each buffer is a distinct LDS object in every
     // access (the compiler then keeps copies into one buffer in flight while
@@ -1010,18 +999,22 @@ __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(ASM ? 8 : 1
     }
     if (n < J.nn) {
         // [X0 X1 | X2 im] [v0 v1 | v2 u0 | u1 u2]; the mixed engine never
-        // rewrites X (the reference position)
-        const D2 p1 = gld2(J.pos, n * POS_REC + 2);
+        // rewrites X (the reference position), so it reads 1/m from its own
+        // array instead of the position record
         const D2 p2 = gld2(J.vu, n * VU_REC), p3 = gld2(J.vu, n * VU_REC + 2), p4 = gld2(J.vu, n * VU_REC + 4);
-        double x[3] = {0.0, 0.0, p1.a};
-        if (!MIXED) {
-            const D2 p0 = gld2(J.pos, n * POS_REC);
+        double x[3] = {0.0, 0.0, 0.0};
+        double im;
+        if (MIXED) {
+            im = gld(J.inv_m, n);
+        } else {
+            const D2 p0 = gld2(J.pos, n * POS_REC), p1 = gld2(J.pos, n * POS_REC + 2);
             x[0] = p0.a;
             x[1] = p0.b;
+            x[2] = p1.a;
+            im = p1.b;
         }
         double v[3] = {p2.a, p2.b, p3.a};
         double u[3] = {p3.b, p4.a, p4.b};
-        const double im = p1.b;
         double acc[3] = {0.0, 0.0, 0.0};
         if (ASM) {
             // block assembly: the node's partials, one per element block that
@@ -1139,7 +1132,7 @@ __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(ASM ? 8 : 1
             // output_results' node state: x and u before this step's update
             double xs[3] = {x[0], x[1], x[2]};
             if (MIXED) {
-                const D2 p0 = gld2(J.pos, n * POS_REC);
+                const D2 p0 = gld2(J.pos, n * POS_REC), p1 = gld2(J.pos, n * POS_REC + 2);
                 xs[0] = p0.a + u[0];
                 xs[1] = p0.b + u[1];
                 xs[2] = p1.a + u[2];
