@@ -470,7 +470,7 @@ ensc_status upload_job(Job& J) {
         if ((st = upload(&dq, none, 12 * es, A))) return st;
         if ((st = upload(&dp, none, es, A))) return st;
         if ((st = upload(&df, none, 8 * es * 3, A))) return st;
-        if ((st = upload(&dx, none, Nn * 4, A))) return st;
+        if ((st = upload(&dx, none, Nn * 3, A))) return st;
         D.Gi = nullptr;
         D.V0 = nullptr;
         D.sigma = ds;

@@ -1159,10 +1159,10 @@ __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(ASM ? 8 : 1
             if (MIXED) {
                 // update_displacement (651-654), compensated (Kahan); the
                 // position is X + u, so move_mesh has nothing to store
-                const float cf = gld(J.ucomp, n * 4 + d);
+                const float cf = gld(J.ucomp, n * 3 + d);
                 const double y = dx - (double)cf;
                 const double t = u[d] + y;
-                gst(J.ucomp, n * 4 + d) = (float)((t - u[d]) - y);
+                gst(J.ucomp, n * 3 + d) = (float)((t - u[d]) - y);
                 u[d] = t;
             } else {
                 u[d] += dx;  // update_displacement (651-654)
