@@ -265,6 +265,30 @@ def test_full_size_cube_fp64_vs_oracle_few_steps():
         assert err <= TOL_FP64, f
 
 
+def test_mixed_vs_fp64_full_plastic_cube_3000_steps():
+    # configuration 3 as specified: 1M elements, J2, run to the end time of
+    # 3000 initial stable steps (the step shrinks as the cube deforms, so a
+    # few more steps are taken).  The mixed engine against the FP64 engine
+    # (itself pinned to the oracle above): displacement within 1e-4, total
+    # energy within 1% of the external work.
+    m = decks.config3(n=100, steps=3000)
+    a = eng.Engine(m, eng.FP64, reorder=True).run()
+    b = eng.Engine(m, eng.MIXED, reorder=True).run()
+    sa, sb = a.scalars(), b.scalars()
+    assert sa["nsteps"] >= 3000 and abs(sb["nsteps"] - sa["nsteps"]) <= 1
+    assert sa["done"] == 1.0 and sb["done"] == 1.0
+    err_u = rel(b.get("u"), a.get("u"))
+    ea = sa["ke"] + sa["ie"] + sa["hg"] + sa["vis"]
+    eb = sb["ke"] + sb["ie"] + sb["hg"] + sb["vis"]
+    e_err = abs(eb - ea) / sa["wext"]
+    plastic = float((a.get("epsp") > 0).mean())
+    print(f"[c3 full mixed vs fp64, {int(sa['nsteps'])} steps] u {err_u:.2e}, energy {e_err:.2e} of W_ext, "
+          f"{100 * plastic:.1f}% of elements yielded", flush=True)
+    assert plastic > 0.0
+    assert err_u <= TOL_MIXED
+    assert e_err <= 0.01
+
+
 def test_bad_models_fail_loudly():
     m = decks.config3(n=4, steps=10)
     bad = decks.config3(n=4, steps=10)
