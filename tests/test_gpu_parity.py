@@ -289,6 +289,24 @@ def test_mixed_vs_fp64_full_plastic_cube_3000_steps():
     assert e_err <= 0.01
 
 
+def test_mixed_vs_fp64_full_jittered_block_c2():
+    # configuration 2 as specified: 200x40x40 jittered block (320k elements),
+    # top face driven at -2 m/s ramped over the first 10% by an amplitude,
+    # 5000 steps; the FP64 GPU engine against the mixed one
+    m = decks.config2()
+    a = eng.Engine(m, eng.FP64, reorder=True).run()
+    b = eng.Engine(m, eng.MIXED, reorder=True).run()
+    sa, sb = a.scalars(), b.scalars()
+    assert sa["done"] == 1.0 and sb["done"] == 1.0 and abs(sb["nsteps"] - sa["nsteps"]) <= 1
+    err_u = rel(b.get("u"), a.get("u"))
+    err_s = rel(b.get("sigma"), a.get("sigma"))
+    e_err = abs((sb["ke"] + sb["ie"] + sb["hg"] + sb["vis"]) - (sa["ke"] + sa["ie"] + sa["hg"] + sa["vis"]))
+    print(f"[c2 full mixed vs fp64, {int(sa['nsteps'])} steps] u {err_u:.2e} sigma {err_s:.2e} "
+          f"energy {e_err / sa['wext']:.2e} of W_ext", flush=True)
+    assert err_u <= TOL_MIXED
+    assert e_err <= 0.01 * sa["wext"]
+
+
 def test_bad_models_fail_loudly():
     m = decks.config3(n=4, steps=10)
     bad = decks.config3(n=4, steps=10)
