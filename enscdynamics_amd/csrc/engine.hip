@@ -313,6 +313,19 @@ ensc_status prepare_job(Job& J, int precision, int reorder, int blk_e, int blk_n
     J.e_old2new.assign(Ne, 0);
     for (size_t i = 0; i < Ne; ++i) J.e_old2new[J.e_new2old[i]] = (uint32_t)i;
     const size_t es = round_up(Ne, 256), ns = round_up(Nn, 256);
+    // Device code addresses every per-job array with a 32-bit byte offset
+    // from the array's base, so each must stay under 4 GiB: the per-corner
+    // forces (24 values per element) and the node records bound the size of
+    // one job (FP64: ~22M elements, mixed: ~44M; larger meshes would be split
+    // across engines or GPUs).
+    {
+        const size_t R = mixed ? sizeof(float) : sizeof(double);
+        const size_t largest = std::max({24 * R * es, (size_t)VU_REC * sizeof(double) * ns,
+                                         (size_t)POS_REC * sizeof(double) * ns});
+        if (Ne >= (1u << 29) || largest >= ((size_t)1 << 32))
+            return fail(ENSC_ERR_ARG, "mesh too large for one engine job: " + std::to_string(Ne) + " elements, " +
+                                          std::to_string(Nn) + " nodes (a per-job array would reach 4 GiB)");
+    }
     J.ne_s = es;
     J.nn_s = ns;
     // element SoA arrays in the new numbering
@@ -328,6 +341,9 @@ ensc_status prepare_job(Job& J, int precision, int reorder, int blk_e, int blk_n
         for (int k = 0; k < 8; ++k) cnt[M.conn[8 * e + k]]++;
     uint32_t W = 0;
     for (uint32_t v : cnt) W = std::max(W, v);
+    if ((size_t)W * ns * sizeof(uint32_t) >= ((size_t)1 << 32))
+        return fail(ENSC_ERR_ARG, "a node is shared by " + std::to_string(W) +
+                                      " elements: the gather table would reach 4 GiB");
     J.ell_w = W;
     std::vector<uint32_t> ell((size_t)W * ns, ELL_EMPTY);
     std::vector<uint32_t> fillc(Nn, 0);
@@ -337,7 +353,6 @@ ensc_status prepare_job(Job& J, int precision, int reorder, int blk_e, int blk_n
             const uint32_t nn_ = J.n_old2new[no];
             ell[(size_t)fillc[no]++ * ns + nn_] = (J.e_old2new[eo] << 3) | (uint32_t)k;
         }
-    if (Ne >= (1u << 29)) return fail(ENSC_ERR_ARG, "too many elements for the 29-bit gather slot");
     // LDS staging lists: per element block, its distinct nodes (ascending new
     // ids) and each corner's 16-bit index into that list.
     std::vector<uint32_t> blk_ptr, blk_nodes;
