@@ -13,7 +13,7 @@ Extensions the reference reader ignores (documented where used):
 from __future__ import annotations
 
 import math
-from typing import Dict, List, Optional, Sequence, Tuple
+from typing import List, Optional, Sequence
 
 import numpy as np
 

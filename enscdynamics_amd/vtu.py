@@ -11,7 +11,7 @@ from __future__ import annotations
 import ctypes as C
 import re
 import xml.etree.ElementTree as ET
-from typing import Dict, Mapping, Tuple
+from typing import Dict, Mapping
 
 import numpy as np
 

@@ -1,5 +1,4 @@
 """The C-ABI library loads and exports every entry point include/*.h declares."""
-import ctypes
 import os
 import re
 

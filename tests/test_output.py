@@ -1,7 +1,6 @@
 """Output files and fields (exDyna3D::output_results, DataOut::write_vtu) on
 the CPU: the .vtu writer, and the oracle's output snapshots pinned by
 independent recomputation and by the nodal-update identities."""
-import os
 
 import numpy as np
 import pytest
