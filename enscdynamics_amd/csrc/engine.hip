@@ -576,6 +576,20 @@ ensc_status upload_job(Job& J) {
     D.factor = M.factor;
     D.yield0 = M.yield0;
     D.hard = M.hard;
+    D.mc.lam = (float)M.lambda;
+    D.mc.two_g = (float)(2.0 * M.G);
+    D.mc.three_g = (float)(3.0 * M.G);
+    D.mc.inv_ret = (float)(1.0 / (3.0 * M.G + M.hard));
+    D.mc.yield0 = (float)M.yield0;
+    D.mc.hard = (float)M.hard;
+    D.mc.dr = (float)(M.c_dr * (4.0 / 3.0 * M.G + M.K) * 0.1 / std::sqrt(8.0));
+    D.mc.cr = (float)(M.c_cr * M.rho * M.wos / 4.0);
+    D.mc.rho = (float)M.rho;
+    D.mc.cq2 = (float)(M.cvisq * M.cvisq);
+    D.mc.clw = (float)(M.cvisl * M.wos);
+    D.mc.pad = 0.0f;
+    D.mc.inv_wos = 1.0 / M.wos;
+    D.mc.cq2_wos = M.cvisq * M.cvisq / M.wos;
     D.plastic = M.plastic;
     D.ell_w = (int32_t)W;
     D.ne = (uint32_t)Ne;

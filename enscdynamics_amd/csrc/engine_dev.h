@@ -117,6 +117,13 @@ struct JobDev {
     double grav_g, grav_dir[3];
     // material and parameters (Material_elastic.cpp:9-14; exDyna3D.h:179-188)
     double lambda, G, K, rho, wos, c_dr, c_cr, cvisl, cvisq, factor, yield0, hard;
+    // the same constants folded for the mixed-precision element body
+    struct MixConst {
+        float lam, two_g, three_g, inv_ret;  // inv_ret = 1 / (3 G + hard)
+        float yield0, hard, dr, cr;          // dr = c_dr (4/3 G + K) / 10 / sqrt 8;  cr = c_cr rho wos / 4
+        float rho, cq2, clw, pad;            // cq2 = cvisq^2;  clw = cvisl wos
+        double inv_wos, cq2_wos;             // 1 / wos;  cvisq^2 / wos
+    } mc;
     int32_t plastic, dyn_bc, ell_w, n_probe;
     int32_t block_asm, pad_asm;  // 1: block assembly (staged mixed engine, conflict-free blocks)
     uint32_t ne, nn, ne_s, nn_s;

@@ -202,6 +202,31 @@ __device__ __forceinline__ void inverse3(const R m[3][3], R r[3][3]) {
     r[0][2] = c20 * invdet;
 }
 
+// The cofactor inverse for the mixed-precision body: contracted products and
+// one hardware reciprocal (1 ulp) in place of the IEEE division.  Returns the
+// determinant (first-column expansion); *invdet gets its reciprocal.
+__device__ __forceinline__ float inverse3_rcp(const float m[3][3], float r[3][3], float* invdet) {
+#pragma clang fp contract(fast)
+    auto cof = [&](int i, int j) {
+        const int i1 = (i + 1) % 3, i2 = (i + 2) % 3, j1 = (j + 1) % 3, j2 = (j + 2) % 3;
+        return m[i1][j1] * m[i2][j2] - m[i1][j2] * m[i2][j1];
+    };
+    const float c00 = cof(0, 0), c10 = cof(1, 0), c20 = cof(2, 0);
+    const float det = c00 * m[0][0] + c10 * m[1][0] + c20 * m[2][0];
+    const float id = __builtin_amdgcn_rcpf(det);
+    r[0][0] = c00 * id;
+    r[0][1] = c10 * id;
+    r[0][2] = c20 * id;
+    r[1][0] = cof(0, 1) * id;
+    r[1][1] = cof(1, 1) * id;
+    r[1][2] = cof(2, 1) * id;
+    r[2][0] = cof(0, 2) * id;
+    r[2][1] = cof(1, 2) * id;
+    r[2][2] = cof(2, 2) * id;
+    *invdet = id;
+    return det;
+}
+
 // Velocity-BC amplitude (extension): piecewise linear in total time, held
 // constant outside the table.
 __device__ inline double amp_value(const JobDev& J, int id, double t) {
