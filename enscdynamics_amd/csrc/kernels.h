@@ -110,6 +110,17 @@ __device__ __forceinline__ void walsh8(const R x[8], R w[8]) {
     for (int i = 0; i < 8; ++i) w[i] = a[i];
 }
 
+// walsh8 of FP64 corner values evaluated in FP64 and rounded once to R at the
+// end (the non-constant sums are differences of nearby values, so this keeps
+// them exact to rounding whatever the common offset).
+template <class R>
+__device__ __forceinline__ void walsh8_d(const double c[8], R w[8]) {
+    double wd[8];
+    walsh8<double>(c, wd);
+#pragma unroll
+    for (int i = 0; i < 8; ++i) w[i] = R(wd[i]);
+}
+
 // f[k] = sum_s c[s] pattern_s(k): the expansion back to the corners.
 template <class R>
 __device__ __forceinline__ void walsh8_expand(const R c[8], R f[8]) {
