@@ -479,20 +479,18 @@ ensc_status upload_job(Job& J) {
     if ((st = upload(&d_conn, I.conn_d, I.conn_d.size(), A))) return st;
     D.conn = d_conn;
     if (mixed) {
-        float *ds, *dq, *dp, *df, *dx;
+        float *ds, *dq, *dp, *df;
         std::vector<float> none;
         if ((st = upload(&ds, none, 6 * es, A))) return st;
         if ((st = upload(&dq, none, 12 * es, A))) return st;
         if ((st = upload(&dp, none, es, A))) return st;
         if ((st = upload(&df, none, 8 * es * 3, A))) return st;
-        if ((st = upload(&dx, none, Nn * 3, A))) return st;
         D.Gi = nullptr;
         D.V0 = nullptr;
         D.sigma = ds;
         D.qia = dq;
         D.epsp = dp;
         D.fel = df;
-        D.ucomp = dx;
         float* dpf;
         uint32_t *dpp, *dpl;
         if ((st = upload(&dpf, none, I.blk_nodes.size() * 3, A))) return st;
@@ -521,7 +519,6 @@ ensc_status upload_job(Job& J) {
         D.qia = dq;
         D.epsp = dp;
         D.fel = df;
-        D.ucomp = nullptr;
     }
     double *d_pos, *d_vu;
     if ((st = upload(&d_pos, I.pos, I.pos.size(), A))) return st;

@@ -100,7 +100,6 @@ struct JobDev {
     // node arrays
     double* pos;           // [nn][POS_REC]  X, invM
     double* vu;            // [nn][VU_REC]   v, u
-    float* ucomp;          // [nn][3] compensation of the u accumulation (mixed) or null
     const double* inv_m;   // [nn] 1/m (mixed; the nodal update reads it without the position record)
     double* rf;            // [nn][3]
     const uint8_t* bcmask; // [nn]

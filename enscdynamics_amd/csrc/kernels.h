@@ -1192,18 +1192,10 @@ __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(ASM ? 8 : 1
             }
             vd += hs * ad / 2.0;  // update_velocity (646-649)
             const double dx = dt_i * vd;
-            if (MIXED) {
-                // update_displacement (651-654), compensated (Kahan); the
-                // position is X + u, so move_mesh has nothing to store
-                const float cf = gld(J.ucomp, n * 3 + d);
-                const double y = dx - (double)cf;
-                const double t = u[d] + y;
-                gst(J.ucomp, n * 3 + d) = (float)((t - u[d]) - y);
-                u[d] = t;
-            } else {
-                u[d] += dx;  // update_displacement (651-654)
-                x[d] += dx;  // move_mesh (688-696)
-            }
+            u[d] += dx;  // update_displacement (651-654)
+            // move_mesh (688-696); the mixed engine's position is X + u, so
+            // it has nothing to store
+            if (!MIXED) x[d] += dx;
             v[d] = vd;
             k2 += vd * vd;
         }
