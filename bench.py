@@ -65,8 +65,8 @@ def alg_bytes(precision: int, plastic: bool, r: float):
     # node values each element pass reads once: X, v, u
     elem += 72 * r
     if precision == eng.MIXED:
-        # v, u read+write, invM
-        node = (48 + 48 + 8) * r
+        # v, u read+write, invM, compensation of u (one packed word) read+write
+        node = (48 + 48 + 8 + 8) * r
     else:
         # x, v, u read+write, invM
         node = (144 + 8) * r

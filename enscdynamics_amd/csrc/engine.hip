@@ -485,6 +485,10 @@ ensc_status upload_job(Job& J) {
         if ((st = upload(&dq, none, 12 * es, A))) return st;
         if ((st = upload(&dp, none, es, A))) return st;
         if ((st = upload(&df, none, 8 * es * 3, A))) return st;
+        uint32_t* dc;
+        std::vector<uint32_t> no_c;
+        if ((st = upload(&dc, no_c, Nn, A))) return st;
+        D.ucomp = dc;
         D.Gi = nullptr;
         D.V0 = nullptr;
         D.sigma = ds;

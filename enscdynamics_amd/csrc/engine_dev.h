@@ -101,6 +101,7 @@ struct JobDev {
     double* pos;           // [nn][POS_REC]  X, invM
     double* vu;            // [nn][VU_REC]   v, u
     const double* inv_m;   // [nn] 1/m (mixed; the nodal update reads it without the position record)
+    uint32_t* ucomp;       // [nn] compensation of the u sum (mixed; 3 x 10-bit fields) or null
     double* rf;            // [nn][3]
     const uint8_t* bcmask; // [nn]
     const double* bcval;   // [nn][3]

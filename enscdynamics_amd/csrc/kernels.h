@@ -238,6 +238,23 @@ __device__ __forceinline__ float inverse3_rcp(const float m[3][3], float r[3][3]
     return det;
 }
 
+// Compensation of the mixed engine's u sum (Kahan): per node one word of
+// three signed 10-bit fields, field d the residual of the last addition to
+// u[d] in units of 2^-10 ulp of u[d] (|residual| <= 1/2 ulp, so the fields
+// keep it to 2^-11 ulp).
+__device__ __forceinline__ double ucomp_get(uint32_t w, int d, double u) {
+    const int q = ((int)(w << (22 - 10 * d))) >> 22;
+    int k;
+    (void)frexp(u, &k);  // ulp(u) = 2^(k - 53)
+    return ldexp((double)q, k - 63);
+}
+__device__ __forceinline__ uint32_t ucomp_put(double c, double t, int d) {
+    int k;
+    (void)frexp(t, &k);
+    const double s = fmin(fmax(rint(ldexp(c, 63 - k)), -512.0), 511.0);
+    return ((uint32_t)(int)s & 0x3FFu) << (10 * d);
+}
+
 // Velocity-BC amplitude (extension): piecewise linear in total time, held
 // constant outside the table.
 __device__ inline double amp_value(const JobDev& J, int id, double t) {
@@ -1163,6 +1180,8 @@ __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(ASM ? 8 : 1
         }
         const double hs = dt_i + dt_i_1;
         double k2 = 0.0;
+        const uint32_t cw = MIXED ? gld(J.ucomp, n) : 0u;
+        uint32_t cw_new = 0;
         double* sn = snap >= 0 ? J.snap_node + ((size_t)snap * J.nn + n) * 12 : nullptr;
         if (sn) {
             // output_results' node state: x and u before this step's update
@@ -1192,14 +1211,22 @@ __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(ASM ? 8 : 1
             }
             vd += hs * ad / 2.0;  // update_velocity (646-649)
             const double dx = dt_i * vd;
-            u[d] += dx;  // update_displacement (651-654)
-            // move_mesh (688-696); the mixed engine's position is X + u, so
-            // it has nothing to store
-            if (!MIXED) x[d] += dx;
+            if (MIXED) {
+                // update_displacement (651-654), compensated (Kahan); the
+                // position is X + u, so move_mesh has nothing to store
+                const double y = dx - ucomp_get(cw, d, u[d]);
+                const double t = u[d] + y;
+                cw_new |= ucomp_put((t - u[d]) - y, t, d);
+                u[d] = t;
+            } else {
+                u[d] += dx;  // update_displacement (651-654)
+                x[d] += dx;  // move_mesh (688-696)
+            }
             v[d] = vd;
             k2 += vd * vd;
         }
         ke = 0.5 * k2 / im;
+        if (MIXED) gst(J.ucomp, n) = cw_new;
         if (!MIXED) {
             gst2(J.pos, n * POS_REC, x[0], x[1]);
             gst2(J.pos, n * POS_REC + 2, x[2], im);

@@ -130,10 +130,10 @@ int64_t footprint(const ModelCopy& c, int precision) {
     // corner forces
     const int64_t elem = 32 + 16 + (mixed ? 0 : 10 * r) + 6 * r + 12 * r + r + 24 * r;
     // record, reactions, BC mask/values/amplitudes, 8-wide gather table,
-    // staging lists (< 2 positions per node); mixed: 1/m and the
-    // block-assembly partials and position lists
+    // staging lists (< 2 positions per node); mixed: u compensation, 1/m
+    // and the block-assembly partials and position lists
     const int64_t node = NODE_BYTES + 24 + 1 + 24 + 6 + 8 * 4 + 8 +
-                         (precision == ENSC_MIXED ? 8 + 4 + 2 * (4 + 12) : 0) + 16;
+                         (precision == ENSC_MIXED ? 4 + 8 + 4 + 2 * (4 + 12) : 0) + 16;
     return es * elem + nn * node + 64 * 1024;
 }
 
