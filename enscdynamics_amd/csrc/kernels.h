@@ -213,8 +213,17 @@ __device__ __forceinline__ void inverse3(const R m[3][3], R r[3][3]) {
     r[0][2] = c20 * invdet;
 }
 
+// Reciprocal for the mixed-precision body: the hardware estimate (1 ulp)
+// refined by one Newton step.  The bare estimate is not correctly rounded,
+// and its one-sided errors add up over thousands of steps where the refined
+// one does not.
+__device__ __forceinline__ float rcp_nr(float x) {
+    const float r = __builtin_amdgcn_rcpf(x);
+    return __builtin_fmaf(r, __builtin_fmaf(-x, r, 1.0f), r);
+}
+
 // The cofactor inverse for the mixed-precision body: contracted products and
-// one hardware reciprocal (1 ulp) in place of the IEEE division.  Returns the
+// one reciprocal (rcp_nr) in place of the IEEE division.  Returns the
 // determinant (first-column expansion); *invdet gets its reciprocal.
 __device__ __forceinline__ float inverse3_rcp(const float m[3][3], float r[3][3], float* invdet) {
 #pragma clang fp contract(fast)
@@ -224,7 +233,7 @@ __device__ __forceinline__ float inverse3_rcp(const float m[3][3], float r[3][3]
     };
     const float c00 = cof(0, 0), c10 = cof(1, 0), c20 = cof(2, 0);
     const float det = c00 * m[0][0] + c10 * m[1][0] + c20 * m[2][0];
-    const float id = __builtin_amdgcn_rcpf(det);
+    const float id = rcp_nr(det);
     r[0][0] = c00 * id;
     r[0][1] = c10 * id;
     r[0][2] = c20 * id;
