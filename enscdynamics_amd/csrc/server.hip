@@ -247,7 +247,16 @@ ensc_status ensc_server_pump(ensc_server* S, int32_t ticks, uint64_t* done, uint
         if (S->used + r->footprint > S->budget) break;
         S->queue.pop_front();
         uint32_t slot = 0;
-        if ((st = engine_add_prepared(S->eng, std::move(r->job), &slot))) return st;
+        if (engine_add_prepared(S->eng, std::move(r->job), &slot)) {
+            // could not be placed on the device (an allocation or upload
+            // failed): retired with its reason, like a failed set-up
+            Record& rc = S->rec[r->ticket];
+            rc.failed = rc.finished = true;
+            rc.err = ensc_last_error();
+            rc.t_admit = rc.t_finish = S->now();
+            report(r->ticket);
+            continue;
+        }
         const int64_t bytes = engine_job_bytes(S->eng, slot);
         S->used += bytes;
         S->live[slot] = {r->ticket, bytes};
