@@ -948,6 +948,12 @@ uint32_t engine_job_blocks(ensc_engine* E, uint32_t s) {
     Job* J = job_at(E, s);
     return J ? J->dev.nblk_e : 0;
 }
+// Make the engine's device current on the calling thread (allocations and
+// plain copies use the current device).
+ensc_status engine_bind_device(ensc_engine* E) {
+    HIPCHK(hipSetDevice(E->device));
+    return ENSC_OK;
+}
 ensc_status engine_set_graph(ensc_engine* E, int32_t ticks) {
     E->graph_ticks = std::max(0, ticks);
     E->graph_valid = false;
@@ -1101,6 +1107,7 @@ ensc_status ensc_engine_sync(ensc_engine* E) {
 
 ensc_status ensc_engine_set_graph(ensc_engine* E, int32_t ticks) {
     if (!E) return fail(ENSC_ERR_ARG, "null engine");
+    HIPCHK(hipSetDevice(E->device));
     ensc_status st = sync(E);
     if (st) return st;
     return engine_set_graph(E, ticks);
@@ -1136,6 +1143,7 @@ ensc_status ensc_engine_history(ensc_engine* E, double* out, int64_t rows, int64
 
 ensc_status ensc_engine_kernel_times(ensc_engine* E, double* elem_ms, double* node_ms, int64_t* launches) {
     if (!E) return fail(ENSC_ERR_ARG, "null engine");
+    HIPCHK(hipSetDevice(E->device));
     ensc_status st = sync(E);
     if (st) return st;
     if (elem_ms) *elem_ms = E->elem_ms;
@@ -1146,6 +1154,7 @@ ensc_status ensc_engine_kernel_times(ensc_engine* E, double* elem_ms, double* no
 
 ensc_status ensc_engine_phase_times(ensc_engine* E, double* ms4) {
     if (!E || !ms4) return fail(ENSC_ERR_ARG, "null argument");
+    HIPCHK(hipSetDevice(E->device));
     ensc_status st = sync(E);
     if (st) return st;
     for (int p = 0; p < 4; ++p) ms4[p] = E->phase_ms[p];
@@ -1154,6 +1163,7 @@ ensc_status ensc_engine_phase_times(ensc_engine* E, double* ms4) {
 
 ensc_status ensc_engine_layout(ensc_engine* E, int64_t* out) {
     if (!E || !out) return fail(ENSC_ERR_ARG, "null argument");
+    HIPCHK(hipSetDevice(E->device));
     ensc_status st = sync(E);
     if (st) return st;
     out[ENSC_L_STAGED] = E->staged ? 1 : 0;
@@ -1167,6 +1177,7 @@ ensc_status ensc_engine_layout(ensc_engine* E, int64_t* out) {
 
 ensc_status ensc_engine_set_output(ensc_engine* E, double interval, int32_t slots, const char* dir) {
     if (!E) return fail(ENSC_ERR_ARG, "null engine");
+    HIPCHK(hipSetDevice(E->device));
     if (E->slots.size() != 1 || !E->slots[0]) return fail(ENSC_ERR_STATE, "output snapshots need a single-job engine");
     if (slots < 0 || slots > 64) return fail(ENSC_ERR_ARG, "slots must be 0..64");
     ensc_status st;
@@ -1202,6 +1213,7 @@ ensc_status ensc_engine_set_output(ensc_engine* E, double interval, int32_t slot
 
 ensc_status ensc_engine_output_pending(ensc_engine* E, int32_t* n, double* time) {
     if (!E || !n) return fail(ENSC_ERR_ARG, "null argument");
+    HIPCHK(hipSetDevice(E->device));
     ensc_status st;
     if ((st = sync(E)) || (st = read_ctl(E))) return st;
     *n = output_engine(E) ? E->h_ctl[0].snap_head - E->h_ctl[0].snap_tail : 0;
@@ -1214,6 +1226,7 @@ ensc_status ensc_engine_output_pending(ensc_engine* E, int32_t* n, double* time)
 
 ensc_status ensc_engine_output_field(ensc_engine* E, const char* field, double* out, int64_t count) {
     if (!E || !field || !out) return fail(ENSC_ERR_ARG, "null argument");
+    HIPCHK(hipSetDevice(E->device));
     ensc_status st;
     if ((st = sync(E)) || (st = read_ctl(E))) return st;
     if (!output_engine(E)) return fail(ENSC_ERR_STATE, "output snapshots are off");
@@ -1253,6 +1266,7 @@ ensc_status ensc_engine_output_field(ensc_engine* E, const char* field, double* 
 
 ensc_status ensc_engine_output_write(ensc_engine* E, const char* path) {
     if (!E || !path) return fail(ENSC_ERR_ARG, "null argument");
+    HIPCHK(hipSetDevice(E->device));
     ensc_status st;
     if ((st = sync(E)) || (st = read_ctl(E))) return st;
     if (!output_engine(E)) return fail(ENSC_ERR_STATE, "output snapshots are off");
@@ -1261,6 +1275,7 @@ ensc_status ensc_engine_output_write(ensc_engine* E, const char* path) {
 
 ensc_status ensc_engine_output_pop(ensc_engine* E) {
     if (!E) return fail(ENSC_ERR_ARG, "null engine");
+    HIPCHK(hipSetDevice(E->device));
     ensc_status st;
     if ((st = sync(E)) || (st = read_ctl(E))) return st;
     if (!output_engine(E)) return fail(ENSC_ERR_STATE, "output snapshots are off");
@@ -1269,6 +1284,7 @@ ensc_status ensc_engine_output_pop(ensc_engine* E) {
 
 ensc_status ensc_engine_set_timing(ensc_engine* E, int32_t on) {
     if (!E) return fail(ENSC_ERR_ARG, "null engine");
+    HIPCHK(hipSetDevice(E->device));
     ensc_status st = sync(E);
     if (st) return st;
     E->timing = on != 0;

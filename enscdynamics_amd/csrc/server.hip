@@ -39,6 +39,7 @@ ensc_status engine_remove_job(ensc_engine* E, uint32_t s);
 int64_t engine_job_bytes(ensc_engine* E, uint32_t s);
 uint32_t engine_job_blocks(ensc_engine* E, uint32_t s);
 ensc_status engine_set_graph(ensc_engine* E, int32_t ticks);
+ensc_status engine_bind_device(ensc_engine* E);
 }  // namespace ensc
 
 using namespace ensc;
@@ -175,6 +176,8 @@ ensc_status ensc_server_destroy(ensc_server* S) {
 
 ensc_status ensc_server_set_graph(ensc_server* S, int32_t ticks) {
     if (!S) return fail(ENSC_ERR_ARG, "null server");
+    ensc_status st = engine_bind_device(S->eng);
+    if (st) return st;
     return engine_set_graph(S->eng, ticks);
 }
 
@@ -208,6 +211,7 @@ ensc_status ensc_server_pump(ensc_server* S, int32_t ticks, uint64_t* done, uint
     if (!S || !n_done) return fail(ENSC_ERR_ARG, "null argument");
     *n_done = 0;
     ensc_status st;
+    if ((st = engine_bind_device(S->eng))) return st;
     auto report = [&](uint64_t t) {
         if (done && *n_done < cap) done[*n_done] = t;
         ++*n_done;

@@ -9,8 +9,11 @@
  *
  * One ensc_engine holds one or more jobs (meshes) resident in device
  * memory.  All calls are host-thread-safe per engine only when serialized by
- * the caller.  Every call returns an ensc_status; ensc_last_error() gives
- * the message of the last failure on the calling thread.
+ * the caller.  Calls that touch the device make the engine's device
+ * (ensc_options.device) current on the calling thread and leave it current,
+ * so engines on different GPUs can be driven from one thread.  Every call
+ * returns an ensc_status; ensc_last_error() gives the message of the last
+ * failure on the calling thread.
  */
 #ifndef ENSC_ENGINE_H
 #define ENSC_ENGINE_H
