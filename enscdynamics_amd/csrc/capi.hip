@@ -1012,32 +1012,36 @@ ensc_status ensc_copy_bandwidth(int32_t device, int64_t bytes, int32_t reps, dou
 ensc_status ensc_engine_create(const ensc_model_desc* model, const ensc_options* opts, ensc_engine** out) {
     if (!out) return fail(ENSC_ERR_ARG, "out is null");
     *out = nullptr;
-    std::unique_ptr<ensc_engine> E;
-    ensc_status st = engine_new(opts, E);
-    if (st) return st;
-    uint32_t slot = 0;
-    if ((st = add_job(E.get(), model, opts, &slot))) return st;
-    E->slots[0]->keep_in_layout = true;
-    if ((st = relayout(E.get()))) return st;
-    *out = E.release();
-    return ENSC_OK;
+    return guarded([&]() -> ensc_status {
+        std::unique_ptr<ensc_engine> E;
+        ensc_status st = engine_new(opts, E);
+        if (st) return st;
+        uint32_t slot = 0;
+        if ((st = add_job(E.get(), model, opts, &slot))) return st;
+        E->slots[0]->keep_in_layout = true;
+        if ((st = relayout(E.get()))) return st;
+        *out = E.release();
+        return ENSC_OK;
+    });
 }
 
 ensc_status ensc_batch_create(const ensc_model_desc* const* models, uint32_t n_models, const ensc_options* opts,
                               ensc_engine** out) {
     if (!out || !models || n_models == 0) return fail(ENSC_ERR_ARG, "no models");
     *out = nullptr;
-    std::unique_ptr<ensc_engine> E;
-    ensc_status st = engine_new(opts, E);
-    if (st) return st;
-    if ((st = grow_ctl(E.get(), n_models))) return st;
-    for (uint32_t i = 0; i < n_models; ++i) {
-        uint32_t slot = 0;
-        if ((st = add_job(E.get(), models[i], nullptr, &slot))) return st;
-    }
-    if ((st = relayout(E.get()))) return st;
-    *out = E.release();
-    return ENSC_OK;
+    return guarded([&]() -> ensc_status {
+        std::unique_ptr<ensc_engine> E;
+        ensc_status st = engine_new(opts, E);
+        if (st) return st;
+        if ((st = grow_ctl(E.get(), n_models))) return st;
+        for (uint32_t i = 0; i < n_models; ++i) {
+            uint32_t slot = 0;
+            if ((st = add_job(E.get(), models[i], nullptr, &slot))) return st;
+        }
+        if ((st = relayout(E.get()))) return st;
+        *out = E.release();
+        return ENSC_OK;
+    });
 }
 
 ensc_status ensc_engine_destroy(ensc_engine* E) {

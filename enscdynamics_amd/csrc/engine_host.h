@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 
 #include <memory>
+
+#include "guard.h"
 #include <string>
 #include <vector>
 

@@ -24,6 +24,7 @@
 #include <vector>
 
 #include "../../include/ensc_deck.h"
+#include "guard.h"
 
 namespace ensc {
 extern thread_local std::string g_last_error;
@@ -1009,13 +1010,15 @@ ensc_status ensc_deck_read(const char* path, const ensc_project* project, ensc_d
         ensc::g_last_error = "null path";
         return ENSC_ERR_ARG;
     }
-    std::ifstream f(path, std::ios::binary);
-    if (!f.is_open()) {
-        ensc::g_last_error = std::string("cannot open deck ") + path;
-        return ENSC_ERR_ARG;
-    }
-    std::string text((std::istreambuf_iterator<char>(f)), std::istreambuf_iterator<char>());
-    return parse_lines(split_lines(text.data(), text.size()), project, out);
+    return ensc::guarded([&]() -> ensc_status {
+        std::ifstream f(path, std::ios::binary);
+        if (!f.is_open()) {
+            ensc::g_last_error = std::string("cannot open deck ") + path;
+            return ENSC_ERR_ARG;
+        }
+        std::string text((std::istreambuf_iterator<char>(f)), std::istreambuf_iterator<char>());
+        return parse_lines(split_lines(text.data(), text.size()), project, out);
+    });
 }
 
 ensc_status ensc_deck_parse(const char* text, size_t len, const ensc_project* project, ensc_deck** out) {
@@ -1023,7 +1026,7 @@ ensc_status ensc_deck_parse(const char* text, size_t len, const ensc_project* pr
         ensc::g_last_error = "null text";
         return ENSC_ERR_ARG;
     }
-    return parse_lines(split_lines(text, len), project, out);
+    return ensc::guarded([&] { return parse_lines(split_lines(text, len), project, out); });
 }
 
 ensc_status ensc_deck_model(const ensc_deck* deck, ensc_model_desc* out) {

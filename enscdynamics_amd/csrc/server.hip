@@ -107,7 +107,7 @@ struct ensc_server {
                 r->state = PREP_RUNNING;
             }
             std::unique_ptr<Job> J;
-            const ensc_status st = engine_prepare_job(eng, std::move(r->model), J);
+            const ensc_status st = guarded([&] { return engine_prepare_job(eng, std::move(r->model), J); });
             const std::string err = st ? std::string(ensc_last_error()) : std::string();
             {
                 std::lock_guard<std::mutex> lk(mu);
@@ -184,124 +184,128 @@ ensc_status ensc_server_set_graph(ensc_server* S, int32_t ticks) {
 ensc_status ensc_server_submit(ensc_server* S, const ensc_model_desc* model, int32_t keep_result, double t_arrival,
                                uint64_t* ticket) {
     if (!S || !ticket) return fail(ENSC_ERR_ARG, "null argument");
-    std::shared_ptr<Request> r(new Request());
-    std::string err;
-    if (!copy_model(model, r->model, err)) return fail(ENSC_ERR_ARG, err);
-    r->footprint = footprint(r->model, S->precision);
-    if (r->footprint > S->budget) return fail(ENSC_ERR_MEM, "job larger than the server's device budget");
-    r->ticket = S->next_ticket++;
-    r->keep = keep_result;
-    r->n_nodes = r->model.Nn;
-    r->t_submit = t_arrival >= 0.0 ? t_arrival : S->now();
-    Record& rc = S->rec[r->ticket];
-    rc.t_submit = r->t_submit;
-    rc.keep = keep_result;
-    rc.n_nodes = r->n_nodes;
-    *ticket = r->ticket;
-    S->queue.push_back(r);
-    {
-        std::lock_guard<std::mutex> lk(S->mu);
-        S->todo.push_back(r);
-    }
-    S->cv.notify_all();
-    return ENSC_OK;
+    return guarded([&]() -> ensc_status {
+        std::shared_ptr<Request> r(new Request());
+        std::string err;
+        if (!copy_model(model, r->model, err)) return fail(ENSC_ERR_ARG, err);
+        r->footprint = footprint(r->model, S->precision);
+        if (r->footprint > S->budget) return fail(ENSC_ERR_MEM, "job larger than the server's device budget");
+        r->ticket = S->next_ticket++;
+        r->keep = keep_result;
+        r->n_nodes = r->model.Nn;
+        r->t_submit = t_arrival >= 0.0 ? t_arrival : S->now();
+        Record& rc = S->rec[r->ticket];
+        rc.t_submit = r->t_submit;
+        rc.keep = keep_result;
+        rc.n_nodes = r->n_nodes;
+        *ticket = r->ticket;
+        S->queue.push_back(r);
+        {
+            std::lock_guard<std::mutex> lk(S->mu);
+            S->todo.push_back(r);
+        }
+        S->cv.notify_all();
+        return ENSC_OK;
+    });
 }
 
 ensc_status ensc_server_pump(ensc_server* S, int32_t ticks, uint64_t* done, uint32_t cap, uint32_t* n_done) {
     if (!S || !n_done) return fail(ENSC_ERR_ARG, "null argument");
-    *n_done = 0;
-    ensc_status st;
-    if ((st = engine_bind_device(S->eng))) return st;
-    auto report = [&](uint64_t t) {
-        if (done && *n_done < cap) done[*n_done] = t;
-        ++*n_done;
-    };
-    auto remove_retired = [&]() -> ensc_status {
-        for (const auto& z : S->retired) {
-            const ensc_status s2 = engine_remove_job(S->eng, z.first);
-            if (s2) return s2;
-            S->used -= z.second;
-        }
-        S->retired.clear();
-        return ENSC_OK;
-    };
-    // 1. admission of prepared jobs by footprint, arrival order; with nothing
-    //    on the device, wait for the next job's preparation
-    std::vector<uint32_t> started;
-    while (!S->queue.empty() && (int32_t)S->live.size() < S->max_live) {
-        const std::shared_ptr<Request> r = S->queue.front();
-        int state;
-        {
-            std::unique_lock<std::mutex> lk(S->mu);
-            if (r->state < PREP_READY && S->live.empty() && started.empty())
-                S->cv.wait(lk, [&] { return r->state >= PREP_READY; });
-            state = r->state;
-        }
-        if (state < PREP_READY) break;
-        if (state == PREP_FAILED) {
-            Record& rc = S->rec[r->ticket];
-            rc.failed = rc.finished = true;
-            rc.err = r->err;
-            rc.t_admit = rc.t_finish = S->now();
+    return guarded([&]() -> ensc_status {
+        *n_done = 0;
+        ensc_status st;
+        if ((st = engine_bind_device(S->eng))) return st;
+        auto report = [&](uint64_t t) {
+            if (done && *n_done < cap) done[*n_done] = t;
+            ++*n_done;
+        };
+        auto remove_retired = [&]() -> ensc_status {
+            for (const auto& z : S->retired) {
+                const ensc_status s2 = engine_remove_job(S->eng, z.first);
+                if (s2) return s2;
+                S->used -= z.second;
+            }
+            S->retired.clear();
+            return ENSC_OK;
+        };
+        // 1. admission of prepared jobs by footprint, arrival order; with nothing
+        //    on the device, wait for the next job's preparation
+        std::vector<uint32_t> started;
+        while (!S->queue.empty() && (int32_t)S->live.size() < S->max_live) {
+            const std::shared_ptr<Request> r = S->queue.front();
+            int state;
+            {
+                std::unique_lock<std::mutex> lk(S->mu);
+                if (r->state < PREP_READY && S->live.empty() && started.empty())
+                    S->cv.wait(lk, [&] { return r->state >= PREP_READY; });
+                state = r->state;
+            }
+            if (state < PREP_READY) break;
+            if (state == PREP_FAILED) {
+                Record& rc = S->rec[r->ticket];
+                rc.failed = rc.finished = true;
+                rc.err = r->err;
+                rc.t_admit = rc.t_finish = S->now();
+                S->queue.pop_front();
+                report(r->ticket);
+                continue;
+            }
+            if (S->used + r->footprint > S->budget && !S->retired.empty() && (st = remove_retired())) return st;
+            if (S->used + r->footprint > S->budget) break;
             S->queue.pop_front();
-            report(r->ticket);
-            continue;
+            uint32_t slot = 0;
+            if (engine_add_prepared(S->eng, std::move(r->job), &slot)) {
+                // could not be placed on the device (an allocation or upload
+                // failed): retired with its reason, like a failed set-up
+                Record& rc = S->rec[r->ticket];
+                rc.failed = rc.finished = true;
+                rc.err = ensc_last_error();
+                rc.t_admit = rc.t_finish = S->now();
+                report(r->ticket);
+                continue;
+            }
+            const int64_t bytes = engine_job_bytes(S->eng, slot);
+            S->used += bytes;
+            S->live[slot] = {r->ticket, bytes};
+            S->rec[r->ticket].t_admit = S->now();
+            started.push_back(slot);
         }
-        if (S->used + r->footprint > S->budget && !S->retired.empty() && (st = remove_retired())) return st;
-        if (S->used + r->footprint > S->budget) break;
-        S->queue.pop_front();
-        uint32_t slot = 0;
-        if (engine_add_prepared(S->eng, std::move(r->job), &slot)) {
-            // could not be placed on the device (an allocation or upload
-            // failed): retired with its reason, like a failed set-up
-            Record& rc = S->rec[r->ticket];
-            rc.failed = rc.finished = true;
-            rc.err = ensc_last_error();
-            rc.t_admit = rc.t_finish = S->now();
-            report(r->ticket);
-            continue;
+        // 2. new jobs: init pass inside the next step iteration, then step 1
+        if (!started.empty() && (st = engine_start_jobs(S->eng, started))) return st;
+        // 3. advance every live job together
+        if (!S->live.empty() && ticks > 0 && (st = engine_ticks(S->eng, ticks))) return st;
+        // 4. retire finished jobs: results to the host.  Their device memory is
+        //    released once the retired jobs hold a quarter of the element blocks
+        //    still in the launch, when nothing live is left, or when an
+        //    admission needs the room (a finished job's blocks exit at once, so
+        //    keeping it in the layout costs little, while every layout change
+        //    rebuilds the block maps and the step graph).
+        std::vector<uint32_t> fin;
+        if ((st = engine_poll(S->eng, fin))) return st;
+        for (uint32_t slot : fin) {
+            auto it = S->live.find(slot);
+            if (it == S->live.end()) continue;
+            const uint64_t t = it->second.first;
+            Record& rc = S->rec[t];
+            if ((st = engine_job_scalars(S->eng, slot, rc.scalars))) return st;
+            if (rc.keep) {
+                rc.u.assign(3 * (size_t)rc.n_nodes, 0.0);
+                if ((st = engine_job_download(S->eng, slot, "u", rc.u.data(), (int64_t)rc.u.size()))) return st;
+            }
+            rc.t_finish = S->now();
+            rc.finished = true;
+            S->retired.emplace_back(slot, it->second.second);
+            S->live.erase(it);
+            report(t);
         }
-        const int64_t bytes = engine_job_bytes(S->eng, slot);
-        S->used += bytes;
-        S->live[slot] = {r->ticket, bytes};
-        S->rec[r->ticket].t_admit = S->now();
-        started.push_back(slot);
-    }
-    // 2. new jobs: init pass inside the next step iteration, then step 1
-    if (!started.empty() && (st = engine_start_jobs(S->eng, started))) return st;
-    // 3. advance every live job together
-    if (!S->live.empty() && ticks > 0 && (st = engine_ticks(S->eng, ticks))) return st;
-    // 4. retire finished jobs: results to the host.  Their device memory is
-    //    released once the retired jobs hold a quarter of the element blocks
-    //    still in the launch, when nothing live is left, or when an
-    //    admission needs the room (a finished job's blocks exit at once, so
-    //    keeping it in the layout costs little, while every layout change
-    //    rebuilds the block maps and the step graph).
-    std::vector<uint32_t> fin;
-    if ((st = engine_poll(S->eng, fin))) return st;
-    for (uint32_t slot : fin) {
-        auto it = S->live.find(slot);
-        if (it == S->live.end()) continue;
-        const uint64_t t = it->second.first;
-        Record& rc = S->rec[t];
-        if ((st = engine_job_scalars(S->eng, slot, rc.scalars))) return st;
-        if (rc.keep) {
-            rc.u.assign(3 * (size_t)rc.n_nodes, 0.0);
-            if ((st = engine_job_download(S->eng, slot, "u", rc.u.data(), (int64_t)rc.u.size()))) return st;
+        if (!S->retired.empty()) {
+            uint64_t zb = 0, lb = 0;
+            for (const auto& z : S->retired) zb += engine_job_blocks(S->eng, z.first);
+            for (const auto& l : S->live) lb += engine_job_blocks(S->eng, l.first);
+            if (S->live.empty() || zb * 4 > zb + lb) return remove_retired();
         }
-        rc.t_finish = S->now();
-        rc.finished = true;
-        S->retired.emplace_back(slot, it->second.second);
-        S->live.erase(it);
-        report(t);
-    }
-    if (!S->retired.empty()) {
-        uint64_t zb = 0, lb = 0;
-        for (const auto& z : S->retired) zb += engine_job_blocks(S->eng, z.first);
-        for (const auto& l : S->live) lb += engine_job_blocks(S->eng, l.first);
-        if (S->live.empty() || zb * 4 > zb + lb) return remove_retired();
-    }
-    return ENSC_OK;
+        return ENSC_OK;
+    });
 }
 
 ensc_status ensc_server_result(ensc_server* S, uint64_t ticket, double* scalars, double* u, int64_t u_count,
