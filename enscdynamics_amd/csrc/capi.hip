@@ -216,15 +216,17 @@ ensc_status relayout(ensc_engine* E) {
     //  - staged: its largest block node list fits 64 KiB of LDS (72 B per
     //    node; the mixed engine adds the block assembly accumulators, 12 B);
     //  - block assembly (mixed): staged and free of corner conflicts;
-    //  - pipelined pass: block assembly, no amplitude velocity BCs, tiles of
-    //    PIPE_BLK elements and at most PIPE_CAP nodes.
+    //  - pipelined pass: block assembly, no amplitude velocity BCs and tiles
+    //    of PIPE_BLK elements; each such tile with at most PIPE_CAP nodes
+    //    (the rest of the job's tiles use the staged pass).  The split
+    //    follows from the job's own tiles, so it too is the same in any mix.
     const size_t lds_per_node = 9 * sizeof(double) + (mixed ? 12 : 0);
     auto job_staged = [&](const Job& J) {
         return E->staging_opt != 2 && (size_t)J.lds_nodes * lds_per_node <= 65536;
     };
     auto job_asm = [&](const Job& J) { return mixed && J.asm_capable && job_staged(J); };
     auto job_pipe = [&](const Job& J) {
-        return E->pipe_opt && job_asm(J) && !J.dev.dyn_bc && E->blk_e == PIPE_BLK && J.lds_nodes <= (uint32_t)PIPE_CAP;
+        return E->pipe_opt && job_asm(J) && !J.dev.dyn_bc && E->blk_e == PIPE_BLK;
     };
     // block maps: (job, block of the job, staged range) per launched block,
     // in XCD order; element blocks grouped pipelined | staged | gathered,
@@ -241,14 +243,16 @@ ensc_status relayout(ensc_engine* E) {
         const JobDev& D = J.dev;
         pe0.push_back(ne_blocks);
         pn0.push_back(n_blocks);
-        const bool as = job_asm(J);
-        std::vector<uint4>& le = job_pipe(J) ? lp : (job_staged(J) ? ls : lg);
-        if (&le == &ls) {
-            lds_nodes = std::max(lds_nodes, J.lds_nodes);
-            any_asm |= as;
+        const bool as = job_asm(J), pipe = job_pipe(J), staged = job_staged(J);
+        for (uint32_t b = 0; b < D.nblk_e; ++b) {
+            const uint32_t nodes = J.blk_ptr_h[b + 1] - J.blk_ptr_h[b];
+            std::vector<uint4>& le = (pipe && nodes <= (uint32_t)PIPE_CAP) ? lp : (staged ? ls : lg);
+            if (&le == &ls) {
+                lds_nodes = std::max(lds_nodes, nodes);
+                any_asm |= as;
+            }
+            le.push_back(make_uint4(i, b, J.blk_ptr_h[b], nodes));
         }
-        for (uint32_t b = 0; b < D.nblk_e; ++b)
-            le.push_back(make_uint4(i, b, J.blk_ptr_h[b], J.blk_ptr_h[b + 1] - J.blk_ptr_h[b]));
         ne_blocks += D.nblk_e;
         std::vector<uint4>& lnj = as ? lna : ln;
         for (uint32_t b = 0; b < D.nblk_n; ++b) lnj.push_back(make_uint4(i, b, 0, 0));

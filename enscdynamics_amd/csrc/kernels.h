@@ -794,9 +794,10 @@ __global__ __launch_bounds__(BLK) void k_fin_element(const JobDev* __restrict__ 
 // is computed from the first; the only exposed latency per tile is one
 // round trip for its element state and the next tile's node ids.  The body
 // is element_mixed_S's (same operations, same assembly order); the two
-// instantiations may contract multiply-adds differently, so a job is given
-// one of the two passes by its own properties and keeps it in any batch.
-// Jobs with amplitude velocity BCs (applied to staged velocities) use
+// instantiations may contract multiply-adds differently, so each tile is
+// given one of the two passes by its job's properties and its own node
+// count, and keeps it in any batch.  Jobs with amplitude velocity BCs
+// (applied to staged velocities) and tiles of more than PIPE_CAP nodes use
 // k_element.
 //
 // LDS per block is exactly the two buffers, 2 x 5 x 16 B x PIPE_CAP = 80 KiB,
