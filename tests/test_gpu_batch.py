@@ -25,9 +25,9 @@ def small_jobs(n, seed=7, plastic_p=0.0):
 
 @pytest.mark.parametrize("precision,staging", [(eng.FP64, 0), (eng.MIXED, 0), (eng.MIXED, 1)])
 def test_batch_alone_equals_in_mix(precision, staging):
-    # staging 0: jobs whose element blocks fit the pipelined pass run it,
-    # the others the one-block-per-tile pass, decided per job; staging 1:
-    # every job the one-block-per-tile pass
+    # staging 0: element blocks that fit the pipelined pass run it, the
+    # others the one-block-per-tile pass, decided from each job alone;
+    # staging 1: every job the one-block-per-tile pass
     jobs = small_jobs(10, seed=3, plastic_p=0.5)
     # a ramped velocity BC (amplitude) keeps this job on the one-block-per-
     # tile pass, so a mixed batch at staging 0 runs both passes per step
