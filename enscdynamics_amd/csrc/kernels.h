@@ -497,29 +497,38 @@ __device__ __forceinline__ void element_mixed_S(ELEM_ARGS) {
 
 // Fixed-order reduction of a job's block partials (value q of block b at
 // part[q * nblk + b]).  fold_partials: thread t folds blocks t, t+BLK,
-// t+2*BLK, ... in ascending order, the node partials (values 0-1, when
-// `node`) and the element partials (values 2-6, when `elem`) in the same
-// loop, loads issued FOLD_U blocks at a time.  combine_partials: a fixed
-// xor-shuffle tree per wave, then thread 0 adds the waves in index order.
-// Same inputs, same result.  Every thread of the block must call both.
-constexpr int FOLD_U = 4;  // blocks per thread whose loads are issued together
-template <int BLK>
-__device__ inline void fold_partials(const JobDev& J, bool node, bool elem, const bool* is_min, double* acc) {
+// t+2*BLK, ... in ascending order, the node partials (values 0-1) and, with
+// ELEM, the element partials (values 2-6) in the same loop, loads issued
+// FOLD_U blocks at a time (one round for jobs of up to FOLD_U * BLK blocks).
+// It reads only the job table, so its loads go out together with those of
+// the job's scalars; sums of a phase that is not open are not used.
+// combine_partials: a fixed xor-shuffle tree per wave, then thread 0 adds
+// the waves in index order.  Same inputs, same result.  Every thread of the
+// block must call both.
+constexpr int FOLD_U = 8;  // blocks per thread whose loads are issued together
+template <int BLK, bool ELEM>
+__device__ inline void fold_partials(const JobDev& J, const bool* is_min, double* acc) {
     const uint32_t t = threadIdx.x;
 #pragma unroll
     for (int q = 0; q < 7; ++q) acc[q] = is_min[q] ? __builtin_inf() : 0.0;
-    const uint32_t nn = node ? J.nblk_n : 0u, ne = elem ? J.nblk_e : 0u;
+    const uint32_t nn = J.nblk_n, ne = ELEM ? J.nblk_e : 0u;
     const uint32_t nmax = nn > ne ? nn : ne;
     for (uint32_t b0 = t; b0 < nmax; b0 += FOLD_U * BLK) {
         double v[FOLD_U][7];
+        // every load reads a valid partial (index clamped), so they issue
+        // together without branches; out-of-range ones are replaced after
 #pragma unroll
         for (int u = 0; u < FOLD_U; ++u) {
             const uint32_t b = b0 + u * BLK;
+            const uint32_t bn = nn ? (b < nn ? b : nn - 1) : 0u, be = ne ? (b < ne ? b : ne - 1) : 0u;
 #pragma unroll
-            for (int q = 0; q < 2; ++q) v[u][q] = (b < nn) ? J.part_n[(size_t)q * nn + b] : 0.0;
+            for (int q = 0; q < 2; ++q) v[u][q] = J.part_n[(size_t)q * nn + bn];
 #pragma unroll
-            for (int q = 2; q < 7; ++q)
-                v[u][q] = (b < ne) ? J.part_e[(size_t)(q - 2) * ne + b] : (is_min[q] ? __builtin_inf() : 0.0);
+            for (int q = 2; q < 7; ++q) v[u][q] = J.part_e[(size_t)(q - 2) * ne + be];
+#pragma unroll
+            for (int q = 0; q < 2; ++q) v[u][q] = (b < nn) ? v[u][q] : 0.0;
+#pragma unroll
+            for (int q = 2; q < 7; ++q) v[u][q] = (b < ne) ? v[u][q] : (is_min[q] ? __builtin_inf() : 0.0);
         }
 #pragma unroll
         for (int u = 0; u < FOLD_U; ++u)
@@ -549,10 +558,12 @@ __device__ inline void combine_partials(double* acc, const bool* is_min, double 
 }
 
 // End of a job's nodal update (thread 0): kinetic energy, external work and
-// the job's history row for the step; marks the update closed.
-__device__ __forceinline__ void node_phase_finish(const JobDev& J, double ke, double pw) {
+// the job's history row for the step; marks the update closed.  c is the
+// job's Ctl or the caller's register copy of it (written back once): updating
+// the device copy field by field makes every later read of it wait for the
+// stores before it, which the per-tick finalize avoids.
+__device__ __forceinline__ void node_phase_finish(const JobDev& J, Ctl& c, double ke, double pw) {
 #pragma clang fp contract(off)
-    Ctl& c = *J.ctl;
     c.ke = ke;
     c.wext += pw * (0.5 * (c.dt_i + c.dt_i_1));
     const long long row = c.nsteps - 1;
@@ -578,10 +589,9 @@ __device__ __forceinline__ void node_phase_finish(const JobDev& J, double ke, do
 }
 
 // End of a job's element pass (thread 0): tot = min edge, min dtStable,
-// stress, hourglass and viscous work of the pass.
-__device__ __forceinline__ void elem_phase_finish(const JobDev& J, const Decision& dc, const double* tot) {
+// stress, hourglass and viscous work of the pass.  c as in node_phase_finish.
+__device__ __forceinline__ void elem_phase_finish(const JobDev& J, Ctl& c, const Decision& dc, const double* tot) {
 #pragma clang fp contract(off)
-    Ctl& c = *J.ctl;
     const double Lmin = tot[0], dtsn = tot[1];
     c.snap_now = 0;
     if (dc.kind == KIND_PREPASS) {
@@ -767,20 +777,25 @@ __global__ __launch_bounds__(BLK) void k_fin_element(const JobDev* __restrict__ 
 #pragma clang fp contract(off)
     const JobDev& J = jobs[blockIdx.x];
     Ctl* ctl = J.ctl;
-    const bool node_open = ctl->kind == KIND_STEP;
-    const Decision dc = decide(*ctl, J.factor, J.wos);
-    const bool elem = dc.kind != KIND_NONE;
-    if (!node_open && !elem) return;
+    // one copy of the job's scalars, loaded at once; nothing else writes them
+    // while this kernel runs, and thread 0 updates the copy and stores it
+    const Ctl c0 = *ctl;
     // values 0-1: node partials (sums); 2-6: element partials (2 minima, 3 sums)
     const bool is_min[7] = {false, false, true, true, false, false, false};
     double acc[7];
-    fold_partials<BLK>(J, node_open, elem, is_min, acc);
+    fold_partials<BLK, true>(J, is_min, acc);
+    const bool node_open = c0.kind == KIND_STEP;
+    const Decision dc = decide(c0, J.factor, J.wos);
+    const bool elem = dc.kind != KIND_NONE;
+    if (!node_open && !elem) return;
     __shared__ double s_red[BLK / 64][7];
     double tot[7];
     combine_partials<7, BLK>(acc, is_min, s_red, tot);
     if (threadIdx.x == 0) {
-        if (node_open) node_phase_finish(J, tot[0], tot[1]);
-        if (elem) elem_phase_finish(J, dc, tot + 2);
+        Ctl c = c0;
+        if (node_open) node_phase_finish(J, c, tot[0], tot[1]);
+        if (elem) elem_phase_finish(J, c, dc, tot + 2);
+        *ctl = c;
     }
 }
 // ---------------------------------------------------------------------------
@@ -1276,11 +1291,12 @@ __global__ __launch_bounds__(BLK) void k_fin_node(const JobDev* __restrict__ job
     if (J.ctl->kind != KIND_STEP) return;
     const bool is_min[7] = {false, false, true, true, false, false, false};
     double acc[7];
-    fold_partials<BLK>(J, true, false, is_min, acc);
+    fold_partials<BLK, false>(J, is_min, acc);
     __shared__ double s_red[BLK / 64][7];
     double tot[7];
     combine_partials<7, BLK>(acc, is_min, s_red, tot);
-    if (threadIdx.x == 0) node_phase_finish(J, tot[0], tot[1]);
+    // once per run of ticks: the device copy is updated in place
+    if (threadIdx.x == 0) node_phase_finish(J, *J.ctl, tot[0], tot[1]);
 }
 
 // setCurrentStep's apply_boundary_condition_vec (exDyna3D.cpp:1911, 379-490)
