@@ -82,6 +82,9 @@ def c3_section(args, m):
             e.set_graph(args.graph)
         e.init()
         e.set_current_step(1)
+        tuned = e.autotune(args.c3_autotune) if args.c3_autotune else None
+        if tuned:
+            log(f"{name}: autotuned {tuned}")
         e.enqueue(args.warmup)
         e.sync()
         n0 = int(e.scalars()["nsteps"])
@@ -97,7 +100,7 @@ def c3_section(args, m):
         e.sync()
         ph = e.phase_times()
         e.set_timing(False)
-        res[name] = dict(engine=e, secs=t1 - t0, phases=ph, layout=e.layout())
+        res[name] = dict(engine=e, secs=t1 - t0, phases=ph, layout=e.layout(), tuned=tuned)
         log(f"{name}: {args.steps} steps in {t1 - t0:.4f} s; per launch (ms): element {ph[0]:.4f}, "
             f"element finalize {ph[1]:.4f}, node {ph[2]:.4f}, node finalize {ph[3]:.4f}")
     u_mix = res["mixed"]["engine"].get("u")
@@ -117,6 +120,7 @@ def c3_section(args, m):
             "node_gbs": bn * m.n_elems / (ph[2] * 1e-3) / 1e9,
             "step_gbs": (be + bn) * m.n_elems * args.steps / d["secs"] / 1e9,
             "layout": d["layout"],
+            "autotuned": d["tuned"],
         }
         d["engine"].close()
     out["rel_disp_err_mixed_vs_fp64"] = rel_err
@@ -199,11 +203,31 @@ def c4_section(args):
     b.close()
     secs = t2 - t1
     log(f"c4: built in {t1 - t0:.2f} s, ran in {secs:.3f} s")
-    return {"workload": f"config 4: {len(models)} hex-block meshes, Ne log-uniform [1e3, 5e4], "
-                        f"{int(ne.sum())} elements, own end times (2000-8000 CFL steps), elastic, mixed",
-            "jobs": len(models), "solves_per_s": len(models) / secs, "seconds": secs,
-            "elem_steps_per_s": float((ne * steps).sum() / secs), "build_seconds": t1 - t0,
-            "graph_ticks": args.graph}
+    out = {"workload": f"config 4: {len(models)} hex-block meshes, Ne log-uniform [1e3, 5e4], "
+                       f"{int(ne.sum())} elements, own end times (2000-8000 CFL steps), elastic, mixed",
+           "jobs": len(models), "solves_per_s": len(models) / secs, "seconds": secs,
+           "elem_steps_per_s": float((ne * steps).sum() / secs), "build_seconds": t1 - t0,
+           "graph_ticks": args.graph}
+    if args.c4_autotune:
+        # the same batch with its launch configuration autotuned after step 1
+        # is set (the tuning's trial steps are undone; its time is reported)
+        b = Batch(models, eng.MIXED, reorder=True, graph_ticks=args.graph)
+        t3 = time.perf_counter()
+        b.init()
+        b.set_current_step(1)
+        t4 = time.perf_counter()
+        cfg = b.autotune(args.c4_autotune)
+        t5 = time.perf_counter()
+        b.advance(0)
+        t6 = time.perf_counter()
+        steps_t = np.array([b.scalars(i)["nsteps"] for i in range(len(models))])
+        assert np.array_equal(steps_t, steps), "the tuned batch took other step counts"
+        b.close()
+        run_t = (t4 - t3) + (t6 - t5)
+        log(f"c4 autotuned: {cfg}; tuning {t5 - t4:.3f} s, ran in {run_t:.3f} s")
+        out["autotuned"] = {"config": cfg, "tune_seconds": t5 - t4, "seconds": run_t,
+                            "solves_per_s": len(models) / run_t}
+    return out
 
 
 def _pump_until(srv, pending, t0, done_set, ticks):
@@ -410,7 +434,11 @@ def main() -> int:
     ap.add_argument("--kernel-steps", type=int, default=50, help="steps timed per launch with events")
     ap.add_argument("--sections", default="c3,out,c4,c5", help="comma list of c3, out, c4, c5")
     ap.add_argument("--graph", type=int, default=20, help="step iterations per captured hipGraph (0: none)")
+    ap.add_argument("--c3-autotune", type=int, default=0,
+                    help="step iterations per autotuner trial before the c3 timing (0: the default launch shapes)")
     ap.add_argument("--c4-jobs", type=int, default=256)
+    ap.add_argument("--c4-autotune", type=int, default=20,
+                    help="step iterations per autotuner trial for a second, tuned c4 run (0: none)")
     ap.add_argument("--c5-jobs", type=int, default=48)
     ap.add_argument("--c5-live", type=int, default=64)
     ap.add_argument("--c5-ticks", type=int, default=50)

@@ -98,6 +98,23 @@ class Batch:
         _eng._check(_lib().ensc_engine_run(self.h))
         return self
 
+    # step by step, as Engine: init, then per step set_current_step and advance
+    def init(self) -> "Batch":
+        _eng._check(_lib().ensc_engine_init(self.h))
+        return self
+
+    def set_current_step(self, i: int) -> None:
+        _eng._check(_lib().ensc_engine_set_step(self.h, int(i)))
+
+    def advance(self, max_steps: int = 0) -> bool:
+        done = C.c_int32()
+        _eng._check(_lib().ensc_engine_advance(self.h, int(max_steps), C.byref(done)))
+        return bool(done.value)
+
+    def autotune(self, ticks: int = 20) -> Dict[str, float]:
+        """ensc_engine_autotune over every job of the batch."""
+        return _eng.autotune_handle(self.h, ticks)
+
     def __len__(self) -> int:
         return int(_lib().ensc_batch_size(self.h))
 

@@ -74,7 +74,8 @@ struct ensc_engine {
     KernelFn kn = nullptr, kn_asm = nullptr;  // k_node per-corner / block partials
     bool staged = false;      // some live job stages its nodes in LDS
     uint32_t n_asm = 0;       // live jobs using block assembly
-    uint32_t pipe_grid = 0;   // the pipelined kernel's block count (2 per CU)
+    uint32_t pipe_grid = 0;   // the pipelined kernel's block count (pipe_per_cu per CU)
+    int ncu = 0, pipe_per_cu = 2;
     size_t lds_bytes = 0;     // dynamic LDS of the staged k_element
     // output snapshots of a single-job engine: files go to out_dir (empty:
     // the caller drains them), numbered from out_count
@@ -414,6 +415,13 @@ ensc_status sync(ensc_engine* E) {
     return finish_timing(E);
 }
 
+// Blocks of the pipelined pass: `per_cu` per CU (80 KiB of LDS each, so at
+// most two), a multiple of the XCD count.
+void set_pipe_grid(ensc_engine* E, int per_cu) {
+    E->pipe_per_cu = per_cu;
+    E->pipe_grid = (uint32_t)std::max(8, (E->ncu * per_cu + 7) / 8 * 8);
+}
+
 ensc_status engine_new(const ensc_options* opts, std::unique_ptr<ensc_engine>& E) {
     ensc_options o{};
     if (opts) o = *opts;
@@ -434,9 +442,8 @@ ensc_status engine_new(const ensc_options* opts, std::unique_ptr<ensc_engine>& E
     E->waves = o.waves ? o.waves : (o.precision == ENSC_FP64 ? 1 : 2);
     HIPCHK(hipSetDevice(E->device));
     HIPCHK(hipStreamCreateWithFlags(&E->stream, hipStreamNonBlocking));
-    int ncu = 0;
-    HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, E->device));
-    E->pipe_grid = (uint32_t)std::max(8, (ncu * 2 + 7) / 8 * 8);  // two per CU, a multiple of the XCD count
+    HIPCHK(hipDeviceGetAttribute(&E->ncu, hipDeviceAttributeMultiprocessorCount, E->device));
+    set_pipe_grid(E.get(), 2);
     E->pipe_opt = o.staging == 0;
     return ENSC_OK;
 }
@@ -963,6 +970,154 @@ ensc_status engine_set_graph(ensc_engine* E, int32_t ticks) {
     E->graph_valid = false;
     return ENSC_OK;
 }
+
+// Autotuner.  Trial runs advance the jobs, so the state a tick changes is
+// copied aside first and put back after every trial: per live job v and u,
+// the u compensation (mixed) or the positions (FP64), sigma, QIA, plastic
+// strain and reactions, and every job's scalars.  Between runs of ticks the
+// nodal update is always closed (each run ends with the node finalize), so
+// no block partials carry over, and the history and output rings are
+// rewritten as the restored steps run again.
+struct SavedState {
+    std::vector<std::pair<void*, size_t>> regions;
+    char* buf = nullptr;
+    ~SavedState() {
+        if (buf) (void)hipFree(buf);
+    }
+};
+ensc_status save_state(ensc_engine* E, SavedState& S) {
+    const size_t R = E->precision == ENSC_MIXED ? sizeof(float) : sizeof(double);
+    for (uint32_t s : E->live) {
+        const Job& J = *E->slots[s];
+        const size_t nn = J.model.Nn, es = J.ne_s;
+        S.regions.push_back({J.dev.vu, nn * VU_REC * sizeof(double)});
+        if (J.dev.ucomp) S.regions.push_back({J.dev.ucomp, nn * sizeof(uint32_t)});
+        if (E->precision != ENSC_MIXED) S.regions.push_back({J.dev.pos, nn * POS_REC * sizeof(double)});
+        S.regions.push_back({J.dev.sigma, 6 * es * R});
+        S.regions.push_back({J.dev.qia, 12 * es * R});
+        S.regions.push_back({J.dev.epsp, es * R});
+        S.regions.push_back({J.dev.rf, nn * 3 * sizeof(double)});
+    }
+    S.regions.push_back({E->d_ctl, (size_t)E->ctl_cap * sizeof(Ctl)});
+    size_t total = 0;
+    for (const auto& r : S.regions) total += round_up(r.second, 256);
+    if (hipMalloc(&S.buf, total) != hipSuccess) {
+        S.buf = nullptr;
+        return fail(ENSC_ERR_MEM, "autotune: no device memory for the saved state (" + std::to_string(total) + " bytes)");
+    }
+    size_t off = 0;
+    for (const auto& r : S.regions) {
+        HIPCHK(hipMemcpyAsync(S.buf + off, r.first, r.second, hipMemcpyDeviceToDevice, E->stream));
+        off += round_up(r.second, 256);
+    }
+    HIPCHK(hipStreamSynchronize(E->stream));
+    return ENSC_OK;
+}
+ensc_status restore_state(ensc_engine* E, const SavedState& S) {
+    size_t off = 0;
+    for (const auto& r : S.regions) {
+        HIPCHK(hipMemcpyAsync(r.first, S.buf + off, r.second, hipMemcpyDeviceToDevice, E->stream));
+        off += round_up(r.second, 256);
+    }
+    HIPCHK(hipStreamSynchronize(E->stream));
+    return read_ctl(E);
+}
+void set_node_block(ensc_engine* E, int blk) {
+    E->blk_n = blk;
+    for (auto& j : E->slots)
+        if (j) j->dev.nblk_n = (uint32_t)((j->model.Nn + blk - 1) / blk);
+    E->dirty = true;
+}
+// Mean milliseconds per tick of `ticks` ticks (after two untimed ones).
+ensc_status time_ticks(ensc_engine* E, int32_t ticks, double* ms) {
+    ensc_status st;
+    if ((st = enqueue_ticks(E, 2)) || (st = sync(E))) return st;
+    hipEvent_t a, b;
+    HIPCHK(hipEventCreate(&a));
+    HIPCHK(hipEventCreate(&b));
+    float t = 0.f;
+    hipError_t e = hipEventRecord(a, E->stream);
+    if (e == hipSuccess && !(st = enqueue_ticks(E, ticks))) {
+        e = hipEventRecord(b, E->stream);
+        if (e == hipSuccess) e = hipEventSynchronize(b);
+        if (e == hipSuccess) e = hipEventElapsedTime(&t, a, b);
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    if (st) return st;
+    if (e != hipSuccess) return fail(ENSC_ERR_HIP, std::string("autotune timing: ") + hipGetErrorString(e));
+    *ms = (double)t / ticks;
+    return ENSC_OK;
+}
+// Coordinate search over the launch choices that leave every job's tables
+// and element arithmetic unchanged: blocks per CU of the pipelined pass,
+// waves per SIMD of the staged element kernel, and the nodal update's block
+// size (which regroups only the kinetic-energy and external-work block
+// sums).  Keeps the fastest; the jobs' state is as before the call.
+ensc_status autotune(ensc_engine* E, int32_t ticks, int64_t* chosen) {
+    ensc_status st;
+    if ((st = sync(E))) return st;
+    if (E->dirty && (st = relayout(E))) return st;
+    if (E->live.empty()) return fail(ENSC_ERR_STATE, "autotune: no job to step");
+    SavedState S;
+    if ((st = save_state(E, S))) return st;
+    const bool any_pipe = E->nblk_pipe > 0, any_staged = E->nblk_e + E->nblk_g > 0;
+    auto trial = [&](double* ms) -> ensc_status {
+        ensc_status s2 = time_ticks(E, ticks, ms);
+        ensc_status s3 = restore_state(E, S);
+        return s2 ? s2 : s3;
+    };
+    double best = 0.0;
+    if ((st = trial(&best))) return st;
+    if (any_pipe) {
+        const int cur = E->pipe_per_cu;
+        int pick = cur;
+        for (int p : {1, 2}) {
+            if (p == cur) continue;
+            set_pipe_grid(E, p);
+            E->graph_valid = false;
+            double ms = 0.0;
+            if ((st = trial(&ms))) return st;
+            if (ms < best) best = ms, pick = p;
+        }
+        set_pipe_grid(E, pick);
+        E->graph_valid = false;
+    }
+    if (any_staged) {
+        const int cur = E->waves;
+        int pick = cur;
+        for (int w : {1, 2, 3}) {
+            if (w == cur) continue;
+            E->waves = w;
+            E->dirty = true;
+            double ms = 0.0;
+            if ((st = trial(&ms))) return st;
+            if (ms < best) best = ms, pick = w;
+        }
+        E->waves = pick;
+        E->dirty = true;
+    }
+    {
+        const int cur = E->blk_n;
+        int pick = cur;
+        for (int b : {64, 128, 256}) {
+            if (b == cur) continue;
+            set_node_block(E, b);
+            double ms = 0.0;
+            if ((st = trial(&ms))) return st;
+            if (ms < best) best = ms, pick = b;
+        }
+        set_node_block(E, pick);
+    }
+    if (E->dirty && (st = relayout(E))) return st;
+    if (chosen) {
+        chosen[0] = E->waves;
+        chosen[1] = E->blk_n;
+        chosen[2] = E->pipe_per_cu;
+        chosen[3] = (int64_t)(best * 1e6);  // ns per tick of the choice
+    }
+    return ENSC_OK;
+}
 }  // namespace ensc
 
 extern "C" {
@@ -1288,6 +1443,13 @@ ensc_status ensc_engine_output_pop(ensc_engine* E) {
     if ((st = sync(E)) || (st = read_ctl(E))) return st;
     if (!output_engine(E)) return fail(ENSC_ERR_STATE, "output snapshots are off");
     return pop_snapshot(E);
+}
+
+ensc_status ensc_engine_autotune(ensc_engine* E, int32_t ticks, int64_t* chosen) {
+    if (!E) return fail(ENSC_ERR_ARG, "null engine");
+    if (ticks < 1 || ticks > 10000) return fail(ENSC_ERR_ARG, "autotune: ticks must be 1..10000");
+    HIPCHK(hipSetDevice(E->device));
+    return guarded([&] { return autotune(E, ticks, chosen); });
 }
 
 ensc_status ensc_engine_set_timing(ensc_engine* E, int32_t on) {

@@ -33,7 +33,7 @@ EXPORTS = [
     "ensc_engine_sync", "ensc_engine_download", "ensc_engine_scalars", "ensc_engine_history",
     "ensc_engine_set_graph", "ensc_copy_bandwidth", "ensc_engine_kernel_times", "ensc_engine_phase_times", "ensc_engine_layout", "ensc_engine_set_timing",
     "ensc_engine_set_output", "ensc_engine_output_pending", "ensc_engine_output_field", "ensc_engine_output_write",
-    "ensc_engine_output_pop", "ensc_vtu_write", "ensc_engine_device_bytes",
+    "ensc_engine_output_pop", "ensc_vtu_write", "ensc_engine_device_bytes", "ensc_engine_autotune",
 ]
 
 
@@ -91,6 +91,7 @@ def load_library(path: str = LIB_PATH):
         ("ensc_engine_output_write", [vp, C.c_char_p]),
         ("ensc_engine_output_pop", [vp]),
         ("ensc_engine_set_timing", [vp, C.c_int32]),
+        ("ensc_engine_autotune", [vp, C.c_int32, C.POINTER(C.c_int64)]),
         ("ensc_engine_set_graph", [vp, C.c_int32]),
     ]:
         f = getattr(L, name)
@@ -132,6 +133,14 @@ def write_history_csv(path: str, rows: np.ndarray, probes: Sequence[int]) -> int
             f.write(",".join([str(i + 1)] + [repr(float(v)) for v in r]) + "\n")
     return rows.shape[0]
 
+
+
+def autotune_handle(h, ticks: int) -> Dict[str, float]:
+    """ensc_engine_autotune on an engine or batch handle."""
+    out = (C.c_int64 * 4)()
+    _check(load_library().ensc_engine_autotune(h, int(ticks), out))
+    return {"waves": int(out[0]), "block_node": int(out[1]), "pipe_per_cu": int(out[2]),
+            "ms_per_tick": out[3] * 1e-6}
 
 class Engine:
     """One job resident on the GPU, driven like exDyna3D."""
@@ -207,6 +216,12 @@ class Engine:
         """Replay enqueued steps as a hipGraph of `ticks` step iterations
         (0: plain launches)."""
         _check(_lib.ensc_engine_set_graph(self.h, int(ticks)))
+
+    def autotune(self, ticks: int = 20) -> Dict[str, float]:
+        """Time the launch candidates on this engine's jobs and keep the
+        fastest (ensc_engine_autotune; call after init and set_current_step,
+        the jobs continue from where they were)."""
+        return autotune_handle(self.h, ticks)
 
     def set_timing(self, on: bool) -> None:
         _check(_lib.ensc_engine_set_timing(self.h, int(on)))

@@ -173,6 +173,23 @@ ensc_status ensc_engine_output_write(ensc_engine* eng, const char* path);
 /* Drop the oldest snapshot. */
 ensc_status ensc_engine_output_pop(ensc_engine* eng);
 
+/* Autotuner (extension; no counterpart in the reference).  Times `ticks`
+ * step iterations (1..10000, after two untimed ones) of the engine's live
+ * jobs under each candidate launch configuration and keeps the fastest:
+ * blocks per CU of the pipelined element pass (1 or 2), waves per SIMD of
+ * the staged element kernel (1, 2 or 3) and the nodal update's block size
+ * (64, 128 or 256), searched one after the other.  None of these changes a
+ * job's tables or its element and nodal arithmetic; the node block size
+ * regroups only the kinetic-energy and external-work block sums, so those
+ * two diagnostics may change in their last bits.  Call it after
+ * ensc_engine_init and ensc_engine_set_step: the trial steps run from the
+ * current state, which is saved on the device first and restored after
+ * every trial, so the jobs continue from where they were.  chosen (may be
+ * null) receives {waves, node block size, pipelined blocks per CU, ns per
+ * step iteration of the choice}. */
+ensc_status ensc_engine_autotune(ensc_engine* eng, int32_t ticks, int64_t* chosen);
+
+
 /* Enable (1) / disable (0) per-kernel event timing for later enqueue calls. */
 ensc_status ensc_engine_set_timing(ensc_engine* eng, int32_t on);
 /* Device bytes held by the engine. */

@@ -144,3 +144,35 @@ def test_server_reports_a_job_that_cannot_be_set_up():
     for t in (tickets[0], tickets[2]):
         s, _, _ = srv.result(t)
         assert s["done"] == 1.0 and s["err"] == 0.0
+
+
+@pytest.mark.parametrize("precision", [eng.MIXED, eng.FP64])
+def test_autotune_leaves_the_solution_unchanged(precision):
+    # ensc_engine_autotune runs trial steps from the current state and puts
+    # it back; its candidates change launch shapes only.  A batch tuned after
+    # step 1 is set must finish with the same fields as an untuned one, bit
+    # for bit; kinetic energy and external work may differ in their last
+    # bits (the nodal update's block size regroups their block sums).
+    jobs = small_jobs(6, seed=31, plastic_p=0.5)
+    jobs.append(decks.config2(nx=12, ny=6, nz=6, steps=300))  # staged pass (amplitude BC)
+    ref = Batch(jobs, precision, reorder=True).run()
+    b = Batch(jobs, precision, reorder=True, graph_ticks=4)
+    b.init()
+    b.set_current_step(1)
+    cfg = b.autotune(6)
+    print(f"[autotune] chose {cfg}", flush=True)
+    assert cfg["waves"] in (1, 2, 3) and cfg["block_node"] in (64, 128, 256) and cfg["pipe_per_cu"] in (1, 2)
+    assert cfg["ms_per_tick"] > 0.0
+    b.advance(0)
+    for si in range(2, max(len(m.steps) for m in jobs)):
+        b.set_current_step(si)
+        b.advance(0)
+    for i in range(len(jobs)):
+        s, sr = b.scalars(i), ref.scalars(i)
+        assert s["done"] == 1.0 and s["nsteps"] == sr["nsteps"], i
+        for f in ("u", "v", "sigma", "qia", "epsp"):
+            assert np.array_equal(b.get(i, f), ref.get(i, f)), (i, f)
+        for k in ("ie", "hg", "vis", "time"):
+            assert s[k] == sr[k], (i, k)
+        for k in ("ke", "wext"):
+            assert s[k] == pytest.approx(sr[k], rel=1e-12, abs=1e-300), (i, k)
