@@ -176,3 +176,27 @@ def test_autotune_leaves_the_solution_unchanged(precision):
             assert s[k] == sr[k], (i, k)
         for k in ("ke", "wext"):
             assert s[k] == pytest.approx(sr[k], rel=1e-12, abs=1e-300), (i, k)
+
+
+def test_mixed_batch_within_tolerance_of_fp64_batch():
+    # every job of a mixed batch (uneven sizes, half J2) against the same job
+    # in an FP64 batch: displacement within 1e-4 relative, total energy
+    # within 1% of the external work (or of the initial kinetic energy when
+    # nothing does work on the job)
+    jobs = small_jobs(8, seed=41, plastic_p=0.5)
+    a = Batch(jobs, eng.FP64, reorder=True).run()
+    b = Batch(jobs, eng.MIXED, reorder=True).run()
+    worst_u, worst_e = 0.0, 0.0
+    for i in range(len(jobs)):
+        sa, sb = a.scalars(i), b.scalars(i)
+        assert sa["done"] == 1.0 and sb["done"] == 1.0 and abs(sa["nsteps"] - sb["nsteps"]) <= 1, i
+        ua, ub = a.get(i, "u"), b.get(i, "u")
+        du = float(np.abs(ub - ua).max() / max(np.abs(ua).max(), 1e-30))
+        ea = sa["ke"] + sa["ie"] + sa["hg"] + sa["vis"]
+        eb = sb["ke"] + sb["ie"] + sb["hg"] + sb["vis"]
+        scale = max(abs(sa["wext"]), sa["ke0"], abs(ea))
+        de = abs(eb - ea) / scale
+        worst_u, worst_e = max(worst_u, du), max(worst_e, de)
+        assert du <= 1e-4, (i, du)
+        assert de <= 0.01, (i, de)
+    print(f"[mixed batch vs fp64 batch] {len(jobs)} jobs: u {worst_u:.2e}, energy {worst_e:.2e}", flush=True)
