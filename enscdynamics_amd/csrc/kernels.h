@@ -1104,10 +1104,12 @@ __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(ASM ? 8 : 1
                 for (int j = 0; j < 4; ++j) {
                     f[j][0] = f[j][1] = f[j][2] = 0.f;
                     if (q + j < q1) {
+                        // one 12-byte record: a single address, three offsets
                         const uint32_t pos = gld(J.pos_list, q + j);
-                        f[j][0] = gld(J.part_f, 3 * pos);
-                        f[j][1] = gld(J.part_f, 3 * pos + 1);
-                        f[j][2] = gld(J.part_f, 3 * pos + 2);
+                        const ENSC_GLB float* pf = (const ENSC_GLB float*)((const ENSC_GLB char*)J.part_f + pos * 12u);
+                        f[j][0] = pf[0];
+                        f[j][1] = pf[1];
+                        f[j][2] = pf[2];
                     }
                 }
 #pragma unroll
