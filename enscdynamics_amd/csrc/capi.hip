@@ -56,6 +56,10 @@ struct ensc_engine {
     int device = 0, precision = ENSC_FP64, blk_e = 256, blk_n = 256, reorder = 0, staging_opt = 0, waves = 2;
     bool pipe_opt = true;  // staging option 0: pipelined element pass where it applies
     hipStream_t stream = nullptr;
+    // second stream for the staged element blocks of a tick that also runs
+    // the pipelined pass (forked from and joined to `stream` by events)
+    hipStream_t side = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     std::vector<std::unique_ptr<Job>> slots;  // job id = slot index; null = free
     // per-slot scalars (device array + pinned host mirror), never moved while jobs live
     Ctl* d_ctl = nullptr;
@@ -317,17 +321,33 @@ void launch_phase(ensc_engine* E, int phase) {
     const uint32_t nj = (uint32_t)E->live.size();
     if (!nj) return;
     switch (phase) {
-        case 0:
+        case 0: {
+            // The pipelined pass takes every CU's LDS, so staged and gathered
+            // blocks launched beside it (on the side stream) start as its
+            // blocks finish and fill its last round instead of following it.
+            // The passes cover disjoint element blocks of the tick.
+            const bool fork = E->nblk_pipe && (E->nblk_e || E->nblk_g);
+            hipStream_t s2 = E->stream;
+            if (fork) {
+                (void)hipEventRecord(E->ev_fork, E->stream);
+                (void)hipStreamWaitEvent(E->side, E->ev_fork, 0);
+                s2 = E->side;
+            }
             if (E->nblk_pipe)
                 hipLaunchKernelGGL(k_element_pipe<2>, dim3(std::min(E->nblk_pipe, E->pipe_grid)), dim3(PIPE_BLK), 0,
                                    E->stream, jobs, static_cast<const uint4*>(E->b2e_buf.p), E->nblk_pipe);
             if (E->nblk_e)
-                hipLaunchKernelGGL(E->ke, dim3(E->nblk_e), dim3(E->blk_e), E->lds_bytes, E->stream, jobs,
+                hipLaunchKernelGGL(E->ke, dim3(E->nblk_e), dim3(E->blk_e), E->lds_bytes, s2, jobs,
                                    static_cast<const uint4*>(E->b2e_buf.p) + E->nblk_pipe);
             if (E->nblk_g)
-                hipLaunchKernelGGL(E->ke_g, dim3(E->nblk_g), dim3(E->blk_e), 0, E->stream, jobs,
+                hipLaunchKernelGGL(E->ke_g, dim3(E->nblk_g), dim3(E->blk_e), 0, s2, jobs,
                                    static_cast<const uint4*>(E->b2e_buf.p) + E->nblk_pipe + E->nblk_e);
+            if (fork) {
+                (void)hipEventRecord(E->ev_join, E->side);
+                (void)hipStreamWaitEvent(E->stream, E->ev_join, 0);
+            }
             break;
+        }
         case 1: hipLaunchKernelGGL(k_fin_element<FIN_BLK>, dim3(nj), dim3(FIN_BLK), 0, E->stream, jobs); break;
         case 2:
             if (E->nblk_n_asm)
@@ -442,6 +462,9 @@ ensc_status engine_new(const ensc_options* opts, std::unique_ptr<ensc_engine>& E
     E->waves = o.waves ? o.waves : (o.precision == ENSC_FP64 ? 1 : 2);
     HIPCHK(hipSetDevice(E->device));
     HIPCHK(hipStreamCreateWithFlags(&E->stream, hipStreamNonBlocking));
+    HIPCHK(hipStreamCreateWithFlags(&E->side, hipStreamNonBlocking));
+    HIPCHK(hipEventCreateWithFlags(&E->ev_fork, hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&E->ev_join, hipEventDisableTiming));
     HIPCHK(hipDeviceGetAttribute(&E->ncu, hipDeviceAttributeMultiprocessorCount, E->device));
     set_pipe_grid(E.get(), 2);
     E->pipe_opt = o.staging == 0;
@@ -1207,12 +1230,16 @@ ensc_status ensc_engine_destroy(ensc_engine* E) {
     if (!E) return ENSC_OK;
     (void)hipSetDevice(E->device);
     (void)hipStreamSynchronize(E->stream);
+    if (E->side) (void)hipStreamSynchronize(E->side);
     for (hipEvent_t ev : E->evs) (void)hipEventDestroy(ev);
     if (E->graph_exec) (void)hipGraphExecDestroy(E->graph_exec);
     if (E->graph) (void)hipGraphDestroy(E->graph);
     E->slots.clear();
     if (E->d_ctl) (void)hipFree(E->d_ctl);
     if (E->h_ctl) (void)hipHostFree(E->h_ctl);
+    if (E->ev_fork) (void)hipEventDestroy(E->ev_fork);
+    if (E->ev_join) (void)hipEventDestroy(E->ev_join);
+    if (E->side) (void)hipStreamDestroy(E->side);
     if (E->stream) (void)hipStreamDestroy(E->stream);
     delete E;
     return ENSC_OK;
