@@ -1051,10 +1051,14 @@ void set_node_block(ensc_engine* E, int blk) {
         if (j) j->dev.nblk_n = (uint32_t)((j->model.Nn + blk - 1) / blk);
     E->dirty = true;
 }
-// Mean milliseconds per tick of `ticks` ticks (after two untimed ones).
+// Mean milliseconds per tick of `ticks` ticks (after two untimed ones; the
+// layout and the step graph of the candidate are built before the clock
+// starts).
 ensc_status time_ticks(ensc_engine* E, int32_t ticks, double* ms) {
     ensc_status st;
     if ((st = enqueue_ticks(E, 2)) || (st = sync(E))) return st;
+    if (E->dirty && (st = relayout(E))) return st;
+    if (E->graph_ticks > 0 && ticks >= E->graph_ticks && !E->graph_valid && (st = build_graph(E))) return st;
     hipEvent_t a, b;
     HIPCHK(hipEventCreate(&a));
     HIPCHK(hipEventCreate(&b));
