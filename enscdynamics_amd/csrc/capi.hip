@@ -54,6 +54,7 @@ struct DevBuf {
 
 struct ensc_engine {
     int device = 0, precision = ENSC_FP64, blk_e = 256, blk_n = 256, reorder = 0, staging_opt = 0, waves = 2;
+    int comp_opt = 0;  // ensc_options.state_comp
     bool pipe_opt = true;  // staging option 0: pipelined element pass where it applies
     hipStream_t stream = nullptr;
     // second stream for the staged element blocks of a tick that also runs
@@ -221,7 +222,8 @@ ensc_status relayout(ensc_engine* E) {
     //  - staged: its largest block node list fits 64 KiB of LDS (72 B per
     //    node; the mixed engine adds the block assembly accumulators, 12 B);
     //  - block assembly (mixed): staged and free of corner conflicts;
-    //  - pipelined pass: block assembly, no amplitude velocity BCs and tiles
+    //  - pipelined pass: block assembly, no amplitude velocity BCs, no
+    //    residual accumulation of sigma and QIA, and tiles
     //    of PIPE_BLK elements; each such tile with at most PIPE_CAP nodes
     //    (the rest of the job's tiles use the staged pass).  The split
     //    follows from the job's own tiles, so it too is the same in any mix.
@@ -231,7 +233,7 @@ ensc_status relayout(ensc_engine* E) {
     };
     auto job_asm = [&](const Job& J) { return mixed && J.asm_capable && job_staged(J); };
     auto job_pipe = [&](const Job& J) {
-        return E->pipe_opt && job_asm(J) && !J.dev.dyn_bc && E->blk_e == PIPE_BLK;
+        return E->pipe_opt && job_asm(J) && !J.dev.dyn_bc && !J.comp && E->blk_e == PIPE_BLK;
     };
     // block maps: (job, block of the job, staged range) per launched block,
     // in XCD order; element blocks grouped pipelined | staged | gathered,
@@ -316,10 +318,16 @@ ensc_status relayout(ensc_engine* E) {
 // per-job node finalize.  With timing on, every tick launches all four
 // phases so each is timed on its own.
 constexpr int FIN_BLK = 512;
-void launch_phase(ensc_engine* E, int phase) {
+// Returns the first error of the phase's runtime calls (kernel launches
+// report theirs through hipGetLastError).
+hipError_t launch_phase(ensc_engine* E, int phase) {
     const JobDev* jobs = static_cast<const JobDev*>(E->jobs_buf.p);
     const uint32_t nj = (uint32_t)E->live.size();
-    if (!nj) return;
+    if (!nj) return hipSuccess;
+    hipError_t err = hipSuccess;
+    auto chk = [&](hipError_t e) {
+        if (err == hipSuccess && e != hipSuccess) err = e;
+    };
     switch (phase) {
         case 0: {
             // The pipelined pass takes every CU's LDS, so staged and gathered
@@ -329,8 +337,8 @@ void launch_phase(ensc_engine* E, int phase) {
             const bool fork = E->nblk_pipe && (E->nblk_e || E->nblk_g);
             hipStream_t s2 = E->stream;
             if (fork) {
-                (void)hipEventRecord(E->ev_fork, E->stream);
-                (void)hipStreamWaitEvent(E->side, E->ev_fork, 0);
+                chk(hipEventRecord(E->ev_fork, E->stream));
+                chk(hipStreamWaitEvent(E->side, E->ev_fork, 0));
                 s2 = E->side;
             }
             if (E->nblk_pipe)
@@ -343,8 +351,8 @@ void launch_phase(ensc_engine* E, int phase) {
                 hipLaunchKernelGGL(E->ke_g, dim3(E->nblk_g), dim3(E->blk_e), 0, s2, jobs,
                                    static_cast<const uint4*>(E->b2e_buf.p) + E->nblk_pipe + E->nblk_e);
             if (fork) {
-                (void)hipEventRecord(E->ev_join, E->side);
-                (void)hipStreamWaitEvent(E->stream, E->ev_join, 0);
+                chk(hipEventRecord(E->ev_join, E->side));
+                chk(hipStreamWaitEvent(E->stream, E->ev_join, 0));
             }
             break;
         }
@@ -359,26 +367,51 @@ void launch_phase(ensc_engine* E, int phase) {
             break;
         default: hipLaunchKernelGGL(k_fin_node<FIN_BLK>, dim3(nj), dim3(FIN_BLK), 0, E->stream, jobs); break;
     }
+    return err;
 }
-void launch_ticks(ensc_engine* E, int64_t n) {
-    for (int64_t i = 0; i < n; ++i)
-        for (int p = 0; p < 3; ++p) launch_phase(E, p);
-    if (n > 0) launch_phase(E, 3);
+hipError_t launch_ticks(ensc_engine* E, int64_t n) {
+    hipError_t err = hipSuccess;
+    for (int64_t i = 0; i < n && err == hipSuccess; ++i)
+        for (int p = 0; p < 3 && err == hipSuccess; ++p) err = launch_phase(E, p);
+    if (n > 0 && err == hipSuccess) err = launch_phase(E, 3);
+    return err;
 }
 
+void drop_graph(ensc_engine* E) {
+    if (E->graph_exec) (void)hipGraphExecDestroy(E->graph_exec);
+    if (E->graph) (void)hipGraphDestroy(E->graph);
+    E->graph_exec = nullptr;
+    E->graph = nullptr;
+    E->graph_valid = false;
+}
+
+// Capture graph_ticks ticks of the current layout.  The previous graph is
+// destroyed only once the stream is idle (a replay of it may still be
+// queued when set_graph or the autotuner invalidated it), and every call
+// made while capturing is checked: a fork/join event call that failed
+// would leave the side stream outside the capture, its kernels running at
+// capture time instead of in the graph.
 ensc_status build_graph(ensc_engine* E) {
-    if (E->graph_exec) {
-        (void)hipGraphExecDestroy(E->graph_exec);
-        E->graph_exec = nullptr;
-    }
-    if (E->graph) {
-        (void)hipGraphDestroy(E->graph);
-        E->graph = nullptr;
-    }
+    HIPCHK(hipStreamSynchronize(E->stream));
+    drop_graph(E);
     HIPCHK(hipStreamBeginCapture(E->stream, hipStreamCaptureModeThreadLocal));
-    launch_ticks(E, E->graph_ticks);
-    HIPCHK(hipStreamEndCapture(E->stream, &E->graph));
-    HIPCHK(hipGraphInstantiate(&E->graph_exec, E->graph, nullptr, nullptr, 0));
+    const hipError_t le = launch_ticks(E, E->graph_ticks);
+    const hipError_t ke = hipGetLastError();
+    hipGraph_t g = nullptr;
+    const hipError_t ee = hipStreamEndCapture(E->stream, &g);
+    if (le != hipSuccess || ke != hipSuccess || ee != hipSuccess) {
+        if (ee == hipSuccess && g) (void)hipGraphDestroy(g);
+        const hipError_t e = le != hipSuccess ? le : (ke != hipSuccess ? ke : ee);
+        return fail(ENSC_ERR_HIP, std::string("step graph capture: ") + hipGetErrorString(e));
+    }
+    hipGraphExec_t x = nullptr;
+    const hipError_t ie = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+    if (ie != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        return fail(ENSC_ERR_HIP, std::string("step graph instantiate: ") + hipGetErrorString(ie));
+    }
+    E->graph = g;
+    E->graph_exec = x;
     E->graph_valid = true;
     return ENSC_OK;
 }
@@ -396,7 +429,7 @@ ensc_status enqueue_ticks(ensc_engine* E, int64_t n) {
         HIPCHK(hipEventRecord(E->evs[0], E->stream));
         for (int64_t i = 0; i < n; ++i)
             for (int p = 0; p < 4; ++p) {
-                launch_phase(E, p);
+                HIPCHK(launch_phase(E, p));
                 HIPCHK(hipEventRecord(E->evs[4 * i + p + 1], E->stream));
             }
         E->pending_ticks = n;
@@ -406,7 +439,7 @@ ensc_status enqueue_ticks(ensc_engine* E, int64_t n) {
             if (!E->graph_valid && (st = build_graph(E))) return st;
             for (; i + E->graph_ticks <= n; i += E->graph_ticks) HIPCHK(hipGraphLaunch(E->graph_exec, E->stream));
         }
-        launch_ticks(E, n - i);
+        HIPCHK(launch_ticks(E, n - i));
         E->pending_ticks = 0;
     }
     HIPCHK(hipGetLastError());
@@ -459,6 +492,8 @@ ensc_status engine_new(const ensc_options* opts, std::unique_ptr<ensc_engine>& E
     E->blk_e = okblk(o.block_elem) ? o.block_elem : (o.precision == ENSC_FP64 ? 128 : 256);
     E->blk_n = okblk(o.block_node) ? o.block_node : 256;
     if (o.waves < 0 || o.waves > 3) return fail(ENSC_ERR_ARG, "waves must be 0 (default), 1, 2 or 3");
+    if (o.state_comp < 0 || o.state_comp > 2) return fail(ENSC_ERR_ARG, "state_comp must be 0 (auto), 1 or 2");
+    E->comp_opt = o.state_comp;
     E->waves = o.waves ? o.waves : (o.precision == ENSC_FP64 ? 1 : 2);
     HIPCHK(hipSetDevice(E->device));
     HIPCHK(hipStreamCreateWithFlags(&E->stream, hipStreamNonBlocking));
@@ -518,7 +553,7 @@ ensc_status add_job(ensc_engine* E, const ensc_model_desc* model, const ensc_opt
     std::unique_ptr<Job> J(new Job());
     std::string err;
     if (!copy_model(model, J->model, err)) return fail(ENSC_ERR_ARG, err);
-    ensc_status st = prepare_job(*J, E->precision, E->reorder, E->blk_e, E->blk_n);
+    ensc_status st = prepare_job(*J, E->precision, E->reorder, E->blk_e, E->blk_n, E->comp_opt);
     if (st) return st;
     return add_prepared(E, std::move(J), opts, slot_out);
 }
@@ -557,9 +592,8 @@ ensc_status job_switch_step(ensc_engine* E, uint32_t slot, uint32_t si) {
     J.self_copy = J.dev;
     HIPCHK(hipMemcpyAsync(J.d_self, &J.self_copy, sizeof(JobDev), hipMemcpyHostToDevice, E->stream));
     const uint32_t n = (uint32_t)nodes.size();
-    const uint32_t blocks = std::max<uint32_t>(1, (n + 255) / 256);
-    hipLaunchKernelGGL(k_apply_bc, dim3(blocks), dim3(256), 0, E->stream, J.d_self, 0u, (const uint32_t*)J.d_bcnodes,
-                       n, J.model.steps[si].period, 1);
+    hipLaunchKernelGGL(k_apply_bc, dim3(1), dim3(APPLY_BC_BLK), 0, E->stream, J.d_self, 0u,
+                       (const uint32_t*)J.d_bcnodes, n, J.model.steps[si].period, 1);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(E->stream));  // host vectors above go out of scope
     J.pending_step = false;
@@ -872,6 +906,12 @@ ensc_status download(ensc_engine* E, Job& J, const std::string& f, double* out, 
         std::vector<float> t32((size_t)nc * es);
         HIPCHK(hipMemcpy(t32.data(), src, t32.size() * sizeof(float), hipMemcpyDeviceToHost));
         for (size_t i = 0; i < t32.size(); ++i) tmp[i] = t32[i];
+        // residual accumulation: the value is hi + lo
+        const float* lo = f == "sigma" ? J.dev.sig_lo : (f == "qia" ? J.dev.qia_lo : nullptr);
+        if (lo) {
+            HIPCHK(hipMemcpy(t32.data(), lo, t32.size() * sizeof(float), hipMemcpyDeviceToHost));
+            for (size_t i = 0; i < t32.size(); ++i) tmp[i] += (double)t32[i];
+        }
     } else {
         HIPCHK(hipMemcpy(tmp.data(), src, tmp.size() * sizeof(double), hipMemcpyDeviceToHost));
     }
@@ -903,7 +943,7 @@ ensc_status engine_add_job(ensc_engine* E, const ensc_model_desc* m, uint32_t* s
 ensc_status engine_prepare_job(const ensc_engine* E, ModelCopy&& model, std::unique_ptr<Job>& out) {
     std::unique_ptr<Job> J(new Job());
     J->model = std::move(model);
-    ensc_status st = prepare_job(*J, E->precision, E->reorder, E->blk_e, E->blk_n);
+    ensc_status st = prepare_job(*J, E->precision, E->reorder, E->blk_e, E->blk_n, E->comp_opt);
     if (st) return st;
     out = std::move(J);
     return ENSC_OK;
@@ -939,17 +979,34 @@ ensc_status engine_start_jobs(ensc_engine* E, const std::vector<uint32_t>& slots
     return ENSC_OK;
 }
 ensc_status engine_ticks(ensc_engine* E, int64_t n) { return enqueue_ticks(E, n); }
+// Jobs that reached the end of their current *Step: those with a later step
+// switch to it (setCurrentStep, as exDyna3D::run's step loop does,
+// exDyna3D.cpp:77-99) and keep running; the others (and jobs with an error
+// flag) are reported finished.
 ensc_status engine_poll(ensc_engine* E, std::vector<uint32_t>& finished) {
     ensc_status st;
     if ((st = sync(E)) || (st = read_ctl(E))) return st;
     finished.clear();
+    std::vector<uint32_t> next;
     for (uint32_t s = 0; s < E->slots.size(); ++s) {
         Job* J = E->slots[s].get();
         if (!J || !J->inited || J->pending_step) continue;
         const Ctl& c = E->h_ctl[s];
         if (c.init_pending) continue;
-        if (!(c.time < c.step_end) || c.err) finished.push_back(s);
+        if (!(c.time < c.step_end) || c.err) {
+            if (!c.err && J->cur_step + 1 < J->model.steps.size())
+                next.push_back(s);
+            else
+                finished.push_back(s);
+        }
     }
+    if (next.empty()) return ENSC_OK;
+    for (uint32_t s : next)
+        if ((st = job_switch_step(E, s, E->slots[s]->cur_step + 1))) return st;
+    if ((st = read_ctl(E))) return st;
+    for (uint32_t s : next) E->h_ctl[s].step_index = (int32_t)E->slots[s]->cur_step;
+    if ((st = write_ctl(E))) return st;
+    E->dirty = true;
     return ENSC_OK;
 }
 ensc_status engine_job_scalars(ensc_engine* E, uint32_t s, double* out) {
@@ -1019,6 +1076,8 @@ ensc_status save_state(ensc_engine* E, SavedState& S) {
         S.regions.push_back({J.dev.sigma, 6 * es * R});
         S.regions.push_back({J.dev.qia, 12 * es * R});
         S.regions.push_back({J.dev.epsp, es * R});
+        if (J.dev.sig_lo) S.regions.push_back({J.dev.sig_lo, 6 * es * sizeof(float)});
+        if (J.dev.qia_lo) S.regions.push_back({J.dev.qia_lo, 12 * es * sizeof(float)});
         S.regions.push_back({J.dev.rf, nn * 3 * sizeof(double)});
     }
     S.regions.push_back({E->d_ctl, (size_t)E->ctl_cap * sizeof(Ctl)});
@@ -1236,8 +1295,7 @@ ensc_status ensc_engine_destroy(ensc_engine* E) {
     (void)hipStreamSynchronize(E->stream);
     if (E->side) (void)hipStreamSynchronize(E->side);
     for (hipEvent_t ev : E->evs) (void)hipEventDestroy(ev);
-    if (E->graph_exec) (void)hipGraphExecDestroy(E->graph_exec);
-    if (E->graph) (void)hipGraphDestroy(E->graph);
+    drop_graph(E);
     E->slots.clear();
     if (E->d_ctl) (void)hipFree(E->d_ctl);
     if (E->h_ctl) (void)hipHostFree(E->h_ctl);

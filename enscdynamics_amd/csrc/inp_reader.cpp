@@ -675,7 +675,7 @@ std::string jstr(const std::string& s) {
     return o + "\"";
 }
 
-bool build_deck(Parser& P, const ensc_project* proj, ensc_deck& D, std::string& err) {
+bool build_deck(Parser& P, const ensc_project* proj, uint32_t flags, ensc_deck& D, std::string& err) {
     // transferToExDyna (dataIn.cpp:1550-1788)
     std::vector<Point3> verts;
     std::vector<std::vector<uint32_t>> elems;
@@ -821,7 +821,9 @@ bool build_deck(Parser& P, const ensc_project* proj, ensc_deck& D, std::string& 
             e.d1 = (int32_t)R.vel[i].d1;
             e.cond = 1;
             e.value = R.vel[i].value;
-            e.amp = amp_index(R.vel[i].amp);
+            // the reference's velocity BCs are constant (BoundaryCondition.h:18,
+            // dataIn.cpp:1114-1195 drops AMPLITUDE=): kept only on request
+            e.amp = (flags & ENSC_DECK_VEL_AMPLITUDE) ? amp_index(R.vel[i].amp) : -1;
             D.vel[s].push_back(e);
         }
         ensc_step_def& sd = D.steps[s];
@@ -855,9 +857,12 @@ bool build_deck(Parser& P, const ensc_project* proj, ensc_deck& D, std::string& 
     m.E = P.E;
     m.nu = P.nu;
     m.rho = P.rho;
-    m.plastic = P.plastic;
-    m.yield0 = P.yield0;
-    m.hard = P.hard;
+    // *Plastic has no effect in the reference (dataIn.cpp:69-92 has no such
+    // keyword): J2 only on request
+    const bool pl = (flags & ENSC_DECK_PLASTIC) && P.plastic;
+    m.plastic = pl ? 1 : 0;
+    m.yield0 = pl ? P.yield0 : 0.0;
+    m.hard = pl ? P.hard : 0.0;
     m.c_dr = proj ? proj->c_dr : 0.013;
     m.c_cr = proj ? proj->c_cr : 0.05;
     m.factor = proj ? proj->factor : 0.71;
@@ -917,7 +922,7 @@ bool build_deck(Parser& P, const ensc_project* proj, ensc_deck& D, std::string& 
     return true;
 }
 
-ensc_status parse_lines(std::vector<std::string>&& lines, const ensc_project* proj, ensc_deck** out) {
+ensc_status parse_lines(std::vector<std::string>&& lines, const ensc_project* proj, uint32_t flags, ensc_deck** out) {
     if (!out) {
         ensc::g_last_error = "out is null";
         return ENSC_ERR_ARG;
@@ -928,7 +933,7 @@ ensc_status parse_lines(std::vector<std::string>&& lines, const ensc_project* pr
     P.parse();
     ensc_deck* D = new ensc_deck();
     std::string err;
-    if (!build_deck(P, proj, *D, err)) {
+    if (!build_deck(P, proj, flags, *D, err)) {
         delete D;
         ensc::g_last_error = err;
         return ENSC_ERR_ARG;
@@ -1005,7 +1010,7 @@ ensc_status ensc_project_read(const char* path, ensc_project* out) {
     return ENSC_OK;
 }
 
-ensc_status ensc_deck_read(const char* path, const ensc_project* project, ensc_deck** out) {
+ensc_status ensc_deck_read_ex(const char* path, const ensc_project* project, uint32_t flags, ensc_deck** out) {
     if (!path) {
         ensc::g_last_error = "null path";
         return ENSC_ERR_ARG;
@@ -1017,16 +1022,25 @@ ensc_status ensc_deck_read(const char* path, const ensc_project* project, ensc_d
             return ENSC_ERR_ARG;
         }
         std::string text((std::istreambuf_iterator<char>(f)), std::istreambuf_iterator<char>());
-        return parse_lines(split_lines(text.data(), text.size()), project, out);
+        return parse_lines(split_lines(text.data(), text.size()), project, flags, out);
     });
 }
 
-ensc_status ensc_deck_parse(const char* text, size_t len, const ensc_project* project, ensc_deck** out) {
+ensc_status ensc_deck_parse_ex(const char* text, size_t len, const ensc_project* project, uint32_t flags,
+                               ensc_deck** out) {
     if (!text) {
         ensc::g_last_error = "null text";
         return ENSC_ERR_ARG;
     }
-    return ensc::guarded([&] { return parse_lines(split_lines(text, len), project, out); });
+    return ensc::guarded([&] { return parse_lines(split_lines(text, len), project, flags, out); });
+}
+
+ensc_status ensc_deck_read(const char* path, const ensc_project* project, ensc_deck** out) {
+    return ensc_deck_read_ex(path, project, 0u, out);
+}
+
+ensc_status ensc_deck_parse(const char* text, size_t len, const ensc_project* project, ensc_deck** out) {
+    return ensc_deck_parse_ex(text, len, project, 0u, out);
 }
 
 ensc_status ensc_deck_model(const ensc_deck* deck, ensc_model_desc* out) {

@@ -462,7 +462,7 @@ __device__ __forceinline__ void element_fp64_G(ELEM_ARGS) {
 template <class R>
 __device__ __forceinline__ void element_mixed_G(ELEM_ARGS) {
 #pragma clang fp contract(fast)
-    constexpr bool MIXED = true, ASM_CAP = false;
+    constexpr bool MIXED = true, ASM_CAP = false, COMP_CAP = true;
 #include "elem_body_mixed.inc"
 }
 #undef NVAL
@@ -484,7 +484,7 @@ __device__ __forceinline__ void element_fp64_S(ELEM_ARGS) {
 template <class R>
 __device__ __forceinline__ void element_mixed_S(ELEM_ARGS) {
 #pragma clang fp contract(fast)
-    constexpr bool MIXED = true, ASM_CAP = true;
+    constexpr bool MIXED = true, ASM_CAP = true, COMP_CAP = true;
 #include "elem_body_mixed.inc"
 }
 #undef NVAL
@@ -557,15 +557,16 @@ __device__ inline void combine_partials(double* acc, const bool* is_min, double 
     }
 }
 
-// End of a job's nodal update (thread 0): kinetic energy, external work and
-// the job's history row for the step; marks the update closed.  c is the
+// End of a job's nodal update (thread 0): kinetic energy, external work (w:
+// the step's work summed over the nodes) and the job's history row for the
+// step; marks the update closed.  c is the
 // job's Ctl or the caller's register copy of it (written back once): updating
 // the device copy field by field makes every later read of it wait for the
 // stores before it, which the per-tick finalize avoids.
-__device__ __forceinline__ void node_phase_finish(const JobDev& J, Ctl& c, double ke, double pw) {
+__device__ __forceinline__ void node_phase_finish(const JobDev& J, Ctl& c, double ke, double w) {
 #pragma clang fp contract(off)
     c.ke = ke;
-    c.wext += pw * (0.5 * (c.dt_i + c.dt_i_1));
+    c.wext += w;
     const long long row = c.nsteps - 1;
     if (J.hist && row >= 0 && row < J.hist_cap) {
         double* h = J.hist + row * (ENSC_H_BASE + 3 * J.n_probe);
@@ -897,7 +898,8 @@ __device__ __forceinline__ void element_mixed_P(const JobDev& J, uint32_t e, con
                                                 uint32_t nbn, PipeNext& pn, ElemState<float>& ESn, uint32_t gn[2]) {
 #pragma clang fp contract(fast)
     typedef float R;
-    constexpr bool MIXED = true, ASM_CAP = true;
+    // jobs with residual accumulation never run the pipelined pass
+    constexpr bool MIXED = true, ASM_CAP = true, COMP_CAP = false;
     ENSC_LDS float* acc = pipe_acc(buf);
 #include "elem_body_mixed.inc"
 }
@@ -1073,7 +1075,9 @@ __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(ASM ? 8 : 1
         const R_* sg = static_cast<const R_*>(J.sigma);
         for (uint32_t e = n; e < J.ne; e += J.nn)
 #pragma unroll
-            for (int q = 0; q < 6; ++q) se[(size_t)e * 6 + q] = (double)sg[(size_t)q * J.ne_s + e];
+            for (int q = 0; q < 6; ++q)
+                se[(size_t)e * 6 + q] = (double)sg[(size_t)q * J.ne_s + e] +
+                                        (MIXED && J.sig_lo ? (double)J.sig_lo[(size_t)q * J.ne_s + e] : 0.0);
     }
     if (n < J.nn) {
         // [X0 X1 | X2 im] [v0 v1 | v2 u0 | u1 u2]; the mixed engine never
@@ -1181,31 +1185,48 @@ __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(ASM ? 8 : 1
             a[1] += ag * J.grav_dir[1] / im;
             a[2] += ag * J.grav_dir[2] / im;
         }
+        // External work of the step (extension; the reference keeps no energy
+        // account), per node:
+        //   wr: the reactions' power at velocity-BC dofs, -(f_int + f_grav) v_c,
+        //       v_c the constrained velocity (the BC value, or 0 where an SPC
+        //       line also holds the dof);
+        //   wg: gravity's power on every dof over g m, dir . (v_old + v_new),
+        //       v_old the velocity after the BCs;
+        //   wj: twice the kinetic energy over m that a constraint adds or
+        //       removes when it sets v (a ramped velocity BC), v_c^2 - v_record^2.
+        // Work = dt_avg (wr + ag m wg / 2) + m wj / 2, dt_avg = (dt_i + dt_i_1) / 2;
+        // the constraint terms are folded into pw where they are formed.
+        const double hs = dt_i + dt_i_1;
+        double wg = 0.0, kj = 0.0;
         const uint8_t mk = gld(J.bcmask, n);
-        double vbc[3] = {0.0, 0.0, 0.0};
+        double vbc[3] = {0.0, 0.0, 0.0};  // constrained velocity of each constrained dof
         if (mk) {  // apply_boundary_condition_a (2029-2145): velocity lines, then SPC lines
             double rfv[3] = {0.0, 0.0, 0.0};
+            double wr = 0.0, wj = 0.0;
 #pragma unroll
             for (int d = 0; d < 3; ++d) {
                 if (mk & (BC_VEL0 << d)) {
                     vbc[d] = gld(J.bcval, n * 3 + d) * amp_value(J, gld(J.bcamp, n * 3 + d), time);
-                    pw += -a[d] * vbc[d];
+                    if (!(mk & (BC_SPC0 << d))) wr += -a[d] * vbc[d];
                     a[d] = 0.0;
                 }
                 if (mk & (BC_SPC0 << d)) {
+                    vbc[d] = 0.0;
                     if (a[d] != 0.0) {
                         rfv[d] = -a[d];
                         a[d] = 0.0;
                     }
                 }
+                if (mk & ((BC_VEL0 | BC_SPC0) << d)) wj += vbc[d] * vbc[d] - v[d] * v[d];
             }
+            pw = 0.5 * hs * wr;
+            kj = wj;
             if (mk & (7u * BC_SPC0)) {
                 gst(J.rf, n * 3) = rfv[0];
                 gst(J.rf, n * 3 + 1) = rfv[1];
                 gst(J.rf, n * 3 + 2) = rfv[2];
             }
         }
-        const double hs = dt_i + dt_i_1;
         double k2 = 0.0;
         const uint32_t cw = MIXED ? gld(J.ucomp, n) : 0u;
         uint32_t cw_new = 0;
@@ -1229,14 +1250,15 @@ __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(ASM ? 8 : 1
         for (int d = 0; d < 3; ++d) {
             const double ad = a[d] * im;  // compute_a (642-644)
             double vd = v[d];
-            if (mk & (BC_VEL0 << d)) vd = vbc[d];
-            if (mk & (BC_SPC0 << d)) vd = 0.0;
+            if (mk & ((BC_VEL0 | BC_SPC0) << d)) vd = vbc[d];
             if (sn) {
                 // a after compute_a; v after the velocity BCs
                 sn[3 + d] = ad;
                 sn[6 + d] = vd;
             }
+            const double vb = vd;
             vd += hs * ad / 2.0;  // update_velocity (646-649)
+            wg += J.grav_dir[d] * (vb + vd);
             const double dx = dt_i * vd;
             if (MIXED) {
                 // update_displacement (651-654), compensated (Kahan); the
@@ -1252,7 +1274,12 @@ __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(ASM ? 8 : 1
             v[d] = vd;
             k2 += vd * vd;
         }
-        ke = 0.5 * k2 / im;
+        // kinetic energy, and the kinetic-energy change of the constrained
+        // dofs, with one division by 1/m
+        const double half_m = 0.5 / im;
+        ke = k2 * half_m;
+        pw += kj * half_m;
+        if (J.grav_on) pw += 0.5 * hs * (ctl->grav_amp * J.grav_g) * wg * half_m;
         if (MIXED) gst(J.ucomp, n) = cw_new;
         if (!MIXED) {
             gst2(J.pos, n * POS_REC, x[0], x[1]);
@@ -1302,24 +1329,45 @@ __global__ __launch_bounds__(BLK) void k_fin_node(const JobDev* __restrict__ job
 }
 
 // setCurrentStep's apply_boundary_condition_vec (exDyna3D.cpp:1911, 379-490)
-// at the current time, plus stepEndTime = time + timePeriod (83).  One
-// thread per listed BC node.
-__global__ void k_apply_bc(JobDev* __restrict__ jobs, uint32_t job, const uint32_t* __restrict__ nodes, uint32_t n,
-                           double period, int set_end) {
+// at the current time, plus stepEndTime = time + timePeriod (83).  One block
+// walks the listed BC nodes.  The kinetic energy the constraints add or
+// remove by setting v (e.g. an initial velocity on nodes a step then fixes)
+// is external work: it is summed in a fixed order (per thread, then a fixed
+// tree) and added to the job's W_ext, so the energy balance
+// KE + IE + HG + VIS - W_ext = KE0 also holds across step switches.
+constexpr int APPLY_BC_BLK = 256;
+__global__ __launch_bounds__(APPLY_BC_BLK) void k_apply_bc(JobDev* __restrict__ jobs, uint32_t job,
+                                                           const uint32_t* __restrict__ nodes, uint32_t n,
+                                                           double period, int set_end) {
 #pragma clang fp contract(off)
     JobDev& J = jobs[job];
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const double time = J.ctl->time;
-    if (i < n) {
+    double wj = 0.0;
+    for (uint32_t i = threadIdx.x; i < n; i += APPLY_BC_BLK) {
         const uint32_t nd = nodes[i];
         const uint8_t mk = J.bcmask[nd];
         double* rec = J.vu + (size_t)nd * VU_REC;
+        double w = 0.0;
         for (int d = 0; d < 3; ++d) {
-            if (mk & (BC_VEL0 << d)) rec[d] = J.bcval[(size_t)nd * 3 + d] * amp_value(J, J.bcamp[(size_t)nd * 3 + d], time);
-            if (mk & (BC_SPC0 << d)) rec[d] = 0.0;
+            const double v0 = rec[d];
+            double v1 = v0;
+            if (mk & (BC_VEL0 << d)) v1 = J.bcval[(size_t)nd * 3 + d] * amp_value(J, J.bcamp[(size_t)nd * 3 + d], time);
+            if (mk & (BC_SPC0 << d)) v1 = 0.0;
+            rec[d] = v1;
+            w += v1 * v1 - v0 * v0;
         }
+        wj += 0.5 * w / J.pos[(size_t)nd * POS_REC + 3];
     }
-    if (set_end && i == 0) J.ctl->step_end = time + period;
+    __shared__ double s_w[APPLY_BC_BLK / 64];
+    wj = wave_sum(wj);
+    if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = wj;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = s_w[0];
+        for (int q = 1; q < APPLY_BC_BLK / 64; ++q) t += s_w[q];
+        J.ctl->wext += t;
+        if (set_end) J.ctl->step_end = time + period;
+    }
 }
 
 // Fields output_results writes for a captured snapshot (exDyna3D.cpp:

@@ -4,6 +4,10 @@
 restatement of the reference's DataIn, Explicit3D/source/dataIn.cpp) and
 returns a ``Model``; ``read_project(path)`` reads project.txt parameters
 (exDyna3D.cpp:698-772).
+
+By default the reader solves what the reference solves: ``*Plastic`` and
+``AMPLITUDE=`` on velocity BCs, which the reference ignores, are dropped.
+``extensions=PLASTIC | VEL_AMPLITUDE`` (or ``ALL``) keeps them.
 """
 from __future__ import annotations
 
@@ -15,6 +19,11 @@ import numpy as np
 
 from . import engine as _eng
 from .model import Bc, Model, ModelDesc, Step
+
+
+PLASTIC = 1        # ENSC_DECK_PLASTIC
+VEL_AMPLITUDE = 2  # ENSC_DECK_VEL_AMPLITUDE
+ALL = PLASTIC | VEL_AMPLITUDE
 
 
 class Project(C.Structure):
@@ -31,6 +40,10 @@ def _lib():
         L.ensc_deck_read.argtypes = [C.c_char_p, vp, C.POINTER(vp)]
         L.ensc_deck_parse.restype = C.c_int
         L.ensc_deck_parse.argtypes = [C.c_char_p, C.c_size_t, vp, C.POINTER(vp)]
+        L.ensc_deck_read_ex.restype = C.c_int
+        L.ensc_deck_read_ex.argtypes = [C.c_char_p, vp, C.c_uint32, C.POINTER(vp)]
+        L.ensc_deck_parse_ex.restype = C.c_int
+        L.ensc_deck_parse_ex.argtypes = [C.c_char_p, C.c_size_t, vp, C.c_uint32, C.POINTER(vp)]
         L.ensc_deck_model.restype = C.c_int
         L.ensc_deck_model.argtypes = [vp, vp]
         L.ensc_deck_json.restype = C.c_int64
@@ -92,21 +105,23 @@ def _model_from_handle(h, L, name: str) -> Model:
     return m
 
 
-def read_deck(path: str, project: Optional[Project] = None) -> Model:
+def read_deck(path: str, project: Optional[Project] = None, extensions: int = 0) -> Model:
     L = _lib()
     h = C.c_void_p()
-    _eng._check(L.ensc_deck_read(path.encode(), C.byref(project) if project else None, C.byref(h)))
+    _eng._check(L.ensc_deck_read_ex(path.encode(), C.byref(project) if project else None, int(extensions),
+                                    C.byref(h)))
     try:
         return _model_from_handle(h, L, path)
     finally:
         L.ensc_deck_free(h)
 
 
-def parse_deck(text: str, project: Optional[Project] = None, name: str = "deck") -> Model:
+def parse_deck(text: str, project: Optional[Project] = None, name: str = "deck", extensions: int = 0) -> Model:
     L = _lib()
     h = C.c_void_p()
     raw = text.encode()
-    _eng._check(L.ensc_deck_parse(raw, len(raw), C.byref(project) if project else None, C.byref(h)))
+    _eng._check(L.ensc_deck_parse_ex(raw, len(raw), C.byref(project) if project else None, int(extensions),
+                                     C.byref(h)))
     try:
         return _model_from_handle(h, L, name)
     finally:

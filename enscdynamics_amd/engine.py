@@ -53,6 +53,8 @@ class Options(C.Structure):
         ("hist_cap", C.c_int64),
         ("staging", C.c_int32),
         ("waves", C.c_int32),
+        ("state_comp", C.c_int32),
+        ("pad_", C.c_int32),
     ]
 
 
@@ -147,7 +149,7 @@ class Engine:
 
     def __init__(self, model: Model, precision: int = FP64, reorder: bool = False, block_elem: int = 0,
                  block_node: int = 0, probes: Sequence[int] = (), hist_cap: int = 0, device: int = 0,
-                 staging: bool = True, waves: int = 0):
+                 staging: bool = True, waves: int = 0, state_comp: int = 0):
         L = load_library()
         self.model = model
         self.precision = precision
@@ -165,6 +167,7 @@ class Engine:
         o.hist_cap = hist_cap
         o.staging = (0 if staging else 2) if isinstance(staging, bool) else int(staging)
         o.waves = waves
+        o.state_comp = state_comp
         h = C.c_void_p()
         _check(L.ensc_engine_create(C.byref(self._desc), C.byref(o), C.byref(h)))
         self.h = h

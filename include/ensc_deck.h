@@ -7,9 +7,16 @@
  * same set/instance/step semantics and the same quirks, producing an
  * ensc_model_desc (include/ensc_model.h) for ensc_engine_create.
  *
- * Extensions the reference ignores: *Plastic (J2 with linear hardening:
- * yield0 = first yield stress, hard = slope between the first two points)
- * and AMPLITUDE= on *Boundary, type=VELOCITY.
+ * Extensions the reference ignores are off unless asked for (flags of the
+ * _ex readers), so a deck read with flags 0 describes the problem the
+ * reference solves:
+ *   ENSC_DECK_PLASTIC       *Plastic: J2 with linear hardening (yield0 =
+ *                           first yield stress, hard = slope between the
+ *                           first two points); the reference has no such
+ *                           keyword (dataIn.cpp:69-92);
+ *   ENSC_DECK_VEL_AMPLITUDE AMPLITUDE= on *Boundary, type=VELOCITY; the
+ *                           reference keeps velocity BCs constant
+ *                           (BoundaryCondition.h:18, dataIn.cpp:1114-1195).
  */
 #ifndef ENSC_DECK_H
 #define ENSC_DECK_H
@@ -35,10 +42,20 @@ typedef struct ensc_project {
 
 ensc_status ensc_project_read(const char* path, ensc_project* out);
 
-/* Read a deck.  project may be NULL (defaults). */
+enum {
+    ENSC_DECK_PLASTIC = 1u,
+    ENSC_DECK_VEL_AMPLITUDE = 2u
+};
+
+/* Read a deck the way the reference's DataIn::read_inp does (no
+ * extensions).  project may be NULL (defaults). */
 ensc_status ensc_deck_read(const char* path, const ensc_project* project, ensc_deck** out);
 /* Read a deck held in memory. */
 ensc_status ensc_deck_parse(const char* text, size_t len, const ensc_project* project, ensc_deck** out);
+/* The same with extensions enabled by flags (ENSC_DECK_*). */
+ensc_status ensc_deck_read_ex(const char* path, const ensc_project* project, uint32_t flags, ensc_deck** out);
+ensc_status ensc_deck_parse_ex(const char* text, size_t len, const ensc_project* project, uint32_t flags,
+                               ensc_deck** out);
 /* The parsed job.  Pointers stay valid until ensc_deck_free. */
 ensc_status ensc_deck_model(const ensc_deck* deck, ensc_model_desc* out);
 /* JSON description of what was parsed (material, parameters, node sets,

@@ -68,6 +68,15 @@ typedef struct ensc_options {
     int32_t waves;          /* element-kernel occupancy target, waves per SIMD
                                (1, 2 or 3; 0 = default for the precision).
                                Staged kernels only. */
+    int32_t state_comp;     /* mixed engine: accumulate sigma and QIA exactly
+                               (FP32 value + FP32 residual, ~48 bits) instead
+                               of in FP32.  0: for quasi-static jobs, whose run
+                               spans >= 512 wave transits (total time x wave
+                               speed / largest extent of the mesh), where
+                               FP32 stress rounding persists in the state and
+                               grows; 1: every job; 2: none.  Decided per job
+                               from the job alone. */
+    int32_t pad_;
 } ensc_options;
 
 /* Scalars of one job (ensc_engine_scalars). */
@@ -218,7 +227,9 @@ ensc_status ensc_batch_job_download(ensc_engine* eng, uint32_t job, const char* 
  * order while their device footprint fits hbm_budget_bytes (and at most
  * max_live_jobs are live), runs their init pass inside the next step
  * iteration of the live batch, advances every live job together for
- * `ticks` step iterations, and retires jobs that reached their end time
+ * `ticks` step iterations, switches a job that reached the end of one
+ * *Step to the next (setCurrentStep, as exDyna3D::run does), and retires
+ * jobs that reached the end of their last step or raised an error flag
  * (results to the host at once; their device memory is freed in groups,
  * when retired jobs hold a quarter of the launch, when nothing is live or
  * when an admission needs the room).  With nothing on the device,

@@ -19,6 +19,9 @@
 //  * J2 radial return with linear isotropic hardening after the Jaumann
 //    update (Simo & Hughes, Computational Inelasticity, 1998, Box 3.1).
 //  * Energy sums (internal, hourglass, bulk-viscous, kinetic, external).
+//    External work is the reactions' work at velocity-BC dofs, gravity's
+//    work on every dof and the kinetic energy constraints add or remove
+//    when they set v, so KE + IE + HG + VIS - W_ext = KE0 closes.
 //  * An amplitude on velocity BCs (piecewise-linear in total time).
 #include "oracle.h"
 
@@ -529,20 +532,49 @@ struct Sim {
     }
 
     // apply_boundary_condition_vec (379-490): velocity lines, then SPC lines.
+    // Energy account (extension): the kinetic energy the constraints add or
+    // remove by setting v is external work, 1/2 m (v_new^2 - v_old^2) per
+    // constrained dof, v_old being the value before this call (a dof named
+    // by several lines counts once).
+    std::vector<uint32_t> bc_stamp;
+    uint32_t bc_tag = 0;
+    std::vector<std::pair<size_t, double>> bc_old;
     void apply_bc_vec() {
+        if (bc_stamp.size() != 3 * Nn) bc_stamp.assign(3 * Nn, 0);
+        ++bc_tag;
+        bc_old.clear();
+        auto touch = [&](size_t idx) {
+            if (bc_stamp[idx] != bc_tag) {
+                bc_stamp[idx] = bc_tag;
+                bc_old.push_back({idx, v[idx]});
+            }
+        };
         for (const ensc_bc_entry* b : cur_vel) {
             const double val = b->value * (b->amp >= 0 ? amp_interp(b->amp, time) : 1.0);
             for (uint32_t i = 0; i < b->n_nodes; ++i)
-                for (int d = b->d0; d <= b->d1; ++d) v[3 * (size_t)b->nodes[i] + d] = val;
+                for (int d = b->d0; d <= b->d1; ++d) {
+                    touch(3 * (size_t)b->nodes[i] + d);
+                    v[3 * (size_t)b->nodes[i] + d] = val;
+                }
         }
         for (const ensc_bc_entry* b : cur_spc) {
             if (b->cond != 1) continue;
             for (uint32_t i = 0; i < b->n_nodes; ++i)
-                for (int d = b->d0; d <= b->d1; ++d) v[3 * (size_t)b->nodes[i] + d] = 0.0;
+                for (int d = b->d0; d <= b->d1; ++d) {
+                    touch(3 * (size_t)b->nodes[i] + d);
+                    v[3 * (size_t)b->nodes[i] + d] = 0.0;
+                }
         }
+        double w = 0.0;
+        for (const auto& p : bc_old) {
+            const double vn = v[p.first];
+            w += 0.5 * (vn * vn - p.second * p.second) / invM[p.first / 3];
+        }
+        wext += w;
     }
 
-    // apply_boundary_condition_a (2029-2145) + external work of velocity BCs.
+    // apply_boundary_condition_a (2029-2145) + the work of the reactions at
+    // velocity-BC dofs, -(f_int + f_grav) v over the step.
     void apply_bc_a(double dt_avg) {
         std::fill(rf.begin(), rf.end(), 0.0);
         double w = 0.0;
@@ -568,12 +600,19 @@ struct Sim {
         wext += w * dt_avg;
     }
 
-    // apply_gravity (800-826)
+    // apply_gravity (800-826).  Remembers the step's amp * g and direction
+    // for the work gravity does in nodal_update.
+    bool g_now = false;
+    double g_ag = 0.0, g_dir[3] = {0, 0, 0};
     void apply_gravity() {
+        g_now = false;
         if (currentStepIndex >= grav.size()) return;
         const Gravity& g = grav[currentStepIndex];
         if (!g.on) return;
         const double amp = g.amp >= 0 ? deal_amp(g.amp) : 1.0;
+        g_now = true;
+        g_ag = amp * g.value;
+        for (int d = 0; d < 3; ++d) g_dir[d] = g.dir[d];
 #pragma omp parallel for num_threads(nthreads) schedule(static)
         for (long n = 0; n < (long)Nn; ++n)
             for (int d = 0; d < 3; ++d) a[3 * n + d] += amp * g.value * g.dir[d] / invM[n];
@@ -581,23 +620,29 @@ struct Sim {
 
     void nodal_update() {
         // compute_a, update_velocity, update_displacement, move_mesh (642-696)
-        double kep = 0.0;
+        // Energy account (extension): gravity's work on every dof over the
+        // step, f_grav (v_old + v_new) / 2 dt_avg.
+        double kep = 0.0, wg = 0.0;
         const double hs = dt_i + dt_i_1;
-#pragma omp parallel for num_threads(nthreads) schedule(static) reduction(+ : kep)
+        const bool gon = g_now;
+#pragma omp parallel for num_threads(nthreads) schedule(static) reduction(+ : kep, wg)
         for (long n = 0; n < (long)Nn; ++n) {
             double k2 = 0.0;
             for (int d = 0; d < 3; ++d) {
                 const size_t i = 3 * n + d;
                 const double acc = a[i] * invM[n];
+                const double vb = v[i];
                 v[i] += hs * acc / 2.0;
                 u[i] += dt_i * v[i];
                 x[i] += dt_i * v[i];
                 a[i] = 0.0;
                 k2 += v[i] * v[i];
+                if (gon) wg += g_ag * g_dir[d] / invM[n] * (vb + v[i]);
             }
             kep += 0.5 * k2 / invM[n];
         }
         ke = kep;
+        wext += 0.25 * hs * wg;
     }
 
     double kinetic() const {

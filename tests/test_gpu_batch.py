@@ -10,6 +10,7 @@
 import numpy as np
 import pytest
 
+import goldens as G
 from enscdynamics_amd import decks
 from enscdynamics_amd import engine as eng
 from enscdynamics_amd.batch import Batch, Server
@@ -215,3 +216,88 @@ def test_mixed_batch_within_tolerance_of_fp64_batch():
         assert du <= 1e-4, (i, du)
         assert de <= 0.01, (i, de)
     print(f"[mixed batch vs fp64 batch] {len(jobs)} jobs: u {worst_u:.2e}, energy {worst_e:.2e}", flush=True)
+
+
+def serve_all(srv, jobs, keep=True, first=2, ticks=48):
+    """Submit jobs two at a time with pumps between, so later jobs arrive
+    while earlier ones run; pump until every ticket is retired."""
+    tickets, done = [], []
+    for i, m in enumerate(jobs):
+        tickets.append(srv.submit(m, keep=keep))
+        if i + 1 >= first and i % 2 == 1:
+            done += srv.pump(ticks)
+    guard = 0
+    while len(done) < len(jobs):
+        done += srv.pump(64)
+        guard += 1
+        assert guard < 100000
+    assert sorted(done) == sorted(tickets)
+    return tickets
+
+
+def test_server_fp64_matches_oracle():
+    # FP64 serving with staggered arrivals: every retired job equals the CPU
+    # oracle (the restatement of the reference's step loop) within 1e-10
+    jobs = small_jobs(8, seed=51, plastic_p=0.5)
+    srv = Server(eng.FP64, budget_bytes=48 << 20, max_live=3, graph_ticks=8)
+    tickets = serve_all(srv, jobs)
+    for t, m in zip(tickets, jobs):
+        s, u, times = srv.result(t, with_u=True)
+        o = Oracle(m, 0).init()
+        o.run(0, [], 0)
+        so = o.scalars()
+        err = float(np.abs(u - o.get("u")).max() / max(np.abs(o.get("u")).max(), 1e-30))
+        assert err <= 1e-10, (t, err)
+        assert s["nsteps"] == so["nsteps"] and s["time"] == so["time"]
+        scale = max(abs(so["wext"]), so["ke0"])
+        for k in ("ke", "ie", "hg", "vis", "wext"):
+            assert abs(s[k] - so[k]) <= 1e-9 * scale, (t, k)
+    print(f"[server fp64] {len(jobs)} staggered jobs equal the oracle", flush=True)
+
+
+def test_server_runs_every_step_of_a_multi_step_deck():
+    # test_step (a reference deck: 3 *Steps, op=NEW, gravity) served among
+    # other jobs: the server switches it from step to step like
+    # exDyna3D::run, and the result equals Engine.run() on the same deck
+    d = G.load("ref_test_step")
+    deck = G.model_from_dump(d)
+    assert len(deck.steps) > 2
+    jobs = small_jobs(3, seed=61)
+    jobs.insert(1, deck)
+    srv = Server(eng.FP64, budget_bytes=64 << 20, max_live=4, graph_ticks=8)
+    tickets = serve_all(srv, jobs)
+    s, u, _ = srv.result(tickets[1], with_u=True)
+    a = eng.Engine(deck, eng.FP64, reorder=True).run()
+    sa = a.scalars()
+    assert s["nsteps"] == sa["nsteps"] == int(d["n_steps"])
+    assert s["step_index"] == len(deck.steps) - 1
+    assert np.array_equal(u, a.get("u"))
+    assert float(np.abs(u - d["final_u"]).max() / np.abs(d["final_u"]).max()) <= 1e-10
+
+
+def test_server_mixed_config5_jobs_vs_fp64():
+    # requests from the configuration-5 generator (sizes log-uniform up to
+    # 5e5 elements, half J2) served in mixed precision, each against the
+    # same job on the FP64 engine: displacement within 1e-4, total energy
+    # within 1% of the energy scale (initial kinetic energy: nothing does
+    # external work on these jobs)
+    jobs = decks.config5(n_jobs=16)
+    print(f"[c5 server] {len(jobs)} jobs, {sum(m.n_elems for m in jobs)} elements, "
+          f"largest {max(m.n_elems for m in jobs)}, J2 {sum(m.plastic for m in jobs)}", flush=True)
+    srv = Server(eng.MIXED, budget_bytes=8 << 30, max_live=8, graph_ticks=20)
+    tickets = serve_all(srv, jobs, ticks=100)
+    worst_u, worst_e = 0.0, 0.0
+    for t, m in zip(tickets, jobs):
+        s, u, _ = srv.result(t, with_u=True)
+        a = eng.Engine(m, eng.FP64, reorder=True).run()
+        sa = a.scalars()
+        assert s["done"] == 1.0 and s["err"] == 0.0 and abs(s["nsteps"] - sa["nsteps"]) <= 1
+        ua = a.get("u")
+        du = float(np.abs(u - ua).max() / max(np.abs(ua).max(), 1e-30))
+        ea = sa["ke"] + sa["ie"] + sa["hg"] + sa["vis"] - sa["wext"]
+        eb = s["ke"] + s["ie"] + s["hg"] + s["vis"] - s["wext"]
+        de = abs(eb - ea) / max(abs(sa["wext"]), sa["ke0"])
+        worst_u, worst_e = max(worst_u, du), max(worst_e, de)
+        assert du <= 1e-4, (t, du)
+        assert de <= 0.01, (t, de)
+    print(f"[c5 server mixed vs fp64] u {worst_u:.2e}, energy {worst_e:.2e}", flush=True)

@@ -57,12 +57,17 @@ def test_reference_reader_with_gpu_step_loop_fp64(case, tmp_path):
     assert rel(s, d["final_sigma"], 1.0) <= 1e-10
 
 
-@pytest.mark.parametrize("case", ["ref_test_single2", "ref_test_step"])
+@pytest.mark.parametrize("case", sorted(DECKS))
 def test_reference_reader_with_gpu_step_loop_mixed(case, tmp_path):
+    # every reference deck is quasi-static (810-7500 wave transits in its
+    # run), so the mixed engine accumulates sigma and QIA with FP32
+    # residuals there (ensc_options.state_comp, automatic): without them
+    # Job-6 and test_part1 drift to 2e-4 and 2e-3
     d, u, s, x, steps = run_dropin(case, "mixed", tmp_path)
     assert steps == int(d["n_steps"])
     err = rel(u, d["final_u"], 1e-12)
-    print(f"[dropin mixed] {case}: u rel err {err:.2e}", flush=True)
+    err_s = rel(s, d["final_sigma"], 1.0)
+    print(f"[dropin mixed] {case}: u rel err {err:.2e}, sigma {err_s:.2e}", flush=True)
     assert err <= 1e-4
 
 
@@ -89,16 +94,3 @@ def test_dropin_writes_the_reference_result_files(tmp_path):
         for f in ("vonMises", "Stress", "Green-Lagrange Strain"):
             ref = o.snapshot(k, f).reshape(r["cell_data"][f].shape)
             assert np.allclose(r["cell_data"][f], ref, rtol=1e-5, atol=1e-6 * max(np.abs(ref).max(), 1e-30)), f
-
-
-def test_mixed_long_tiny_strain_run_stays_near_fp64(tmp_path):
-    # Job-6: 40,004 steps of ~1e-7 strain.  FP32 element arithmetic and
-    # state (sigma, QIA) carry rounding that this long, lightly damped run
-    # amplifies; the mixed engine lands at ~2e-4 of the FP64 reference here,
-    # above the 1e-4 bar that holds for the benchmark configurations.  The
-    # FP64 engine is the drop-in for such runs (<= 1e-10 above).
-    d, u, s, x, steps = run_dropin("ref_Job_6", "mixed", tmp_path)
-    assert steps == int(d["n_steps"])
-    err = rel(u, d["final_u"], 1e-12)
-    print(f"[dropin mixed] ref_Job_6: u rel err {err:.2e}", flush=True)
-    assert err <= 5e-4

@@ -12,6 +12,7 @@ import math
 import numpy as np
 import pytest
 
+import energy_cases as EC
 import goldens as G
 from enscdynamics_amd import decks
 from enscdynamics_amd.model import Bc, Model, Step
@@ -220,3 +221,96 @@ def test_energy_conservation_c1():
     total = s["ke"] + s["ie"] + s["hg"] + s["vis"] - s["wext"]
     assert abs(total - s["ke0"]) <= 0.01 * s["ke0"]
     assert s["vis"] >= 0.0
+
+
+# ---------------------------------------------------------------------------
+# external work and the velocity-BC amplitude (extensions: no reference
+# counterpart, pinned by closed forms)
+# ---------------------------------------------------------------------------
+
+def test_energy_free_fall_gravity_work():
+    """Free fall: nothing deforms, so the balance is exact; W_ext is the work
+    of gravity, which is M g times the drop; v_z = -g * sum(dt_avg)."""
+    g = 9.81
+    m = EC.free_fall(steps=200, g=g)
+    o = Oracle(m, 1).init()
+    n, h = o.run(0, [0], 10 ** 5)
+    s = o.scalars()
+    M = float((1.0 / o.get("invM")).sum())
+    assert M == pytest.approx(m.rho * 0.03 * 0.02 * 0.02, rel=1e-12)
+    assert abs(s["ie"]) + abs(s["hg"]) + abs(s["vis"]) <= 1e-20
+    assert abs(EC.balance(s)) <= 1e-12 * s["wext"]
+    dta = 0.5 * (h[:, 1] + h[:, 2])
+    v = o.get("v")
+    assert np.allclose(v[:, 2], -g * dta.sum(), rtol=1e-12, atol=0)
+    drop = -float(o.get("u")[:, 2].mean())
+    assert s["wext"] == pytest.approx(M * g * drop, rel=2.0 / n)  # O(dt / T)
+    assert s["ke"] == pytest.approx(0.5 * M * (g * dta.sum()) ** 2, rel=1e-12)
+
+
+def test_energy_free_fall_with_initial_velocity():
+    m = EC.free_fall(steps=150, v0=(1.0, 2.0, -3.0))
+    o = Oracle(m, 1).init()
+    o.run(0, [], 0)
+    s = o.scalars()
+    M = float((1.0 / o.get("invM")).sum())
+    assert s["ke0"] == pytest.approx(0.5 * M * 14.0, rel=1e-13)
+    assert abs(EC.balance(s)) <= 1e-12 * s["ke0"]
+    vz = o.get("v")[0, 2]
+    assert s["wext"] == pytest.approx(0.5 * M * (vz * vz - 9.0), rel=1e-9)
+
+
+def test_energy_driven_column_slow_limit():
+    """Uniaxial strain driven slowly to strain 1e-3: the reaction's work
+    approaches 1/2 (lambda + 2G) strain^2 V and the balance closes."""
+    m = EC.driven_column(nz=10, strain=1e-3, transits=20.0)
+    o = Oracle(m, 1).init()
+    o.run(0, [], 0)
+    s = o.scalars()
+    G_, K_, lam, c = m.material_consts()
+    V = 0.01 * 0.01 * m.height
+    eps = m.speed * s["time"] / m.height
+    assert eps == pytest.approx(1e-3, rel=1e-6)
+    assert s["wext"] == pytest.approx(0.5 * (lam + 2 * G_) * eps ** 2 * V, rel=2e-2)
+    assert abs(EC.balance(s)) <= 0.01 * s["wext"]
+    assert s["vis"] >= 0.0 and s["ke"] < 0.01 * s["wext"]
+
+
+def test_velocity_amplitude_matches_linear_interpolation():
+    """The driven velocity equals value * amp(time) at every step, amp the
+    piecewise-linear table (numpy.interp), and the driven displacement is
+    the sum of dt_i * value * amp(time_i) over the history (u advances with
+    dt_i, time with the lagged dt_i_1)."""
+    m = EC.ramped_block(steps=150)
+    tab = m.amplitudes[0][1]
+    o = Oracle(m, 1).init()
+    top = int(m.top[0])
+    rows = []
+    for _ in range(70):
+        n, h = o.run(1, [top], 1)
+        assert n == 1
+        rows.append(h[0])
+        t = o.scalars()["time"]
+        assert o.get("v")[m.top, 2] == pytest.approx(m.value * np.interp(t, tab[:, 0], tab[:, 1]), rel=1e-14, abs=0)
+    n, h = o.run(0, [top], 10 ** 5)
+    H = np.vstack([np.array(rows), h])
+    expect = float(np.sum(H[:, 1] * m.value * np.interp(H[:, 0], tab[:, 0], tab[:, 1])))
+    assert o.get("u")[top, 2] == pytest.approx(expect, rel=1e-12)
+    s = o.scalars()
+    assert s["time"] == pytest.approx(m.total_time, rel=1e-14)
+    assert abs(EC.balance(s)) <= 0.01 * s["wext"]
+
+
+def test_energy_constraint_jump_at_step_switch():
+    """Fixing a moving face at the step switch removes its kinetic energy:
+    W_ext = -1/2 sum(m v0^2) over the face (no other external work here).
+    The balance is looser than the smooth cases: the element pass estimates
+    dissipation with the start-of-step velocity, which overstates it in the
+    near-Nyquist hourglass modes an impulsive stop excites."""
+    m = EC.fixed_after_release(steps=120)
+    o = Oracle(m, 1).init()
+    o.run(0, [], 0)
+    s = o.scalars()
+    mass = 1.0 / o.get("invM")
+    assert s["wext"] == pytest.approx(-0.5 * float(mass[m.left].sum()) * 9.0, rel=1e-13)
+    assert abs(EC.balance(s)) <= 0.1 * s["ke0"]

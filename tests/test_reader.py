@@ -13,6 +13,7 @@ import pytest
 
 import goldens as G
 from enscdynamics_amd import decks
+from enscdynamics_amd import deck
 from enscdynamics_amd.deck import parse_deck, read_project
 
 CASES = G.CASES_REF + G.CASES_MICRO
@@ -61,9 +62,31 @@ def test_reader_extensions_plastic_and_amplitude(tmp_path):
     m.plastic, m.yield0, m.hard = True, 250e6, 1e9
     p = tmp_path / "c2.inp"
     decks.write_inp(m, str(p))
-    r = parse_deck(p.read_text())
+    r = parse_deck(p.read_text(), extensions=deck.ALL)
     assert r.plastic and r.yield0 == 250e6 and r.hard == 1e9
     assert r.steps[1].vel[0].amp == 0
+    assert np.array_equal(r.amplitudes[0][1], m.amplitudes[0][1])
+    r = parse_deck(p.read_text(), extensions=deck.VEL_AMPLITUDE)
+    assert not r.plastic and r.steps[1].vel[0].amp == 0
+
+
+def test_reader_default_parse_is_the_references(tmp_path):
+    """By default a velocity BC with AMPLITUDE= stays constant and *Plastic
+    is ignored, as in the reference (BoundaryCondition.h:18; vel_nodes hold
+    a value only, dataIn.cpp:1114-1195; no *Plastic keyword,
+    dataIn.cpp:69-92): the default parse describes the reference's problem."""
+    m = decks.config2(nx=4, ny=3, nz=3, steps=20)
+    m.plastic, m.yield0, m.hard = True, 250e6, 1e9
+    p = tmp_path / "c2.inp"
+    decks.write_inp(m, str(p))
+    text = p.read_text()
+    assert "amplitude=" in text.lower() and "*plastic" in text.lower()
+    r = parse_deck(text)
+    assert not r.plastic and r.yield0 == 0.0 and r.hard == 0.0
+    vel = r.steps[1].vel[0]
+    assert vel.amp == -1 and vel.value == -2.0
+    assert np.array_equal(np.sort(vel.nodes), np.sort(m.steps[1].vel[0].nodes))
+    # the amplitude table itself is still read (gravity may use it)
     assert np.array_equal(r.amplitudes[0][1], m.amplitudes[0][1])
 
 
