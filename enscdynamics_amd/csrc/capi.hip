@@ -54,7 +54,8 @@ struct DevBuf {
 
 struct ensc_engine {
     int device = 0, precision = ENSC_FP64, blk_e = 256, blk_n = 256, reorder = 0, staging_opt = 0, waves = 2;
-    int comp_opt = 0;  // ensc_options.state_comp
+    int fp64_rule = 0;  // ensc_options.fp64_jobs (per-job precision of a mixed engine)
+    int waves64 = 1;    // occupancy target of the FP64 staged element kernel
     bool pipe_opt = true;  // staging option 0: pipelined element pass where it applies
     hipStream_t stream = nullptr;
     // second stream for the staged element blocks of a tick that also runs
@@ -67,21 +68,28 @@ struct ensc_engine {
     Ctl* h_ctl = nullptr;
     uint32_t ctl_cap = 0;
     // layout of the jobs currently stepped (relayout): the element block
-    // map holds the pipelined jobs' blocks, then the staged k_element
-    // blocks, then the gathered ones; the node block map the block-assembly
-    // jobs' blocks, then the others
+    // map holds, in this order, the blocks of the pipelined pass, then the
+    // staged and the gathered blocks of mixed-precision jobs, then the staged
+    // and the gathered blocks of FP64 jobs; the node block map the
+    // block-assembly jobs' blocks, then the other mixed jobs', then the FP64
+    // jobs'
     std::vector<uint32_t> live;
     DevBuf jobs_buf, b2e_buf, b2n_buf, pe_buf, pn_buf;
-    uint32_t nblk_pipe = 0, nblk_e = 0, nblk_g = 0;  // element blocks: pipelined, staged, gathered
-    uint32_t nblk_n_asm = 0, nblk_n = 0;             // node blocks: block partials, per-corner forces
+    // where the kernels find the job table and block maps
+    void *jobs_p = nullptr, *b2e_p = nullptr, *b2n_p = nullptr;
+    uint32_t nblk_pipe = 0, nblk_e = 0, nblk_g = 0;  // element blocks: pipelined, staged, gathered (mixed)
+    uint32_t nblk_e64 = 0, nblk_g64 = 0;             // element blocks: staged, gathered (FP64)
+    uint32_t nblk_n_asm = 0, nblk_n = 0, nblk_n64 = 0;  // node blocks: block partials, per-corner (mixed, FP64)
     bool dirty = true;
-    KernelFn ke = nullptr, ke_g = nullptr;   // k_element staged / gathered
-    KernelFn kn = nullptr, kn_asm = nullptr;  // k_node per-corner / block partials
+    KernelFn ke = nullptr, ke_g = nullptr;      // k_element staged / gathered, mixed jobs
+    KernelFn ke64 = nullptr, ke64_g = nullptr;  // the same, FP64 jobs
+    KernelFn kn = nullptr, kn_asm = nullptr, kn64 = nullptr;  // k_node per-corner (mixed) / block partials / FP64
     bool staged = false;      // some live job stages its nodes in LDS
     uint32_t n_asm = 0;       // live jobs using block assembly
+    uint32_t n_fp64 = 0;      // live jobs running in FP64
     uint32_t pipe_grid = 0;   // the pipelined kernel's block count (pipe_per_cu per CU)
     int ncu = 0, pipe_per_cu = 2;
-    size_t lds_bytes = 0;     // dynamic LDS of the staged k_element
+    size_t lds_bytes = 0, lds_bytes64 = 0;  // dynamic LDS of the staged k_element (mixed, FP64)
     // output snapshots of a single-job engine: files go to out_dir (empty:
     // the caller drains them), numbered from out_count
     std::string out_dir;
@@ -215,99 +223,122 @@ ensc_status relayout(ensc_engine* E) {
         const bool idle = J->inited && !c.init_pending && !(c.time < c.step_end) && !J->pending_step;
         if (!idle || J->keep_in_layout) E->live.push_back(s);
     }
-    const bool mixed = E->precision == ENSC_MIXED;
     // Every choice below depends on the job alone (and the engine options),
     // so a job runs the same kernels, with the same arithmetic, alone and in
     // any mix:
+    //  - precision: the job's own (job_precision);
     //  - staged: its largest block node list fits 64 KiB of LDS (72 B per
-    //    node; the mixed engine adds the block assembly accumulators, 12 B);
+    //    node; mixed jobs add the block assembly accumulators, 12 B);
     //  - block assembly (mixed): staged and free of corner conflicts;
-    //  - pipelined pass: block assembly, no amplitude velocity BCs, no
-    //    residual accumulation of sigma and QIA, and tiles
+    //  - pipelined pass: block assembly, no amplitude velocity BCs and tiles
     //    of PIPE_BLK elements; each such tile with at most PIPE_CAP nodes
     //    (the rest of the job's tiles use the staged pass).  The split
     //    follows from the job's own tiles, so it too is the same in any mix.
-    const size_t lds_per_node = 9 * sizeof(double) + (mixed ? 12 : 0);
+    auto lds_per_node = [](const Job& J) { return 9 * sizeof(double) + (J.precision == ENSC_MIXED ? 12 : 0); };
     auto job_staged = [&](const Job& J) {
-        return E->staging_opt != 2 && (size_t)J.lds_nodes * lds_per_node <= 65536;
+        return E->staging_opt != 2 && (size_t)J.lds_nodes * lds_per_node(J) <= 65536;
     };
-    auto job_asm = [&](const Job& J) { return mixed && J.asm_capable && job_staged(J); };
+    auto job_asm = [&](const Job& J) { return J.precision == ENSC_MIXED && J.asm_capable && job_staged(J); };
     auto job_pipe = [&](const Job& J) {
-        return E->pipe_opt && job_asm(J) && !J.dev.dyn_bc && !J.comp && E->blk_e == PIPE_BLK;
+        return E->pipe_opt && job_asm(J) && !J.dev.dyn_bc && E->blk_e == PIPE_BLK;
     };
     // block maps: (job, block of the job, staged range) per launched block,
-    // in XCD order; element blocks grouped pipelined | staged | gathered,
-    // node blocks block-assembly | per-corner forces
-    std::vector<uint4> lp, ls, lg, lna, ln;
+    // in XCD order, grouped as described at ensc_engine::live
+    std::vector<uint4> lp, ls, lg, ls64, lg64, lna, ln, ln64;
     std::vector<JobDev> tab;
     std::vector<size_t> pe0, pn0;
     size_t ne_blocks = 0, n_blocks = 0;
-    uint32_t lds_nodes = 0;
+    uint32_t lds_nodes = 0, lds_nodes64 = 0;
     bool any_asm = false;
     E->n_asm = 0;
+    E->n_fp64 = 0;
     for (uint32_t i = 0; i < E->live.size(); ++i) {
         Job& J = *E->slots[E->live[i]];
         const JobDev& D = J.dev;
         pe0.push_back(ne_blocks);
         pn0.push_back(n_blocks);
+        const bool m = J.precision == ENSC_MIXED;
         const bool as = job_asm(J), pipe = job_pipe(J), staged = job_staged(J);
         for (uint32_t b = 0; b < D.nblk_e; ++b) {
             const uint32_t nodes = J.blk_ptr_h[b + 1] - J.blk_ptr_h[b];
-            std::vector<uint4>& le = (pipe && nodes <= (uint32_t)PIPE_CAP) ? lp : (staged ? ls : lg);
-            if (&le == &ls) {
+            std::vector<uint4>* le;
+            if (!m)
+                le = staged ? &ls64 : &lg64;
+            else
+                le = (pipe && nodes <= (uint32_t)PIPE_CAP) ? &lp : (staged ? &ls : &lg);
+            if (le == &ls) {
                 lds_nodes = std::max(lds_nodes, nodes);
                 any_asm |= as;
+            } else if (le == &ls64) {
+                lds_nodes64 = std::max(lds_nodes64, nodes);
             }
-            le.push_back(make_uint4(i, b, J.blk_ptr_h[b], nodes));
+            le->push_back(make_uint4(i, b, J.blk_ptr_h[b], nodes));
         }
         ne_blocks += D.nblk_e;
-        std::vector<uint4>& lnj = as ? lna : ln;
+        std::vector<uint4>& lnj = !m ? ln64 : (as ? lna : ln);
         for (uint32_t b = 0; b < D.nblk_n; ++b) lnj.push_back(make_uint4(i, b, 0, 0));
         n_blocks += D.nblk_n;
         J.dev.block_asm = as ? 1 : 0;
         E->n_asm += as;
+        E->n_fp64 += !m;
     }
-    std::vector<uint4> b2e;
-    for (const std::vector<uint4>* l : {&lp, &ls, &lg}) {
+    std::vector<uint4> b2e, b2n;
+    for (const std::vector<uint4>* l : {&lp, &ls, &lg, &ls64, &lg64}) {
         const std::vector<uint4> r = xcd_order(*l);
         b2e.insert(b2e.end(), r.begin(), r.end());
     }
-    std::vector<uint4> b2n = xcd_order(lna);
-    {
-        const std::vector<uint4> r = xcd_order(ln);
+    for (const std::vector<uint4>* l : {&lna, &ln, &ln64}) {
+        const std::vector<uint4> r = xcd_order(*l);
         b2n.insert(b2n.end(), r.begin(), r.end());
     }
     E->nblk_pipe = (uint32_t)lp.size();
     E->nblk_e = (uint32_t)ls.size();
     E->nblk_g = (uint32_t)lg.size();
+    E->nblk_e64 = (uint32_t)ls64.size();
+    E->nblk_g64 = (uint32_t)lg64.size();
     E->nblk_n_asm = (uint32_t)lna.size();
     E->nblk_n = (uint32_t)ln.size();
-    if (!E->jobs_buf.ensure(std::max<size_t>(1, E->live.size()) * sizeof(JobDev)) ||
-        !E->b2e_buf.ensure(std::max<size_t>(1, b2e.size()) * sizeof(uint4)) ||
-        !E->b2n_buf.ensure(std::max<size_t>(1, b2n.size()) * sizeof(uint4)) ||
-        !E->pe_buf.ensure(std::max<size_t>(1, ne_blocks) * 5 * sizeof(double)) ||
-        !E->pn_buf.ensure(std::max<size_t>(1, n_blocks) * 2 * sizeof(double)))
-        return fail(ENSC_ERR_MEM, "layout allocation failed");
+    E->nblk_n64 = (uint32_t)ln64.size();
+    const size_t sz[5] = {std::max<size_t>(1, E->live.size()) * sizeof(JobDev),
+                          std::max<size_t>(1, b2e.size()) * sizeof(uint4), std::max<size_t>(1, b2n.size()) * sizeof(uint4),
+                          std::max<size_t>(1, ne_blocks) * 5 * sizeof(double),
+                          std::max<size_t>(1, n_blocks) * 2 * sizeof(double)};
+    void* pe = nullptr;
+    void* pn = nullptr;
+    {
+        if (!E->jobs_buf.ensure(sz[0]) || !E->b2e_buf.ensure(sz[1]) || !E->b2n_buf.ensure(sz[2]) ||
+            !E->pe_buf.ensure(sz[3]) || !E->pn_buf.ensure(sz[4]))
+            return fail(ENSC_ERR_MEM, "layout allocation failed");
+        E->jobs_p = E->jobs_buf.p;
+        E->b2e_p = E->b2e_buf.p;
+        E->b2n_p = E->b2n_buf.p;
+        pe = E->pe_buf.p;
+        pn = E->pn_buf.p;
+    }
     for (uint32_t i = 0; i < E->live.size(); ++i) {
         Job& J = *E->slots[E->live[i]];
         J.dev.ctl = E->d_ctl + E->live[i];
-        J.dev.part_e = static_cast<double*>(E->pe_buf.p) + pe0[i] * 5;
-        J.dev.part_n = static_cast<double*>(E->pn_buf.p) + pn0[i] * 2;
+        J.dev.part_e = static_cast<double*>(pe) + pe0[i] * 5;
+        J.dev.part_n = static_cast<double*>(pn) + pn0[i] * 2;
         tab.push_back(J.dev);
     }
     if (!tab.empty())
-        HIPCHK(hipMemcpyAsync(E->jobs_buf.p, tab.data(), tab.size() * sizeof(JobDev), hipMemcpyHostToDevice, E->stream));
+        HIPCHK(hipMemcpyAsync(E->jobs_p, tab.data(), tab.size() * sizeof(JobDev), hipMemcpyHostToDevice, E->stream));
     if (!b2e.empty())
-        HIPCHK(hipMemcpyAsync(E->b2e_buf.p, b2e.data(), b2e.size() * sizeof(uint4), hipMemcpyHostToDevice, E->stream));
+        HIPCHK(hipMemcpyAsync(E->b2e_p, b2e.data(), b2e.size() * sizeof(uint4), hipMemcpyHostToDevice, E->stream));
     if (!b2n.empty())
-        HIPCHK(hipMemcpyAsync(E->b2n_buf.p, b2n.data(), b2n.size() * sizeof(uint4), hipMemcpyHostToDevice, E->stream));
+        HIPCHK(hipMemcpyAsync(E->b2n_p, b2n.data(), b2n.size() * sizeof(uint4), hipMemcpyHostToDevice, E->stream));
     HIPCHK(hipStreamSynchronize(E->stream));
-    E->staged = E->nblk_pipe + E->nblk_e > 0;
-    E->lds_bytes = (size_t)std::max<uint32_t>(lds_nodes, 1) * (any_asm ? lds_per_node : 9 * sizeof(double));
-    E->ke = mixed ? pick_e<true>(E->blk_e, true, E->waves) : pick_e<false>(E->blk_e, true, E->waves);
-    E->ke_g = mixed ? pick_e<true>(E->blk_e, false, E->waves) : pick_e<false>(E->blk_e, false, E->waves);
-    E->kn = mixed ? pick_n<true, false>(E->blk_n) : pick_n<false, false>(E->blk_n);
+    E->staged = E->nblk_pipe + E->nblk_e + E->nblk_e64 > 0;
+    E->lds_bytes = (size_t)std::max<uint32_t>(lds_nodes, 1) * (any_asm ? 9 * sizeof(double) + 12 : 9 * sizeof(double));
+    E->lds_bytes64 = (size_t)std::max<uint32_t>(lds_nodes64, 1) * 9 * sizeof(double);
+    E->ke = pick_e<true>(E->blk_e, true, E->waves);
+    E->ke_g = pick_e<true>(E->blk_e, false, E->waves);
+    E->ke64 = pick_e<false>(E->blk_e, true, E->waves64);
+    E->ke64_g = pick_e<false>(E->blk_e, false, E->waves64);
+    E->kn = pick_n<true, false>(E->blk_n);
     E->kn_asm = pick_n<true, true>(E->blk_n);
+    E->kn64 = pick_n<false, false>(E->blk_n);
     E->dirty = false;
     E->graph_valid = false;
     return ENSC_OK;
@@ -321,7 +352,7 @@ constexpr int FIN_BLK = 512;
 // Returns the first error of the phase's runtime calls (kernel launches
 // report theirs through hipGetLastError).
 hipError_t launch_phase(ensc_engine* E, int phase) {
-    const JobDev* jobs = static_cast<const JobDev*>(E->jobs_buf.p);
+    const JobDev* jobs = static_cast<const JobDev*>(E->jobs_p);
     const uint32_t nj = (uint32_t)E->live.size();
     if (!nj) return hipSuccess;
     hipError_t err = hipSuccess;
@@ -334,22 +365,27 @@ hipError_t launch_phase(ensc_engine* E, int phase) {
             // blocks launched beside it (on the side stream) start as its
             // blocks finish and fill its last round instead of following it.
             // The passes cover disjoint element blocks of the tick.
-            const bool fork = E->nblk_pipe && (E->nblk_e || E->nblk_g);
+            const bool fork = E->nblk_pipe && (E->nblk_e || E->nblk_g || E->nblk_e64 || E->nblk_g64);
             hipStream_t s2 = E->stream;
             if (fork) {
                 chk(hipEventRecord(E->ev_fork, E->stream));
                 chk(hipStreamWaitEvent(E->side, E->ev_fork, 0));
                 s2 = E->side;
             }
+            const uint4* map = static_cast<const uint4*>(E->b2e_p);
             if (E->nblk_pipe)
                 hipLaunchKernelGGL(k_element_pipe<2>, dim3(std::min(E->nblk_pipe, E->pipe_grid)), dim3(PIPE_BLK), 0,
-                                   E->stream, jobs, static_cast<const uint4*>(E->b2e_buf.p), E->nblk_pipe);
+                                   E->stream, jobs, map, E->nblk_pipe);
+            map += E->nblk_pipe;
             if (E->nblk_e)
-                hipLaunchKernelGGL(E->ke, dim3(E->nblk_e), dim3(E->blk_e), E->lds_bytes, s2, jobs,
-                                   static_cast<const uint4*>(E->b2e_buf.p) + E->nblk_pipe);
-            if (E->nblk_g)
-                hipLaunchKernelGGL(E->ke_g, dim3(E->nblk_g), dim3(E->blk_e), 0, s2, jobs,
-                                   static_cast<const uint4*>(E->b2e_buf.p) + E->nblk_pipe + E->nblk_e);
+                hipLaunchKernelGGL(E->ke, dim3(E->nblk_e), dim3(E->blk_e), E->lds_bytes, s2, jobs, map);
+            map += E->nblk_e;
+            if (E->nblk_g) hipLaunchKernelGGL(E->ke_g, dim3(E->nblk_g), dim3(E->blk_e), 0, s2, jobs, map);
+            map += E->nblk_g;
+            if (E->nblk_e64)
+                hipLaunchKernelGGL(E->ke64, dim3(E->nblk_e64), dim3(E->blk_e), E->lds_bytes64, s2, jobs, map);
+            map += E->nblk_e64;
+            if (E->nblk_g64) hipLaunchKernelGGL(E->ke64_g, dim3(E->nblk_g64), dim3(E->blk_e), 0, s2, jobs, map);
             if (fork) {
                 chk(hipEventRecord(E->ev_join, E->side));
                 chk(hipStreamWaitEvent(E->stream, E->ev_join, 0));
@@ -357,14 +393,15 @@ hipError_t launch_phase(ensc_engine* E, int phase) {
             break;
         }
         case 1: hipLaunchKernelGGL(k_fin_element<FIN_BLK>, dim3(nj), dim3(FIN_BLK), 0, E->stream, jobs); break;
-        case 2:
-            if (E->nblk_n_asm)
-                hipLaunchKernelGGL(E->kn_asm, dim3(E->nblk_n_asm), dim3(E->blk_n), 0, E->stream, jobs,
-                                   static_cast<const uint4*>(E->b2n_buf.p));
-            if (E->nblk_n)
-                hipLaunchKernelGGL(E->kn, dim3(E->nblk_n), dim3(E->blk_n), 0, E->stream, jobs,
-                                   static_cast<const uint4*>(E->b2n_buf.p) + E->nblk_n_asm);
+        case 2: {
+            const uint4* map = static_cast<const uint4*>(E->b2n_p);
+            if (E->nblk_n_asm) hipLaunchKernelGGL(E->kn_asm, dim3(E->nblk_n_asm), dim3(E->blk_n), 0, E->stream, jobs, map);
+            map += E->nblk_n_asm;
+            if (E->nblk_n) hipLaunchKernelGGL(E->kn, dim3(E->nblk_n), dim3(E->blk_n), 0, E->stream, jobs, map);
+            map += E->nblk_n;
+            if (E->nblk_n64) hipLaunchKernelGGL(E->kn64, dim3(E->nblk_n64), dim3(E->blk_n), 0, E->stream, jobs, map);
             break;
+        }
         default: hipLaunchKernelGGL(k_fin_node<FIN_BLK>, dim3(nj), dim3(FIN_BLK), 0, E->stream, jobs); break;
     }
     return err;
@@ -492,9 +529,12 @@ ensc_status engine_new(const ensc_options* opts, std::unique_ptr<ensc_engine>& E
     E->blk_e = okblk(o.block_elem) ? o.block_elem : (o.precision == ENSC_FP64 ? 128 : 256);
     E->blk_n = okblk(o.block_node) ? o.block_node : 256;
     if (o.waves < 0 || o.waves > 3) return fail(ENSC_ERR_ARG, "waves must be 0 (default), 1, 2 or 3");
-    if (o.state_comp < 0 || o.state_comp > 2) return fail(ENSC_ERR_ARG, "state_comp must be 0 (auto), 1 or 2");
-    E->comp_opt = o.state_comp;
+    if (o.fp64_jobs < 0 || o.fp64_jobs > 2) return fail(ENSC_ERR_ARG, "fp64_jobs must be 0 (auto), 1 or 2");
+    E->fp64_rule = o.fp64_jobs;
     E->waves = o.waves ? o.waves : (o.precision == ENSC_FP64 ? 1 : 2);
+    // FP64 jobs: the engine's occupancy target in an FP64 engine, one wave
+    // per SIMD in a mixed one (the FP64 body needs more than 128 registers)
+    E->waves64 = o.precision == ENSC_FP64 ? E->waves : 1;
     HIPCHK(hipSetDevice(E->device));
     HIPCHK(hipStreamCreateWithFlags(&E->stream, hipStreamNonBlocking));
     HIPCHK(hipStreamCreateWithFlags(&E->side, hipStreamNonBlocking));
@@ -553,7 +593,7 @@ ensc_status add_job(ensc_engine* E, const ensc_model_desc* model, const ensc_opt
     std::unique_ptr<Job> J(new Job());
     std::string err;
     if (!copy_model(model, J->model, err)) return fail(ENSC_ERR_ARG, err);
-    ensc_status st = prepare_job(*J, E->precision, E->reorder, E->blk_e, E->blk_n, E->comp_opt);
+    ensc_status st = prepare_job(*J, E->precision, E->reorder, E->blk_e, E->blk_n, E->fp64_rule);
     if (st) return st;
     return add_prepared(E, std::move(J), opts, slot_out);
 }
@@ -688,7 +728,7 @@ ensc_status snapshot_fields(ensc_engine* E, std::vector<double>* node, std::vect
         J.self_copy = J.dev;
         HIPCHK(hipMemcpyAsync(J.d_self, &J.self_copy, sizeof(JobDev), hipMemcpyHostToDevice, E->stream));
         const uint32_t blocks = (uint32_t)((Ne + 255) / 256);
-        if (E->precision == ENSC_MIXED)
+        if (J.precision == ENSC_MIXED)
             hipLaunchKernelGGL(k_post<true>, dim3(blocks), dim3(256), 0, E->stream, J.d_self, slot, J.d_post);
         else
             hipLaunchKernelGGL(k_post<false>, dim3(blocks), dim3(256), 0, E->stream, J.d_self, slot, J.d_post);
@@ -797,7 +837,7 @@ ensc_status do_advance(ensc_engine* E, int64_t max_steps, int32_t* done) {
         if (n_live_blocks && done_blocks * 4 > n_live_blocks) E->dirty = true;
         if (E->dirty) {
             if ((st = relayout(E))) return st;
-            n_live_blocks = (size_t)E->nblk_pipe + E->nblk_e + E->nblk_g;
+            n_live_blocks = (size_t)E->nblk_pipe + E->nblk_e + E->nblk_g + E->nblk_e64 + E->nblk_g64;
         }
         int64_t n = chunk;
         if (output_engine(E)) {
@@ -857,7 +897,7 @@ void fill_scalars(const Ctl& c, double* out) {
 
 ensc_status download(ensc_engine* E, Job& J, const std::string& f, double* out, int64_t count) {
     const size_t Nn = J.model.Nn, Ne = J.model.Ne, es = J.ne_s;
-    const bool mixed = E->precision == ENSC_MIXED;
+    const bool mixed = J.precision == ENSC_MIXED;
     if (f == "u" || f == "v" || f == "x") {
         if (count < (int64_t)(3 * Nn)) return fail(ENSC_ERR_ARG, "buffer too small");
         std::vector<double> pos(Nn * POS_REC), vu(Nn * VU_REC);
@@ -906,12 +946,6 @@ ensc_status download(ensc_engine* E, Job& J, const std::string& f, double* out, 
         std::vector<float> t32((size_t)nc * es);
         HIPCHK(hipMemcpy(t32.data(), src, t32.size() * sizeof(float), hipMemcpyDeviceToHost));
         for (size_t i = 0; i < t32.size(); ++i) tmp[i] = t32[i];
-        // residual accumulation: the value is hi + lo
-        const float* lo = f == "sigma" ? J.dev.sig_lo : (f == "qia" ? J.dev.qia_lo : nullptr);
-        if (lo) {
-            HIPCHK(hipMemcpy(t32.data(), lo, t32.size() * sizeof(float), hipMemcpyDeviceToHost));
-            for (size_t i = 0; i < t32.size(); ++i) tmp[i] += (double)t32[i];
-        }
     } else {
         HIPCHK(hipMemcpy(tmp.data(), src, tmp.size() * sizeof(double), hipMemcpyDeviceToHost));
     }
@@ -943,7 +977,7 @@ ensc_status engine_add_job(ensc_engine* E, const ensc_model_desc* m, uint32_t* s
 ensc_status engine_prepare_job(const ensc_engine* E, ModelCopy&& model, std::unique_ptr<Job>& out) {
     std::unique_ptr<Job> J(new Job());
     J->model = std::move(model);
-    ensc_status st = prepare_job(*J, E->precision, E->reorder, E->blk_e, E->blk_n, E->comp_opt);
+    ensc_status st = prepare_job(*J, E->precision, E->reorder, E->blk_e, E->blk_n, E->fp64_rule);
     if (st) return st;
     out = std::move(J);
     return ENSC_OK;
@@ -1066,18 +1100,16 @@ struct SavedState {
     }
 };
 ensc_status save_state(ensc_engine* E, SavedState& S) {
-    const size_t R = E->precision == ENSC_MIXED ? sizeof(float) : sizeof(double);
     for (uint32_t s : E->live) {
         const Job& J = *E->slots[s];
+        const size_t R = J.precision == ENSC_MIXED ? sizeof(float) : sizeof(double);
         const size_t nn = J.model.Nn, es = J.ne_s;
         S.regions.push_back({J.dev.vu, nn * VU_REC * sizeof(double)});
         if (J.dev.ucomp) S.regions.push_back({J.dev.ucomp, nn * sizeof(uint32_t)});
-        if (E->precision != ENSC_MIXED) S.regions.push_back({J.dev.pos, nn * POS_REC * sizeof(double)});
+        if (J.precision != ENSC_MIXED) S.regions.push_back({J.dev.pos, nn * POS_REC * sizeof(double)});
         S.regions.push_back({J.dev.sigma, 6 * es * R});
         S.regions.push_back({J.dev.qia, 12 * es * R});
         S.regions.push_back({J.dev.epsp, es * R});
-        if (J.dev.sig_lo) S.regions.push_back({J.dev.sig_lo, 6 * es * sizeof(float)});
-        if (J.dev.qia_lo) S.regions.push_back({J.dev.qia_lo, 12 * es * sizeof(float)});
         S.regions.push_back({J.dev.rf, nn * 3 * sizeof(double)});
     }
     S.regions.push_back({E->d_ctl, (size_t)E->ctl_cap * sizeof(Ctl)});
@@ -1147,7 +1179,7 @@ ensc_status autotune(ensc_engine* E, int32_t ticks, int64_t* chosen) {
     if (E->live.empty()) return fail(ENSC_ERR_STATE, "autotune: no job to step");
     SavedState S;
     if ((st = save_state(E, S))) return st;
-    const bool any_pipe = E->nblk_pipe > 0, any_staged = E->nblk_e + E->nblk_g > 0;
+    const bool any_pipe = E->nblk_pipe > 0, any_staged = E->nblk_e + E->nblk_g + E->nblk_e64 + E->nblk_g64 > 0;
     auto trial = [&](double* ms) -> ensc_status {
         ensc_status s2 = time_ticks(E, ticks, ms);
         ensc_status s3 = restore_state(E, S);
@@ -1420,10 +1452,11 @@ ensc_status ensc_engine_layout(ensc_engine* E, int64_t* out) {
     if (st) return st;
     out[ENSC_L_STAGED] = E->staged ? 1 : 0;
     out[ENSC_L_PIPE_BLOCKS] = E->nblk_pipe;
-    out[ENSC_L_ELEM_BLOCKS] = E->nblk_e + E->nblk_g;
-    out[ENSC_L_NODE_BLOCKS] = E->nblk_n_asm + E->nblk_n;
+    out[ENSC_L_ELEM_BLOCKS] = E->nblk_e + E->nblk_g + E->nblk_e64 + E->nblk_g64;
+    out[ENSC_L_NODE_BLOCKS] = E->nblk_n_asm + E->nblk_n + E->nblk_n64;
     out[ENSC_L_LDS_BYTES] = (int64_t)E->lds_bytes;
     out[ENSC_L_ASM_JOBS] = E->n_asm;
+    out[ENSC_L_FP64_JOBS] = E->n_fp64;
     return ENSC_OK;
 }
 

@@ -289,17 +289,18 @@ DeviceArena::~DeviceArena() {
     for (void* p : ptrs) (void)hipFree(p);
 }
 
-// The residual accumulation of sigma and QIA (mixed engine).  A quasi-static
-// run - many wave transits, so the stress changes little per step - keeps
-// every FP32 rounding of the stored stress in its state, and on small,
-// lightly restrained meshes those roundings grow into the displacement
-// (measured on the reference's own decks: 1e-4 to 2e-3 after 9k-40k steps,
-// 810-7500 transits), while dynamic runs of a few hundred transits stay
-// near 1e-7.  The rule reads the job alone: total time x wave speed over
-// the largest extent of the mesh.
-bool wants_comp(const ModelCopy& m, int precision, int comp_opt) {
-    if (precision != ENSC_MIXED || comp_opt == 2) return false;
-    if (comp_opt == 1) return true;
+// Per-job precision of the mixed engine.  A quasi-static run - many wave
+// transits, so the state changes little per step - keeps the per-step FP32
+// errors of the element arithmetic in its stress, and on small, lightly
+// restrained meshes they grow into the displacement: on the reference's own
+// decks (810-7500 transits) FP32 element arithmetic reaches 1e-4 to 3e-3,
+// with the stress and QIA accumulated exactly or not (measured on a CPU
+// emulation and on the GPU), while dynamic runs of a few hundred transits
+// stay near 1e-7.  Such jobs run the FP64 kernels.  The rule reads the job
+// alone: total time x wave speed over the largest extent of the mesh.
+int job_precision(const ModelCopy& m, int engine_precision, int fp64_rule) {
+    if (engine_precision != ENSC_MIXED || fp64_rule == 2) return ENSC_FP64;
+    if (fp64_rule == 1) return ENSC_MIXED;
     double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
     for (size_t i = 0; i < m.Nn; ++i)
         for (int d = 0; d < 3; ++d) {
@@ -307,15 +308,18 @@ bool wants_comp(const ModelCopy& m, int precision, int comp_opt) {
             hi[d] = std::max(hi[d], m.x0[3 * i + d]);
         }
     const double ext = std::max(hi[0] - lo[0], std::max(hi[1] - lo[1], hi[2] - lo[2]));
-    return ext > 0.0 && m.total_time * m.wos / ext >= COMP_TRANSITS;
+    const bool quasi_static = ext > 0.0 && m.total_time * m.wos / ext >= QUASI_STATIC_TRANSITS;
+    return quasi_static ? ENSC_FP64 : ENSC_MIXED;
 }
 
 // Host half of building a job: setup arithmetic, numbering and every table
 // the device needs, into J.img.  Touches nothing but J (no HIP calls), so
 // it may run on any thread.
-ensc_status prepare_job(Job& J, int precision, int reorder, int blk_e, int blk_n, int comp_opt) {
+ensc_status prepare_job(Job& J, int engine_precision, int reorder, int blk_e, int blk_n, int fp64_rule) {
     const ModelCopy& M = J.model;
     const size_t Ne = M.Ne, Nn = M.Nn;
+    const int precision = job_precision(M, engine_precision, fp64_rule);
+    J.precision = precision;
     const bool mixed = precision == ENSC_MIXED;
     std::string err;
     std::vector<double> G8, V0, invM;
@@ -450,7 +454,6 @@ ensc_status prepare_job(Job& J, int precision, int reorder, int blk_e, int blk_n
         ke0 += 0.5 * k2 / invM[n];
     }
     J.ke0 = ke0;
-    J.comp = wants_comp(M, precision, comp_opt);
     JobImage& I = *(J.img = std::unique_ptr<JobImage>(new JobImage()));
     I.precision = precision;
     I.blk_e = blk_e;
@@ -525,14 +528,7 @@ ensc_status upload_job(Job& J) {
         D.part_f = dpf;
         D.pos_ptr = dpp;
         D.pos_list = dpl;
-        if (J.comp) {
-            float *dsl, *dql;
-            if ((st = upload(&dsl, none, 6 * es, A))) return st;
-            if ((st = upload(&dql, none, 12 * es, A))) return st;
-            D.sig_lo = dsl;
-            D.qia_lo = dql;
-            D.comp = 1;
-        }
+
         std::vector<double> im(Nn);
         for (size_t i = 0; i < Nn; ++i) im[i] = I.pos[i * POS_REC + 3];
         double* dim;

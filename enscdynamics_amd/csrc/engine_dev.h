@@ -84,8 +84,6 @@ struct JobDev {
     void* sigma;           // [6][ne_s]
     void* qia;             // [12][ne_s]
     void* epsp;            // [ne_s]
-    float* sig_lo;         // [6][ne_s]  residuals of sigma (comp jobs; mixed engine)
-    float* qia_lo;         // [12][ne_s] residuals of QIA
     void* fel;             // [8][ne_s][3] element nodal forces (corner-major, packed xyz);
                            //   gathered assembly only
     // LDS staging of each element block's nodes (null when not staged)
@@ -127,8 +125,7 @@ struct JobDev {
         double inv_wos, cq2_wos;             // 1 / wos;  cvisq^2 / wos
     } mc;
     int32_t plastic, dyn_bc, ell_w, n_probe;
-    int32_t block_asm;  // 1: block assembly (staged mixed engine, conflict-free blocks)
-    int32_t comp;       // 1: sigma and QIA carry FP32 residuals (sig_lo, qia_lo): value = hi + lo
+    int32_t block_asm, pad_asm;  // 1: block assembly (staged mixed job, conflict-free blocks)
     uint32_t ne, nn, ne_s, nn_s;
     uint32_t nblk_e, nblk_n;
     uint32_t probe[MAX_PROBE];

@@ -79,7 +79,7 @@ struct Job {
     uint32_t ell_w = 0;
     uint32_t lds_nodes = 0;  // largest distinct-node count of an element block
     bool asm_capable = false; // mixed engine, every element block free of corner conflicts
-    bool comp = false;        // mixed engine: sigma and QIA accumulated as FP32 value + residual
+    int precision = ENSC_FP64;  // the job's own precision (job_precision)
     std::vector<uint32_t> blk_ptr_h;  // host copy of the staging list offsets (block maps)
     std::unique_ptr<JobImage> img;    // between prepare_job and upload_job
     double ke0 = 0.0;
@@ -105,12 +105,12 @@ struct Job {
 };
 
 bool copy_model(const ensc_model_desc* m, ModelCopy& c, std::string& err);
-ensc_status prepare_job(Job& J, int precision, int reorder, int blk_e, int blk_n, int comp_opt);
+ensc_status prepare_job(Job& J, int precision, int reorder, int blk_e, int blk_n, int fp64_rule);
 ensc_status upload_job(Job& J);
-// Whether a job of the mixed engine accumulates sigma and QIA with residuals
-// (ensc_options.state_comp: 0 auto, 1 always, 2 never).
-bool wants_comp(const ModelCopy& m, int precision, int comp_opt);
-constexpr double COMP_TRANSITS = 512.0;
+// The precision a job runs in, from the engine's precision and the rule of
+// ensc_options.fp64_jobs (0 auto, 1 mixed, 2 FP64), reading the job alone.
+int job_precision(const ModelCopy& m, int engine_precision, int fp64_rule);
+constexpr double QUASI_STATIC_TRANSITS = 512.0;
 bool job_set_step(Job& J, uint32_t si);
 void job_bc_arrays(const Job& J, std::vector<uint8_t>& mask, std::vector<double>& val, std::vector<int16_t>& amp,
                    std::vector<uint32_t>& nodes);

@@ -462,7 +462,7 @@ __device__ __forceinline__ void element_fp64_G(ELEM_ARGS) {
 template <class R>
 __device__ __forceinline__ void element_mixed_G(ELEM_ARGS) {
 #pragma clang fp contract(fast)
-    constexpr bool MIXED = true, ASM_CAP = false, COMP_CAP = true;
+    constexpr bool MIXED = true, ASM_CAP = false;
 #include "elem_body_mixed.inc"
 }
 #undef NVAL
@@ -484,7 +484,7 @@ __device__ __forceinline__ void element_fp64_S(ELEM_ARGS) {
 template <class R>
 __device__ __forceinline__ void element_mixed_S(ELEM_ARGS) {
 #pragma clang fp contract(fast)
-    constexpr bool MIXED = true, ASM_CAP = true, COMP_CAP = true;
+    constexpr bool MIXED = true, ASM_CAP = true;
 #include "elem_body_mixed.inc"
 }
 #undef NVAL
@@ -898,8 +898,7 @@ __device__ __forceinline__ void element_mixed_P(const JobDev& J, uint32_t e, con
                                                 uint32_t nbn, PipeNext& pn, ElemState<float>& ESn, uint32_t gn[2]) {
 #pragma clang fp contract(fast)
     typedef float R;
-    // jobs with residual accumulation never run the pipelined pass
-    constexpr bool MIXED = true, ASM_CAP = true, COMP_CAP = false;
+    constexpr bool MIXED = true, ASM_CAP = true;
     ENSC_LDS float* acc = pipe_acc(buf);
 #include "elem_body_mixed.inc"
 }
@@ -1076,8 +1075,7 @@ __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(ASM ? 8 : 1
         for (uint32_t e = n; e < J.ne; e += J.nn)
 #pragma unroll
             for (int q = 0; q < 6; ++q)
-                se[(size_t)e * 6 + q] = (double)sg[(size_t)q * J.ne_s + e] +
-                                        (MIXED && J.sig_lo ? (double)J.sig_lo[(size_t)q * J.ne_s + e] : 0.0);
+                se[(size_t)e * 6 + q] = (double)sg[(size_t)q * J.ne_s + e];
     }
     if (n < J.nn) {
         // [X0 X1 | X2 im] [v0 v1 | v2 u0 | u1 u2]; the mixed engine never

@@ -75,7 +75,7 @@ struct Record {
 
 struct ensc_server {
     ensc_engine* eng = nullptr;
-    int precision = ENSC_FP64, comp_opt = 0;
+    int precision = ENSC_FP64, fp64_rule = 0;
     int64_t budget = 0, used = 0;
     int32_t max_live = 0;
     std::chrono::steady_clock::time_point t0;
@@ -123,8 +123,9 @@ struct ensc_server {
 namespace {
 
 // Device bytes a job will hold (matches build_job's allocations).
-int64_t footprint(const ModelCopy& c, int precision, int comp_opt) {
+int64_t footprint(const ModelCopy& c, int engine_precision, int fp64_rule) {
     const int64_t es = (int64_t)round_up(c.Ne, 256), nn = c.Nn;
+    const int precision = job_precision(c, engine_precision, fp64_rule);
     const bool mixed = precision == ENSC_MIXED;
     const int64_t r = mixed ? 4 : 8;
     // conn, lconn, Gi and V0 (FP64 engine), sigma, QIA, plastic strain,
@@ -135,9 +136,7 @@ int64_t footprint(const ModelCopy& c, int precision, int comp_opt) {
     // and the block-assembly partials and position lists
     const int64_t node = NODE_BYTES + 24 + 1 + 24 + 6 + 8 * 4 + 8 +
                          (precision == ENSC_MIXED ? 4 + 8 + 4 + 2 * (4 + 12) : 0) + 16;
-    // residuals of sigma and QIA (mixed jobs accumulating them)
-    const int64_t comp = wants_comp(c, precision, comp_opt) ? 18 * 4 : 0;
-    return es * (elem + comp) + nn * node + 64 * 1024;
+    return es * elem + nn * node + 64 * 1024;
 }
 
 }  // namespace
@@ -152,7 +151,7 @@ ensc_status ensc_server_create(const ensc_options* opts, int64_t hbm_budget_byte
     ensc_status st = engine_new_public(opts, &S->eng);
     if (st) return st;
     S->precision = opts ? opts->precision : ENSC_FP64;
-    S->comp_opt = opts ? opts->state_comp : 0;
+    S->fp64_rule = opts ? opts->fp64_jobs : 0;
     S->budget = hbm_budget_bytes > 0 ? hbm_budget_bytes : (int64_t)64 << 30;
     S->max_live = max_live_jobs > 0 ? max_live_jobs : 1024;
     S->t0 = std::chrono::steady_clock::now();
@@ -191,7 +190,7 @@ ensc_status ensc_server_submit(ensc_server* S, const ensc_model_desc* model, int
         std::shared_ptr<Request> r(new Request());
         std::string err;
         if (!copy_model(model, r->model, err)) return fail(ENSC_ERR_ARG, err);
-        r->footprint = footprint(r->model, S->precision, S->comp_opt);
+        r->footprint = footprint(r->model, S->precision, S->fp64_rule);
         if (r->footprint > S->budget) return fail(ENSC_ERR_MEM, "job larger than the server's device budget");
         r->ticket = S->next_ticket++;
         r->keep = keep_result;

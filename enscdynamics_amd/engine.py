@@ -53,7 +53,7 @@ class Options(C.Structure):
         ("hist_cap", C.c_int64),
         ("staging", C.c_int32),
         ("waves", C.c_int32),
-        ("state_comp", C.c_int32),
+        ("fp64_jobs", C.c_int32),
         ("pad_", C.c_int32),
     ]
 
@@ -149,7 +149,7 @@ class Engine:
 
     def __init__(self, model: Model, precision: int = FP64, reorder: bool = False, block_elem: int = 0,
                  block_node: int = 0, probes: Sequence[int] = (), hist_cap: int = 0, device: int = 0,
-                 staging: bool = True, waves: int = 0, state_comp: int = 0):
+                 staging: bool = True, waves: int = 0, fp64_jobs: int = 0):
         L = load_library()
         self.model = model
         self.precision = precision
@@ -167,7 +167,7 @@ class Engine:
         o.hist_cap = hist_cap
         o.staging = (0 if staging else 2) if isinstance(staging, bool) else int(staging)
         o.waves = waves
-        o.state_comp = state_comp
+        o.fp64_jobs = fp64_jobs
         h = C.c_void_p()
         _check(L.ensc_engine_create(C.byref(self._desc), C.byref(o), C.byref(h)))
         self.h = h
@@ -243,9 +243,9 @@ class Engine:
 
     def layout(self) -> Dict[str, int]:
         """Launch layout of the last step iteration (ensc_engine_layout)."""
-        out = np.zeros(6, dtype=np.int64)
+        out = np.zeros(7, dtype=np.int64)
         _check(_lib.ensc_engine_layout(self.h, out.ctypes.data))
-        keys = ("staged", "pipe_blocks", "elem_blocks", "node_blocks", "lds_bytes", "asm_jobs")
+        keys = ("staged", "pipe_blocks", "elem_blocks", "node_blocks", "lds_bytes", "asm_jobs", "fp64_jobs")
         return {k: int(v) for k, v in zip(keys, out)}
 
     # --- output snapshots (exDyna3D::output_results) ---

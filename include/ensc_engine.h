@@ -38,7 +38,8 @@ typedef enum ensc_status {
 typedef enum ensc_precision {
     ENSC_FP64 = 0,   /* everything FP64: parity path (<=1e-10 vs the reference) */
     ENSC_MIXED = 1   /* FP32 element kinematics/constitutive, FP64 nodal state,
-                        energies and dt; compensated position accumulation */
+                        energies and dt; compensated position accumulation.
+                        Quasi-static jobs run in FP64 (ensc_options.fp64_jobs) */
 } ensc_precision;
 
 /* Device error flags of a job (ensc_engine_scalars ENSC_S_ERR). */
@@ -68,14 +69,16 @@ typedef struct ensc_options {
     int32_t waves;          /* element-kernel occupancy target, waves per SIMD
                                (1, 2 or 3; 0 = default for the precision).
                                Staged kernels only. */
-    int32_t state_comp;     /* mixed engine: accumulate sigma and QIA exactly
-                               (FP32 value + FP32 residual, ~48 bits) instead
-                               of in FP32.  0: for quasi-static jobs, whose run
-                               spans >= 512 wave transits (total time x wave
-                               speed / largest extent of the mesh), where
-                               FP32 stress rounding persists in the state and
-                               grows; 1: every job; 2: none.  Decided per job
-                               from the job alone. */
+    int32_t fp64_jobs;      /* mixed engine, per-job precision rule (decided
+                               from each job alone, so a job computes the same
+                               alone and in any mix):
+                               0: quasi-static jobs - runs spanning >= 512
+                                  wave transits (total time x wave speed /
+                                  largest extent of the mesh) - run the FP64
+                                  kernels, every other job mixed precision;
+                               1: every job mixed precision;
+                               2: every job FP64.
+                               FP64 engines run every job in FP64. */
     int32_t pad_;
 } ensc_options;
 
@@ -149,10 +152,11 @@ ensc_status ensc_engine_phase_times(ensc_engine* eng, double* ms4);
 /* Launch layout of the last step iteration (ENSC_L_COUNT values): whether
  * element blocks stage their nodes in LDS, the element blocks run by the
  * pipelined pass and by the one-block-per-tile pass, the node blocks, the
- * dynamic LDS bytes per element block and the jobs using block assembly. */
+ * dynamic LDS bytes per element block of the mixed jobs, the jobs using
+ * block assembly and the jobs running in FP64. */
 enum {
     ENSC_L_STAGED = 0, ENSC_L_PIPE_BLOCKS, ENSC_L_ELEM_BLOCKS, ENSC_L_NODE_BLOCKS, ENSC_L_LDS_BYTES,
-    ENSC_L_ASM_JOBS, ENSC_L_COUNT
+    ENSC_L_ASM_JOBS, ENSC_L_FP64_JOBS, ENSC_L_COUNT
 };
 ensc_status ensc_engine_layout(ensc_engine* eng, int64_t* out /* ENSC_L_COUNT */);
 /* ---- output snapshots: exDyna3D::output_results (exDyna3D.cpp:207-336) ----

@@ -301,3 +301,31 @@ def test_server_mixed_config5_jobs_vs_fp64():
         assert du <= 1e-4, (t, du)
         assert de <= 0.01, (t, de)
     print(f"[c5 server mixed vs fp64] u {worst_u:.2e}, energy {worst_e:.2e}", flush=True)
+
+
+def test_mixed_engine_runs_quasi_static_jobs_in_fp64():
+    # per-job precision: in a mixed batch, the two quasi-static reference
+    # decks (thousands of wave transits) run the FP64 kernels and equal the
+    # FP64 engine's displacement and stress bit for bit (the gather follows
+    # the canonical (corner, element) order whatever the block partition),
+    # while the other jobs run mixed and equal themselves alone
+    quasi = [G.model_from_dump(G.load(c)) for c in ("ref_test_part1", "ref_Job_6")]
+    jobs = small_jobs(4, seed=71, plastic_p=0.5)
+    jobs = [jobs[0], quasi[0], jobs[1], jobs[2], quasi[1], jobs[3]]
+    b = Batch(jobs, eng.MIXED, reorder=True).run()
+    for i, m in enumerate(jobs):
+        s = b.scalars(i)
+        assert s["done"] == 1.0 and s["err"] == 0.0, i
+        if any(m is q for q in quasi):
+            a = eng.Engine(m, eng.FP64, reorder=True).run()
+            assert a.layout()["fp64_jobs"] == 1
+        else:
+            a = eng.Engine(m, eng.MIXED, reorder=True).run()
+            assert a.layout()["fp64_jobs"] == 0
+        for f in ("u", "v", "sigma", "qia"):
+            assert np.array_equal(b.get(i, f), a.get(f)), (i, f)
+        assert s["nsteps"] == a.scalars()["nsteps"], i
+    forced = eng.Engine(quasi[0], eng.MIXED, reorder=True, fp64_jobs=1)
+    forced.run_steps(3)
+    assert forced.layout()["fp64_jobs"] == 0
+    print(f"[per-job precision] {len(jobs)} jobs, 2 quasi-static in FP64", flush=True)

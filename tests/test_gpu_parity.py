@@ -409,7 +409,10 @@ def test_mixed_rigid_translation_u_is_the_compensated_sum():
     T = 5000 * decks.stable_dt0(m)
     m.steps = [Step("Initial", 0.0), Step("Step-1", T)]
     m.total_time = T
-    e = eng.Engine(m, eng.MIXED, reorder=True, hist_cap=8000).run()
+    # 5000 steps on a 4-element cube span ~830 wave transits: the automatic
+    # rule would run this quasi-static job in FP64, so mixed is forced
+    e = eng.Engine(m, eng.MIXED, reorder=True, hist_cap=8000, fp64_jobs=1).run()
+    assert e.layout()["fp64_jobs"] == 0
     h = e.history()
     n = int(e.scalars()["nsteps"])
     assert h.shape[0] == n and n >= 5000
