@@ -114,6 +114,12 @@ class Batch:
         _eng._check(_lib().ensc_engine_advance(self.h, int(max_steps), C.byref(done)))
         return bool(done.value)
 
+    def advance_static(self, max_steps: int = 0) -> bool:
+        """advance() through ensc_engine_advance_static (allocates nothing)."""
+        done = C.c_int32()
+        _eng._check(_lib().ensc_engine_advance_static(self.h, int(max_steps), C.byref(done)))
+        return bool(done.value)
+
     def autotune(self, ticks: int = 20) -> Dict[str, float]:
         """ensc_engine_autotune over every job of the batch."""
         return _eng.autotune_handle(self.h, ticks)

@@ -12,6 +12,7 @@
 #include <cstring>
 #include <limits>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -30,6 +31,12 @@ using namespace ensc;
 typedef void (*KernelFn)(const JobDev*, const uint4*);
 
 namespace {
+
+// Size of the static scratch array of ensc_engine_advance_static: the job
+// table (~0.6 KB per job), the block maps (16 B per block) and the block
+// partials (40 B per element block, 16 B per node block) of one engine; 4 MiB
+// hold a single mesh of ~10M elements.
+constexpr size_t STATIC_SCRATCH_BYTES = (size_t)4 << 20;
 
 // Device buffer that grows (contents are not preserved).
 struct DevBuf {
@@ -75,8 +82,12 @@ struct ensc_engine {
     // jobs'
     std::vector<uint32_t> live;
     DevBuf jobs_buf, b2e_buf, b2n_buf, pe_buf, pn_buf;
-    // where the kernels find the job table and block maps
+    // where the kernels find the job table and block maps: the DevBufs, or
+    // the static scratch array (ensc_engine_advance_static)
     void *jobs_p = nullptr, *b2e_p = nullptr, *b2n_p = nullptr;
+    bool in_scratch = false;      // the call in progress places its tables in scratch
+    bool layout_scratch = false;  // the current layout lives in scratch
+    char* scratch_base = nullptr;
     uint32_t nblk_pipe = 0, nblk_e = 0, nblk_g = 0;  // element blocks: pipelined, staged, gathered (mixed)
     uint32_t nblk_e64 = 0, nblk_g64 = 0;             // element blocks: staged, gathered (FP64)
     uint32_t nblk_n_asm = 0, nblk_n = 0, nblk_n64 = 0;  // node blocks: block partials, per-corner (mixed, FP64)
@@ -305,7 +316,28 @@ ensc_status relayout(ensc_engine* E) {
                           std::max<size_t>(1, n_blocks) * 2 * sizeof(double)};
     void* pe = nullptr;
     void* pn = nullptr;
-    {
+    if (E->in_scratch) {
+        // the static scratch array: job table, maps and block partials back
+        // to back, each 256-byte aligned; nothing is allocated
+        size_t need = 0;
+        for (size_t b : sz) need += round_up(b, 256);
+        if (need > STATIC_SCRATCH_BYTES)
+            return fail(ENSC_ERR_ARG, "ensc_engine_advance_static: the launch tables of this engine need " +
+                                          std::to_string(need) + " bytes, more than the " +
+                                          std::to_string(STATIC_SCRATCH_BYTES) +
+                                          "-byte static scratch (use ensc_engine_advance)");
+        char* q = E->scratch_base;
+        void* part[5];
+        for (int k = 0; k < 5; ++k) {
+            part[k] = q;
+            q += round_up(sz[k], 256);
+        }
+        E->jobs_p = part[0];
+        E->b2e_p = part[1];
+        E->b2n_p = part[2];
+        pe = part[3];
+        pn = part[4];
+    } else {
         if (!E->jobs_buf.ensure(sz[0]) || !E->b2e_buf.ensure(sz[1]) || !E->b2n_buf.ensure(sz[2]) ||
             !E->pe_buf.ensure(sz[3]) || !E->pn_buf.ensure(sz[4]))
             return fail(ENSC_ERR_MEM, "layout allocation failed");
@@ -315,6 +347,7 @@ ensc_status relayout(ensc_engine* E) {
         pe = E->pe_buf.p;
         pn = E->pn_buf.p;
     }
+    E->layout_scratch = E->in_scratch;
     for (uint32_t i = 0; i < E->live.size(); ++i) {
         Job& J = *E->slots[E->live[i]];
         J.dev.ctl = E->d_ctl + E->live[i];
@@ -455,7 +488,7 @@ ensc_status build_graph(ensc_engine* E) {
 
 ensc_status enqueue_ticks(ensc_engine* E, int64_t n) {
     ensc_status st;
-    if (E->dirty && (st = relayout(E))) return st;
+    if ((E->dirty || E->layout_scratch != E->in_scratch) && (st = relayout(E))) return st;
     if (E->timing) {
         const size_t need = 4 * (size_t)n + 1;
         while (E->evs.size() < need) {
@@ -472,7 +505,9 @@ ensc_status enqueue_ticks(ensc_engine* E, int64_t n) {
         E->pending_ticks = n;
     } else {
         int64_t i = 0;
-        if (E->graph_ticks > 0 && n >= E->graph_ticks) {
+        // graph instantiation allocates: the allocation-free entry point
+        // launches plainly
+        if (E->graph_ticks > 0 && n >= E->graph_ticks && !E->in_scratch) {
             if (!E->graph_valid && (st = build_graph(E))) return st;
             for (; i + E->graph_ticks <= n; i += E->graph_ticks) HIPCHK(hipGraphLaunch(E->graph_exec, E->stream));
         }
@@ -1238,6 +1273,35 @@ ensc_status autotune(ensc_engine* E, int32_t ticks, int64_t* chosen) {
 }
 }  // namespace ensc
 
+// ---- allocation-free stepping (ensc_engine_advance_static) ----
+// The job table, the block maps and the block partials of the call live in
+// this file-scope device array instead of engine buffers.  Each GPU holds its
+// own copy for the life of the process; in host code its name is only a
+// stand-in, so kernels and copies get its device address from
+// hipGetSymbolAddress, taken once per device on the first call there.
+static __device__ __align__(256) unsigned char scratch[STATIC_SCRATCH_BYTES];
+
+namespace {
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+char* g_scratch[MAX_DEVICES] = {};
+
+// The scratch array's address on `device`, which must be current.
+ensc_status scratch_on(int device, char** out) {
+    if (device < 0 || device >= MAX_DEVICES) return fail(ENSC_ERR_ARG, "device ordinal out of range");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch[device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(ENSC_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch[device] = static_cast<char*>(p);
+    }
+    *out = g_scratch[device];
+    return ENSC_OK;
+}
+}  // namespace
+
 extern "C" {
 
 const char* ensc_last_error(void) { return g_last_error.c_str(); }
@@ -1355,6 +1419,29 @@ ensc_status ensc_engine_advance(ensc_engine* E, int64_t max_steps, int32_t* done
     if (!E) return fail(ENSC_ERR_ARG, "null engine");
     HIPCHK(hipSetDevice(E->device));
     return do_advance(E, max_steps, done);
+}
+
+ensc_status ensc_engine_advance_static(ensc_engine* E, int64_t max_steps, int32_t* done) {
+    if (!E) return fail(ENSC_ERR_ARG, "null engine");
+    HIPCHK(hipSetDevice(E->device));
+    return guarded([&]() -> ensc_status {
+        ensc_status st;
+        if ((st = sync(E))) return st;
+        if (output_engine(E))
+            return fail(ENSC_ERR_STATE, "ensc_engine_advance_static: output snapshots are on (they copy and write "
+                                        "files); use ensc_engine_advance");
+        char* base = nullptr;
+        if ((st = scratch_on(E->device, &base))) return st;
+        // another engine's call may have left its tables in the array: lay
+        // this engine's out again, into it
+        E->scratch_base = base;
+        E->in_scratch = true;
+        E->dirty = true;
+        st = do_advance(E, max_steps, done);
+        E->in_scratch = false;
+        E->dirty = true;  // the next call lays out in the engine's own buffers
+        return st;
+    });
 }
 
 ensc_status ensc_engine_run(ensc_engine* E) {

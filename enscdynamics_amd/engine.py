@@ -29,7 +29,8 @@ H_NAMES = ["time", "dt_i", "dt_i_1", "dts_min", "ke", "ie", "hg", "vis", "wext"]
 
 EXPORTS = [
     "ensc_last_error", "ensc_device_count", "ensc_engine_create", "ensc_engine_destroy", "ensc_engine_init",
-    "ensc_engine_set_step", "ensc_engine_advance", "ensc_engine_run", "ensc_engine_enqueue_steps",
+    "ensc_engine_set_step", "ensc_engine_advance", "ensc_engine_advance_static", "ensc_engine_run",
+    "ensc_engine_enqueue_steps",
     "ensc_engine_sync", "ensc_engine_download", "ensc_engine_scalars", "ensc_engine_history",
     "ensc_engine_set_graph", "ensc_copy_bandwidth", "ensc_engine_kernel_times", "ensc_engine_phase_times", "ensc_engine_layout", "ensc_engine_set_timing",
     "ensc_engine_set_output", "ensc_engine_output_pending", "ensc_engine_output_field", "ensc_engine_output_write",
@@ -78,6 +79,7 @@ def load_library(path: str = LIB_PATH):
         ("ensc_engine_init", [vp]),
         ("ensc_engine_set_step", [vp, C.c_uint32]),
         ("ensc_engine_advance", [vp, C.c_int64, C.POINTER(C.c_int32)]),
+        ("ensc_engine_advance_static", [vp, C.c_int64, C.POINTER(C.c_int32)]),
         ("ensc_engine_run", [vp]),
         ("ensc_engine_enqueue_steps", [vp, C.c_int64]),
         ("ensc_engine_sync", [vp]),
@@ -200,6 +202,12 @@ class Engine:
     def run(self) -> "Engine":
         _check(_lib.ensc_engine_run(self.h))
         return self
+
+    def advance_static(self, max_steps: int = 0) -> bool:
+        """advance() through ensc_engine_advance_static (allocates nothing)."""
+        done = C.c_int32(0)
+        _check(_lib.ensc_engine_advance_static(self.h, int(max_steps), C.byref(done)))
+        return bool(done.value)
 
     def run_steps(self, max_steps: int) -> int:
         """init + step 1 + up to max_steps step iterations; returns steps taken."""

@@ -125,6 +125,19 @@ ensc_status ensc_engine_set_step(ensc_engine* eng, uint32_t step_index);
  * step ends).  *done is set to 1 when time has reached stepEndTime. */
 ensc_status ensc_engine_advance(ensc_engine* eng, int64_t max_steps, int32_t* done);
 
+/* ensc_engine_advance for callers that cannot allocate device memory (code
+ * run inside stream capture, tight serving loops, hosts that forbid
+ * hipMalloc).  No counterpart in the reference.  Same loop, same results
+ * bit for bit; the call allocates nothing: the job table, block maps and
+ * block partials of its step iterations live in a static 4 MiB device array
+ * (one per GPU, held until the process ends; every call on a GPU shares it
+ * and rewrites it first, so calls that overlap on one GPU race), and it
+ * launches plainly instead of instantiating a hipGraph.  Fails with
+ * ENSC_ERR_ARG when the engine's tables need more than the array (about 10M
+ * elements in one mesh) and with ENSC_ERR_STATE when output snapshots are
+ * on.  Later ensc_engine_advance calls use the engine's own buffers again. */
+ensc_status ensc_engine_advance_static(ensc_engine* eng, int64_t max_steps, int32_t* done);
+
 /* exDyna3D::run (exDyna3D.cpp:54-102): init, then every *Step. */
 ensc_status ensc_engine_run(ensc_engine* eng);
 
