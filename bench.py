@@ -52,6 +52,23 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def survey_bytes(precision: int, plastic: bool, r: float):
+    """Algorithmic bytes per element-step as the hot-path contract fixes
+    them: the once-per-step read and write of the persistent state at the
+    stored precision, element part and node part.
+      FP64:  conn 32 + B0 192 + V0 8 read; sigma 48 + QIA 96 read and
+             written (+ plastic strain 8 read and written) = 520 (+16);
+             nodes: x, v, u read and written, 1/m read = 152 per node.
+      mixed: conn 32 + B0 96 + V0 4 read; sigma 24 + QIA 48 read and
+             written (+ plastic strain 4 read and written) = 276 (+8);
+             nodes: x, v, u FP64 read and written, 1/m, a compensation
+             residual for x (24) read and written = 200 per node.
+    Returns (element part, node part) per element (r = nodes / elements)."""
+    if precision == eng.MIXED:
+        return 276.0 + (8.0 if plastic else 0.0), 200.0 * r
+    return 520.0 + (16.0 if plastic else 0.0), 152.0 * r
+
+
 def alg_bytes(precision: int, plastic: bool, r: float):
     """Algorithmic bytes per element-step: the compulsory once-per-step read
     and write of persistent state as this engine stores it (force exchange,
@@ -110,15 +127,22 @@ def c3_section(args, m):
     for name, prec in (("mixed", eng.MIXED), ("fp64", eng.FP64)):
         d = res[name]
         be, bn = alg_bytes(prec, m.plastic, r)
+        se, sn = survey_bytes(prec, m.plastic, r)
         ph = d["phases"]
         out[name] = {
             "value": m.n_elems * args.steps / d["secs"],
             "ms_per_step": 1e3 * d["secs"] / args.steps,
             "launch_ms": {"element": ph[0], "element_finalize": ph[1], "node": ph[2], "node_finalize": ph[3]},
+            # contract bytes (state at the stored precision) and this engine's
+            # own stored bytes (no stored B0/V0 in mixed; node reads of the
+            # element pass counted there)
+            "contract_bytes_per_elem": {"element": se, "node": sn, "step": se + sn},
             "alg_bytes_per_elem": {"element": be, "node": bn},
-            "element_gbs": be * m.n_elems / (ph[0] * 1e-3) / 1e9,
-            "node_gbs": bn * m.n_elems / (ph[2] * 1e-3) / 1e9,
-            "step_gbs": (be + bn) * m.n_elems * args.steps / d["secs"] / 1e9,
+            "element_gbs": se * m.n_elems / (ph[0] * 1e-3) / 1e9,
+            "node_gbs": sn * m.n_elems / (ph[2] * 1e-3) / 1e9,
+            "step_gbs": (se + sn) * m.n_elems * args.steps / d["secs"] / 1e9,
+            "engine_bytes_gbs": {"element": be * m.n_elems / (ph[0] * 1e-3) / 1e9,
+                                 "node": bn * m.n_elems / (ph[2] * 1e-3) / 1e9},
             "layout": d["layout"],
             "autotuned": d["tuned"],
         }
@@ -282,7 +306,10 @@ def c5_section(args):
     secs_p, lat_p = serve(list(arr))
     log(f"c5: Poisson {rate:.2f}/s run {secs_p:.3f} s, p50 {np.percentile(lat_p, 50):.3f} s, "
         f"p99 {np.percentile(lat_p, 99):.3f} s")
-    # the same jobs one at a time, one engine each (build, run, results)
+    # the same jobs one at a time, one engine each: (a) build, run and read
+    # the results inside the clock, as a caller without a server would;
+    # (b) engines built before the clock (in groups, to bound memory), so the
+    # ratio compares device work alone
     t0 = time.perf_counter()
     lat_seq = []
     for m in models:
@@ -296,11 +323,32 @@ def c5_section(args):
         e.close()
         lat_seq.append(time.perf_counter() - a)
     secs_seq = time.perf_counter() - t0
-    log(f"c5: one at a time {secs_seq:.3f} s")
+    log(f"c5: one at a time {secs_seq:.3f} s (engine build inside the clock)")
+    secs_pre, lat_pre = 0.0, []
+    for g0 in range(0, len(models), 40):
+        group = []
+        for m in models[g0:g0 + 40]:
+            e = eng.Engine(m, eng.MIXED, reorder=True)
+            if args.graph:
+                e.set_graph(args.graph)
+            group.append(e)
+        for e in group:
+            a = time.perf_counter()
+            e.run()
+            s = e.scalars()
+            assert s["err"] == 0.0
+            dt = time.perf_counter() - a
+            secs_pre += dt
+            lat_pre.append(dt)
+        for e in group:
+            e.close()
+    log(f"c5: one at a time {secs_pre:.3f} s (engines built before the clock)")
     return {"workload": f"config 5: {len(models)} requests from the config-4 generator, Ne log-uniform "
                         f"[1e3, 5e5], half J2, mixed, admitted by HBM footprint (budget {budget >> 30} GiB, "
                         f"<= {args.c5_live} live)",
             "jobs": len(models),
+            "jobs_note": f"{len(models)} completions per leg keep the whole bench within a few minutes "
+                         f"(p99 of {len(models)} latencies); --c5-jobs 1000 runs the full stream",
             "capacity_solves_per_s": cap,
             "poisson_rate_per_s": rate,
             "poisson_solves_per_s": len(models) / secs_p,
@@ -310,8 +358,85 @@ def c5_section(args):
             "one_at_a_time_solves_per_s": len(models) / secs_seq,
             "one_at_a_time_latency_s": {"p50": float(np.percentile(lat_seq, 50)),
                                         "p99": float(np.percentile(lat_seq, 99))},
+            "one_at_a_time_prebuilt_solves_per_s": len(models) / secs_pre,
+            "one_at_a_time_prebuilt_latency_s": {"p50": float(np.percentile(lat_pre, 50)),
+                                                 "p99": float(np.percentile(lat_pre, 99))},
             "speedup_vs_one_at_a_time": cap / (len(models) / secs_seq),
+            "speedup_vs_one_at_a_time_prebuilt": cap / (len(models) / secs_pre),
             "ticks_per_pump": args.c5_ticks}
+
+
+def gpu_rate(m, precision, steps, warm, graph):
+    """elem-steps/s of one job on the GPU: `steps` full step iterations
+    timed after `warm` (steps <= 0: the whole run, setup excluded)."""
+    e = eng.Engine(m, precision, reorder=True)
+    if graph:
+        e.set_graph(graph)
+    e.init()
+    e.set_current_step(1)
+    if steps <= 0:
+        t0 = time.perf_counter()
+        e.advance(0)
+        secs = time.perf_counter() - t0
+        n = int(e.scalars()["nsteps"])
+    else:
+        e.enqueue(warm)
+        e.sync()
+        n0 = int(e.scalars()["nsteps"])
+        t0 = time.perf_counter()
+        e.enqueue(steps)
+        e.sync()
+        secs = time.perf_counter() - t0
+        n = int(e.scalars()["nsteps"]) - n0
+    e.close()
+    return m.n_elems * n / secs, n
+
+
+def configs_section(args):
+    """Configurations 1, 2 and 4 beside the headline: GPU element-steps/s
+    (FP64 and mixed) and CPU baselines on bounded samples."""
+    from oracle.oracle import Oracle  # CPU baseline leg only
+
+    out = {}
+    c1 = decks.config1()
+    c2 = decks.config2()
+    for name, m, steps in (("c1", c1, 0), ("c2", c2, 200)):
+        r = {}
+        for prec, pn in ((eng.FP64, "fp64"), (eng.MIXED, "mixed")):
+            v, n = gpu_rate(m, prec, steps, 20, args.graph)
+            r[f"gpu_{pn}_elem_steps_per_s"] = v
+            r["gpu_steps"] = n
+        log(f"{name}: gpu fp64 {r['gpu_fp64_elem_steps_per_s']:.3e}, mixed {r['gpu_mixed_elem_steps_per_s']:.3e} "
+            f"elem-steps/s ({r['gpu_steps']} steps)")
+        out[name] = r
+    if not args.no_cpu:
+        threads = os.cpu_count() or 1
+        # C1: all 2000 steps; C2: 10 warm-up + 100 timed steps
+        for name, m, steps, warm, ref_steps in (("c1", c1, 2000, 0, 2000), ("c2", c2, 100, 10, 10)):
+            o = Oracle(m, threads).init()
+            if warm:
+                o.time_steps(warm)
+            secs = o.time_steps(steps)
+            out[name]["cpu_restatement"] = {"value": m.n_elems * steps / secs, "cores": o.threads,
+                                            "sample": f"{warm} warm-up + {steps} timed steps"}
+            if os.path.exists(REF_TIMER):
+                try:
+                    out[name]["cpu_reference"] = cpu_reference(m, ref_steps, warm=2)
+                except (RuntimeError, OSError, subprocess.SubprocessError) as ex:
+                    log(f"{name}: cpu reference not timed: {ex}")
+            log(f"{name}: cpu restatement {out[name]['cpu_restatement']['value']:.3e} elem-steps/s")
+        # C4: 20 steps per mesh over all 256 meshes, the restatement (one
+        # reference process per mesh would time its deck parsing instead)
+        models = decks.config4(args.c4_jobs)
+        tot, secs = 0, 0.0
+        for m in models:
+            o = Oracle(m, threads).init()
+            secs += o.time_steps(20)
+            tot += m.n_elems * 20
+        out["c4"] = {"cpu_restatement": {"value": tot / secs, "cores": threads,
+                                         "sample": f"20 steps of each of the {len(models)} meshes"}}
+        log(f"c4: cpu restatement {tot / secs:.3e} elem-steps/s")
+    return out
 
 
 REF_TIMER = os.path.join(ROOT, "oracle", "_ref", "ref_timer")
@@ -432,14 +557,14 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--kernel-steps", type=int, default=50, help="steps timed per launch with events")
-    ap.add_argument("--sections", default="c3,out,c4,c5", help="comma list of c3, out, c4, c5")
+    ap.add_argument("--sections", default="c3,out,c4,c5,cfg", help="comma list of c3, out, c4, c5, cfg")
     ap.add_argument("--graph", type=int, default=20, help="step iterations per captured hipGraph (0: none)")
     ap.add_argument("--c3-autotune", type=int, default=0,
                     help="step iterations per autotuner trial before the c3 timing (0: the default launch shapes)")
     ap.add_argument("--c4-jobs", type=int, default=256)
     ap.add_argument("--c4-autotune", type=int, default=20,
                     help="step iterations per autotuner trial for a second, tuned c4 run (0: none)")
-    ap.add_argument("--c5-jobs", type=int, default=48)
+    ap.add_argument("--c5-jobs", type=int, default=240)
     ap.add_argument("--c5-live", type=int, default=64)
     ap.add_argument("--c5-ticks", type=int, default=50)
     ap.add_argument("--no-cpu", action="store_true")
@@ -470,7 +595,7 @@ def main() -> int:
     log(f"workload {m.name}: {m.n_elems} elements, {m.n_nodes} nodes")
     c3 = c3_section(args, m)
     mixed, fp64 = c3["mixed"], c3["fp64"]
-    be, _ = mixed["alg_bytes_per_elem"]["element"], None
+    be = mixed["contract_bytes_per_elem"]["element"]
     achieved = mixed["element_gbs"]
     piped = mixed["layout"]["pipe_blocks"] > 0
     ekey = "k_element_pipe_mixed" if piped else "k_element_mixed"
@@ -496,7 +621,10 @@ def main() -> int:
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_entry(ekey),
                      "kernel": ename,
-                     "alg_bytes_per_launch": be * m.n_elems, "avg_launch_ms": mixed["launch_ms"]["element"]},
+                     "alg_bytes_per_launch": be * m.n_elems, "avg_launch_ms": mixed["launch_ms"]["element"],
+                     "bytes_basis": "contract state bytes: mixed 276 + 8 (J2) per element; whole step "
+                                    "(276 + 8 + 200 r) per element-step in mixed.step_gbs",
+                     "step_frac": mixed["step_gbs"] / HBM_PEAK_GBS},
         "mixed": mixed,
         "fp64": fp64,
         "mixed_over_fp64": mixed["value"] / fp64["value"],
@@ -509,6 +637,8 @@ def main() -> int:
         out["batched"] = c4_section(args)
     if "c5" in sections:
         out["serving"] = c5_section(args)
+    if "cfg" in sections:
+        out["configs"] = configs_section(args)
     if not args.no_cpu:
         base, port = cpu_baseline(m, args)
         out["cpu_baseline"] = base

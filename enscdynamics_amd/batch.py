@@ -53,7 +53,7 @@ def _lib():
 
 
 def _options(precision: int, reorder: bool, block_elem: int, staging: bool, waves: int = 0,
-             fp64_jobs: int = 0) -> _eng.Options:
+             fp64_jobs: int = 0, fp64_order: int = 0) -> _eng.Options:
     o = _eng.Options()
     o.precision = precision
     o.reorder = int(reorder)
@@ -66,6 +66,7 @@ def _options(precision: int, reorder: bool, block_elem: int, staging: bool, wave
     o.staging = (0 if staging else 2) if isinstance(staging, bool) else int(staging)
     o.waves = waves
     o.fp64_jobs = fp64_jobs
+    o.fp64_order = fp64_order
     return o
 
 
@@ -74,12 +75,12 @@ class Batch:
 
     def __init__(self, models: Sequence[Model], precision: int = _eng.FP64, reorder: bool = True,
                  block_elem: int = 0, graph_ticks: int = 0, staging: bool = True, waves: int = 0,
-                 fp64_jobs: int = 0):
+                 fp64_jobs: int = 0, fp64_order: int = 0):
         L = _lib()
         self.models = list(models)
         self._descs = [m.desc() for m in self.models]
         arr = (C.c_void_p * len(self._descs))(*[C.addressof(d) for d in self._descs])
-        self._opts = _options(precision, reorder, block_elem, staging, waves, fp64_jobs)
+        self._opts = _options(precision, reorder, block_elem, staging, waves, fp64_jobs, fp64_order)
         h = C.c_void_p()
         _eng._check(L.ensc_batch_create(arr, len(self._descs), C.byref(self._opts), C.byref(h)))
         self.h = h
@@ -146,9 +147,9 @@ class Server:
 
     def __init__(self, precision: int = _eng.FP64, budget_bytes: int = 64 << 30, max_live: int = 1024,
                  reorder: bool = True, graph_ticks: int = 0, block_elem: int = 0, staging: bool = True,
-                 waves: int = 0, fp64_jobs: int = 0):
+                 waves: int = 0, fp64_jobs: int = 0, fp64_order: int = 0):
         L = _lib()
-        self._opts = _options(precision, reorder, block_elem, staging, waves, fp64_jobs)
+        self._opts = _options(precision, reorder, block_elem, staging, waves, fp64_jobs, fp64_order)
         h = C.c_void_p()
         _eng._check(L.ensc_server_create(C.byref(self._opts), int(budget_bytes), int(max_live), C.byref(h)))
         self.h = h

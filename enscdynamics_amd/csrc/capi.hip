@@ -62,6 +62,7 @@ struct DevBuf {
 struct ensc_engine {
     int device = 0, precision = ENSC_FP64, blk_e = 256, blk_n = 256, reorder = 0, staging_opt = 0, waves = 2;
     int fp64_rule = 0;  // ensc_options.fp64_jobs (per-job precision of a mixed engine)
+    int fp64_order = 0; // ensc_options.fp64_order (0: block partials, 1: the reference's order)
     int waves64 = 1;    // occupancy target of the FP64 staged element kernel
     bool pipe_opt = true;  // staging option 0: pipelined element pass where it applies
     hipStream_t stream = nullptr;
@@ -77,9 +78,9 @@ struct ensc_engine {
     // layout of the jobs currently stepped (relayout): the element block
     // map holds, in this order, the blocks of the pipelined pass, then the
     // staged and the gathered blocks of mixed-precision jobs, then the staged
-    // and the gathered blocks of FP64 jobs; the node block map the
-    // block-assembly jobs' blocks, then the other mixed jobs', then the FP64
-    // jobs'
+    // and the gathered blocks of FP64 jobs; the node block map the mixed
+    // block-assembly jobs' blocks, the other mixed jobs', the FP64
+    // block-assembly jobs' and the other FP64 jobs'
     std::vector<uint32_t> live;
     DevBuf jobs_buf, b2e_buf, b2n_buf, pe_buf, pn_buf;
     // where the kernels find the job table and block maps: the DevBufs, or
@@ -90,11 +91,13 @@ struct ensc_engine {
     char* scratch_base = nullptr;
     uint32_t nblk_pipe = 0, nblk_e = 0, nblk_g = 0;  // element blocks: pipelined, staged, gathered (mixed)
     uint32_t nblk_e64 = 0, nblk_g64 = 0;             // element blocks: staged, gathered (FP64)
-    uint32_t nblk_n_asm = 0, nblk_n = 0, nblk_n64 = 0;  // node blocks: block partials, per-corner (mixed, FP64)
+    uint32_t nblk_n_asm = 0, nblk_n = 0;      // node blocks, mixed jobs: block partials, per-corner forces
+    uint32_t nblk_n64_asm = 0, nblk_n64 = 0;  // node blocks, FP64 jobs: block partials, per-corner forces
     bool dirty = true;
     KernelFn ke = nullptr, ke_g = nullptr;      // k_element staged / gathered, mixed jobs
     KernelFn ke64 = nullptr, ke64_g = nullptr;  // the same, FP64 jobs
-    KernelFn kn = nullptr, kn_asm = nullptr, kn64 = nullptr;  // k_node per-corner (mixed) / block partials / FP64
+    KernelFn kn = nullptr, kn_asm = nullptr;      // k_node, mixed jobs: per-corner / block partials
+    KernelFn kn64 = nullptr, kn64_asm = nullptr;  // k_node, FP64 jobs
     bool staged = false;      // some live job stages its nodes in LDS
     uint32_t n_asm = 0;       // live jobs using block assembly
     uint32_t n_fp64 = 0;      // live jobs running in FP64
@@ -239,28 +242,32 @@ ensc_status relayout(ensc_engine* E) {
     // any mix:
     //  - precision: the job's own (job_precision);
     //  - staged: its largest block node list fits 64 KiB of LDS (72 B per
-    //    node; mixed jobs add the block assembly accumulators, 12 B);
-    //  - block assembly (mixed): staged and free of corner conflicts;
+    //    node, plus the block assembly accumulators: 12 B mixed, 24 B FP64);
+    //  - block assembly: staged and free of corner conflicts (FP64 jobs:
+    //    unless the reference's summation order is asked for);
     //  - pipelined pass: block assembly, no amplitude velocity BCs and tiles
     //    of PIPE_BLK elements; each such tile with at most PIPE_CAP nodes
     //    (the rest of the job's tiles use the staged pass).  The split
     //    follows from the job's own tiles, so it too is the same in any mix.
-    auto lds_per_node = [](const Job& J) { return 9 * sizeof(double) + (J.precision == ENSC_MIXED ? 12 : 0); };
+    auto asm_wanted = [&](const Job& J) { return J.precision == ENSC_MIXED || E->fp64_order == 0; };
+    auto lds_per_node = [&](const Job& J) {
+        return 9 * sizeof(double) + (asm_wanted(J) ? (J.precision == ENSC_MIXED ? 12 : 24) : 0);
+    };
     auto job_staged = [&](const Job& J) {
         return E->staging_opt != 2 && (size_t)J.lds_nodes * lds_per_node(J) <= 65536;
     };
-    auto job_asm = [&](const Job& J) { return J.precision == ENSC_MIXED && J.asm_capable && job_staged(J); };
+    auto job_asm = [&](const Job& J) { return asm_wanted(J) && J.asm_capable && job_staged(J); };
     auto job_pipe = [&](const Job& J) {
         return E->pipe_opt && job_asm(J) && !J.dev.dyn_bc && E->blk_e == PIPE_BLK;
     };
     // block maps: (job, block of the job, staged range) per launched block,
     // in XCD order, grouped as described at ensc_engine::live
-    std::vector<uint4> lp, ls, lg, ls64, lg64, lna, ln, ln64;
+    std::vector<uint4> lp, ls, lg, ls64, lg64, lna, ln, lna64, ln64;
     std::vector<JobDev> tab;
     std::vector<size_t> pe0, pn0;
     size_t ne_blocks = 0, n_blocks = 0;
     uint32_t lds_nodes = 0, lds_nodes64 = 0;
-    bool any_asm = false;
+    bool any_asm = false, any_asm64 = false;
     E->n_asm = 0;
     E->n_fp64 = 0;
     for (uint32_t i = 0; i < E->live.size(); ++i) {
@@ -282,11 +289,12 @@ ensc_status relayout(ensc_engine* E) {
                 any_asm |= as;
             } else if (le == &ls64) {
                 lds_nodes64 = std::max(lds_nodes64, nodes);
+                any_asm64 |= as;
             }
             le->push_back(make_uint4(i, b, J.blk_ptr_h[b], nodes));
         }
         ne_blocks += D.nblk_e;
-        std::vector<uint4>& lnj = !m ? ln64 : (as ? lna : ln);
+        std::vector<uint4>& lnj = !m ? (as ? lna64 : ln64) : (as ? lna : ln);
         for (uint32_t b = 0; b < D.nblk_n; ++b) lnj.push_back(make_uint4(i, b, 0, 0));
         n_blocks += D.nblk_n;
         J.dev.block_asm = as ? 1 : 0;
@@ -298,7 +306,7 @@ ensc_status relayout(ensc_engine* E) {
         const std::vector<uint4> r = xcd_order(*l);
         b2e.insert(b2e.end(), r.begin(), r.end());
     }
-    for (const std::vector<uint4>* l : {&lna, &ln, &ln64}) {
+    for (const std::vector<uint4>* l : {&lna, &ln, &lna64, &ln64}) {
         const std::vector<uint4> r = xcd_order(*l);
         b2n.insert(b2n.end(), r.begin(), r.end());
     }
@@ -309,6 +317,7 @@ ensc_status relayout(ensc_engine* E) {
     E->nblk_g64 = (uint32_t)lg64.size();
     E->nblk_n_asm = (uint32_t)lna.size();
     E->nblk_n = (uint32_t)ln.size();
+    E->nblk_n64_asm = (uint32_t)lna64.size();
     E->nblk_n64 = (uint32_t)ln64.size();
     const size_t sz[5] = {std::max<size_t>(1, E->live.size()) * sizeof(JobDev),
                           std::max<size_t>(1, b2e.size()) * sizeof(uint4), std::max<size_t>(1, b2n.size()) * sizeof(uint4),
@@ -364,7 +373,7 @@ ensc_status relayout(ensc_engine* E) {
     HIPCHK(hipStreamSynchronize(E->stream));
     E->staged = E->nblk_pipe + E->nblk_e + E->nblk_e64 > 0;
     E->lds_bytes = (size_t)std::max<uint32_t>(lds_nodes, 1) * (any_asm ? 9 * sizeof(double) + 12 : 9 * sizeof(double));
-    E->lds_bytes64 = (size_t)std::max<uint32_t>(lds_nodes64, 1) * 9 * sizeof(double);
+    E->lds_bytes64 = (size_t)std::max<uint32_t>(lds_nodes64, 1) * (9 * sizeof(double) + (any_asm64 ? 24 : 0));
     E->ke = pick_e<true>(E->blk_e, true, E->waves);
     E->ke_g = pick_e<true>(E->blk_e, false, E->waves);
     E->ke64 = pick_e<false>(E->blk_e, true, E->waves64);
@@ -372,6 +381,7 @@ ensc_status relayout(ensc_engine* E) {
     E->kn = pick_n<true, false>(E->blk_n);
     E->kn_asm = pick_n<true, true>(E->blk_n);
     E->kn64 = pick_n<false, false>(E->blk_n);
+    E->kn64_asm = pick_n<false, true>(E->blk_n);
     E->dirty = false;
     E->graph_valid = false;
     return ENSC_OK;
@@ -432,6 +442,9 @@ hipError_t launch_phase(ensc_engine* E, int phase) {
             map += E->nblk_n_asm;
             if (E->nblk_n) hipLaunchKernelGGL(E->kn, dim3(E->nblk_n), dim3(E->blk_n), 0, E->stream, jobs, map);
             map += E->nblk_n;
+            if (E->nblk_n64_asm)
+                hipLaunchKernelGGL(E->kn64_asm, dim3(E->nblk_n64_asm), dim3(E->blk_n), 0, E->stream, jobs, map);
+            map += E->nblk_n64_asm;
             if (E->nblk_n64) hipLaunchKernelGGL(E->kn64, dim3(E->nblk_n64), dim3(E->blk_n), 0, E->stream, jobs, map);
             break;
         }
@@ -566,6 +579,8 @@ ensc_status engine_new(const ensc_options* opts, std::unique_ptr<ensc_engine>& E
     if (o.waves < 0 || o.waves > 3) return fail(ENSC_ERR_ARG, "waves must be 0 (default), 1, 2 or 3");
     if (o.fp64_jobs < 0 || o.fp64_jobs > 2) return fail(ENSC_ERR_ARG, "fp64_jobs must be 0 (auto), 1 or 2");
     E->fp64_rule = o.fp64_jobs;
+    if (o.fp64_order < 0 || o.fp64_order > 1) return fail(ENSC_ERR_ARG, "fp64_order must be 0 or 1");
+    E->fp64_order = o.fp64_order;
     E->waves = o.waves ? o.waves : (o.precision == ENSC_FP64 ? 1 : 2);
     // FP64 jobs: the engine's occupancy target in an FP64 engine, one wave
     // per SIMD in a mixed one (the FP64 body needs more than 128 registers)
@@ -1540,8 +1555,8 @@ ensc_status ensc_engine_layout(ensc_engine* E, int64_t* out) {
     out[ENSC_L_STAGED] = E->staged ? 1 : 0;
     out[ENSC_L_PIPE_BLOCKS] = E->nblk_pipe;
     out[ENSC_L_ELEM_BLOCKS] = E->nblk_e + E->nblk_g + E->nblk_e64 + E->nblk_g64;
-    out[ENSC_L_NODE_BLOCKS] = E->nblk_n_asm + E->nblk_n + E->nblk_n64;
-    out[ENSC_L_LDS_BYTES] = (int64_t)E->lds_bytes;
+    out[ENSC_L_NODE_BLOCKS] = E->nblk_n_asm + E->nblk_n + E->nblk_n64_asm + E->nblk_n64;
+    out[ENSC_L_LDS_BYTES] = (int64_t)std::max(E->lds_bytes, E->lds_bytes64);
     out[ENSC_L_ASM_JOBS] = E->n_asm;
     out[ENSC_L_FP64_JOBS] = E->n_fp64;
     return ENSC_OK;

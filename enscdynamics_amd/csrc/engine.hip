@@ -404,10 +404,10 @@ ensc_status prepare_job(Job& J, int engine_precision, int reorder, int blk_e, in
             J.lds_nodes = std::max<uint32_t>(J.lds_nodes, (uint32_t)tmp.size());
         }
     }
-    // Block assembly (mixed engine) needs every element block to be free of
-    // corner conflicts: no two of its elements share a node at the same
-    // corner index.  Hex meshes of one orientation always are.
-    J.asm_capable = mixed;
+    // Block assembly needs every element block to be free of corner
+    // conflicts: no two of its elements share a node at the same corner
+    // index.  Hex meshes of one orientation always are.
+    J.asm_capable = true;
     {
         const size_t nb = blk_ptr.size() - 1;
         std::vector<uint32_t> stamp(J.lds_nodes + 1, 0);
@@ -514,21 +514,15 @@ ensc_status upload_job(Job& J) {
         std::vector<uint32_t> no_c;
         if ((st = upload(&dc, no_c, Nn, A))) return st;
         D.ucomp = dc;
+        float* dpf;
+        if ((st = upload(&dpf, none, I.blk_nodes.size() * 3, A))) return st;
+        D.part_f = dpf;
         D.Gi = nullptr;
         D.V0 = nullptr;
         D.sigma = ds;
         D.qia = dq;
         D.epsp = dp;
         D.fel = df;
-        float* dpf;
-        uint32_t *dpp, *dpl;
-        if ((st = upload(&dpf, none, I.blk_nodes.size() * 3, A))) return st;
-        if ((st = upload(&dpp, I.pos_ptr, I.pos_ptr.size(), A))) return st;
-        if ((st = upload(&dpl, I.pos_list, I.pos_list.size(), A))) return st;
-        D.part_f = dpf;
-        D.pos_ptr = dpp;
-        D.pos_list = dpl;
-
         std::vector<double> im(Nn);
         for (size_t i = 0; i < Nn; ++i) im[i] = I.pos[i * POS_REC + 3];
         double* dim;
@@ -549,6 +543,17 @@ ensc_status upload_job(Job& J) {
         D.qia = dq;
         D.epsp = dp;
         D.fel = df;
+        double* dpf;
+        if ((st = upload(&dpf, none, I.blk_nodes.size() * 3, A))) return st;
+        D.part_f = dpf;
+    }
+    // block assembly: positions of each node's partials
+    {
+        uint32_t *dpp, *dpl;
+        if ((st = upload(&dpp, I.pos_ptr, I.pos_ptr.size(), A))) return st;
+        if ((st = upload(&dpl, I.pos_list, I.pos_list.size(), A))) return st;
+        D.pos_ptr = dpp;
+        D.pos_list = dpl;
     }
     double *d_pos, *d_vu;
     if ((st = upload(&d_pos, I.pos, I.pos.size(), A))) return st;

@@ -90,11 +90,12 @@ struct JobDev {
     const uint16_t* lconn;     // [8][ne_s] corner -> index in the block's node list
     const uint32_t* blk_ptr;   // [nblk_e + 1] offsets into blk_nodes
     const uint32_t* blk_nodes; // node ids of each element block, ascending
-    // block assembly (mixed engine): each element block sums its elements'
-    // corner forces per block node in LDS and writes one partial per
-    // (block, node) at the node's position in blk_nodes; the nodal update
-    // adds a node's partials in ascending position order.
-    float* part_f;             // [n_pos][3] partial nodal forces
+    // block assembly: each element block sums its elements' corner forces
+    // per block node in LDS and writes one partial per (block, node) at the
+    // node's position in blk_nodes; the nodal update adds a node's partials
+    // in ascending position order.  FP32 partials for mixed jobs, FP64 for
+    // FP64 jobs.
+    void* part_f;              // [n_pos][3] partial nodal forces
     const uint32_t* pos_ptr;   // [nn + 1] offsets into pos_list
     const uint32_t* pos_list;  // positions in blk_nodes of each node, ascending
     // node arrays

@@ -78,7 +78,7 @@ struct Job {
     size_t ne_s = 0, nn_s = 0;
     uint32_t ell_w = 0;
     uint32_t lds_nodes = 0;  // largest distinct-node count of an element block
-    bool asm_capable = false; // mixed engine, every element block free of corner conflicts
+    bool asm_capable = false; // every element block free of corner conflicts (block assembly possible)
     int precision = ENSC_FP64;  // the job's own precision (job_precision)
     std::vector<uint32_t> blk_ptr_h;  // host copy of the staging list offsets (block maps)
     std::unique_ptr<JobImage> img;    // between prepare_job and upload_job

@@ -438,7 +438,7 @@ __device__ __noinline__ V3 bc_override(const JobDev& J, uint32_t g, double t, V3
 #define ELEM_ARGS                                                                                          \
     const JobDev &J, uint32_t e, const ElemState<R> &ES, double dt, int32_t kind, double time_n, bool act, \
         bool asmode, double &Le, double &dts, double *en, uint32_t &errf, const ENSC_LDS double *lds,     \
-        ENSC_LDS float *acc
+        ENSC_LDS R *acc
 #define NVAL(k, i)                                                                                      \
     ((i) < 3 ? (MIXED ? gld((const double*)J.pos, ES.nd[k] * POS_REC + (i)) +                           \
                             gld((const double*)J.vu, ES.nd[k] * VU_REC + 3 + (i))                           \
@@ -478,7 +478,7 @@ __device__ __forceinline__ void element_mixed_G(ELEM_ARGS) {
 template <class R>
 __device__ __forceinline__ void element_fp64_S(ELEM_ARGS) {
 #pragma clang fp contract(off)
-    constexpr bool MIXED = false, ASM_CAP = false;
+    constexpr bool MIXED = false, ASM_CAP = true;
 #include "elem_body.inc"
 }
 template <class R>
@@ -661,10 +661,10 @@ __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(WPE))) void
     ElemState<R> ES;
     load_state<R, STAGED>(J, act ? e : J.ne - 1, dc.kind != KIND_PREPASS, ES);
     ENSC_LDS double* lds = (ENSC_LDS double*)s_nodes;
-    // block assembly: staged mixed jobs whose blocks are conflict-free, on
-    // ticks that compute forces (uniform over the block)
-    const bool asmode = MIXED && STAGED && J.block_asm && dc.kind != KIND_PREPASS;
-    ENSC_LDS float* acc = (ENSC_LDS float*)(lds + nbn * 9);  // block assembly: FP32 sum per block node
+    // block assembly: staged jobs whose blocks are conflict-free, on ticks
+    // that compute forces (uniform over the block)
+    const bool asmode = STAGED && J.block_asm && dc.kind != KIND_PREPASS;
+    ENSC_LDS R* acc = (ENSC_LDS R*)(lds + nbn * 9);  // block assembly: sum per block node, precision R
     if (STAGED) {
         // stage the block's node records once (X, v, u); velocity BCs with an
         // amplitude are applied here, once per node
@@ -686,9 +686,9 @@ __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(WPE))) void
             d[7] = p4.a;
             d[8] = p4.b;
             if (asmode) {
-                acc[3 * i] = 0.f;
-                acc[3 * i + 1] = 0.f;
-                acc[3 * i + 2] = 0.f;
+                acc[3 * i] = R(0);
+                acc[3 * i + 1] = R(0);
+                acc[3 * i + 2] = R(0);
             }
         }
         lds_sync();
@@ -702,9 +702,12 @@ __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(WPE))) void
         if constexpr (MIXED && STAGED)
             element_mixed_S<R>(J, act ? e : J.ne - 1, ES, dc.dt_i_1, dc.kind, dc.time_n, act, true, Le, dts, en, errf,
                                lds, acc);
+        else if constexpr (STAGED)
+            element_fp64_S<R>(J, act ? e : J.ne - 1, ES, dc.dt_i_1, dc.kind, dc.time_n, act, true, Le, dts, en, errf,
+                              lds, acc);
         lds_sync();
         for (uint32_t i = threadIdx.x; i < nbn; i += BLK) {
-            float* pf = J.part_f + (size_t)(n0 + i) * 3;
+            R* pf = static_cast<R*>(J.part_f) + (size_t)(n0 + i) * 3;
             gst(pf, 0u) = acc[3 * i];
             gst(pf, 1u) = acc[3 * i + 1];
             gst(pf, 2u) = acc[3 * i + 2];
@@ -961,7 +964,7 @@ __device__ __forceinline__ bool pipe_tile(const JobDev* __restrict__ jobs, const
             const ENSC_LDS float* acc = pipe_acc(cur);
             lds_sync();
             for (uint32_t i = threadIdx.x; i < nbn; i += PIPE_BLK) {
-                float* pf = J.part_f + (size_t)(bm.z + i) * 3;
+                float* pf = static_cast<float*>(J.part_f) + (size_t)(bm.z + i) * 3;
                 gst(pf, 0u) = acc[4 * i];
                 gst(pf, 1u) = acc[4 * i + 1];
                 gst(pf, 2u) = acc[4 * i + 2];
@@ -1055,7 +1058,7 @@ __global__ __launch_bounds__(PIPE_BLK) __attribute__((amdgpu_waves_per_eu(WPE)))
 // ASM: the launch covers block-assembly jobs only (mixed engine) and sums
 // their block partials; otherwise the per-corner forces through the ELL.
 template <bool MIXED, int BLK, bool ASM>
-__global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(ASM ? 8 : 1))) void k_node(
+__global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(ASM ? (MIXED ? 8 : 4) : 1))) void k_node(
     const JobDev* __restrict__ jobs, const uint4* __restrict__ blkmap) {
 #pragma clang fp contract(off)
     const uint4 bm = blkmap[blockIdx.x];  // (job, block of the job, -, -)
@@ -1099,16 +1102,18 @@ __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(ASM ? 8 : 1
         if (ASM) {
             // block assembly: the node's partials, one per element block that
             // holds it, in ascending block order
+            typedef typename std::conditional<MIXED, float, double>::type P;
             const uint32_t q0 = gld(J.pos_ptr, n), q1 = gld(J.pos_ptr, n + 1);
             for (uint32_t q = q0; q < q1; q += 4) {
-                float f[4][3];
+                P f[4][3];
 #pragma unroll
                 for (int j = 0; j < 4; ++j) {
-                    f[j][0] = f[j][1] = f[j][2] = 0.f;
+                    f[j][0] = f[j][1] = f[j][2] = P(0);
                     if (q + j < q1) {
-                        // one 12-byte record: a single address, three offsets
+                        // one record: a single address, three offsets
                         const uint32_t pos = gld(J.pos_list, q + j);
-                        const ENSC_GLB float* pf = (const ENSC_GLB float*)((const ENSC_GLB char*)J.part_f + pos * 12u);
+                        const ENSC_GLB P* pf =
+                            (const ENSC_GLB P*)((const ENSC_GLB char*)J.part_f + pos * (uint32_t)(3 * sizeof(P)));
                         f[j][0] = pf[0];
                         f[j][1] = pf[1];
                         f[j][2] = pf[2];

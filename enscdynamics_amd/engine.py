@@ -55,7 +55,7 @@ class Options(C.Structure):
         ("staging", C.c_int32),
         ("waves", C.c_int32),
         ("fp64_jobs", C.c_int32),
-        ("pad_", C.c_int32),
+        ("fp64_order", C.c_int32),
     ]
 
 
@@ -151,7 +151,7 @@ class Engine:
 
     def __init__(self, model: Model, precision: int = FP64, reorder: bool = False, block_elem: int = 0,
                  block_node: int = 0, probes: Sequence[int] = (), hist_cap: int = 0, device: int = 0,
-                 staging: bool = True, waves: int = 0, fp64_jobs: int = 0):
+                 staging: bool = True, waves: int = 0, fp64_jobs: int = 0, fp64_order: int = 0):
         L = load_library()
         self.model = model
         self.precision = precision
@@ -170,6 +170,7 @@ class Engine:
         o.staging = (0 if staging else 2) if isinstance(staging, bool) else int(staging)
         o.waves = waves
         o.fp64_jobs = fp64_jobs
+        o.fp64_order = fp64_order
         h = C.c_void_p()
         _check(L.ensc_engine_create(C.byref(self._desc), C.byref(o), C.byref(h)))
         self.h = h

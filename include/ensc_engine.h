@@ -79,7 +79,15 @@ typedef struct ensc_options {
                                1: every job mixed precision;
                                2: every job FP64.
                                FP64 engines run every job in FP64. */
-    int32_t pad_;
+    int32_t fp64_order;     /* how FP64 jobs sum a node's element forces:
+                               0: each element block sums its elements'
+                               corner forces in LDS and the node adds the
+                               blocks' partials (no per-corner forces through
+                               HBM; <= 1e-10 of the reference, rounding of the
+                               other order); 1: the reference's order, per
+                               corner then ascending element (exDyna3D.cpp:
+                               584-639) through the gather table - bit
+                               identical to the CPU restatement. */
 } ensc_options;
 
 /* Scalars of one job (ensc_engine_scalars). */

@@ -305,10 +305,10 @@ def test_server_mixed_config5_jobs_vs_fp64():
 
 def test_mixed_engine_runs_quasi_static_jobs_in_fp64():
     # per-job precision: in a mixed batch, the two quasi-static reference
-    # decks (thousands of wave transits) run the FP64 kernels and equal the
-    # FP64 engine's displacement and stress bit for bit (the gather follows
-    # the canonical (corner, element) order whatever the block partition),
-    # while the other jobs run mixed and equal themselves alone
+    # decks (thousands of wave transits) run the FP64 kernels and equal an
+    # FP64 engine with the same element blocks (256 elements, the mixed
+    # engine's) bit for bit, while the other jobs run mixed and equal
+    # themselves alone
     quasi = [G.model_from_dump(G.load(c)) for c in ("ref_test_part1", "ref_Job_6")]
     jobs = small_jobs(4, seed=71, plastic_p=0.5)
     jobs = [jobs[0], quasi[0], jobs[1], jobs[2], quasi[1], jobs[3]]
@@ -317,7 +317,7 @@ def test_mixed_engine_runs_quasi_static_jobs_in_fp64():
         s = b.scalars(i)
         assert s["done"] == 1.0 and s["err"] == 0.0, i
         if any(m is q for q in quasi):
-            a = eng.Engine(m, eng.FP64, reorder=True).run()
+            a = eng.Engine(m, eng.FP64, reorder=True, block_elem=256).run()
             assert a.layout()["fp64_jobs"] == 1
         else:
             a = eng.Engine(m, eng.MIXED, reorder=True).run()

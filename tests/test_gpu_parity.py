@@ -140,13 +140,42 @@ def test_fp64_matches_oracle_plastic_cube():
 
 
 def test_reorder_is_bit_exact():
+    # the reference's summation order (fp64_order=1): per node, corner by
+    # corner, ascending original element - independent of the numbering
     m = decks.config2(nx=20, ny=7, nz=5, steps=200, seed=11)
-    a = eng.Engine(m, eng.FP64, reorder=False)
+    a = eng.Engine(m, eng.FP64, reorder=False, fp64_order=1)
     a.run()
-    b = eng.Engine(m, eng.FP64, reorder=True)
+    b = eng.Engine(m, eng.FP64, reorder=True, fp64_order=1)
     b.run()
     for f in ["u", "v", "x", "sigma", "qia", "rf"]:
         assert np.array_equal(a.get(f), b.get(f)), f
+    # the oracle sums in the same order; the device's cbrt may differ from
+    # glibc's by an ulp, so agreement is to rounding
+    o = Oracle(m, 0).init()
+    o.run(0, [], 0)
+    for f in ["u", "v", "sigma", "qia"]:
+        assert rel(a.get(f), o.get(f), 1e-12) <= 1e-12, f
+
+
+def test_fp64_block_assembly_vs_reference_order():
+    # default FP64 assembly (block partials summed per node) against the
+    # reference's order: the same forces added in another order, so the
+    # results differ by rounding only, renumbered or not
+    m = decks.config3(n=14, steps=400)
+    a = eng.Engine(m, eng.FP64, reorder=True, fp64_order=1).run()
+    for reorder in (True, False):
+        b = eng.Engine(m, eng.FP64, reorder=reorder)
+        b.init()
+        b.set_current_step(1)
+        b.advance(3)
+        lay = b.layout()
+        b.advance(0)
+        assert lay["asm_jobs"] == 1 and lay["staged"] == 1, lay
+        assert b.scalars()["nsteps"] == a.scalars()["nsteps"]
+        for f in ("u", "v", "sigma", "qia", "rf"):
+            err = rel(b.get(f), a.get(f), 1e-12)
+            assert err <= 1e-12, (reorder, f, err)
+    print(f"[fp64 block assembly] within 1e-12 of the reference order", flush=True)
 
 
 def test_determinism_two_runs():
