@@ -299,6 +299,15 @@ def c5_section(args):
     secs_sat, lat_sat = serve([0.0] * len(models))
     cap = len(models) / secs_sat
     log(f"c5: saturated {secs_sat:.3f} s ({cap:.2f} solves/s)")
+    # the same jobs as one batch, all resident from the start (no admission,
+    # no pumps): the device-side bound of the saturated server
+    t0 = time.perf_counter()
+    b = Batch(models, eng.MIXED, reorder=True, graph_ticks=args.graph)
+    t1 = time.perf_counter()
+    b.run()
+    secs_batch = time.perf_counter() - t1
+    b.close()
+    log(f"c5: as one batch {secs_batch:.3f} s (built in {t1 - t0:.2f} s)")
     rate = 0.8 * cap
     rng = np.random.default_rng(55)
     arr = np.cumsum(rng.exponential(1.0 / rate, size=len(models)))
@@ -350,6 +359,7 @@ def c5_section(args):
             "jobs_note": f"{len(models)} completions per leg keep the whole bench within a few minutes "
                          f"(p99 of {len(models)} latencies); --c5-jobs 1000 runs the full stream",
             "capacity_solves_per_s": cap,
+            "one_batch_solves_per_s": len(models) / secs_batch,
             "poisson_rate_per_s": rate,
             "poisson_solves_per_s": len(models) / secs_p,
             "latency_s": {"p50": float(np.percentile(lat_p, 50)), "p99": float(np.percentile(lat_p, 99)),

@@ -1,10 +1,13 @@
 // Test infrastructure only: the reference solver with its step loop on the
 // MI355X engine.  The reference's own reader (DataIn, dataIn.cpp:300-558)
-// fills an exDyna3D_MI355X (exDyna3D_mi355x.h, the subclass a maintainer
-// adds to the reference), whose run() executes on the GPU through the C ABI; the
-// final state is written for comparison with the CPU reference's goldens.
+// fills an exDyna3D_MI355X (enscdynamics_amd/binding/exDyna3D_mi355x.h, the
+// subclass a maintainer adds to the reference), whose run() executes on the
+// GPU through the C ABI; the final state is written for comparison with the
+// CPU reference's goldens.  With "stepwise" the binding's per-step mode runs,
+// and this subclass hooks computeSate() the way the reference's own
+// subclasses do, counting the step iterations it sees.
 //
-// usage: ref_dropin WORKDIR DECK OUTDIR fp64|mixed [c_dr c_cr factor]
+// usage: ref_dropin WORKDIR DECK OUTDIR fp64|mixed [c_dr c_cr factor [stepwise]]
 #include "dataIn.h"
 #include "exDyna3D_mi355x.h"
 #include "npy.h"
@@ -19,6 +22,11 @@ namespace {
 class DropIn : public EnSC::exDyna3D_MI355X {
 public:
     using exDyna3D_MI355X::exDyna3D_MI355X;
+    long hooked = 0;  // computeSate() calls seen by this subclass
+    void computeSate() override {
+        exDyna3D_MI355X::computeSate();
+        ++hooked;
+    }
     void write(const std::string& out) const {
         const size_t Nn = vertices.size(), Ne = hexahedron_elements.size();
         std::vector<double> u(3 * Nn), x(3 * Nn), s(6 * Ne);
@@ -32,14 +40,14 @@ public:
         npy::npy_f64(out + "/sigma.npy", {Ne, 6}, s.data());
         std::ofstream f(out + "/scalars.txt");
         f.precision(17);
-        f << "time " << time << "\nsteps " << steps_run() << "\n";
+        f << "time " << time << "\nsteps " << steps_run() << "\nhooked " << hooked << "\n";
     }
 };
 }  // namespace
 
 int main(int argc, char** argv) {
     if (argc < 5) {
-        std::fprintf(stderr, "usage: %s WORKDIR DECK OUTDIR fp64|mixed [c_dr c_cr factor]\n", argv[0]);
+        std::fprintf(stderr, "usage: %s WORKDIR DECK OUTDIR fp64|mixed [c_dr c_cr factor [stepwise]]\n", argv[0]);
         return 2;
     }
     const std::string work = argv[1], deck = argv[2], out = argv[3], prec = argv[4];
@@ -56,7 +64,8 @@ int main(int argc, char** argv) {
             p << "#\n#c_dr\n" << argv[5] << "\n#\n#c_cr\n" << argv[6] << "\n#\n#factor\n" << argv[7]
               << "\n#\n#time_interval\n1.0\n";
     }
-    DropIn sim(prec == "mixed" ? ENSC_MIXED : ENSC_FP64);
+    const bool stepwise = argc >= 9 && std::string(argv[8]) == "stepwise";
+    DropIn sim(prec == "mixed" ? ENSC_MIXED : ENSC_FP64, true, stepwise);
     EnSC::DataIn reader(sim);
     reader.read_inp(deck);
     try {

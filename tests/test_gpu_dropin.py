@@ -1,5 +1,5 @@
 """Drop-in check: the reference's own reader and solver class, with the
-MI355X binding (oracle/ref_harness/exDyna3D_mi355x.h, the subclass a
+MI355X binding (enscdynamics_amd/binding/exDyna3D_mi355x.h, the subclass a
 maintainer adds to the reference) replacing its CPU step loop.
 
 `make -C oracle shim` links the reference's sources (where /root/reference
@@ -32,19 +32,24 @@ def rel(a, b, floor=0.0):
     return float(np.abs(a - b).max()) / s if s > 0 else float(np.abs(a - b).max())
 
 
-def run_dropin(case, precision, tmp_path):
+def run_dropin(case, precision, tmp_path, stepwise=False):
     if not os.path.exists(DROPIN):
         pytest.skip("oracle/_ref/ref_dropin not built (needs the reference sources: make -C oracle shim)")
     d = G.load(case)
     j = json.loads(str(d["model_json"]))
+    tmp_path.mkdir(parents=True, exist_ok=True)
     with gzip.open(os.path.join(G.GOLDEN, "decks", DECKS[case] + ".inp.gz"), "rt") as f:
         (tmp_path / "deck.inp").write_text(f.read())
     args = [DROPIN, str(tmp_path), "deck.inp", str(tmp_path), precision,
-            repr(j["c_dr"]), repr(j["c_cr"]), repr(j["factor"])]
-    print(f"[dropin] {case} {precision}", flush=True)
+            repr(j["c_dr"]), repr(j["c_cr"]), repr(j["factor"])] + (["stepwise"] if stepwise else [])
+    print(f"[dropin] {case} {precision}{' stepwise' if stepwise else ''}", flush=True)
     r = subprocess.run(args, capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stderr[-2000:]
-    steps = int(open(tmp_path / "scalars.txt").read().split("steps")[1].split()[0])
+    sc = open(tmp_path / "scalars.txt").read().split()
+    steps = int(sc[sc.index("steps") + 1])
+    if stepwise:
+        # the subclass's computeSate() hook saw every step iteration
+        assert int(sc[sc.index("hooked") + 1]) == steps
     return d, np.load(tmp_path / "u.npy"), np.load(tmp_path / "sigma.npy"), np.load(tmp_path / "x.npy"), steps
 
 
@@ -94,3 +99,15 @@ def test_dropin_writes_the_reference_result_files(tmp_path):
         for f in ("vonMises", "Stress", "Green-Lagrange Strain"):
             ref = o.snapshot(k, f).reshape(r["cell_data"][f].shape)
             assert np.allclose(r["cell_data"][f], ref, rtol=1e-5, atol=1e-6 * max(np.abs(ref).max(), 1e-30)), f
+
+
+def test_stepwise_binding_hooks_every_step(tmp_path):
+    # the binding's per-step mode: run() calls computeSate() once per step
+    # iteration, the subclass hooks it, and the result equals the whole-run
+    # mode bit for bit (the same device steps) and the reference's golden
+    case = "ref_test_step"
+    d, u, s, x, steps = run_dropin(case, "fp64", tmp_path / "whole")
+    d, u2, s2, x2, steps2 = run_dropin(case, "fp64", tmp_path / "stepwise", stepwise=True)
+    assert steps2 == steps == int(d["n_steps"])
+    assert np.array_equal(u2, u) and np.array_equal(s2, s) and np.array_equal(x2, x)
+    assert rel(u2, d["final_u"], 1e-12) <= 1e-10
