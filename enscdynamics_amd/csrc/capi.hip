@@ -82,10 +82,11 @@ struct ensc_engine {
     // block-assembly jobs' blocks, the other mixed jobs', the FP64
     // block-assembly jobs' and the other FP64 jobs'
     std::vector<uint32_t> live;
-    DevBuf jobs_buf, b2e_buf, b2n_buf, pe_buf, pn_buf;
+    DevBuf jobs_buf, b2e_buf, b2n_buf, b2f_buf, pe_buf, pn_buf;
     // where the kernels find the job table and block maps: the DevBufs, or
     // the static scratch array (ensc_engine_advance_static)
-    void *jobs_p = nullptr, *b2e_p = nullptr, *b2n_p = nullptr;
+    void *jobs_p = nullptr, *b2e_p = nullptr, *b2n_p = nullptr, *b2f_p = nullptr;
+    uint32_t nblk_f = 0;  // face blocks of the surface-pressure kernel
     bool in_scratch = false;      // the call in progress places its tables in scratch
     bool layout_scratch = false;  // the current layout lives in scratch
     char* scratch_base = nullptr;
@@ -262,7 +263,7 @@ ensc_status relayout(ensc_engine* E) {
     };
     // block maps: (job, block of the job, staged range) per launched block,
     // in XCD order, grouped as described at ensc_engine::live
-    std::vector<uint4> lp, ls, lg, ls64, lg64, lna, ln, lna64, ln64;
+    std::vector<uint4> lp, ls, lg, ls64, lg64, lna, ln, lna64, ln64, lf;
     std::vector<JobDev> tab;
     std::vector<size_t> pe0, pn0;
     size_t ne_blocks = 0, n_blocks = 0;
@@ -297,6 +298,7 @@ ensc_status relayout(ensc_engine* E) {
         std::vector<uint4>& lnj = !m ? (as ? lna64 : ln64) : (as ? lna : ln);
         for (uint32_t b = 0; b < D.nblk_n; ++b) lnj.push_back(make_uint4(i, b, 0, 0));
         n_blocks += D.nblk_n;
+        for (uint32_t b = 0; b < (D.n_dsf + FACE_BLK - 1) / FACE_BLK; ++b) lf.push_back(make_uint4(i, b, 0, 0));
         J.dev.block_asm = as ? 1 : 0;
         E->n_asm += as;
         E->n_fp64 += !m;
@@ -319,10 +321,11 @@ ensc_status relayout(ensc_engine* E) {
     E->nblk_n = (uint32_t)ln.size();
     E->nblk_n64_asm = (uint32_t)lna64.size();
     E->nblk_n64 = (uint32_t)ln64.size();
-    const size_t sz[5] = {std::max<size_t>(1, E->live.size()) * sizeof(JobDev),
+    E->nblk_f = (uint32_t)lf.size();
+    const size_t sz[6] = {std::max<size_t>(1, E->live.size()) * sizeof(JobDev),
                           std::max<size_t>(1, b2e.size()) * sizeof(uint4), std::max<size_t>(1, b2n.size()) * sizeof(uint4),
                           std::max<size_t>(1, ne_blocks) * 5 * sizeof(double),
-                          std::max<size_t>(1, n_blocks) * 2 * sizeof(double)};
+                          std::max<size_t>(1, n_blocks) * 2 * sizeof(double), std::max<size_t>(1, lf.size()) * sizeof(uint4)};
     void* pe = nullptr;
     void* pn = nullptr;
     if (E->in_scratch) {
@@ -336,8 +339,8 @@ ensc_status relayout(ensc_engine* E) {
                                           std::to_string(STATIC_SCRATCH_BYTES) +
                                           "-byte static scratch (use ensc_engine_advance)");
         char* q = E->scratch_base;
-        void* part[5];
-        for (int k = 0; k < 5; ++k) {
+        void* part[6];
+        for (int k = 0; k < 6; ++k) {
             part[k] = q;
             q += round_up(sz[k], 256);
         }
@@ -346,13 +349,15 @@ ensc_status relayout(ensc_engine* E) {
         E->b2n_p = part[2];
         pe = part[3];
         pn = part[4];
+        E->b2f_p = part[5];
     } else {
         if (!E->jobs_buf.ensure(sz[0]) || !E->b2e_buf.ensure(sz[1]) || !E->b2n_buf.ensure(sz[2]) ||
-            !E->pe_buf.ensure(sz[3]) || !E->pn_buf.ensure(sz[4]))
+            !E->pe_buf.ensure(sz[3]) || !E->pn_buf.ensure(sz[4]) || !E->b2f_buf.ensure(sz[5]))
             return fail(ENSC_ERR_MEM, "layout allocation failed");
         E->jobs_p = E->jobs_buf.p;
         E->b2e_p = E->b2e_buf.p;
         E->b2n_p = E->b2n_buf.p;
+        E->b2f_p = E->b2f_buf.p;
         pe = E->pe_buf.p;
         pn = E->pn_buf.p;
     }
@@ -370,6 +375,8 @@ ensc_status relayout(ensc_engine* E) {
         HIPCHK(hipMemcpyAsync(E->b2e_p, b2e.data(), b2e.size() * sizeof(uint4), hipMemcpyHostToDevice, E->stream));
     if (!b2n.empty())
         HIPCHK(hipMemcpyAsync(E->b2n_p, b2n.data(), b2n.size() * sizeof(uint4), hipMemcpyHostToDevice, E->stream));
+    if (!lf.empty())
+        HIPCHK(hipMemcpyAsync(E->b2f_p, lf.data(), lf.size() * sizeof(uint4), hipMemcpyHostToDevice, E->stream));
     HIPCHK(hipStreamSynchronize(E->stream));
     E->staged = E->nblk_pipe + E->nblk_e + E->nblk_e64 > 0;
     E->lds_bytes = (size_t)std::max<uint32_t>(lds_nodes, 1) * (any_asm ? 9 * sizeof(double) + 12 : 9 * sizeof(double));
@@ -437,6 +444,10 @@ hipError_t launch_phase(ensc_engine* E, int phase) {
         }
         case 1: hipLaunchKernelGGL(k_fin_element<FIN_BLK>, dim3(nj), dim3(FIN_BLK), 0, E->stream, jobs); break;
         case 2: {
+            // surface-pressure forces of the step, then the nodal update
+            if (E->nblk_f)
+                hipLaunchKernelGGL(k_face, dim3(E->nblk_f), dim3(FACE_BLK), 0, E->stream, jobs,
+                                   static_cast<const uint4*>(E->b2f_p));
             const uint4* map = static_cast<const uint4*>(E->b2n_p);
             if (E->nblk_n_asm) hipLaunchKernelGGL(E->kn_asm, dim3(E->nblk_n_asm), dim3(E->blk_n), 0, E->stream, jobs, map);
             map += E->nblk_n_asm;
@@ -669,6 +680,35 @@ ensc_status job_switch_step(ensc_engine* E, uint32_t slot, uint32_t si) {
     }
     if (!nodes.empty())
         HIPCHK(hipMemcpyAsync(J.d_bcnodes, nodes.data(), nodes.size() * 4, hipMemcpyHostToDevice, E->stream));
+    // surface pressures of the step (fresh arrays: switches are rare)
+    {
+        std::vector<uint32_t> fnodes, nptr, nlist;
+        std::vector<double> fval;
+        std::vector<int32_t> famp;
+        job_dsload_arrays(J, fnodes, fval, famp, nptr, nlist);
+        const uint32_t nf = (uint32_t)fval.size();
+        J.dev.n_dsf = nf;
+        if (nf) {
+            uint32_t* d_fn = static_cast<uint32_t*>(J.arena.alloc(fnodes.size() * 4));
+            double* d_fv = static_cast<double*>(J.arena.alloc(fval.size() * 8));
+            int32_t* d_fa = static_cast<int32_t*>(J.arena.alloc(famp.size() * 4));
+            double* d_ff = static_cast<double*>(J.arena.alloc((size_t)nf * 3 * 8));
+            uint32_t* d_np = static_cast<uint32_t*>(J.arena.alloc(nptr.size() * 4));
+            uint32_t* d_nl = static_cast<uint32_t*>(J.arena.alloc(nlist.size() * 4));
+            if (!d_fn || !d_fv || !d_fa || !d_ff || !d_np || !d_nl) return fail(ENSC_ERR_MEM, "*Dsload arrays: allocation failed");
+            HIPCHK(hipMemcpy(d_fn, fnodes.data(), fnodes.size() * 4, hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(d_fv, fval.data(), fval.size() * 8, hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(d_fa, famp.data(), famp.size() * 4, hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(d_np, nptr.data(), nptr.size() * 4, hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(d_nl, nlist.data(), nlist.size() * 4, hipMemcpyHostToDevice));
+            J.dev.dsf_nodes = d_fn;
+            J.dev.dsf_val = d_fv;
+            J.dev.dsf_amp = d_fa;
+            J.dev.dsf_force = d_ff;
+            J.dev.ds_ptr = d_np;
+            J.dev.ds_list = d_nl;
+        }
+    }
     const GravState& g = J.grav[si];
     J.dev.grav_on = g.on;
     J.dev.grav_amp = g.amp;
@@ -1690,7 +1730,8 @@ int64_t ensc_engine_device_bytes(ensc_engine* E) {
     int64_t t = 0;
     for (auto& j : E->slots)
         if (j) t += (int64_t)j->arena.total;
-    return t + (int64_t)(E->jobs_buf.cap + E->b2e_buf.cap + E->b2n_buf.cap + E->pe_buf.cap + E->pn_buf.cap) +
+    return t + (int64_t)(E->jobs_buf.cap + E->b2e_buf.cap + E->b2n_buf.cap + E->b2f_buf.cap + E->pe_buf.cap +
+                         E->pn_buf.cap) +
            (int64_t)E->ctl_cap * (int64_t)sizeof(Ctl);
 }
 

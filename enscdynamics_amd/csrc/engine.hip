@@ -250,6 +250,22 @@ bool copy_model(const ensc_model_desc* m, ModelCopy& c, std::string& err) {
             return true;
         };
         if (!cp(sd.spc, sd.n_spc, sc.spc) || !cp(sd.vel, sd.n_vel, sc.vel)) return false;
+        sc.reset_dsload = sd.reset_dsload;
+        for (uint32_t i = 0; i < sd.n_dsload; ++i) {
+            const ensc_dsload_entry& d = sd.dsload[i];
+            DsloadCopy dc;
+            dc.elems.assign(d.elems, d.elems + d.n_faces);
+            dc.faces.assign(d.faces, d.faces + d.n_faces);
+            for (size_t q = 0; q < dc.elems.size(); ++q)
+                if (dc.elems[q] >= c.Ne || dc.faces[q] < 0 || dc.faces[q] > 5) {
+                    err = "*Dsload face references element " + std::to_string(dc.elems[q]) + " face " +
+                          std::to_string(dc.faces[q]) + " (need element < n_elems, face 0..5)";
+                    return false;
+                }
+            dc.amp = d.amp;
+            dc.value = d.value;
+            sc.dsload.push_back(std::move(dc));
+        }
     }
     c.amp_off.push_back(0);
     for (uint32_t i = 0; i < m->n_amps; ++i) {
@@ -677,8 +693,46 @@ bool job_set_step(Job& J, uint32_t si) {
         ids(si, st.vel, J.cur_vel);
         if (!st.reset_vel) J.cur_vel.insert(J.cur_vel.end(), J.prev_vel.begin(), J.prev_vel.end());
         if (!st.reset_dload && !J.grav[si].on && J.grav[si - 1].on) J.grav[si] = J.grav[si - 1];
+        // *Dsload lines (exDyna3D.cpp:1897-1907): the step's own lines; with
+        // none, the previous step's unless op=NEW
+        if (!st.dsload.empty())
+            J.dsl_step = (int32_t)si;
+        else if (st.reset_dsload)
+            J.dsl_step = -1;
     }
     return true;
+}
+
+// C3D8 face k (S(k+1)) as element corners, ordered so that the right-hand
+// rule gives the normal pointing into the element (the face numbering of
+// Element_HexN8.cpp:50-89's node order: S1 = 1-2-3-4, S2 = 5-8-7-6,
+// S3 = 1-5-6-2, S4 = 2-6-7-3, S5 = 3-7-8-4, S6 = 4-8-5-1).
+static const int FACE_CORNERS[6][4] = {{0, 1, 2, 3}, {4, 7, 6, 5}, {0, 4, 5, 1},
+                                       {1, 5, 6, 2}, {2, 6, 7, 3}, {3, 7, 4, 0}};
+
+// The loaded faces of the job's current *Dsload lines (new numbering): per
+// face its 4 nodes, pressure and amplitude, and per node its faces in
+// ascending face order (the order the nodal update adds their forces).
+void job_dsload_arrays(const Job& J, std::vector<uint32_t>& fnodes, std::vector<double>& fval,
+                       std::vector<int32_t>& famp, std::vector<uint32_t>& nptr, std::vector<uint32_t>& nlist) {
+    const ModelCopy& M = J.model;
+    fnodes.clear();
+    fval.clear();
+    famp.clear();
+    if (J.dsl_step >= 0)
+        for (const DsloadCopy& d : M.steps[J.dsl_step].dsload)
+            for (size_t q = 0; q < d.elems.size(); ++q) {
+                for (int c = 0; c < 4; ++c)
+                    fnodes.push_back(J.n_old2new[M.conn[8 * (size_t)d.elems[q] + FACE_CORNERS[d.faces[q]][c]]]);
+                fval.push_back(d.value);
+                famp.push_back(d.amp);
+            }
+    nptr.assign((size_t)M.Nn + 1, 0);
+    for (uint32_t n : fnodes) nptr[n + 1]++;
+    for (size_t i = 0; i < M.Nn; ++i) nptr[i + 1] += nptr[i];
+    nlist.assign(fnodes.size(), 0);
+    std::vector<uint32_t> fill(nptr.begin(), nptr.end() - 1);
+    for (size_t i = 0; i < fnodes.size(); ++i) nlist[fill[fnodes[i]]++] = (uint32_t)(i / 4);
 }
 
 // Per-node BC arrays for the job's current BC lists (new numbering).

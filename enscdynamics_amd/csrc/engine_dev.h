@@ -113,6 +113,17 @@ struct JobDev {
     const double* amp_a;
     const int32_t* amp_off;
     int32_t n_amp;
+    // surface pressures of the current step (extension; *Dsload): per loaded
+    // face its 4 nodes (inward-normal order), pressure and amplitude; the
+    // face-force kernel writes each face's per-node force, and the nodal
+    // update adds a node's face forces in ascending face order
+    uint32_t n_dsf, pad_dsf;
+    const uint32_t* dsf_nodes;  // [n_dsf][4]
+    const double* dsf_val;      // [n_dsf]
+    const int32_t* dsf_amp;     // [n_dsf]
+    double* dsf_force;          // [n_dsf][3]
+    const uint32_t* ds_ptr;     // [nn + 1] offsets into ds_list
+    const uint32_t* ds_list;    // faces of each node, ascending
     // gravity of the current step (StepData.h:22)
     int32_t grav_on, grav_amp;
     double grav_g, grav_dir[3];

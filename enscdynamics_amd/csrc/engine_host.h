@@ -24,11 +24,19 @@ struct BcCopy {
     double value = 0.0;
 };
 
+struct DsloadCopy {
+    std::vector<uint32_t> elems;  // original numbering
+    std::vector<int32_t> faces;   // 0..5 = S1..S6
+    int32_t amp = -1;
+    double value = 0.0;
+};
+
 struct StepCopy {
     double period = 0.0;
-    int32_t reset_spc = 0, reset_vel = 0, reset_dload = 0, grav_on = 0, grav_amp = -1;
+    int32_t reset_spc = 0, reset_vel = 0, reset_dload = 0, grav_on = 0, grav_amp = -1, reset_dsload = 0;
     double grav_value = 0.0, grav_dir[3] = {0, 0, 0};
     std::vector<BcCopy> spc, vel;
+    std::vector<DsloadCopy> dsload;
 };
 
 // The caller's model, copied (include/ensc_model.h).
@@ -87,6 +95,7 @@ struct Job {
     uint32_t cur_step = 0;
     std::vector<BcRef> cur_spc, cur_vel, prev_spc, prev_vel;
     std::vector<GravState> grav;
+    int32_t dsl_step = -1;        // step whose *Dsload lines are current (-1: none)
     uint8_t* d_bcmask = nullptr;
     double* d_bcval = nullptr;
     int16_t* d_bcamp = nullptr;
@@ -114,5 +123,7 @@ constexpr double QUASI_STATIC_TRANSITS = 512.0;
 bool job_set_step(Job& J, uint32_t si);
 void job_bc_arrays(const Job& J, std::vector<uint8_t>& mask, std::vector<double>& val, std::vector<int16_t>& amp,
                    std::vector<uint32_t>& nodes);
+void job_dsload_arrays(const Job& J, std::vector<uint32_t>& fnodes, std::vector<double>& fval,
+                       std::vector<int32_t>& famp, std::vector<uint32_t>& nptr, std::vector<uint32_t>& nlist);
 
 }  // namespace ensc

@@ -68,11 +68,16 @@ struct VelLine {
     double value;
     string amp;
 };
+struct DsloadLine {
+    string surf, amp;
+    double value;
+};
 struct StepRec {
     string name;
     double period = 0.0;
     vector<SpcLine> spc;
     vector<VelLine> vel;
+    vector<DsloadLine> dsl;
     bool reset_spc = false, reset_vel = false, reset_dload = false, reset_dsload = false;
     bool grav = false;
     string grav_amp;
@@ -100,6 +105,8 @@ struct Parser {
     vector<StepRec> steps;
     vector<IniVel> ini;
     map<string, vector<std::pair<double, double>>> amps;
+    // *Surface, type=ELEMENT: name -> (element set, face 0..5 for S1..S6)
+    map<string, vector<std::pair<string, int>>> surfaces;
     double E = 2.06e11, nu = 0.3, rho = 7850.0;  // Material_elastic.h defaults
     double cvisl = 0.06, cvisq = 1.2, total_time = 0.0;
     bool plastic = false;
@@ -414,9 +421,24 @@ struct Parser {
     void DSLOAD() {
         const bool op_new = str.find("OP=NEW") != string::npos;
         if (op_new && state == STEP && !steps.empty()) steps.back().reset_dsload = true;
+        const string amp = name_of(str, "AMPLITUDE=");
         while (more() && peek() != '*') {
             getline();
-            if (state == STEP && !steps.empty()) steps.back().n_dsload++;
+            upper(str);
+            // surface, load type, value (dataIn.cpp:1441-1456); pressure (P) only
+            Fields f(str);
+            string surf, type, t;
+            f.next(surf);
+            f.next(type);
+            f.next(t);
+            surf = nospace(surf);
+            type = nospace(type);
+            double v = 0.0;
+            to_double(t, v);
+            if (state == STEP && !steps.empty()) {
+                steps.back().n_dsload++;
+                if (type == "P") steps.back().dsl.push_back({surf, amp, v});
+            }
         }
     }
 
@@ -462,8 +484,23 @@ struct Parser {
         to_double(t, cvisq);
     }
 
+    // *Surface, type=ELEMENT, name=S: lines "element set, Sn" (extension:
+    // every line is kept; the reference keeps the last, dataIn.cpp:1339-1414)
     void SURFACE() {
-        while (more() && peek() != '*') getline();
+        const string name = name_of(str, "NAME=");
+        while (more() && peek() != '*') {
+            getline();
+            upper(str);
+            Fields f(str);
+            string set, face;
+            f.next(set);
+            f.next(face);
+            set = nospace(set);
+            face = nospace(face);
+            if (name.empty() || set.empty() || face.size() != 2 || face[0] != 'S') continue;
+            const int k = face[1] - '1';
+            if (k >= 0 && k < 6) surfaces[name].push_back({set, k});
+        }
     }
 
     void OUTPUT() {
@@ -648,7 +685,11 @@ struct ensc_deck {
     // flattened model
     std::vector<double> x0, v_init;
     std::vector<uint32_t> conn;
-    std::map<std::string, std::vector<uint32_t>> nsets;
+    std::map<std::string, std::vector<uint32_t>> nsets, esets;
+    // surface pressures (ENSC_DECK_DSLOAD): per step, per line
+    std::vector<std::vector<ensc_dsload_entry>> dsl;
+    std::vector<std::vector<std::vector<uint32_t>>> dsl_elems;
+    std::vector<std::vector<std::vector<int32_t>>> dsl_faces;
     std::vector<std::string> amp_names;
     std::vector<std::vector<double>> amp_t, amp_a;
     std::vector<ensc_amplitude> amps;
@@ -715,6 +756,20 @@ bool build_deck(Parser& P, const ensc_project* proj, uint32_t flags, ensc_deck& 
                     }
                 }
                 D.nsets[ns.first] = glob;
+            }
+            for (const auto& es : A.esets) {
+                std::vector<uint32_t> glob;
+                for (const auto& ip : es.second) {
+                    if (ip.first == "__GLOBAL__") {
+                        for (size_t v : ip.second) glob.push_back((uint32_t)v);
+                    } else {
+                        auto ii = std::find_if(A.inst.begin(), A.inst.end(),
+                                               [&](const Instance& x) { return x.name == ip.first; });
+                        if (ii != A.inst.end())
+                            for (size_t v : ip.second) glob.push_back((uint32_t)(v + ii->elem0));
+                    }
+                }
+                D.esets[es.first] = glob;
             }
         }
     } else {
@@ -840,6 +895,45 @@ bool build_deck(Parser& P, const ensc_project* proj, uint32_t flags, ensc_deck& 
         sd.spc = D.spc[s].data();
         sd.n_vel = (uint32_t)D.vel[s].size();
         sd.vel = D.vel[s].data();
+        sd.reset_dsload = R.reset_dsload;
+    }
+    // surface pressures (extension, on request): each *Dsload line's surface
+    // resolved to (element, face) pairs through its element sets
+    D.dsl.assign(S, {});
+    D.dsl_elems.assign(S, {});
+    D.dsl_faces.assign(S, {});
+    if (flags & ENSC_DECK_DSLOAD) {
+        for (size_t s = 0; s < S; ++s) {
+            const StepRec& R = P.steps[s];
+            for (const auto& ln : R.dsl) {
+                std::vector<uint32_t> el;
+                std::vector<int32_t> fc;
+                auto sf = P.surfaces.find(ln.surf);
+                if (sf != P.surfaces.end())
+                    for (const auto& part : sf->second) {
+                        auto es = D.esets.find(part.first);
+                        if (es == D.esets.end()) continue;
+                        for (uint32_t e : es->second)
+                            if (e < Ne) {
+                                el.push_back(e);
+                                fc.push_back(part.second);
+                            }
+                    }
+                D.dsl_elems[s].push_back(std::move(el));
+                D.dsl_faces[s].push_back(std::move(fc));
+                ensc_dsload_entry en{};
+                en.value = ln.value;
+                en.amp = amp_index(ln.amp);
+                D.dsl[s].push_back(en);
+            }
+            for (size_t i = 0; i < D.dsl[s].size(); ++i) {
+                D.dsl[s][i].elems = D.dsl_elems[s][i].data();
+                D.dsl[s][i].faces = D.dsl_faces[s][i].data();
+                D.dsl[s][i].n_faces = (uint32_t)D.dsl_elems[s][i].size();
+            }
+            D.steps[s].n_dsload = (uint32_t)D.dsl[s].size();
+            D.steps[s].dsload = D.dsl[s].empty() ? nullptr : D.dsl[s].data();
+        }
     }
     for (size_t i = 0; i < D.amp_t.size(); ++i) {
         ensc_amplitude a{};
@@ -915,7 +1009,10 @@ bool build_deck(Parser& P, const ensc_project* proj, uint32_t flags, ensc_deck& 
                  std::to_string(R.vel[i].d1) + ", " + jnum(R.vel[i].value) + "]";
         j += "], \"gravity\": [" + std::string(R.grav ? "1" : "0") + ", " + jstr(R.grav_amp) + ", " +
              jnum(R.grav_v) + ", " + jnum(R.gdir[0]) + ", " + jnum(R.gdir[1]) + ", " + jnum(R.gdir[2]) +
-             "], \"n_dsload\": " + std::to_string(R.n_dsload) + "}";
+             "], \"n_dsload\": " + std::to_string(R.n_dsload) + ", \"dsload\": [";
+        for (size_t i = 0; i < R.dsl.size(); ++i)
+            j += (i ? ", [" : "[") + jstr(R.dsl[i].surf) + ", " + jnum(R.dsl[i].value) + ", " + jstr(R.dsl[i].amp) + "]";
+        j += "]}";
     }
     j += "]}";
     D.json = j;

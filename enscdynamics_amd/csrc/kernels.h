@@ -1188,6 +1188,20 @@ __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(ASM ? (MIXE
             a[1] += ag * J.grav_dir[1] / im;
             a[2] += ag * J.grav_dir[2] / im;
         }
+        // surface pressures (extension): the forces of the node's loaded faces,
+        // in ascending face order; their work is counted like gravity's (the
+        // update loop sums the forces again, so nothing stays live between)
+        uint32_t f0 = 0, f1 = 0;
+        if (J.n_dsf) {
+            f0 = J.ds_ptr[n];
+            f1 = J.ds_ptr[n + 1];
+            for (uint32_t q = f0; q < f1; ++q) {
+                const double* ff = J.dsf_force + (size_t)J.ds_list[q] * 3;
+#pragma unroll
+                for (int d = 0; d < 3; ++d) a[d] += ff[d];
+            }
+        }
+        double wp = 0.0;
         // External work of the step (extension; the reference keeps no energy
         // account), per node:
         //   wr: the reactions' power at velocity-BC dofs, -(f_int + f_grav) v_c,
@@ -1262,6 +1276,11 @@ __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(ASM ? (MIXE
             const double vb = vd;
             vd += hs * ad / 2.0;  // update_velocity (646-649)
             wg += J.grav_dir[d] * (vb + vd);
+            if (f1 != f0) {
+                double fp = 0.0;
+                for (uint32_t q = f0; q < f1; ++q) fp += J.dsf_force[(size_t)J.ds_list[q] * 3 + d];
+                wp += fp * (vb + vd);
+            }
             const double dx = dt_i * vd;
             if (MIXED) {
                 // update_displacement (651-654), compensated (Kahan); the
@@ -1283,6 +1302,7 @@ __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(ASM ? (MIXE
         ke = k2 * half_m;
         pw += kj * half_m;
         if (J.grav_on) pw += 0.5 * hs * (ctl->grav_amp * J.grav_g) * wg * half_m;
+        if (f1 != f0) pw += 0.25 * hs * wp;
         if (MIXED) gst(J.ucomp, n) = cw_new;
         if (!MIXED) {
             gst2(J.pos, n * POS_REC, x[0], x[1]);
@@ -1311,6 +1331,41 @@ __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(ASM ? (MIXE
         p[0] = r0;
         p[J.nblk_n] = r1;
     }
+}
+
+// Surface pressures (extension; *Dsload): one thread per loaded face of the
+// block's job computes the force its pressure puts on each of its 4 nodes
+// this step, p amp A / 4 with A = (x2 - x0) x (x3 - x1) / 2 the face's vector
+// area (into the element) on the positions at the start of the step, and the
+// amplitude averaged over the step as for gravity (deal_amp).  Runs between
+// the element finalize (which sets the step's time and dt) and the nodal
+// update, which adds the forces of each node's faces.
+constexpr int FACE_BLK = 256;
+__global__ __launch_bounds__(FACE_BLK) void k_face(const JobDev* __restrict__ jobs, const uint4* __restrict__ blkmap) {
+#pragma clang fp contract(off)
+    const uint4 bm = blkmap[blockIdx.x];  // (job, block of the job's faces, -, -)
+    const JobDev& J = jobs[bm.x];
+    const Ctl* ctl = J.ctl;
+    if (ctl->kind != KIND_STEP) return;
+    const uint32_t f = bm.y * FACE_BLK + threadIdx.x;
+    if (f >= J.n_dsf) return;
+    const bool moving_x = J.inv_m == nullptr;  // FP64 jobs store the current position; mixed ones X (+ u)
+    double x[4][3];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        const uint32_t n = J.dsf_nodes[4 * f + c];
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+            x[c][d] = J.pos[(size_t)n * POS_REC + d] + (moving_x ? 0.0 : J.vu[(size_t)n * VU_REC + 3 + d]);
+    }
+    const double a[3] = {x[2][0] - x[0][0], x[2][1] - x[0][1], x[2][2] - x[0][2]};
+    const double b[3] = {x[3][0] - x[1][0], x[3][1] - x[1][1], x[3][2] - x[1][2]};
+    const double amp = J.dsf_amp[f] >= 0 ? deal_amp(J, J.dsf_amp[f], ctl->time, ctl->dt_i) : 1.0;
+    const double s = J.dsf_val[f] * amp * 0.125;  // p amp / 4 times the 1/2 of the cross product
+    double* out = J.dsf_force + (size_t)f * 3;
+    out[0] = s * (a[1] * b[2] - a[2] * b[1]);
+    out[1] = s * (a[2] * b[0] - a[0] * b[2]);
+    out[2] = s * (a[0] * b[1] - a[1] * b[0]);
 }
 
 // Per-job node finalize (one block per job in the job table): closes the

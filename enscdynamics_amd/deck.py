@@ -5,9 +5,10 @@ restatement of the reference's DataIn, Explicit3D/source/dataIn.cpp) and
 returns a ``Model``; ``read_project(path)`` reads project.txt parameters
 (exDyna3D.cpp:698-772).
 
-By default the reader solves what the reference solves: ``*Plastic`` and
-``AMPLITUDE=`` on velocity BCs, which the reference ignores, are dropped.
-``extensions=PLASTIC | VEL_AMPLITUDE`` (or ``ALL``) keeps them.
+By default the reader solves what the reference solves: ``*Plastic``,
+``AMPLITUDE=`` on velocity BCs and ``*Dsload`` pressures, which the
+reference ignores, are dropped.  ``extensions=PLASTIC | VEL_AMPLITUDE |
+DSLOAD`` (or ``ALL``) keeps them.
 """
 from __future__ import annotations
 
@@ -18,12 +19,13 @@ from typing import Optional
 import numpy as np
 
 from . import engine as _eng
-from .model import Bc, Model, ModelDesc, Step
+from .model import Bc, Dsload, Model, ModelDesc, Step
 
 
 PLASTIC = 1        # ENSC_DECK_PLASTIC
 VEL_AMPLITUDE = 2  # ENSC_DECK_VEL_AMPLITUDE
-ALL = PLASTIC | VEL_AMPLITUDE
+DSLOAD = 4         # ENSC_DECK_DSLOAD
+ALL = PLASTIC | VEL_AMPLITUDE | DSLOAD
 
 
 class Project(C.Structure):
@@ -98,6 +100,13 @@ def _model_from_handle(h, L, name: str) -> Model:
                              name=js["vel"][i][0]))
         if sd.grav_on:
             st.gravity = (sd.grav_value, sd.grav_dir[0], sd.grav_dir[1], sd.grav_dir[2], sd.grav_amp)
+        st.reset_dsload = bool(sd.reset_dsload)
+        for i in range(sd.n_dsload):
+            e = sd.dsload[i]
+            n = e.n_faces
+            el = np.ctypeslib.as_array(e.elems, shape=(n,)).copy() if n else np.zeros(0, np.uint32)
+            fc = np.ctypeslib.as_array(e.faces, shape=(n,)).copy() if n else np.zeros(0, np.int32)
+            st.dsload.append(Dsload(el.astype(np.uint32), fc.astype(np.int32), e.value, e.amp))
         steps.append(st)
     m.steps = steps
     m.node_sets = {k: np.asarray(v_, dtype=np.uint32) for k, v_ in info["node_sets"].items()}

@@ -30,6 +30,16 @@ class BcEntry(C.Structure):
     ]
 
 
+class DsloadEntry(C.Structure):
+    _fields_ = [
+        ("elems", C.POINTER(C.c_uint32)),
+        ("faces", C.POINTER(C.c_int32)),
+        ("n_faces", C.c_uint32),
+        ("amp", C.c_int32),
+        ("value", C.c_double),
+    ]
+
+
 class StepDef(C.Structure):
     _fields_ = [
         ("period", C.c_double),
@@ -45,6 +55,9 @@ class StepDef(C.Structure):
         ("n_vel", C.c_uint32),
         ("pad_", C.c_int32),
         ("vel", C.POINTER(BcEntry)),
+        ("reset_dsload", C.c_int32),
+        ("n_dsload", C.c_uint32),
+        ("dsload", C.POINTER(DsloadEntry)),
     ]
 
 
@@ -104,6 +117,18 @@ class Bc:
 
 
 @dataclass
+class Dsload:
+    """One *Dsload pressure line with its surface resolved (extension: the
+    reference parses *Dsload and never applies it).  Face k of element e is
+    C3D8 face S(k+1); a positive value pushes into the element."""
+
+    elems: np.ndarray
+    faces: np.ndarray
+    value: float
+    amp: int = -1
+
+
+@dataclass
 class Step:
     """One *Step (StepData.h:14-34)."""
 
@@ -115,6 +140,8 @@ class Step:
     reset_vel: bool = False
     reset_dload: bool = False
     gravity: Optional[Tuple[float, float, float, float, int]] = None  # g, dx, dy, dz, amp
+    dsload: List[Dsload] = field(default_factory=list)
+    reset_dsload: bool = False
 
 
 @dataclass
@@ -203,6 +230,17 @@ class Model:
                 st.grav_amp = -1
             sp = entries(s.spc)
             ve = entries(s.vel)
+            ds = (DsloadEntry * max(1, len(s.dsload)))()
+            for i, d in enumerate(s.dsload):
+                ds[i].elems = dptr(np.asarray(d.elems).reshape(-1), np.uint32, C.c_uint32)
+                ds[i].faces = dptr(np.asarray(d.faces).reshape(-1), np.int32, C.c_int32)
+                ds[i].n_faces = len(d.elems)
+                ds[i].amp = d.amp
+                ds[i].value = d.value
+            keep.append(ds)
+            st.reset_dsload = int(s.reset_dsload)
+            st.n_dsload = len(s.dsload)
+            st.dsload = C.cast(ds, C.POINTER(DsloadEntry))
             st.n_spc = len(s.spc)
             st.spc = C.cast(sp, C.POINTER(BcEntry))
             st.n_vel = len(s.vel)

@@ -16,7 +16,10 @@
  *                           keyword (dataIn.cpp:69-92);
  *   ENSC_DECK_VEL_AMPLITUDE AMPLITUDE= on *Boundary, type=VELOCITY; the
  *                           reference keeps velocity BCs constant
- *                           (BoundaryCondition.h:18, dataIn.cpp:1114-1195).
+ *                           (BoundaryCondition.h:18, dataIn.cpp:1114-1195);
+ *   ENSC_DECK_DSLOAD        *Dsload pressures on *Surface element faces; the
+ *                           reference parses them (dataIn.cpp:1339-1478) and
+ *                           never applies them.
  */
 #ifndef ENSC_DECK_H
 #define ENSC_DECK_H
@@ -44,7 +47,8 @@ ensc_status ensc_project_read(const char* path, ensc_project* out);
 
 enum {
     ENSC_DECK_PLASTIC = 1u,
-    ENSC_DECK_VEL_AMPLITUDE = 2u
+    ENSC_DECK_VEL_AMPLITUDE = 2u,
+    ENSC_DECK_DSLOAD = 4u
 };
 
 /* Read a deck the way the reference's DataIn::read_inp does (no

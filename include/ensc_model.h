@@ -36,6 +36,18 @@ typedef struct ensc_bc_entry {
     int32_t pad_;
 } ensc_bc_entry;
 
+/* One *Dsload line with its surface resolved to element faces (extension:
+ * the reference parses *Dsload and *Surface, dataIn.cpp:1339-1478, and never
+ * applies them).  A pressure `value` (times the amplitude) acts on each face
+ * over its current area, into the element for a positive value. */
+typedef struct ensc_dsload_entry {
+    const uint32_t* elems;  /* element of each loaded face */
+    const int32_t* faces;   /* its face: 0..5 = S1..S6 (C3D8 face numbering) */
+    uint32_t n_faces;
+    int32_t amp;            /* amplitude index, -1 for none */
+    double value;           /* pressure */
+} ensc_dsload_entry;
+
 /* One *Step (StepData.h:14-34).  steps[0] is the reference's "Initial"
  * step holding BCs defined outside any *Step (dataIn.cpp:323-326). */
 typedef struct ensc_step_def {
@@ -52,6 +64,12 @@ typedef struct ensc_step_def {
     uint32_t n_vel;
     int32_t pad_;
     const ensc_bc_entry* vel;
+    /* surface pressures (extension); inherited like the reference's dsload
+     * list (exDyna3D.cpp:1897-1907): a step without lines keeps the previous
+     * step's unless reset_dsload (*Dsload, op=NEW) */
+    int32_t reset_dsload;
+    uint32_t n_dsload;
+    const ensc_dsload_entry* dsload;
 } ensc_step_def;
 
 /* *Amplitude table: pairs (t_i, a_i) (dataIn.cpp:1242-1271). */

@@ -23,6 +23,13 @@
 //    work on every dof and the kinetic energy constraints add or remove
 //    when they set v, so KE + IE + HG + VIS - W_ext = KE0 closes.
 //  * An amplitude on velocity BCs (piecewise-linear in total time).
+//  * Surface pressures (*Dsload P lines, which the reference parses and
+//    inherits across steps, exDyna3D.cpp:1897-1907, but never applies): each
+//    loaded face puts p amp A / 4 on each of its 4 nodes, A the face's vector
+//    area (into the element) on the positions at the start of the step, amp
+//    averaged over the step like gravity's; the forces are added after
+//    gravity, per node in ascending face order, and their work enters the
+//    external work like gravity's.
 #include "oracle.h"
 
 #include <omp.h>
@@ -43,6 +50,19 @@ struct Gravity {
     double value = 0, dir[3] = {0, 0, 0};
 };
 
+struct Dsload {
+    std::vector<uint32_t> elems;
+    std::vector<int32_t> faces;
+    int amp = -1;
+    double value = 0;
+};
+
+// C3D8 face k (S(k+1)) as element corners, ordered so that the right-hand
+// rule gives the normal into the element: S1 = 1-2-3-4, S2 = 5-8-7-6,
+// S3 = 1-5-6-2, S4 = 2-6-7-3, S5 = 3-7-8-4, S6 = 4-8-5-1 in the corner
+// numbering of Element_HexN8.cpp:50-89.
+const int FACE_CORNERS[6][4] = {{0, 1, 2, 3}, {4, 7, 6, 5}, {0, 4, 5, 1}, {1, 5, 6, 2}, {2, 6, 7, 3}, {3, 7, 4, 0}};
+
 struct Sim {
     // --- model ---
     size_t Nn = 0, Ne = 0;
@@ -55,6 +75,8 @@ struct Sim {
     std::vector<double> v_init;
     std::vector<ensc_step_def> steps;
     std::vector<Gravity> grav;  // per step, mutated by inheritance (exDyna3D.cpp:1889-1892)
+    std::vector<std::vector<Dsload>> dsl;  // per step: its own *Dsload lines
+    std::vector<int32_t> dsl_reset;
     std::vector<std::vector<double>> amp_t, amp_a;
     // --- state (exDyna3D.h:141-206) ---
     std::vector<double> x;             // [Nn][3] vertices
@@ -161,6 +183,20 @@ struct Sim {
             grav[s].amp = steps[s].grav_amp;
             grav[s].value = steps[s].grav_value;
             for (int d = 0; d < 3; ++d) grav[s].dir[d] = steps[s].grav_dir[d];
+        }
+        dsl.assign(steps.size(), {});
+        dsl_reset.assign(steps.size(), 0);
+        for (size_t s = 0; s < steps.size(); ++s) {
+            dsl_reset[s] = steps[s].reset_dsload;
+            for (uint32_t i = 0; i < steps[s].n_dsload; ++i) {
+                const ensc_dsload_entry& d = steps[s].dsload[i];
+                Dsload L;
+                L.elems.assign(d.elems, d.elems + d.n_faces);
+                L.faces.assign(d.faces, d.faces + d.n_faces);
+                L.amp = d.amp;
+                L.value = d.value;
+                dsl[s].push_back(std::move(L));
+            }
         }
         amp_t.resize(m->n_amps);
         amp_a.resize(m->n_amps);
@@ -618,14 +654,77 @@ struct Sim {
             for (int d = 0; d < 3; ++d) a[3 * n + d] += amp * g.value * g.dir[d] / invM[n];
     }
 
+    // Surface pressures of the current *Dsload lines (extension, see the
+    // header): the faces of the step's lines (the step whose lines are
+    // current: its own, else the previous step's unless op=NEW) in line
+    // order; per node its faces in ascending face order.
+    int32_t dsl_step = -1;
+    std::vector<uint32_t> ds_nodes, ds_ptr, ds_list;
+    std::vector<double> ds_val, ds_force, ds_fp;
+    std::vector<int32_t> ds_amp;
+    void set_dsload(size_t si) {
+        if (!dsl[si].empty())
+            dsl_step = (int32_t)si;
+        else if (dsl_reset[si])
+            dsl_step = -1;
+        ds_nodes.clear();
+        ds_val.clear();
+        ds_amp.clear();
+        if (dsl_step >= 0)
+            for (const Dsload& L : dsl[(size_t)dsl_step])
+                for (size_t q = 0; q < L.elems.size(); ++q) {
+                    for (int c = 0; c < 4; ++c)
+                        ds_nodes.push_back(conn[8 * (size_t)L.elems[q] + FACE_CORNERS[L.faces[q]][c]]);
+                    ds_val.push_back(L.value);
+                    ds_amp.push_back(L.amp);
+                }
+        ds_ptr.assign(Nn + 1, 0);
+        for (uint32_t n : ds_nodes) ds_ptr[n + 1]++;
+        for (size_t i = 0; i < Nn; ++i) ds_ptr[i + 1] += ds_ptr[i];
+        ds_list.assign(ds_nodes.size(), 0);
+        std::vector<uint32_t> fill(ds_ptr.begin(), ds_ptr.end() - 1);
+        for (size_t i = 0; i < ds_nodes.size(); ++i) ds_list[fill[ds_nodes[i]]++] = (uint32_t)(i / 4);
+        ds_force.assign(3 * ds_val.size(), 0.0);
+        ds_fp.assign(ds_val.empty() ? 0 : 3 * Nn, 0.0);
+    }
+
+    // After apply_gravity: each face's force p amp A / 4, A = (x2 - x0) x
+    // (x3 - x1) / 2 on the current positions, added to a per node in
+    // ascending face order; ds_fp keeps the per-dof pressure force for the
+    // work in nodal_update.
+    void apply_dsload() {
+        if (ds_val.empty()) return;
+        for (size_t f = 0; f < ds_val.size(); ++f) {
+            const double* p[4];
+            for (int c = 0; c < 4; ++c) p[c] = &x[3 * (size_t)ds_nodes[4 * f + c]];
+            const double e[3] = {p[2][0] - p[0][0], p[2][1] - p[0][1], p[2][2] - p[0][2]};
+            const double g[3] = {p[3][0] - p[1][0], p[3][1] - p[1][1], p[3][2] - p[1][2]};
+            const double amp = ds_amp[f] >= 0 ? deal_amp(ds_amp[f]) : 1.0;
+            const double s = ds_val[f] * amp * 0.125;
+            ds_force[3 * f + 0] = s * (e[1] * g[2] - e[2] * g[1]);
+            ds_force[3 * f + 1] = s * (e[2] * g[0] - e[0] * g[2]);
+            ds_force[3 * f + 2] = s * (e[0] * g[1] - e[1] * g[0]);
+        }
+        for (size_t n = 0; n < Nn; ++n) {
+            double fp[3] = {0.0, 0.0, 0.0};
+            for (uint32_t q = ds_ptr[n]; q < ds_ptr[n + 1]; ++q)
+                for (int d = 0; d < 3; ++d) {
+                    a[3 * n + d] += ds_force[3 * (size_t)ds_list[q] + d];
+                    fp[d] += ds_force[3 * (size_t)ds_list[q] + d];
+                }
+            for (int d = 0; d < 3; ++d) ds_fp[3 * n + d] = fp[d];
+        }
+    }
+
     void nodal_update() {
         // compute_a, update_velocity, update_displacement, move_mesh (642-696)
-        // Energy account (extension): gravity's work on every dof over the
-        // step, f_grav (v_old + v_new) / 2 dt_avg.
-        double kep = 0.0, wg = 0.0;
+        // Energy account (extension): gravity's and the pressures' work on
+        // every dof over the step, f (v_old + v_new) / 2 dt_avg.
+        double kep = 0.0, wg = 0.0, wp = 0.0;
         const double hs = dt_i + dt_i_1;
         const bool gon = g_now;
-#pragma omp parallel for num_threads(nthreads) schedule(static) reduction(+ : kep, wg)
+        const bool pon = !ds_val.empty();
+#pragma omp parallel for num_threads(nthreads) schedule(static) reduction(+ : kep, wg, wp)
         for (long n = 0; n < (long)Nn; ++n) {
             double k2 = 0.0;
             for (int d = 0; d < 3; ++d) {
@@ -638,11 +737,13 @@ struct Sim {
                 a[i] = 0.0;
                 k2 += v[i] * v[i];
                 if (gon) wg += g_ag * g_dir[d] / invM[n] * (vb + v[i]);
+                if (pon) wp += ds_fp[i] * (vb + v[i]);
             }
             kep += 0.5 * k2 / invM[n];
         }
         ke = kep;
         wext += 0.25 * hs * wg;
+        wext += 0.25 * hs * wp;
     }
 
     double kinetic() const {
@@ -701,6 +802,7 @@ struct Sim {
             if (!st.reset_dload) {
                 if (!grav[si].on && grav[si - 1].on) grav[si] = grav[si - 1];
             }
+            set_dsload(si);
         }
         apply_bc_vec();
     }
@@ -792,6 +894,7 @@ struct Sim {
         apply_bc_vec();
         add_inForce_to_rhs(dt_i_1);
         apply_gravity();
+        apply_dsload();
         apply_bc_a(0.5 * (dt_i + dt_i_1));
         output_results();
         nodal_update();

@@ -8,7 +8,7 @@ from __future__ import annotations
 import numpy as np
 
 from enscdynamics_amd import decks
-from enscdynamics_amd.model import Bc, Model, Step
+from enscdynamics_amd.model import Bc, Dsload, Model, Step
 
 STEEL = dict(E=200e9, nu=0.3, rho=7800.0)
 
@@ -88,6 +88,64 @@ def fixed_after_release(steps: int = 120) -> Model:
     m.steps = [Step("Initial", 0.0), Step("Step-1", T, spc=[Bc("spc", left, 0, 2)])]
     m.total_time = T
     m.left = left  # type: ignore[attr-defined]
+    return m
+
+
+def top_faces(nx: int, ny: int, nz: int) -> np.ndarray:
+    """Elements of the top layer (k = nz - 1) of decks.block_mesh."""
+    return np.arange(nx * ny * (nz - 1), nx * ny * nz, dtype=np.uint32)
+
+
+def pressed_block(steps: int = 120, p: float = 1e6, ramp: bool = False) -> Model:
+    """A free 2x3x2 block (no BCs) with pressure p on its top faces (face
+    S2 = corners 5-8-7-6 of each top element, the *Dsload extension): the
+    surface force is p A0 along -z while the faces stay flat, so the
+    momentum is -p A0 sum(dt_avg) up to the faces' area change (strain
+    p / E ~ 5e-6).  With ramp, the pressure follows an amplitude 0 -> 1
+    over the run, so the impulse halves."""
+    nx, ny, nz = 2, 3, 2
+    x0, conn, _ = decks.block_mesh(nx, ny, nz, 0.02, 0.03, 0.02)
+    m = Model(x0=x0, conn=conn, **STEEL, name="pressed_block")
+    m.v_init = np.zeros_like(x0)
+    dt0 = decks.stable_dt0(m)
+    T = steps * dt0
+    el = top_faces(nx, ny, nz)
+    amp = -1
+    if ramp:
+        m.amplitudes = [("Ramp", np.array([[0.0, 0.0], [T, 1.0]]))]
+        amp = 0
+    m.steps = [Step("Initial", 0.0),
+               Step("Step-1", T, dsload=[Dsload(el, np.full(len(el), 1, np.int32), p, amp)])]
+    m.total_time = T
+    m.area = 0.02 * 0.03  # type: ignore[attr-defined]
+    m.p = p  # type: ignore[attr-defined]
+    return m
+
+
+def pressure_column(nz: int = 100, p: float = 1e6, transits: float = 0.5) -> Model:
+    """Uniaxial-strain column of nz elements (h = 0.01 m, x and y fixed on
+    every node, z fixed at z = 0) with a step pressure p on the top face:
+    a compression front runs down at c = sqrt((lambda + 2G) / rho) with the
+    particle velocity p / (rho c) behind it.  Runs `transits` transits of
+    the column; the area stays h^2, so the momentum is exactly -p h^2 t
+    while the front has not reached the bottom."""
+    h = 0.01
+    H = nz * h
+    x0, conn, ijk = decks.block_mesh(1, 1, nz, h, h, H)
+    m = Model(x0=x0, conn=conn, **STEEL, name="pressure_column")
+    _, _, _, c = m.material_consts()
+    T = transits * H / c
+    alln = np.arange(m.n_nodes, dtype=np.uint32)
+    bot = decks.face_nodes(ijk, 2, 0, nz)
+    el = top_faces(1, 1, nz)
+    m.v_init = np.zeros_like(x0)
+    m.steps = [Step("Initial", 0.0, spc=[Bc("spc", alln, 0, 1), Bc("spc", bot, 2, 2)]),
+               Step("Step-1", T, dsload=[Dsload(el, np.full(len(el), 1, np.int32), p)])]
+    m.total_time = T
+    m.height = H  # type: ignore[attr-defined]
+    m.c = c  # type: ignore[attr-defined]
+    m.p = p  # type: ignore[attr-defined]
+    m.area = h * h  # type: ignore[attr-defined]
     return m
 
 

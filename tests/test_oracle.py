@@ -15,7 +15,7 @@ import pytest
 import energy_cases as EC
 import goldens as G
 from enscdynamics_amd import decks
-from enscdynamics_amd.model import Bc, Model, Step
+from enscdynamics_amd.model import Bc, Dsload, Model, Step
 from oracle.oracle import HIST_BASE, Oracle
 
 TOL_REF = 1e-10  # FP64 bar vs the reference
@@ -314,3 +314,125 @@ def test_energy_constraint_jump_at_step_switch():
     mass = 1.0 / o.get("invM")
     assert s["wext"] == pytest.approx(-0.5 * float(mass[m.left].sum()) * 9.0, rel=1e-13)
     assert abs(EC.balance(s)) <= 0.1 * s["ke0"]
+
+
+# ---------------------------------------------------------------------------
+# surface pressures (*Dsload extension: the reference parses and inherits
+# the lines but applies no force; pinned by closed forms)
+# ---------------------------------------------------------------------------
+
+def test_pressure_impulse_on_free_block():
+    """Free block pressed on its top: total momentum -p A0 sum(dt_avg) (the
+    internal forces sum to zero); a wrong corner order, sign or the 1/4 per
+    node would change it by O(1)."""
+    m = EC.pressed_block(steps=120)
+    o = Oracle(m, 1).init()
+    n, h = o.run(0, [0], 10 ** 5)
+    mass = 1.0 / o.get("invM")
+    v = o.get("v")
+    P = (mass[:, None] * v).sum(axis=0)
+    dta = 0.5 * (h[:, 1] + h[:, 2])
+    J = -m.p * m.area * dta.sum()
+    assert P[2] == pytest.approx(J, rel=1e-4)
+    assert abs(P[0]) <= 1e-9 * abs(J) and abs(P[1]) <= 1e-9 * abs(J)
+    s = o.scalars()
+    assert s["wext"] > 0.0
+    assert abs(EC.balance(s)) <= 0.01 * s["wext"]
+
+
+def test_pressure_amplitude_halves_the_impulse():
+    """The pressure's amplitude is averaged over each step like gravity's
+    (deal_amp): a ramp 0 -> 1 over the run gives half the impulse."""
+    m = EC.pressed_block(steps=120, ramp=True)
+    o = Oracle(m, 1).init()
+    n, h = o.run(0, [0], 10 ** 5)
+    P = ((1.0 / o.get("invM"))[:, None] * o.get("v")).sum(axis=0)
+    dta = 0.5 * (h[:, 1] + h[:, 2])
+    assert P[2] == pytest.approx(-0.5 * m.p * m.area * dta.sum(), rel=2e-2)
+
+
+def test_pressure_wave_in_uniaxial_column():
+    """Step pressure on a constrained column: after half a transit the front
+    is at mid-height, the material behind it moves at p / (rho c) and the
+    material ahead of it is at rest; the momentum is -p h^2 sum(dt_avg)."""
+    m = EC.pressure_column(nz=100, transits=0.5)
+    o = Oracle(m, 1).init()
+    n, h = o.run(0, [0], 10 ** 5)
+    s = o.scalars()
+    t = s["time"]
+    x0 = m.x0
+    v = o.get("v")
+    mass = 1.0 / o.get("invM")
+    dta = 0.5 * (h[:, 1] + h[:, 2])
+    assert float((mass * v[:, 2]).sum()) == pytest.approx(-m.p * m.area * dta.sum(), rel=1e-6)
+    vp = m.p / (m.rho * m.c)
+    z = x0[:, 2] / m.height
+    front = 1.0 - m.c * t / m.height
+    assert front == pytest.approx(0.5, rel=1e-6)
+    behind = (z > front + 0.15) & (z < 0.95)
+    ahead = z < front - 0.15
+    assert float(v[behind, 2].mean()) == pytest.approx(-vp, rel=0.03)
+    assert float(np.abs(v[ahead, 2]).max()) <= 0.01 * vp
+    # the work p h^2 u_top, shared equally by kinetic and strain energy in a
+    # travelling wave, and the balance
+    assert s["wext"] == pytest.approx(m.p * m.area * -float(o.get("u")[m.n_nodes - 1, 2]), rel=2e-2)
+    assert s["ke"] == pytest.approx(0.5 * s["wext"], rel=0.03)
+    assert abs(EC.balance(s)) <= 0.01 * s["wext"]
+
+
+def test_pressure_inheritance_across_steps():
+    """Lines inherit like the reference's dsload list (exDyna3D.cpp:
+    1897-1907): a step without lines keeps the previous step's, op=NEW
+    drops them.  Three equal steps: pressed, inherited, released."""
+    m = EC.pressed_block(steps=60)
+    T = m.steps[1].period
+    m.steps.append(Step("Step-2", T))
+    m.steps.append(Step("Step-3", T, reset_dsload=True))
+    m.total_time = 3 * T
+    o = Oracle(m, 1).init()
+    P = []
+    rows = []
+    while True:
+        n, h = o.run(1, [0], 1)
+        if n == 0:
+            break
+        rows.append(h[0])
+        P.append(((1.0 / o.get("invM"))[:, None] * o.get("v")).sum(axis=0)[2])
+    H = np.array(rows)
+    P = np.array(P)
+    t = H[:, 0]
+    dta = 0.5 * (H[:, 1] + H[:, 2])
+    i1 = int(np.searchsorted(t, T * (1 - 1e-9)))
+    i2 = int(np.searchsorted(t, 2 * T * (1 - 1e-9)))
+    # pressed through steps 1 and 2, free in step 3: the momentum stops
+    # changing once the lines are dropped
+    assert P[i2] == pytest.approx(-m.p * m.area * dta[: i2 + 1].sum(), rel=1e-4)
+    assert np.allclose(P[i2 + 1:], P[i2 + 1], rtol=1e-9, atol=0)
+
+
+# Abaqus C3D8 faces S1..S6 (1-based nodes 1-2-3-4, 5-8-7-6, 1-5-6-2, 2-6-7-3,
+# 3-7-8-4, 4-8-5-1) on the unit cube of decks.block_mesh's corner order: the
+# inward normal of each, along which a positive pressure pushes
+INWARD = [(0, 0, 1), (0, 0, -1), (0, 1, 0), (-1, 0, 0), (0, -1, 0), (1, 0, 0)]
+
+
+@pytest.mark.parametrize("face", range(6))
+def test_pressure_face_numbering(face):
+    """Pressure on face S(face+1) of the first element of a free 2x2x2
+    block pushes the block along that face's inward normal with impulse
+    p A sum(dt_avg).  (A lone element would not do: at the reference's time
+    step, factor 0.71 of L / c, one free hexahedron with lumped mass is
+    unstable.)"""
+    x0, conn, _ = decks.block_mesh(2, 2, 2, 0.02, 0.02, 0.02)
+    m = Model(x0=x0, conn=conn, **EC.STEEL, name="cube")
+    m.v_init = np.zeros_like(x0)
+    T = 40 * decks.stable_dt0(m)
+    p = 1e6
+    m.steps = [Step("Initial", 0.0),
+               Step("Step-1", T, dsload=[Dsload(np.array([0], np.uint32), np.array([face], np.int32), p)])]
+    m.total_time = T
+    o = Oracle(m, 1).init()
+    n, h = o.run(0, [0], 10 ** 5)
+    P = ((1.0 / o.get("invM"))[:, None] * o.get("v")).sum(axis=0)
+    J = p * 1e-4 * (0.5 * (h[:, 1] + h[:, 2])).sum()
+    assert np.allclose(P, J * np.array(INWARD[face], dtype=float), rtol=0, atol=1e-4 * J)

@@ -121,3 +121,30 @@ def test_reader_quirks():
     assert list(init.spc[1].nodes) == [4] and (init.spc[1].d0, init.spc[1].d1) == (1, 1)
     assert m.steps[1].reset_spc and m.total_time == 0.5
     assert (m.E, m.nu, m.rho) == (1e9, 0.25, 1000.0)
+
+
+# Abaqus C3D8 faces (Abaqus Analysis User's Guide, "Three-dimensional solid
+# element library"): S1 = 1-2-3-4, S2 = 5-8-7-6, S3 = 1-5-6-2, S4 = 2-6-7-3,
+# S5 = 3-7-8-4, S6 = 4-8-5-1 (1-based element node numbers)
+ABAQUS_C3D8_FACES = [[1, 2, 3, 4], [5, 8, 7, 6], [1, 5, 6, 2], [2, 6, 7, 3], [3, 7, 8, 4], [4, 8, 5, 1]]
+
+
+def test_reader_dsload_extension():
+    """GenVirPa's *Dsload (Surf-1 = elset _Surf-1_S6, elements 1..7 step 2,
+    face S6; P 1.0).  The default parse applies nothing, as the reference;
+    with DSLOAD the step holds the faces, which lie on the mesh boundary."""
+    text = G.deck_path_text("GenVirPa")
+    r = parse_deck(text)
+    assert all(not st.dsload for st in r.steps)
+    r = parse_deck(text, extensions=deck.DSLOAD)
+    loaded = [st for st in r.steps if st.dsload]
+    assert len(loaded) == 1 and len(loaded[0].dsload) == 1
+    d = loaded[0].dsload[0]
+    assert list(d.elems) == [0, 2, 4, 6] and list(d.faces) == [5, 5, 5, 5]
+    assert d.value == 1.0 and d.amp == -1
+    elem_sets = [set(int(n) for n in row) for row in r.conn]
+    for e, f in zip(d.elems, d.faces):
+        face = {int(r.conn[e][k - 1]) for k in ABAQUS_C3D8_FACES[f]}
+        assert len(face) == 4
+        owners = [i for i, es in enumerate(elem_sets) if face <= es]
+        assert owners == [int(e)]
