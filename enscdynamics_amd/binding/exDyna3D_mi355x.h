@@ -15,6 +15,14 @@
 //     computeSate() around exDyna3D_MI355X::computeSate() - the reference's
 //     own pattern for hooks - sees every step; pull_state() copies the
 //     fields back into the reference's members when a hook needs them.
+// FSI coupling: run() generates the virtual particles as the reference's
+// run() does (get_fsiSph_virtualParticles_and_vel(0, 1/10),
+// exDyna3D.cpp:67; fsi_layers_ and fsi_dist_ hold those arguments, so a
+// coupler subclass may ask for particles), and setup_fsi() regenerates them
+// on the current state.  The public fsi_share_data (exDyna3D.h:54) works as
+// in the reference: the particle forces a coupler writes there are applied
+// by the next step, and the particles' positions and velocities are copied
+// back after every step (per-step mode) or after the run.
 // The oracle's drop-in test (oracle/ref_harness/dropin.cpp) compiles this
 // header against the reference's sources and runs both modes.
 #pragma once
@@ -64,14 +72,16 @@ public:
             ::mkdir("ResultData", 0755);
             check(ensc_engine_set_output(eng_, time_interval, 4, "ResultData"));
         }
-        if (!stepwise_) {
-            check(ensc_engine_run(eng_));
-        } else {
-            check(ensc_engine_init(eng_));
-            for (uint32_t s = 1; s < (uint32_t)steps.size(); ++s) {
-                check(ensc_engine_set_step(eng_, s));
-                step_done_ = 0;
+        check(ensc_engine_init(eng_));
+        setup_fsi(fsi_layers_, fsi_dist_);
+        for (uint32_t s = 1; s < (uint32_t)steps.size(); ++s) {
+            check(ensc_engine_set_step(eng_, s));
+            step_done_ = 0;
+            if (stepwise_) {
                 while (!step_done_) computeSate();
+            } else {
+                push_fsi_forces();
+                check(ensc_engine_advance(eng_, 0, &step_done_));
             }
         }
         pull_state();
@@ -85,10 +95,48 @@ protected:
     // stepwise mode, which run() drives.
     void computeSate() override {
         if (!eng_) throw std::runtime_error("EnSC-MI355X: computeSate outside run()");
+        push_fsi_forces();
         int32_t done = 0;
         check(ensc_engine_advance(eng_, 1, &done));
         step_done_ = done;
         pull_scalars();
+        pull_fsi_particles();
+    }
+
+    // get_fsiSph_virtualParticles_and_vel (exDyna3D.cpp:1363-1383) on the
+    // engine: n_layers particles per FSI face, dist apart, on the current
+    // state; fsi_share_data is sized to them, forces zero.
+    void setup_fsi(int n_layers, double dist) {
+        int64_t n = 0;
+        check(ensc_engine_fsi_setup(eng_, n_layers, dist, &n));
+        fsi_n_ = n;
+        fsi_share_data.FSI_virtualParticles_coordinates.assign((size_t)n, {0.0, 0.0, 0.0});
+        fsi_share_data.FSI_virtualParticles_velocity.assign((size_t)n, {0.0, 0.0, 0.0});
+        fsi_share_data.FSI_virtualParticles_nodeForce.assign((size_t)n, {0.0, 0.0, 0.0});
+        pull_fsi_particles();
+    }
+
+    void push_fsi_forces() {
+        if (!fsi_n_) return;
+        const auto& F = fsi_share_data.FSI_virtualParticles_nodeForce;
+        if ((int64_t)F.size() != fsi_n_) throw std::runtime_error("EnSC-MI355X: FSI_virtualParticles_nodeForce resized");
+        fsi_buf_.resize(3 * F.size());
+        for (size_t p = 0; p < F.size(); ++p)
+            for (int d = 0; d < 3; ++d) fsi_buf_[3 * p + d] = F[p][d];
+        check(ensc_engine_fsi_set_forces(eng_, fsi_buf_.data(), fsi_n_));
+    }
+
+    void pull_fsi_particles() {
+        if (!fsi_n_) return;
+        std::vector<double> x(3 * (size_t)fsi_n_), v(3 * (size_t)fsi_n_);
+        check(ensc_engine_fsi_get(eng_, x.data(), v.data(), fsi_n_));
+        auto& X = fsi_share_data.FSI_virtualParticles_coordinates;
+        auto& V = fsi_share_data.FSI_virtualParticles_velocity;
+        for (size_t p = 0; p < (size_t)fsi_n_; ++p)
+            for (int d = 0; d < 3; ++d) {
+                X[p][d] = x[3 * p + d];
+                V[p][d] = v[3 * p + d];
+            }
     }
 
     // Copy the device state into the reference's members: solution_u, v, rf,
@@ -102,6 +150,7 @@ protected:
         check(ensc_engine_download(eng_, "x", buf.data(), (int64_t)buf.size()));
         for (size_t i = 0; i < Nn; ++i)
             for (int d = 0; d < 3; ++d) vertices[i][d] = buf[3 * i + d];
+        pull_fsi_particles();
         std::vector<double> s(6 * Ne), qv(12 * Ne);
         check(ensc_engine_download(eng_, "sigma", s.data(), (int64_t)s.size()));
         check(ensc_engine_download(eng_, "qia", qv.data(), (int64_t)qv.size()));
@@ -128,6 +177,11 @@ protected:
     long steps_run_ = 0;
     int32_t step_done_ = 0;
     ensc_engine* eng_ = nullptr;
+    // the arguments of run()'s particle generation (exDyna3D.cpp:67)
+    int fsi_layers_ = 0;
+    double fsi_dist_ = 1.0 / 10.0;
+    int64_t fsi_n_ = 0;
+    std::vector<double> fsi_buf_;
 
 private:
     // The state DataIn filled (exDyna3D.h:141-220), flattened into the C

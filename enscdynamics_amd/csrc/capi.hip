@@ -87,6 +87,7 @@ struct ensc_engine {
     // the static scratch array (ensc_engine_advance_static)
     void *jobs_p = nullptr, *b2e_p = nullptr, *b2n_p = nullptr, *b2f_p = nullptr;
     uint32_t nblk_f = 0;  // face blocks of the surface-pressure kernel
+    uint32_t fsi_np = 0, fsi_nn = 0;  // virtual particles and FSI nodes of the single job
     bool in_scratch = false;      // the call in progress places its tables in scratch
     bool layout_scratch = false;  // the current layout lives in scratch
     char* scratch_base = nullptr;
@@ -444,10 +445,16 @@ hipError_t launch_phase(ensc_engine* E, int phase) {
         }
         case 1: hipLaunchKernelGGL(k_fin_element<FIN_BLK>, dim3(nj), dim3(FIN_BLK), 0, E->stream, jobs); break;
         case 2: {
-            // surface-pressure forces of the step, then the nodal update
+            // surface-pressure forces and the virtual particles' nodal forces
+            // of the step, then the nodal update, then the particles follow
+            // the mesh
             if (E->nblk_f)
                 hipLaunchKernelGGL(k_face, dim3(E->nblk_f), dim3(FACE_BLK), 0, E->stream, jobs,
                                    static_cast<const uint4*>(E->b2f_p));
+            const bool fsi = nj == 1 && E->slots.size() == 1;
+            if (fsi && E->fsi_nn)
+                hipLaunchKernelGGL(k_fsi_force, dim3((E->fsi_nn + FSI_BLK - 1) / FSI_BLK), dim3(FSI_BLK), 0, E->stream,
+                                   jobs);
             const uint4* map = static_cast<const uint4*>(E->b2n_p);
             if (E->nblk_n_asm) hipLaunchKernelGGL(E->kn_asm, dim3(E->nblk_n_asm), dim3(E->blk_n), 0, E->stream, jobs, map);
             map += E->nblk_n_asm;
@@ -457,6 +464,9 @@ hipError_t launch_phase(ensc_engine* E, int phase) {
                 hipLaunchKernelGGL(E->kn64_asm, dim3(E->nblk_n64_asm), dim3(E->blk_n), 0, E->stream, jobs, map);
             map += E->nblk_n64_asm;
             if (E->nblk_n64) hipLaunchKernelGGL(E->kn64, dim3(E->nblk_n64), dim3(E->blk_n), 0, E->stream, jobs, map);
+            if (fsi && E->fsi_np)
+                hipLaunchKernelGGL(k_fsi_particles, dim3((E->fsi_np + FSI_BLK - 1) / FSI_BLK), dim3(FSI_BLK), 0,
+                                   E->stream, jobs);
             break;
         }
         default: hipLaunchKernelGGL(k_fin_node<FIN_BLK>, dim3(nj), dim3(FIN_BLK), 0, E->stream, jobs); break;
@@ -1722,6 +1732,133 @@ ensc_status ensc_engine_set_timing(ensc_engine* E, int32_t on) {
     ensc_status st = sync(E);
     if (st) return st;
     E->timing = on != 0;
+    return ENSC_OK;
+}
+
+// ---- fluid-structure coupling (exDyna3D.cpp:1281-1758) ----
+
+ensc_status ensc_engine_fsi_setup(ensc_engine* E, int32_t n_layers, double dist, int64_t* n_particles) {
+    if (!E) return fail(ENSC_ERR_ARG, "null engine");
+    HIPCHK(hipSetDevice(E->device));
+    return guarded([&]() -> ensc_status {
+        if (E->slots.size() != 1 || !job_at(E, 0))
+            return fail(ENSC_ERR_ARG, "ensc_engine_fsi_setup: FSI coupling needs a single-job engine");
+        if (n_layers < 0 || !(dist > 0.0) || !std::isfinite(dist))
+            return fail(ENSC_ERR_ARG, "ensc_engine_fsi_setup: need n_layers >= 0 and a finite dist > 0");
+        Job& J = *job_at(E, 0);
+        if (!J.inited) return fail(ENSC_ERR_STATE, "ensc_engine_fsi_setup: call ensc_engine_init first");
+        ensc_status st;
+        if ((st = sync(E))) return st;
+        const ModelCopy& M = J.model;
+        // the particles are placed on the current positions
+        std::vector<double> x(3 * (size_t)M.Nn);
+        if ((st = download(E, J, "x", x.data(), (int64_t)x.size()))) return st;
+        FsiParticles P;
+        fsi_particles(M, x, n_layers, dist, P);
+        const uint32_t np = (uint32_t)(P.unit.size() / 3), ns = (uint32_t)P.elems.size();
+        // per particle its FSI element slot, per slot its nodes (engine numbering)
+        std::vector<uint32_t> pelem(np), econn(8 * (size_t)ns);
+        for (uint32_t s = 0; s < ns; ++s) {
+            for (uint32_t p = P.eptr[s]; p < P.eptr[s + 1]; ++p) pelem[p] = s;
+            for (int k = 0; k < 8; ++k) econn[8 * (size_t)s + k] = J.n_old2new[M.conn[8 * (size_t)P.elems[s] + k]];
+        }
+        // the FSI nodes and their (slot, corner) entries in slot order, which
+        // is ascending original element order
+        std::vector<int32_t> node_of(M.Nn, -1);
+        std::vector<uint32_t> cnt(M.Nn, 0), nptr(1, 0);
+        for (uint32_t n : econn) cnt[n]++;
+        uint32_t nn = 0;
+        for (uint32_t n = 0; n < M.Nn; ++n)
+            if (cnt[n]) {
+                node_of[n] = (int32_t)nn++;
+                nptr.push_back(nptr.back() + cnt[n]);
+            }
+        std::vector<uint32_t> nent(nptr.back()), fill(nptr.begin(), nptr.end() - 1);
+        for (uint32_t s = 0; s < ns; ++s)
+            for (uint32_t k = 0; k < 8; ++k) nent[fill[node_of[econn[8 * (size_t)s + k]]]++] = s << 3 | k;
+        JobDev& D = J.dev;
+        D.n_fsi_part = 0;
+        D.n_fsi_node = 0;
+        E->fsi_np = E->fsi_nn = 0;
+        if (np) {
+            auto put = [&](const void* src, size_t bytes) -> void* {
+                void* p = J.arena.alloc(std::max<size_t>(bytes, 8));
+                if (p && bytes && hipMemcpy(p, src, bytes, hipMemcpyHostToDevice) != hipSuccess) return nullptr;
+                return p;
+            };
+            D.fsi_pelem = static_cast<const uint32_t*>(put(pelem.data(), pelem.size() * 4));
+            D.fsi_unit = static_cast<const double*>(put(P.unit.data(), P.unit.size() * 8));
+            D.fsi_econn = static_cast<const uint32_t*>(put(econn.data(), econn.size() * 4));
+            D.fsi_eptr = static_cast<const uint32_t*>(put(P.eptr.data(), P.eptr.size() * 4));
+            D.fsi_nptr = static_cast<const uint32_t*>(put(nptr.data(), nptr.size() * 4));
+            D.fsi_nent = static_cast<const uint32_t*>(put(nent.data(), nent.size() * 4));
+            D.fsi_node_of = static_cast<const int32_t*>(put(node_of.data(), node_of.size() * 4));
+            double* force = static_cast<double*>(J.arena.alloc((size_t)np * 24));
+            D.fsi_nforce = static_cast<double*>(J.arena.alloc((size_t)nn * 24));
+            D.fsi_coord = static_cast<double*>(J.arena.alloc((size_t)np * 24));
+            D.fsi_vel = static_cast<double*>(J.arena.alloc((size_t)np * 24));
+            if (!D.fsi_pelem || !D.fsi_unit || !D.fsi_econn || !D.fsi_eptr || !D.fsi_nptr || !D.fsi_nent ||
+                !D.fsi_node_of || !force || !D.fsi_nforce || !D.fsi_coord || !D.fsi_vel)
+                return fail(ENSC_ERR_MEM, "ensc_engine_fsi_setup: allocation failed");
+            // the particle forces start at zero (resizeFsiSharedData, 1540-1567)
+            HIPCHK(hipMemset(force, 0, (size_t)np * 24));
+            D.fsi_force = force;
+            D.n_fsi_part = np;
+            D.n_fsi_node = nn;
+        }
+        E->dirty = true;
+        if ((st = relayout(E))) return st;
+        E->fsi_np = np;
+        E->fsi_nn = nn;
+        if (np && !E->live.empty()) {
+            // the particles' positions and velocities now (generation,
+            // generateVirtualParticlesForFace 1495-1520)
+            hipLaunchKernelGGL(k_fsi_particles, dim3((np + FSI_BLK - 1) / FSI_BLK), dim3(FSI_BLK), 0, E->stream,
+                               static_cast<const JobDev*>(E->jobs_p));
+            HIPCHK(hipGetLastError());
+        }
+        if ((st = sync(E))) return st;
+        if (n_particles) *n_particles = np;
+        return ENSC_OK;
+    });
+}
+
+ensc_status ensc_engine_fsi_set_forces(ensc_engine* E, const double* forces, int64_t n) {
+    if (!E || (!forces && n)) return fail(ENSC_ERR_ARG, "null argument");
+    HIPCHK(hipSetDevice(E->device));
+    Job* J = E->slots.size() == 1 ? job_at(E, 0) : nullptr;
+    if (!J || n != (int64_t)J->dev.n_fsi_part)
+        return fail(ENSC_ERR_ARG, "ensc_engine_fsi_set_forces: n must equal the particle count of ensc_engine_fsi_setup");
+    if (!n) return ENSC_OK;
+    // ordered after the queued steps on the engine's stream; returns once copied
+    HIPCHK(hipMemcpyAsync(const_cast<double*>(J->dev.fsi_force), forces, (size_t)n * 24, hipMemcpyHostToDevice,
+                          E->stream));
+    HIPCHK(hipStreamSynchronize(E->stream));
+    return ENSC_OK;
+}
+
+ensc_status ensc_engine_fsi_get(ensc_engine* E, double* coords, double* vel, int64_t n) {
+    if (!E) return fail(ENSC_ERR_ARG, "null engine");
+    HIPCHK(hipSetDevice(E->device));
+    Job* J = E->slots.size() == 1 ? job_at(E, 0) : nullptr;
+    if (!J || n != (int64_t)J->dev.n_fsi_part)
+        return fail(ENSC_ERR_ARG, "ensc_engine_fsi_get: n must equal the particle count of ensc_engine_fsi_setup");
+    ensc_status st = sync(E);
+    if (st) return st;
+    if (n && coords) HIPCHK(hipMemcpy(coords, J->dev.fsi_coord, (size_t)n * 24, hipMemcpyDeviceToHost));
+    if (n && vel) HIPCHK(hipMemcpy(vel, J->dev.fsi_vel, (size_t)n * 24, hipMemcpyDeviceToHost));
+    return ENSC_OK;
+}
+
+ensc_status ensc_engine_fsi_device(ensc_engine* E, double** forces, const double** coords, const double** vel,
+                                   void** stream) {
+    if (!E) return fail(ENSC_ERR_ARG, "null engine");
+    Job* J = E->slots.size() == 1 ? job_at(E, 0) : nullptr;
+    if (!J || !J->dev.n_fsi_part) return fail(ENSC_ERR_STATE, "ensc_engine_fsi_device: no particles (ensc_engine_fsi_setup)");
+    if (forces) *forces = const_cast<double*>(J->dev.fsi_force);
+    if (coords) *coords = J->dev.fsi_coord;
+    if (vel) *vel = J->dev.fsi_vel;
+    if (stream) *stream = E->stream;
     return ENSC_OK;
 }
 

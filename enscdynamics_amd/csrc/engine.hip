@@ -766,4 +766,83 @@ void job_bc_arrays(const Job& J, std::vector<uint8_t>& mask, std::vector<double>
         if (mask[n]) nodes.push_back(n);
 }
 
+// ---- fluid-structure coupling: the virtual particles ----
+//
+// exDyna3D::get_fsi_sph_Ele_face_index (exDyna3D.cpp:1282-1361): a node is
+// on the FSI boundary when fewer than 8 elements hold it; an element face
+// is an FSI face when its 4 corners all are.  exDyna3D::processInteraction-
+// Elements and generateVirtualParticlesForFace (1428-1638): per element in
+// ascending order, per FSI face in face order, n_layers particles step
+// inward from the face centre along the face's unit normal, `dist` apart
+// in space: the unit step is dist / |J n|, J = dN(face centre) X the 3x3
+// product of the shape-function derivatives with the node positions and n
+// the unit normal (the reference's product J n, not J^T n).  The face
+// numbering is Element_HexN8::Ele_face_indices (Element_HexN8.cpp:21-26):
+// faces 0..5 are unit-coordinate faces xi = -1, xi = +1, eta = -1, eta = +1,
+// zeta = -1, zeta = +1.
+static const int FSI_FACE_CORNERS[6][4] = {{3, 0, 4, 7}, {1, 2, 6, 5}, {0, 1, 5, 4},
+                                           {2, 3, 7, 6}, {1, 0, 3, 2}, {4, 5, 6, 7}};
+
+// the corner's unit coordinates (Element_HexN8.cpp:46-93)
+static void hex_corner(int k, double c[3]) {
+    c[0] = ((k + 1) & 2) ? 1.0 : -1.0;
+    c[1] = (k & 2) ? 1.0 : -1.0;
+    c[2] = (k & 4) ? 1.0 : -1.0;
+}
+
+void fsi_particles(const ModelCopy& M, const std::vector<double>& x, int n_layers, double dist, FsiParticles& out) {
+    out.elems.clear();
+    out.eptr.assign(1, 0);
+    out.unit.clear();
+    std::vector<uint32_t> held(M.Nn, 0);
+    for (size_t q = 0; q < 8 * (size_t)M.Ne; ++q) held[M.conn[q]]++;
+    auto on_boundary = [&](uint32_t n) { return held[n] > 0 && held[n] < 8; };
+    for (uint32_t e = 0; e < M.Ne; ++e) {
+        const uint32_t* cn = &M.conn[8 * (size_t)e];
+        const size_t before = out.unit.size();
+        for (int f = 0; f < 6; ++f) {
+            bool all = true;
+            for (int c = 0; c < 4; ++c) all = all && on_boundary(cn[FSI_FACE_CORNERS[f][c]]);
+            if (!all) continue;
+            // face centre and inward unit normal in unit coordinates
+            double pc[3] = {0.0, 0.0, 0.0}, nu[3] = {0.0, 0.0, 0.0};
+            pc[f / 2] = (f & 1) ? 1.0 : -1.0;
+            nu[f / 2] = (f & 1) ? -1.0 : 1.0;
+            // J(i, j) = sum_k dN_k/dxi_i (face centre) X_k(j)
+            double J[3][3];
+            for (int i = 0; i < 3; ++i)
+                for (int j = 0; j < 3; ++j) {
+                    double s = 0.0;
+                    for (int k = 0; k < 8; ++k) {
+                        double c[3];
+                        hex_corner(k, c);
+                        double dn;
+                        if (i == 0)
+                            dn = 1.0 / 8.0 * c[0] * (1 + c[1] * pc[1]) * (1 + c[2] * pc[2]);
+                        else if (i == 1)
+                            dn = 1.0 / 8.0 * (1 + c[0] * pc[0]) * c[1] * (1 + c[2] * pc[2]);
+                        else
+                            dn = 1.0 / 8.0 * (1 + c[0] * pc[0]) * (1 + c[1] * pc[1]) * c[2];
+                        const double t = dn * x[3 * (size_t)cn[k] + j];
+                        s = k ? s + t : t;
+                    }
+                    J[i][j] = s;
+                }
+            double s2 = 0.0;
+            for (int i = 0; i < 3; ++i) {
+                const double ni = J[i][0] * nu[0] + J[i][1] * nu[1] + J[i][2] * nu[2];
+                s2 = i ? s2 + ni * ni : ni * ni;
+            }
+            if (!(s2 > 1e-24)) continue;  // degenerate face: no particles (1475-1481)
+            const double step = dist / std::sqrt(s2);
+            for (int l = 0; l < n_layers; ++l)
+                for (int d = 0; d < 3; ++d) out.unit.push_back(pc[d] + l * step * nu[d]);
+        }
+        if (out.unit.size() > before) {
+            out.elems.push_back(e);
+            out.eptr.push_back((uint32_t)(out.unit.size() / 3));
+        }
+    }
+}
+
 }  // namespace ensc

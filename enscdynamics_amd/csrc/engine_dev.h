@@ -124,6 +124,22 @@ struct JobDev {
     double* dsf_force;          // [n_dsf][3]
     const uint32_t* ds_ptr;     // [nn + 1] offsets into ds_list
     const uint32_t* ds_list;    // faces of each node, ascending
+    // fluid-structure coupling through virtual particles (exDyna3D.cpp:
+    // 1281-1758; single-job engines): each particle sits at fixed unit
+    // coordinates of its FSI element; per FSI node the (element, corner)
+    // entries in ascending original element order
+    uint32_t n_fsi_part, n_fsi_node;
+    const uint32_t* fsi_pelem;   // [n_fsi_part] FSI element slot of each particle
+    const double* fsi_unit;      // [n_fsi_part][3] unit coordinates
+    const uint32_t* fsi_econn;   // [slots][8] node ids of each FSI element
+    const uint32_t* fsi_eptr;    // [slots + 1] particle range of each FSI element
+    const uint32_t* fsi_nptr;    // [n_fsi_node + 1] offsets into fsi_nent
+    const uint32_t* fsi_nent;    // slot << 3 | corner
+    const int32_t* fsi_node_of;  // [nn] FSI node index, -1 for none
+    const double* fsi_force;     // [n_fsi_part][3] particle forces (set by the coupler)
+    double* fsi_nforce;          // [n_fsi_node][3] nodal force of the particles
+    double* fsi_coord;           // [n_fsi_part][3] particle positions
+    double* fsi_vel;             // [n_fsi_part][3] particle velocities
     // gravity of the current step (StepData.h:22)
     int32_t grav_on, grav_amp;
     double grav_g, grav_dir[3];

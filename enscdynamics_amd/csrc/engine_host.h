@@ -113,6 +113,15 @@ struct Job {
     JobDev self_copy{};
 };
 
+// Virtual particles of the FSI boundary (host side of the coupling): the
+// elements that received particles, ascending (original numbering), the
+// particle range of each and every particle's unit coordinates.
+struct FsiParticles {
+    std::vector<uint32_t> elems, eptr;
+    std::vector<double> unit;  // [n][3]
+};
+void fsi_particles(const ModelCopy& M, const std::vector<double>& x, int n_layers, double dist, FsiParticles& out);
+
 bool copy_model(const ensc_model_desc* m, ModelCopy& c, std::string& err);
 ensc_status prepare_job(Job& J, int precision, int reorder, int blk_e, int blk_n, int fp64_rule);
 ensc_status upload_job(Job& J);

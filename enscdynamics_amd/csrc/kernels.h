@@ -1181,7 +1181,18 @@ __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(ASM ? (MIXE
             acc[2] += grp[2];
         }
         }
-        double a[3] = {0.0 + acc[0], 0.0 + acc[1], 0.0 + acc[2]};
+        // apply_fsiSph_nodeForce (1635-1689) runs before add_inForce_to_rhs:
+        // a = (sum of the particles' nodal forces) + (the corner sum)
+        int32_t mf = -1;
+        if (J.n_fsi_node) mf = J.fsi_node_of[n];
+        double a[3];
+        if (mf >= 0) {
+#pragma unroll
+            for (int d = 0; d < 3; ++d) a[d] = J.fsi_nforce[(size_t)mf * 3 + d] + acc[d];
+        } else {
+#pragma unroll
+            for (int d = 0; d < 3; ++d) a[d] = 0.0 + acc[d];
+        }
         if (J.grav_on) {  // apply_gravity (800-826)
             const double ag = ctl->grav_amp * J.grav_g;
             a[0] += ag * J.grav_dir[0] / im;
@@ -1281,6 +1292,7 @@ __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(ASM ? (MIXE
                 for (uint32_t q = f0; q < f1; ++q) fp += J.dsf_force[(size_t)J.ds_list[q] * 3 + d];
                 wp += fp * (vb + vd);
             }
+            if (mf >= 0) wp += J.fsi_nforce[(size_t)mf * 3 + d] * (vb + vd);  // the particles' work
             const double dx = dt_i * vd;
             if (MIXED) {
                 // update_displacement (651-654), compensated (Kahan); the
@@ -1302,7 +1314,7 @@ __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(ASM ? (MIXE
         ke = k2 * half_m;
         pw += kj * half_m;
         if (J.grav_on) pw += 0.5 * hs * (ctl->grav_amp * J.grav_g) * wg * half_m;
-        if (f1 != f0) pw += 0.25 * hs * wp;
+        if (f1 != f0 || mf >= 0) pw += 0.25 * hs * wp;
         if (MIXED) gst(J.ucomp, n) = cw_new;
         if (!MIXED) {
             gst2(J.pos, n * POS_REC, x[0], x[1]);
@@ -1366,6 +1378,81 @@ __global__ __launch_bounds__(FACE_BLK) void k_face(const JobDev* __restrict__ jo
     out[0] = s * (a[1] * b[2] - a[2] * b[1]);
     out[1] = s * (a[2] * b[0] - a[0] * b[2]);
     out[2] = s * (a[0] * b[1] - a[1] * b[0]);
+}
+
+// Trilinear shape function of corner k at unit point p
+// (Element_HexN8::get_shapeFunctionValue, Element_HexN8.cpp:46-93: corner k
+// at (-1|1, -1|1, -1|1) in the order 0 (-,-,-), 1 (+,-,-), 2 (+,+,-),
+// 3 (-,+,-), 4..7 the same at +1 in the third coordinate).
+__device__ inline double hex_shape(int k, const double* p) {
+#pragma clang fp contract(off)
+    const double c0 = ((k + 1) & 2) ? 1.0 : -1.0;
+    const double c1 = (k & 2) ? 1.0 : -1.0;
+    const double c2 = (k & 4) ? 1.0 : -1.0;
+    return 1.0 / 8.0 * (1 + c0 * p[0]) * (1 + c1 * p[1]) * (1 + c2 * p[2]);
+}
+
+// Virtual-particle forces onto the nodes (exDyna3D::apply_fsiSph_nodeForce,
+// exDyna3D.cpp:1635-1689): one thread per FSI node sums, over its
+// (element, corner) entries in ascending element order, the element's
+// nodal force at that corner, sum over the element's particles of
+// f_p N_corner(unit_p) in particle order.  The nodal update adds the result
+// before the internal forces.  Runs on the single job of the engine (job 0).
+constexpr int FSI_BLK = 256;
+__global__ __launch_bounds__(FSI_BLK) void k_fsi_force(const JobDev* __restrict__ jobs) {
+#pragma clang fp contract(off)
+    const JobDev& J = jobs[0];
+    if (J.ctl->kind != KIND_STEP) return;
+    const uint32_t m = blockIdx.x * FSI_BLK + threadIdx.x;
+    if (m >= J.n_fsi_node) return;
+    double f[3] = {0.0, 0.0, 0.0};
+    for (uint32_t q = J.fsi_nptr[m]; q < J.fsi_nptr[m + 1]; ++q) {
+        const uint32_t ent = J.fsi_nent[q], s = ent >> 3;
+        const int k = (int)(ent & 7u);
+        double nf[3] = {0.0, 0.0, 0.0};
+        for (uint32_t p = J.fsi_eptr[s]; p < J.fsi_eptr[s + 1]; ++p) {
+            const double N = hex_shape(k, J.fsi_unit + (size_t)p * 3);
+#pragma unroll
+            for (int d = 0; d < 3; ++d) nf[d] += J.fsi_force[(size_t)p * 3 + d] * N;
+        }
+#pragma unroll
+        for (int d = 0; d < 3; ++d) f[d] += nf[d];
+    }
+#pragma unroll
+    for (int d = 0; d < 3; ++d) J.fsi_nforce[(size_t)m * 3 + d] = f[d];
+}
+
+// Virtual-particle positions and velocities from the element's nodes
+// (exDyna3D::update_virParticles_coor_vel, exDyna3D.cpp:1691-1758, and the
+// generation in generateVirtualParticlesForFace, 1428-1521): x_p = sum_j
+// N_j(unit_p) x_j, v_p likewise, after the nodal update.  One thread per
+// particle of job 0.
+__global__ __launch_bounds__(FSI_BLK) void k_fsi_particles(const JobDev* __restrict__ jobs) {
+#pragma clang fp contract(off)
+    const JobDev& J = jobs[0];
+    const uint32_t p = blockIdx.x * FSI_BLK + threadIdx.x;
+    if (p >= J.n_fsi_part) return;
+    const uint32_t* en = J.fsi_econn + (size_t)J.fsi_pelem[p] * 8;
+    const double* up = J.fsi_unit + (size_t)p * 3;
+    const bool moving_x = J.inv_m == nullptr;  // FP64 jobs store the current position; mixed ones X (+ u)
+    double x[3], v[3];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        const double N = hex_shape(j, up);
+        const size_t n = en[j];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const double xj = J.pos[n * POS_REC + d] + (moving_x ? 0.0 : J.vu[n * VU_REC + 3 + d]);
+            const double vj = J.vu[n * VU_REC + d];
+            x[d] = j ? x[d] + N * xj : N * xj;
+            v[d] = j ? v[d] + N * vj : N * vj;
+        }
+    }
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        J.fsi_coord[(size_t)p * 3 + d] = x[d];
+        J.fsi_vel[(size_t)p * 3 + d] = v[d];
+    }
 }
 
 // Per-job node finalize (one block per job in the job table): closes the

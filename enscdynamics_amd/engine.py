@@ -97,6 +97,10 @@ def load_library(path: str = LIB_PATH):
         ("ensc_engine_set_timing", [vp, C.c_int32]),
         ("ensc_engine_autotune", [vp, C.c_int32, C.POINTER(C.c_int64)]),
         ("ensc_engine_set_graph", [vp, C.c_int32]),
+        ("ensc_engine_fsi_setup", [vp, C.c_int32, C.c_double, C.POINTER(C.c_int64)]),
+        ("ensc_engine_fsi_set_forces", [vp, vp, C.c_int64]),
+        ("ensc_engine_fsi_get", [vp, vp, vp, C.c_int64]),
+        ("ensc_engine_fsi_device", [vp, C.POINTER(vp), C.POINTER(vp), C.POINTER(vp), C.POINTER(vp)]),
     ]:
         f = getattr(L, name)
         f.restype = C.c_int
@@ -311,3 +315,26 @@ class Engine:
 
     def device_bytes(self) -> int:
         return int(_lib.ensc_engine_device_bytes(self.h))
+
+    # --- fluid-structure coupling (exDyna3D.cpp:1281-1758) ---
+    def fsi_setup(self, n_layers: int, dist: float) -> int:
+        """Virtual particles on the FSI boundary (get_fsiSph_virtualParticles_
+        and_vel): n_layers per boundary face, dist apart; returns the count.
+        After init()."""
+        n = C.c_int64()
+        _check(_lib.ensc_engine_fsi_setup(self.h, int(n_layers), float(dist), C.byref(n)))
+        self._n_fsi = int(n.value)
+        return self._n_fsi
+
+    def fsi_set_forces(self, forces: np.ndarray) -> None:
+        """The particles' forces (FSI_virtualParticles_nodeForce), (n, 3)."""
+        f = np.ascontiguousarray(forces, dtype=np.float64).reshape(-1, 3)
+        _check(_lib.ensc_engine_fsi_set_forces(self.h, f.ctypes.data, f.shape[0]))
+
+    def fsi_particles(self) -> Tuple[np.ndarray, np.ndarray]:
+        """The particles' positions and velocities, each (n, 3)."""
+        n = getattr(self, "_n_fsi", 0)
+        x = np.zeros((n, 3))
+        v = np.zeros((n, 3))
+        _check(_lib.ensc_engine_fsi_get(self.h, x.ctypes.data, v.ctypes.data, n))
+        return x, v

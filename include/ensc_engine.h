@@ -224,6 +224,38 @@ ensc_status ensc_engine_output_pop(ensc_engine* eng);
 ensc_status ensc_engine_autotune(ensc_engine* eng, int32_t ticks, int64_t* chosen);
 
 
+/* ---- fluid-structure coupling through virtual particles ----
+ * The reference's FSI interface (exDyna3D.cpp:1281-1758): the public
+ * fsi_share_data member (exDyna3D.h:54, FSI_share_data.h:27-36) holds one
+ * position, velocity and force per virtual particle; a fluid code writes
+ * the forces and reads the particles between steps.  Single-job engines.
+ *
+ * fsi_setup replaces get_fsiSph_virtualParticles_and_vel(nLayers, dist)
+ * (1363-1383; run() calls it with 0 layers, 67): boundary nodes are those
+ * held by fewer than 8 elements, FSI faces those with 4 boundary corners
+ * (1282-1361), and each FSI face gets n_layers particles stepping inward
+ * from its centre, dist apart (1428-1521), on the current positions;
+ * particles are ordered by element, face, layer as in the reference.  Call
+ * after ensc_engine_init; calling again replaces the particles.  The
+ * particle forces start at zero and persist until set again.  Each step
+ * adds them to the nodes through the shape functions before the internal
+ * forces (apply_fsiSph_nodeForce, 1635-1689, called at 783) and moves the
+ * particles with the mesh after the nodal update
+ * (update_virParticles_coor_vel, 1691-1758, called at 792); their work
+ * enters the external work. */
+ensc_status ensc_engine_fsi_setup(ensc_engine* eng, int32_t n_layers, double dist, int64_t* n_particles);
+/* fsi_share_data.FSI_virtualParticles_nodeForce <- forces [n][3] (host),
+ * n = the particle count; returns once copied. */
+ensc_status ensc_engine_fsi_set_forces(ensc_engine* eng, const double* forces, int64_t n);
+/* FSI_virtualParticles_coordinates / _velocity -> coords, vel [n][3] (host;
+ * either may be NULL), after the queued steps. */
+ensc_status ensc_engine_fsi_get(ensc_engine* eng, double* coords, double* vel, int64_t n);
+/* Device pointers of the same arrays ([n][3] FP64) and the engine's stream
+ * (a hipStream_t), for a coupler that runs on the GPU: work it queues on
+ * that stream between steps is ordered with the engine's (no host copies). */
+ensc_status ensc_engine_fsi_device(ensc_engine* eng, double** forces, const double** coords, const double** vel,
+                                   void** stream);
+
 /* Enable (1) / disable (0) per-kernel event timing for later enqueue calls. */
 ensc_status ensc_engine_set_timing(ensc_engine* eng, int32_t on);
 /* Device bytes held by the engine. */

@@ -38,6 +38,9 @@ int oracle_init(void* h);
  * hist_rows allows.  Returns the number of steps taken. */
 int64_t oracle_run(void* h, int64_t max_steps, const uint32_t* probe, uint32_t n_probe,
                    double* hist, int64_t hist_rows);
+/* Enter the next *Step (setCurrentStep and its end time) without stepping,
+ * as oracle_run does before its first step there. */
+void oracle_begin_step(void* h);
 /* Time n_steps iterations of the hot loop body (dt update + computeSate)
  * inside the current *Step, with a steady clock around the loop only.
  * Returns seconds.  Used as the CPU baseline. */
@@ -52,6 +55,13 @@ int oracle_threads(void* h);
  * time (interval <= 0: calculate_time_interval's rule, 1947-1977). */
 void oracle_set_output(void* h, double interval);
 int64_t oracle_snapshots(void* h);
+/* Fluid-structure coupling (exDyna3D.cpp:1281-1758): virtual particles on
+ * the current state, n_layers per FSI face, dist apart (returns the count);
+ * their forces [n][3], applied at every later step; their positions and
+ * velocities [n][3] (returns n). */
+int64_t oracle_fsi_setup(void* h, int n_layers, double dist);
+int oracle_fsi_set_forces(void* h, const double* forces, int64_t n);
+int64_t oracle_fsi_get(void* h, double* coords, double* vel);
 /* Field of snapshot k: "time" [1]; "x","a","v","u" [Nn][3]; "vonMises" [Ne];
  * "Stress", "Green-Lagrange Strain" [Ne][6] (xx yy zz xy yz xz).  Returns
  * the count written or -1. */

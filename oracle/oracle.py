@@ -52,6 +52,13 @@ def lib():
         L.oracle_snapshots.argtypes = [C.c_void_p]
         L.oracle_snapshot.restype = C.c_int64
         L.oracle_snapshot.argtypes = [C.c_void_p, C.c_int64, C.c_char_p, C.c_void_p]
+        L.oracle_begin_step.argtypes = [C.c_void_p]
+        L.oracle_fsi_setup.restype = C.c_int64
+        L.oracle_fsi_setup.argtypes = [C.c_void_p, C.c_int, C.c_double]
+        L.oracle_fsi_set_forces.restype = C.c_int
+        L.oracle_fsi_set_forces.argtypes = [C.c_void_p, C.c_void_p, C.c_int64]
+        L.oracle_fsi_get.restype = C.c_int64
+        L.oracle_fsi_get.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
         _lib = L
     return _lib
 
@@ -112,6 +119,28 @@ class Oracle:
         if lib().oracle_snapshot(self.h, int(k), name.encode(), out.ctypes.data) < 0:
             raise KeyError((k, name))
         return out
+
+    def begin_step(self) -> "Oracle":
+        """Enter the next *Step (setCurrentStep) without stepping."""
+        lib().oracle_begin_step(self.h)
+        return self
+
+    # fluid-structure coupling (exDyna3D.cpp:1281-1758)
+    def fsi_setup(self, n_layers: int, dist: float) -> int:
+        self._n_fsi = int(lib().oracle_fsi_setup(self.h, int(n_layers), float(dist)))
+        return self._n_fsi
+
+    def fsi_set_forces(self, forces: np.ndarray) -> None:
+        f = np.ascontiguousarray(forces, dtype=np.float64).reshape(-1, 3)
+        if lib().oracle_fsi_set_forces(self.h, f.ctypes.data, f.shape[0]) != 0:
+            raise ValueError("force count differs from the particle count")
+
+    def fsi_particles(self):
+        n = getattr(self, "_n_fsi", 0)
+        x = np.zeros((n, 3))
+        v = np.zeros((n, 3))
+        lib().oracle_fsi_get(self.h, x.ctypes.data, v.ctypes.data)
+        return x, v
 
     def scalars(self) -> Dict[str, float]:
         o = np.zeros(len(OS_NAMES))

@@ -5,7 +5,11 @@
 // GPU through the C ABI; the final state is written for comparison with the
 // CPU reference's goldens.  With "stepwise" the binding's per-step mode runs,
 // and this subclass hooks computeSate() the way the reference's own
-// subclasses do, counting the step iterations it sees.
+// subclasses do, counting the step iterations it sees.  With
+// ENSC_FSI=layers,dist,scale it is also an FSI coupler, as the reference
+// harness's subclass (ref_harness/harness.cpp): particles generated at the
+// first step, force scale * sin(0.37 (p + 1) + 1.1 d + 0.05 n) written into
+// fsi_share_data before step n; the particles are written at the end.
 //
 // usage: ref_dropin WORKDIR DECK OUTDIR fp64|mixed [c_dr c_cr factor [stepwise]]
 #include "dataIn.h"
@@ -13,7 +17,9 @@
 #include "npy.h"
 #include "spdlog/spdlog.h"
 
+#include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <fstream>
 #include <string>
 #include <unistd.h>
@@ -23,7 +29,15 @@ class DropIn : public EnSC::exDyna3D_MI355X {
 public:
     using exDyna3D_MI355X::exDyna3D_MI355X;
     long hooked = 0;  // computeSate() calls seen by this subclass
+    int fsi_layers = 0;
+    double fsi_dist = 0.0, fsi_scale = 0.0;
     void computeSate() override {
+        if (fsi_layers > 0) {
+            if (hooked == 0) setup_fsi(fsi_layers, fsi_dist);
+            auto& F = fsi_share_data.FSI_virtualParticles_nodeForce;
+            for (size_t p = 0; p < F.size(); ++p)
+                for (int d = 0; d < 3; ++d) F[p][d] = fsi_scale * std::sin(0.37 * (double)(p + 1) + 1.1 * d + 0.05 * (double)hooked);
+        }
         exDyna3D_MI355X::computeSate();
         ++hooked;
     }
@@ -38,6 +52,18 @@ public:
         npy::npy_f64(out + "/u.npy", {Nn, 3}, u.data());
         npy::npy_f64(out + "/x.npy", {Nn, 3}, x.data());
         npy::npy_f64(out + "/sigma.npy", {Ne, 6}, s.data());
+        if (fsi_layers > 0) {
+            const auto& X = fsi_share_data.FSI_virtualParticles_coordinates;
+            const auto& V = fsi_share_data.FSI_virtualParticles_velocity;
+            std::vector<double> px(3 * X.size()), pv(3 * V.size());
+            for (size_t p = 0; p < X.size(); ++p)
+                for (int d = 0; d < 3; ++d) {
+                    px[3 * p + d] = X[p][d];
+                    pv[3 * p + d] = V[p][d];
+                }
+            npy::npy_f64(out + "/fsi_x.npy", {X.size(), 3}, px.data());
+            npy::npy_f64(out + "/fsi_v.npy", {V.size(), 3}, pv.data());
+        }
         std::ofstream f(out + "/scalars.txt");
         f.precision(17);
         f << "time " << time << "\nsteps " << steps_run() << "\nhooked " << hooked << "\n";
@@ -66,6 +92,12 @@ int main(int argc, char** argv) {
     }
     const bool stepwise = argc >= 9 && std::string(argv[8]) == "stepwise";
     DropIn sim(prec == "mixed" ? ENSC_MIXED : ENSC_FP64, true, stepwise);
+    if (const char* f = std::getenv("ENSC_FSI")) {
+        if (std::sscanf(f, "%d,%lf,%lf", &sim.fsi_layers, &sim.fsi_dist, &sim.fsi_scale) != 3 || !stepwise) {
+            std::fprintf(stderr, "ENSC_FSI must be layers,dist,scale, with stepwise\n");
+            return 2;
+        }
+    }
     EnSC::DataIn reader(sim);
     reader.read_inp(deck);
     try {

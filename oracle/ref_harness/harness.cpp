@@ -11,11 +11,19 @@
 // Stepping is observed by subclassing exDyna3D and overriding the virtual
 // computeSate() (Explicit3D/include/exDyna3D.h:75): the base step runs
 // unchanged, then the state is recorded.
+//
+// With ENSC_FSI=layers,dist,scale in the environment the subclass also acts
+// as an FSI coupler: before the first step it generates the virtual
+// particles with the reference's own get_fsiSph_virtualParticles_and_vel
+// (exDyna3D.cpp:1363-1383) and, before every step n, writes the force
+// scale * sin(0.37 (p + 1) + 1.1 d + 0.05 n) on particle p, component d,
+// into fsi_share_data (exDyna3D.h:54), which the base step applies.
 #include "exDyna3D.h"
 #include "dataIn.h"
 #include "dataOut.h"
 #include "spdlog/spdlog.h"
 
+#include <cmath>
 #include <cstdio>
 #include <cstdint>
 #include <cstdlib>
@@ -181,7 +189,32 @@ public:
             for (int d = 0; d < 3; ++d) hist.push_back(solution_u[3 * (size_t)n + d]);
     }
 
+    int fsi_layers = 0;
+    double fsi_dist = 0.0, fsi_scale = 0.0;
+
+    void dump_particles(const std::string& tag) {
+        const auto& X = fsi_share_data.FSI_virtualParticles_coordinates;
+        const auto& V = fsi_share_data.FSI_virtualParticles_velocity;
+        std::vector<double> x(3 * X.size()), v(3 * V.size());
+        for (size_t p = 0; p < X.size(); ++p)
+            for (int d = 0; d < 3; ++d) {
+                x[3 * p + d] = X[p][d];
+                v[3 * p + d] = V[p][d];
+            }
+        npy_f64(out + "/" + tag + "_fsi_x.npy", {X.size(), 3}, x.data());
+        npy_f64(out + "/" + tag + "_fsi_v.npy", {V.size(), 3}, v.data());
+    }
+
     void computeSate() override {
+        if (fsi_layers > 0) {
+            if (nstep == 0) {
+                get_fsiSph_virtualParticles_and_vel(fsi_layers, fsi_dist);
+                dump_particles("setup");
+            }
+            auto& F = fsi_share_data.FSI_virtualParticles_nodeForce;
+            for (size_t p = 0; p < F.size(); ++p)
+                for (int d = 0; d < 3; ++d) F[p][d] = fsi_scale * std::sin(0.37 * (double)(p + 1) + 1.1 * d + 0.05 * (double)nstep);
+        }
         EnSC::exDyna3D::computeSate();
         ++nstep;
         record();
@@ -194,6 +227,7 @@ public:
         size_t w = 4 + 3 * probe_nodes.size();
         npy_f64(out + "/history.npy", {hist.size() / w, w}, hist.data());
         dump_state("final");
+        if (fsi_layers > 0) dump_particles("final");
     }
 
     void do_init() { init(); }
@@ -234,6 +268,12 @@ int main(int argc, char** argv) {
     if (argc >= 6) sim.snap_steps = parse_list(argv[5]);
     if (argc >= 7)
         for (long n : parse_list(argv[6])) sim.probe_nodes.push_back((uint32_t)n);
+    if (const char* f = std::getenv("ENSC_FSI")) {
+        if (std::sscanf(f, "%d,%lf,%lf", &sim.fsi_layers, &sim.fsi_dist, &sim.fsi_scale) != 3) {
+            std::fprintf(stderr, "ENSC_FSI must be layers,dist,scale\n");
+            return 2;
+        }
+    }
     EnSC::DataIn reader(sim);
     reader.read_inp(deck);
     sim.do_init();

@@ -30,11 +30,14 @@
 //    averaged over the step like gravity's; the forces are added after
 //    gravity, per node in ascending face order, and their work enters the
 //    external work like gravity's.
+//  * The external work of the virtual particles' forces (the FSI coupling
+//    itself, exDyna3D.cpp:1281-1758, is restated below).
 #include "oracle.h"
 
 #include <omp.h>
 
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -716,15 +719,144 @@ struct Sim {
         }
     }
 
+    // ---- fluid-structure coupling through virtual particles ----
+    // get_fsi_sph_Ele_face_index (1282-1361): boundary nodes are held by
+    // fewer than 8 elements; an element face is an FSI face when its 4
+    // corners (Element_HexN8::Ele_face_indices, Element_HexN8.cpp:21-26) are
+    // boundary nodes.  processInteractionElements (1584-1638) and
+    // generateVirtualParticlesForFace (1428-1521) place nLayers particles per
+    // FSI face; the map element -> particles (std::map, ascending element)
+    // drives apply_fsiSph_nodeForce (1635-1689) and
+    // update_virParticles_coor_vel (1691-1758).
+    struct FsiElem {
+        uint32_t e;
+        std::vector<std::array<double, 3>> unit;  // unit coordinates of its particles
+        std::vector<size_t> index;                // their global indices
+    };
+    std::vector<FsiElem> fsi;                     // ascending element
+    std::vector<std::array<double, 3>> fsi_coord, fsi_vel, fsi_force;
+    std::vector<double> fsi_fp;                   // [3Nn] the particles' nodal force of the step
+
+    static double shape_value(int k, const double p[3]) {
+        // Element_HexN8::get_shapeFunctionValue (Element_HexN8.cpp:46-93)
+        static const double c[8][3] = {{-1, -1, -1}, {1, -1, -1}, {1, 1, -1}, {-1, 1, -1},
+                                       {-1, -1, 1},  {1, -1, 1},  {1, 1, 1},  {-1, 1, 1}};
+        return 1.0 / 8.0 * (1 + c[k][0] * p[0]) * (1 + c[k][1] * p[1]) * (1 + c[k][2] * p[2]);
+    }
+    static double shape_deriv(int k, int dir, const double p[3]) {
+        // get_shapeFunctionDerivativeValue (Element_HexN8.cpp:103-161)
+        static const double c[8][3] = {{-1, -1, -1}, {1, -1, -1}, {1, 1, -1}, {-1, 1, -1},
+                                       {-1, -1, 1},  {1, -1, 1},  {1, 1, 1},  {-1, 1, 1}};
+        if (dir == 0) return 1.0 / 8.0 * c[k][0] * (1 + c[k][1] * p[1]) * (1 + c[k][2] * p[2]);
+        if (dir == 1) return 1.0 / 8.0 * (1 + c[k][0] * p[0]) * c[k][1] * (1 + c[k][2] * p[2]);
+        return 1.0 / 8.0 * (1 + c[k][0] * p[0]) * (1 + c[k][1] * p[1]) * c[k][2];
+    }
+    // N(unit) . (nodal field of element e), sequential over the 8 corners
+    void interp(uint32_t e, const double p[3], const std::vector<double>& f, double out[3]) const {
+        for (int d = 0; d < 3; ++d) out[d] = 0.0;
+        for (int k = 0; k < 8; ++k) {
+            const double N = shape_value(k, p);
+            for (int d = 0; d < 3; ++d) {
+                const double t = N * f[3 * (size_t)conn[8 * (size_t)e + k] + d];
+                out[d] = k ? out[d] + t : t;
+            }
+        }
+    }
+
+    size_t fsi_setup(int nLayers, double dist) {
+        static const int faces[6][4] = {{3, 0, 4, 7}, {1, 2, 6, 5}, {0, 1, 5, 4},
+                                        {2, 3, 7, 6}, {1, 0, 3, 2}, {4, 5, 6, 7}};
+        static const double surf[6][3] = {{-1, 0, 0}, {1, 0, 0}, {0, -1, 0}, {0, 1, 0}, {0, 0, -1}, {0, 0, 1}};
+        static const double nrm[6][3] = {{1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1}};
+        std::vector<int> held(Nn, 0);
+        for (uint32_t n : conn) held[n]++;
+        fsi.clear();
+        fsi_coord.clear();
+        fsi_vel.clear();
+        for (uint32_t e = 0; e < Ne; ++e) {
+            FsiElem E;
+            E.e = e;
+            for (int f = 0; f < 6; ++f) {
+                int on = 0;
+                for (int i = 0; i < 4; ++i) {
+                    const int h = held[conn[8 * (size_t)e + faces[f][i]]];
+                    if (h > 0 && h < 8) ++on;
+                }
+                if (on != 4) continue;
+                double Jm[3][3];
+                for (int i = 0; i < 3; ++i)
+                    for (int j = 0; j < 3; ++j) {
+                        double s = 0.0;
+                        for (int k = 0; k < 8; ++k) {
+                            const double t = shape_deriv(k, i, surf[f]) * x[3 * (size_t)conn[8 * (size_t)e + k] + j];
+                            s = k ? s + t : t;
+                        }
+                        Jm[i][j] = s;
+                    }
+                double nr[3];
+                for (int i = 0; i < 3; ++i) nr[i] = Jm[i][0] * nrm[f][0] + Jm[i][1] * nrm[f][1] + Jm[i][2] * nrm[f][2];
+                const double s2 = nr[0] * nr[0] + nr[1] * nr[1] + nr[2] * nr[2];
+                if (!(s2 > 1e-24)) continue;
+                const double unit_step = dist / std::sqrt(s2);
+                for (int k = 0; k < nLayers; ++k) {
+                    std::array<double, 3> p;
+                    for (int d = 0; d < 3; ++d) p[d] = surf[f][d] + k * unit_step * nrm[f][d];
+                    double c[3], w[3];
+                    interp(e, p.data(), x, c);
+                    interp(e, p.data(), v, w);
+                    E.unit.push_back(p);
+                    E.index.push_back(fsi_coord.size());
+                    fsi_coord.push_back({c[0], c[1], c[2]});
+                    fsi_vel.push_back({w[0], w[1], w[2]});
+                }
+            }
+            if (!E.unit.empty()) fsi.push_back(std::move(E));
+        }
+        fsi_force.assign(fsi_coord.size(), {0.0, 0.0, 0.0});
+        fsi_fp.assign(fsi_coord.empty() ? 0 : 3 * Nn, 0.0);
+        return fsi_coord.size();
+    }
+
+    // apply_fsiSph_nodeForce (1635-1689), on a = 0 before the internal forces
+    void apply_fsi() {
+        if (fsi_coord.empty()) return;
+        for (const FsiElem& E : fsi) {
+            double nf[8][3] = {};
+            for (size_t q = 0; q < E.unit.size(); ++q) {
+                const auto& F = fsi_force[E.index[q]];
+                for (int k = 0; k < 8; ++k) {
+                    const double N = shape_value(k, E.unit[q].data());
+                    for (int d = 0; d < 3; ++d) nf[k][d] += F[d] * N;
+                }
+            }
+            for (int k = 0; k < 8; ++k)
+                for (int d = 0; d < 3; ++d) a[3 * (size_t)conn[8 * (size_t)E.e + k] + d] += nf[k][d];
+        }
+        fsi_fp = a;  // a held 0 before: the particles' nodal force
+    }
+
+    // update_virParticles_coor_vel (1691-1758), after move_mesh
+    void update_particles() {
+        for (const FsiElem& E : fsi)
+            for (size_t q = 0; q < E.unit.size(); ++q) {
+                double c[3], w[3];
+                interp(E.e, E.unit[q].data(), x, c);
+                interp(E.e, E.unit[q].data(), v, w);
+                fsi_coord[E.index[q]] = {c[0], c[1], c[2]};
+                fsi_vel[E.index[q]] = {w[0], w[1], w[2]};
+            }
+    }
+
     void nodal_update() {
         // compute_a, update_velocity, update_displacement, move_mesh (642-696)
         // Energy account (extension): gravity's and the pressures' work on
         // every dof over the step, f (v_old + v_new) / 2 dt_avg.
-        double kep = 0.0, wg = 0.0, wp = 0.0;
+        double kep = 0.0, wg = 0.0, wp = 0.0, wf = 0.0;
         const double hs = dt_i + dt_i_1;
         const bool gon = g_now;
         const bool pon = !ds_val.empty();
-#pragma omp parallel for num_threads(nthreads) schedule(static) reduction(+ : kep, wg, wp)
+        const bool fon = !fsi_coord.empty();
+#pragma omp parallel for num_threads(nthreads) schedule(static) reduction(+ : kep, wg, wp, wf)
         for (long n = 0; n < (long)Nn; ++n) {
             double k2 = 0.0;
             for (int d = 0; d < 3; ++d) {
@@ -738,12 +870,14 @@ struct Sim {
                 k2 += v[i] * v[i];
                 if (gon) wg += g_ag * g_dir[d] / invM[n] * (vb + v[i]);
                 if (pon) wp += ds_fp[i] * (vb + v[i]);
+                if (fon) wf += fsi_fp[i] * (vb + v[i]);
             }
             kep += 0.5 * k2 / invM[n];
         }
         ke = kep;
         wext += 0.25 * hs * wg;
         wext += 0.25 * hs * wp;
+        wext += 0.25 * hs * wf;
     }
 
     double kinetic() const {
@@ -892,12 +1026,14 @@ struct Sim {
         time += dt_i_1;
         if (time > totalTime) time = totalTime;
         apply_bc_vec();
+        apply_fsi();
         add_inForce_to_rhs(dt_i_1);
         apply_gravity();
         apply_dsload();
         apply_bc_a(0.5 * (dt_i + dt_i_1));
         output_results();
         nodal_update();
+        update_particles();
         ++nsteps;
     }
 
@@ -965,14 +1101,19 @@ int64_t oracle_run(void* h, int64_t max_steps, const uint32_t* probe, uint32_t n
     return static_cast<Sim*>(h)->run(max_steps, probe, n_probe, hist, hist_rows);
 }
 
-double oracle_time_steps(void* h, int64_t n_steps) {
+void oracle_begin_step(void* h) {
     Sim* s = static_cast<Sim*>(h);
     if (s->run_step == 0) s->run_step = 1;
-    if (!s->in_step) {
+    if (!s->in_step && s->run_step < s->steps.size()) {
         s->setCurrentStep(s->run_step);
         s->step_end = s->time + s->steps[s->run_step].period;
         s->in_step = true;
     }
+}
+
+double oracle_time_steps(void* h, int64_t n_steps) {
+    Sim* s = static_cast<Sim*>(h);
+    oracle_begin_step(h);
     const auto t0 = std::chrono::steady_clock::now();
     for (int64_t i = 0; i < n_steps; ++i) {
         s->update_dt();
@@ -1045,6 +1186,28 @@ void oracle_set_output(void* h, double interval) {
     }
     s->time_interval = ti;
     s->snaps.clear();
+}
+
+int64_t oracle_fsi_setup(void* h, int n_layers, double dist) {
+    return (int64_t)static_cast<Sim*>(h)->fsi_setup(n_layers, dist);
+}
+
+int oracle_fsi_set_forces(void* h, const double* f, int64_t n) {
+    Sim* s = static_cast<Sim*>(h);
+    if (n != (int64_t)s->fsi_force.size()) return -1;
+    for (int64_t p = 0; p < n; ++p)
+        for (int d = 0; d < 3; ++d) s->fsi_force[(size_t)p][d] = f[3 * p + d];
+    return 0;
+}
+
+int64_t oracle_fsi_get(void* h, double* coords, double* vel) {
+    Sim* s = static_cast<Sim*>(h);
+    for (size_t p = 0; p < s->fsi_coord.size(); ++p)
+        for (int d = 0; d < 3; ++d) {
+            if (coords) coords[3 * p + d] = s->fsi_coord[p][d];
+            if (vel) vel[3 * p + d] = s->fsi_vel[p][d];
+        }
+    return (int64_t)s->fsi_coord.size();
 }
 
 int64_t oracle_snapshots(void* h) { return (int64_t)static_cast<Sim*>(h)->snaps.size(); }

@@ -6,7 +6,7 @@ oracle/Makefile, target ``ref``) on each deck and packs what it records into
 history and end-of-run fields.  Needs /root/reference; run it here, never on
 a machine without the reference:
 
-    make -C oracle ref && python tests/golden/make_golden.py
+    make -C oracle ref && python tests/golden/make_golden.py [--only NAME ...]
 
 Decks: synthetic decks written by enscdynamics_amd.decks (configuration 1
 and micro decks built to exercise single terms), plus the reference's own
@@ -36,7 +36,7 @@ HARNESS = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
 DECKDIR = os.path.join(HERE, "decks")
 
 
-def run_ref(deck_path: str, max_steps=0, snaps=(), probes=(), params=None):
+def run_ref(deck_path: str, max_steps=0, snaps=(), probes=(), params=None, fsi=None):
     work = tempfile.mkdtemp(prefix="gold_")
     shutil.copy(deck_path, os.path.join(work, "deck.inp"))
     args = [HARNESS, work, "deck.inp", work, str(max_steps), ",".join(map(str, snaps)) or "0",
@@ -44,6 +44,8 @@ def run_ref(deck_path: str, max_steps=0, snaps=(), probes=(), params=None):
     if params is not None:
         args += [repr(params[0]), repr(params[1]), repr(params[2])]
     env = dict(os.environ, OMP_NUM_THREADS="8")
+    if fsi is not None:
+        env["ENSC_FSI"] = ",".join(repr(v) for v in fsi)
     r = subprocess.run(args, capture_output=True, text=True, env=env)
     if r.returncode != 0:
         raise RuntimeError(r.stderr[-2000:])
@@ -139,11 +141,67 @@ def micro_multistep() -> Model:
     return m
 
 
-def main() -> None:
+# FSI coupling cases: (deck builder, layers, dist as a fraction of the
+# element size, force scale in N).  The harness generates the particles
+# with the reference's own code before the first step and sets force
+# scale * sin(0.37 (p + 1) + 1.1 d + 0.05 n) on particle p before step n.
+def micro_fsi_block() -> Model:
+    """3x2x2 block, x = 0 face fixed: every FSI face is an outer face."""
+    x0, conn, ijk = decks.block_mesh(3, 2, 2, 0.03, 0.02, 0.02)
+    fixed = decks.face_nodes(ijk, 0, 0, 3)
+    m = Model(x0=x0, conn=conn, E=200e9, nu=0.3, rho=7800.0, name="micro_fsi_block")
+    m.node_sets = {"Set-1": fixed}
+    m.v_init = np.zeros_like(x0)
+    T = 150 * decks.stable_dt0(m)
+    m.steps = [Step("Initial", 0.0, spc=[Bc("spc", fixed, 0, 2, 1, name="Set-1")]), Step("Step-1", T)]
+    m.total_time = T
+    m.fsi = (2, 0.25 * 0.01, 50.0)  # type: ignore[attr-defined]
+    return m
+
+
+def micro_fsi_slab() -> Model:
+    """3x3x1 jittered slab moving with a linear velocity field, one edge
+    fixed: every node is a boundary node, so every face (inner ones too)
+    is an FSI face; distorted elements make J n differ from J^T n."""
+    x0, conn, ijk = decks.block_mesh(3, 3, 1, 0.03, 0.03, 0.01, jitter=0.15, seed=3)
+    edge = np.nonzero((ijk[:, 0] == 0) & (ijk[:, 1] == 0))[0].astype(np.uint32)
+    m = Model(x0=x0, conn=conn, E=70e9, nu=0.33, rho=2700.0, name="micro_fsi_slab")
+    free = np.setdiff1d(np.arange(x0.shape[0], dtype=np.uint32), edge)
+    m.node_sets = {"Set-1": edge}
+    v = np.zeros_like(x0)
+    v[free, 2] = 2.0 * x0[free, 0] / 0.03 - 0.5 * x0[free, 1] / 0.03
+    v[free, 0] = 0.3
+    for i, n in enumerate(free):
+        m.node_sets[f"Set-{i + 2}"] = np.array([n], dtype=np.uint32)
+    m.v_init = v
+    m.ic = [(f"Set-{i + 2}", d, float(v[n, d])) for i, n in enumerate(free) for d in range(3) if v[n, d] != 0.0]
+    T = 120 * decks.stable_dt0(m)
+    m.steps = [Step("Initial", 0.0, spc=[Bc("spc", edge, 0, 2, 1, name="Set-1")]), Step("Step-1", T)]
+    m.total_time = T
+    m.fsi = (3, 0.2 * 0.01, 20.0)  # type: ignore[attr-defined]
+    return m
+
+
+def main(only=None) -> None:
     if not os.path.exists(HARNESS):
         subprocess.run(["make", "-C", os.path.join(ROOT, "oracle"), "ref"], check=True)
     os.makedirs(DECKDIR, exist_ok=True)
     tmp = tempfile.mkdtemp(prefix="decks_")
+    # ---- FSI coupling (virtual particles) ----
+    for mk in (micro_fsi_block, micro_fsi_slab):
+        mm = mk()
+        if only and mm.name not in only:
+            continue
+        p = os.path.join(tmp, mm.name + ".inp")
+        decks.write_inp(mm, p)
+        with open(p, "rb") as fh, gzip.open(os.path.join(DECKDIR, mm.name + ".inp.gz"), "wb") as gz:
+            gz.write(fh.read())
+        d = run_ref(p, 0, (), (), (mm.c_dr, mm.c_cr, mm.factor), fsi=mm.fsi)
+        d["fsi"] = np.array(mm.fsi)
+        save(mm.name, d, ["model_json", "x0", "conn", "history", "fsi", "final_", "setup_"])
+    if only:
+        shutil.rmtree(tmp)
+        return
     # ---- configuration 1 ----
     m = decks.config1()
     p = os.path.join(tmp, "c1.inp")
@@ -181,4 +239,4 @@ def main() -> None:
 
 
 if __name__ == "__main__":
-    main()
+    main(sys.argv[sys.argv.index("--only") + 1:] if "--only" in sys.argv else None)

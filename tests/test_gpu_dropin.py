@@ -32,18 +32,21 @@ def rel(a, b, floor=0.0):
     return float(np.abs(a - b).max()) / s if s > 0 else float(np.abs(a - b).max())
 
 
-def run_dropin(case, precision, tmp_path, stepwise=False):
+def run_dropin(case, precision, tmp_path, stepwise=False, fsi=None, deck=None):
     if not os.path.exists(DROPIN):
         pytest.skip("oracle/_ref/ref_dropin not built (needs the reference sources: make -C oracle shim)")
     d = G.load(case)
     j = json.loads(str(d["model_json"]))
     tmp_path.mkdir(parents=True, exist_ok=True)
-    with gzip.open(os.path.join(G.GOLDEN, "decks", DECKS[case] + ".inp.gz"), "rt") as f:
+    with gzip.open(os.path.join(G.GOLDEN, "decks", (deck or DECKS[case]) + ".inp.gz"), "rt") as f:
         (tmp_path / "deck.inp").write_text(f.read())
     args = [DROPIN, str(tmp_path), "deck.inp", str(tmp_path), precision,
             repr(j["c_dr"]), repr(j["c_cr"]), repr(j["factor"])] + (["stepwise"] if stepwise else [])
-    print(f"[dropin] {case} {precision}{' stepwise' if stepwise else ''}", flush=True)
-    r = subprocess.run(args, capture_output=True, text=True, timeout=300)
+    print(f"[dropin] {case} {precision}{' stepwise' if stepwise else ''}{' fsi' if fsi else ''}", flush=True)
+    env = dict(os.environ)
+    if fsi is not None:
+        env["ENSC_FSI"] = ",".join(repr(float(v)) if i else str(int(v)) for i, v in enumerate(fsi))
+    r = subprocess.run(args, capture_output=True, text=True, timeout=300, env=env)
     assert r.returncode == 0, r.stderr[-2000:]
     sc = open(tmp_path / "scalars.txt").read().split()
     steps = int(sc[sc.index("steps") + 1])
@@ -111,3 +114,22 @@ def test_stepwise_binding_hooks_every_step(tmp_path):
     assert steps2 == steps == int(d["n_steps"])
     assert np.array_equal(u2, u) and np.array_equal(s2, s) and np.array_equal(x2, x)
     assert rel(u2, d["final_u"], 1e-12) <= 1e-10
+
+
+@pytest.mark.parametrize("case", ["micro_fsi_block", "micro_fsi_slab"])
+def test_dropin_fsi_coupler_matches_the_reference(case, tmp_path):
+    # an FSI coupler written against the reference's interface (a subclass
+    # that calls the particle setup at its first step and writes
+    # fsi_share_data's forces before every step) runs unchanged on the
+    # binding: the state and the particles equal the reference's own
+    # coupling run of the same subclass
+    d, u, s, x, steps = run_dropin(case, "fp64", tmp_path, stepwise=True, fsi=tuple(d for d in G.load(case)["fsi"]),
+                                   deck=case)
+    assert steps == d["history"].shape[0]
+    assert rel(u, d["final_u"], 1e-12) <= 1e-10
+    assert rel(x, d["final_x"]) <= 1e-10
+    assert rel(s, d["final_sigma"], 1.0) <= 1e-10
+    for f in ("fsi_x", "fsi_v"):
+        got = np.load(tmp_path / (f + ".npy"))
+        assert got.shape == d["final_" + f].shape
+        assert rel(got, d["final_" + f]) <= 1e-10, f

@@ -13,6 +13,7 @@ import numpy as np
 import pytest
 
 import energy_cases as EC
+import fsi_cases as FC
 import goldens as G
 from enscdynamics_amd import decks
 from enscdynamics_amd.model import Bc, Dsload, Model, Step
@@ -436,3 +437,178 @@ def test_pressure_face_numbering(face):
     P = ((1.0 / o.get("invM"))[:, None] * o.get("v")).sum(axis=0)
     J = p * 1e-4 * (0.5 * (h[:, 1] + h[:, 2])).sum()
     assert np.allclose(P, J * np.array(INWARD[face], dtype=float), rtol=0, atol=1e-4 * J)
+
+
+# ---------------------------------------------------------------------------
+# FSI virtual particles (exDyna3D.cpp:1281-1758): the reference's own
+# coupling run (fixtures from the compiled reference), then closed forms
+# ---------------------------------------------------------------------------
+
+@pytest.mark.parametrize("case", FC.CASES)
+def test_oracle_fsi_matches_reference(case):
+    d, m, layers, dist, scale = FC.load(case)
+    o = Oracle(m, 1).init()
+    x0, v0, steps = FC.run_oracle(o, layers, dist, scale)
+    assert steps == d["history"].shape[0]
+    assert x0.shape == d["setup_fsi_x"].shape
+    assert np.abs(x0 - d["setup_fsi_x"]).max() <= 1e-14 * np.abs(d["setup_fsi_x"]).max()
+    assert np.abs(v0 - d["setup_fsi_v"]).max() <= 1e-14 * max(np.abs(d["setup_fsi_v"]).max(), 1e-30)
+    x1, v1 = o.fsi_particles()
+    for name, got in (("u", o.get("u")), ("v", o.get("v")), ("x", o.get("x")), ("sigma", o.get("sigma")),
+                      ("fsi_x", x1), ("fsi_v", v1)):
+        ref = d["final_" + name]
+        err = np.abs(got - ref).max() / max(np.abs(ref).max(), 1e-30)
+        assert err <= 1e-10, (name, err)
+
+
+def _box(nx, ny, nz, h=0.01, v0=None):
+    x0, conn, ijk = decks.block_mesh(nx, ny, nz, nx * h, ny * h, nz * h)
+    m = Model(x0=x0, conn=conn, **EC.STEEL, name="fsi_box")
+    m.v_init = np.zeros_like(x0) if v0 is None else v0(x0)
+    T = 30 * decks.stable_dt0(m)
+    m.steps = [Step("Initial", 0.0), Step("Step-1", T)]
+    m.total_time = T
+    return m
+
+
+def test_fsi_particles_on_a_box_closed_form():
+    """On a 3x2x2 box of h-cubes the FSI faces are the outer faces and the
+    particles of a face sit at its centre + l * dist along the inward normal,
+    l = 0..layers-1, ordered by element, then face (unit faces xi-, xi+,
+    eta-, eta+, zeta-, zeta+), then layer."""
+    nx, ny, nz, h, L, dist = 3, 2, 2, 0.01, 3, 0.002
+    m = _box(nx, ny, nz, h)
+    o = Oracle(m, 1).init()
+    n = o.fsi_setup(L, dist)
+    assert n == L * 2 * (nx * ny + ny * nz + nz * nx)
+    x, v = o.fsi_particles()
+    want = []
+    for k in range(nz):
+        for j in range(ny):
+            for i in range(nx):
+                c = (np.array([i, j, k]) + 0.5) * h
+                for ax, side, nmax in ((0, 0, nx), (0, 1, nx), (1, 0, ny), (1, 1, ny), (2, 0, nz), (2, 1, nz)):
+                    idx = (i, j, k)[ax]
+                    if (side == 0 and idx != 0) or (side == 1 and idx != nmax - 1):
+                        continue
+                    nrm = np.zeros(3)
+                    nrm[ax] = 1.0 if side == 0 else -1.0
+                    fc = c.copy()
+                    fc[ax] = idx * h if side == 0 else (idx + 1) * h
+                    for layer in range(L):
+                        want.append(fc + layer * dist * nrm)
+    assert np.allclose(x, np.array(want), rtol=0, atol=1e-15)
+    assert not v.any()
+
+
+def test_fsi_thin_slab_counts_every_face():
+    """The boundary test is by node: in a one-element-thick slab every node
+    is held by fewer than 8 elements, so every face of every element
+    (inner faces included) carries particles (exDyna3D.cpp:1290-1354)."""
+    m = _box(3, 3, 1)
+    o = Oracle(m, 1).init()
+    assert o.fsi_setup(2, 0.001) == 2 * 6 * 9
+
+
+def test_fsi_particles_follow_linear_fields():
+    """The particles interpolate the nodal fields with the trilinear shape
+    functions, which reproduce fields linear in the position on any hex: on
+    a jittered mesh moving with v = A x + b, every particle has v = A x_p + b
+    (x_p itself being the interpolated position)."""
+    x0, conn, _ = decks.block_mesh(3, 3, 2, 0.03, 0.03, 0.02, jitter=0.2, seed=7)
+    A = np.array([[0.3, -1.0, 0.2], [1.0, 0.1, -0.4], [0.5, 0.7, -0.2]]) * 10.0
+    b = np.array([0.4, -0.2, 0.9])
+    m = Model(x0=x0, conn=conn, **EC.STEEL, name="fsi_linear")
+    m.v_init = x0 @ A.T + b
+    T = 20 * decks.stable_dt0(m)
+    m.steps = [Step("Initial", 0.0), Step("Step-1", T)]
+    m.total_time = T
+    o = Oracle(m, 1).init()
+    n = o.fsi_setup(2, 0.002)
+    assert n > 0
+    x, v = o.fsi_particles()
+    assert np.allclose(v, x @ A.T + b, rtol=0, atol=1e-13 * np.abs(v).max())
+
+
+def test_fsi_rigid_drift():
+    """Uniform velocity, no loads: the particles keep that velocity and move
+    with the nodes' displacement."""
+    vel = np.array([1.5, -0.5, 2.0])
+    m = _box(3, 2, 2, v0=lambda x0: np.tile(vel, (x0.shape[0], 1)))
+    o = Oracle(m, 1).init()
+    o.fsi_setup(2, 0.002)
+    x0p, v0p = o.fsi_particles()
+    o.run(0, [], 0)
+    x1, v1 = o.fsi_particles()
+    u = o.get("u")
+    assert np.abs(u - u[0]).max() <= 1e-15
+    assert np.allclose(v1, vel, rtol=1e-14, atol=0)
+    assert np.allclose(x1 - x0p, u[0], rtol=0, atol=1e-15)
+
+
+def test_fsi_forces_on_the_nodes():
+    """From rest the first step's nodal forces are the particles' only:
+    m v / ((dt_i + dt_i_1) / 2) per node.  Their sum is the particles' total
+    force (the shape functions sum to 1), their moment is sum x_p x f_p
+    (linear precision), and a node receives sum_p N_node(xi_p) f_p, xi_p the
+    particle's unit coordinates in its box element."""
+    nx, ny, nz, h = 3, 2, 2, 0.01
+    m = _box(nx, ny, nz, h)
+    o = Oracle(m, 1).init()
+    o.begin_step()
+    n = o.fsi_setup(2, 0.003)
+    xp, _ = o.fsi_particles()
+    rng = np.random.default_rng(11)
+    f = rng.normal(size=(n, 3)) * 40.0
+    o.fsi_set_forces(f)
+    taken, hist = o.run(1, [0], 1)
+    assert taken == 1
+    hs2 = 0.5 * (hist[0, 1] + hist[0, 2])
+    F = (1.0 / o.get("invM"))[:, None] * o.get("v") / hs2
+    x0 = m.x0
+    assert np.allclose(F.sum(axis=0), f.sum(axis=0), rtol=1e-12, atol=1e-12 * np.abs(f).sum())
+    assert np.allclose(np.cross(x0, F).sum(axis=0), np.cross(xp, f).sum(axis=0), rtol=1e-11,
+                       atol=1e-12 * np.abs(np.cross(xp, f)).sum())
+    # explicit shape functions: particle p lies in the element whose box
+    # holds it (faces sit on element boundaries: take the element by order)
+    want = np.zeros_like(F)
+    corners = decks.CORNERS * 2 - 1
+    per_face = 2
+    p = 0
+    for k in range(nz):
+        for j in range(ny):
+            for i in range(nx):
+                e = i + nx * (j + ny * k)
+                nface = sum(1 for ax, side, nmax in ((0, 0, nx), (0, 1, nx), (1, 0, ny), (1, 1, ny), (2, 0, nz), (2, 1, nz))
+                            if (side == 0 and (i, j, k)[ax] == 0) or (side == 1 and (i, j, k)[ax] == nmax - 1))
+                lo = np.array([i, j, k]) * h
+                for q in range(nface * per_face):
+                    xi = 2.0 * (xp[p] - lo) / h - 1.0
+                    N = 0.125 * np.prod(1.0 + corners * xi, axis=1)
+                    want[m.conn[e]] += N[:, None] * f[p]
+                    p += 1
+    assert p == n
+    assert np.allclose(F, want, rtol=1e-10, atol=1e-12 * np.abs(want).max())
+
+
+def test_fsi_energy_balance():
+    """The particles' forces do work, counted in W_ext.  The balance
+    residual on a cantilevered block under a steady particle load is the
+    scheme's own O(dt) error (the element pass estimates the hourglass
+    dissipation with the start-of-step velocity, and this load excites the
+    hourglass modes): it halves with the time step, which a missing or
+    mis-scaled work term would not."""
+    d, m, layers, dist, scale = FC.load("micro_fsi_block")
+    res = []
+    for fac in (0.9, 0.225):
+        m.factor = fac
+        o = Oracle(m, 1).init()
+        o.begin_step()
+        n = o.fsi_setup(layers, dist)
+        o.fsi_set_forces(np.tile([0.0, 3.0, -5.0], (n, 1)))
+        o.run(0, [], 0)
+        s = o.scalars()
+        assert s["wext"] > 0.0
+        res.append(abs(EC.balance(s)) / s["wext"])
+    assert res[0] <= 0.025 and res[1] <= 0.0075
+    assert res[0] / res[1] >= 3.0
