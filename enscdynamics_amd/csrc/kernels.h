@@ -1058,7 +1058,7 @@ __global__ __launch_bounds__(PIPE_BLK) __attribute__((amdgpu_waves_per_eu(WPE)))
 // ASM: the launch covers block-assembly jobs only (mixed engine) and sums
 // their block partials; otherwise the per-corner forces through the ELL.
 template <bool MIXED, int BLK, bool ASM>
-__global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(ASM ? (MIXED ? 8 : 4) : 1))) void k_node(
+__global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(ASM ? (MIXED ? 7 : 4) : 1))) void k_node(
     const JobDev* __restrict__ jobs, const uint4* __restrict__ blkmap) {
 #pragma clang fp contract(off)
     const uint4 bm = blkmap[blockIdx.x];  // (job, block of the job, -, -)
