@@ -490,7 +490,11 @@ ensc_status prepare_job(Job& J, int engine_precision, int reorder, int blk_e, in
     }
     I.conn_d = std::move(conn_d);
     I.ell = std::move(ell);
-    I.lconn = std::move(lconn);
+    // device layout: corners 2p and 2p + 1 of an element side by side, so
+    // one 32-bit load reads a pair ([4][ne_s] words)
+    I.lconn.assign(8 * es, 0);
+    for (size_t k = 0; k < 8; ++k)
+        for (size_t e = 0; e < es; ++e) I.lconn[((k >> 1) * es + e) * 2 + (k & 1)] = lconn[k * es + e];
     I.blk_ptr = std::move(blk_ptr);
     I.blk_nodes = std::move(blk_nodes);
     I.pos_ptr = std::move(pos_ptr);

@@ -87,7 +87,8 @@ struct JobDev {
     void* fel;             // [8][ne_s][3] element nodal forces (corner-major, packed xyz);
                            //   gathered assembly only
     // LDS staging of each element block's nodes (null when not staged)
-    const uint16_t* lconn;     // [8][ne_s] corner -> index in the block's node list
+    const uint16_t* lconn;     // [4][ne_s][2] corner -> index in the block's node list, corners
+                               //   2p and 2p + 1 of an element side by side (one 32-bit word)
     const uint32_t* blk_ptr;   // [nblk_e + 1] offsets into blk_nodes
     const uint32_t* blk_nodes; // node ids of each element block, ascending
     // block assembly: each element block sums its elements' corner forces

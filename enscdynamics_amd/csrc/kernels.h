@@ -146,7 +146,11 @@ __device__ __forceinline__ void walsh8_expand(const R c[8], R f[8]) {
 // the block's node staging, so the two sets of loads overlap).
 template <class R>
 struct ElemState {
-    uint32_t nd[8];  // corner node ids (gathered path) or LDS slots (staged path)
+    // gathered path: the corner node ids; staged path: the corners' LDS slots
+    // in pairs, corners 2p (low 16 bits) and 2p + 1 (high) in nd[p], so the
+    // pipelined pass prefetches them as they are stored and unpacks them at
+    // use (NODE(k))
+    uint32_t nd[8];
     R Gi[3][3];      // FP64 engine only (the mixed body rebuilds Gi and V0)
     R V0;
     R s[6];
@@ -157,8 +161,14 @@ struct ElemState {
 template <class R, bool STAGED>
 __device__ __forceinline__ void load_state(const JobDev& J, uint32_t e, bool full, ElemState<R>& S) {
     const uint32_t es = J.ne_s;
+    if (STAGED) {
+        const uint32_t* lc = reinterpret_cast<const uint32_t*>(J.lconn);
 #pragma unroll
-    for (int k = 0; k < 8; ++k) S.nd[k] = STAGED ? (uint32_t)gld(J.lconn, k * es + e) : gld(J.conn, k * es + e);
+        for (int p = 0; p < 4; ++p) S.nd[p] = gld(lc, p * es + e);
+    } else {
+#pragma unroll
+        for (int k = 0; k < 8; ++k) S.nd[k] = gld(J.conn, k * es + e);
+    }
     if (!full) return;
     const R* Sp = static_cast<const R*>(J.sigma);
     const R* Qp = static_cast<const R*>(J.qia);
@@ -439,20 +449,21 @@ __device__ __noinline__ V3 bc_override(const JobDev& J, uint32_t g, double t, V3
     const JobDev &J, uint32_t e, const ElemState<R> &ES, double dt, int32_t kind, double time_n, bool act, \
         bool asmode, double &Le, double &dts, double *en, uint32_t &errf, const ENSC_LDS double *lds,     \
         ENSC_LDS R *acc
+#define NODE(k) ES.nd[k]
 #define NVAL(k, i)                                                                                      \
-    ((i) < 3 ? (MIXED ? gld((const double*)J.pos, ES.nd[k] * POS_REC + (i)) +                           \
-                            gld((const double*)J.vu, ES.nd[k] * VU_REC + 3 + (i))                           \
-                      : gld((const double*)J.pos, ES.nd[k] * POS_REC + (i)))                              \
-             : gld((const double*)J.vu, ES.nd[k] * VU_REC + (i)-3))
+    ((i) < 3 ? (MIXED ? gld((const double*)J.pos, NODE(k) * POS_REC + (i)) +                           \
+                            gld((const double*)J.vu, NODE(k) * VU_REC + 3 + (i))                           \
+                      : gld((const double*)J.pos, NODE(k) * POS_REC + (i)))                              \
+             : gld((const double*)J.vu, NODE(k) * VU_REC + (i)-3))
 #define DYN_BC_OVERRIDE(k, v)                                              \
     if (J.dyn_bc && kind == KIND_STEP) {                                   \
-        const V3 o_ = bc_override(J, ES.nd[k], time_n, V3{v[0], v[1], v[2]}); \
+        const V3 o_ = bc_override(J, NODE(k), time_n, V3{v[0], v[1], v[2]}); \
         v[0] = o_.x;                                                       \
         v[1] = o_.y;                                                       \
         v[2] = o_.z;                                                       \
     }
 #define CORNER_V(k, i) \
-    ((J.dyn_bc && kind == KIND_STEP) ? bc_override_c(J, ES.nd[k], time_n, (i), NVAL(k, 3 + (i))) : NVAL(k, 3 + (i)))
+    ((J.dyn_bc && kind == KIND_STEP) ? bc_override_c(J, NODE(k), time_n, (i), NVAL(k, 3 + (i))) : NVAL(k, 3 + (i)))
 template <class R>
 __device__ __forceinline__ void element_fp64_G(ELEM_ARGS) {
 #pragma clang fp contract(off)
@@ -465,14 +476,16 @@ __device__ __forceinline__ void element_mixed_G(ELEM_ARGS) {
     constexpr bool MIXED = true, ASM_CAP = false;
 #include "elem_body_mixed.inc"
 }
+#undef NODE
 #undef NVAL
 #undef DYN_BC_OVERRIDE
 #undef CORNER_V
 
 // Staged record in LDS: [x0 x1 x2 v0 v1 v2 u0 u1 u2] as copied from the node
 // record (x0..2 = the record's X).
+#define NODE(k) (((k)&1) ? (ES.nd[(k) >> 1] >> 16) : (ES.nd[(k) >> 1] & 0xFFFFu))
 #define NVAL(k, i) \
-    ((MIXED && (i) < 3) ? lds[ES.nd[k] * 9 + (i)] + lds[ES.nd[k] * 9 + 6 + (i)] : lds[ES.nd[k] * 9 + (i)])
+    ((MIXED && (i) < 3) ? lds[NODE(k) * 9 + (i)] + lds[NODE(k) * 9 + 6 + (i)] : lds[NODE(k) * 9 + (i)])
 #define DYN_BC_OVERRIDE(k, v)  // applied once per node while staging
 #define CORNER_V(k, i) NVAL(k, 3 + (i))
 template <class R>
@@ -487,6 +500,7 @@ __device__ __forceinline__ void element_mixed_S(ELEM_ARGS) {
     constexpr bool MIXED = true, ASM_CAP = true;
 #include "elem_body_mixed.inc"
 }
+#undef NODE
 #undef NVAL
 #undef DYN_BC_OVERRIDE
 #undef CORNER_V
@@ -889,8 +903,9 @@ __device__ __forceinline__ void pipe_prefetch(PipeNext& p, ElemState<float>& ESn
 #define PX(n, i) ((i) == 0 ? buf[0][n].a : ((i) == 1 ? buf[0][n].b : buf[1][n].a))
 #define PV(n, i) ((i) == 0 ? buf[2][n].a : ((i) == 1 ? buf[2][n].b : buf[3][n].a))
 #define PU(n, i) ((i) == 0 ? buf[3][n].b : ((i) == 1 ? buf[4][n].a : buf[4][n].b))
+#define NODE(k) (((k)&1) ? (ES.nd[(k) >> 1] >> 16) : (ES.nd[(k) >> 1] & 0xFFFFu))
 #define NVAL(k, i) \
-    ((i) < 3 ? PX(ES.nd[k], (i)) + PU(ES.nd[k], (i)) : ((i) < 6 ? PV(ES.nd[k], (i)-3) : PU(ES.nd[k], (i)-6)))
+    ((i) < 3 ? PX(NODE(k), (i)) + PU(NODE(k), (i)) : ((i) < 6 ? PV(NODE(k), (i)-3) : PU(NODE(k), (i)-6)))
 #define CORNER_V(k, i) NVAL(k, 3 + (i))
 #define ACC_W 4
 #define ASM_BEGIN pipe_acc_clear(acc, nbn)
@@ -905,6 +920,7 @@ __device__ __forceinline__ void element_mixed_P(const JobDev& J, uint32_t e, con
     ENSC_LDS float* acc = pipe_acc(buf);
 #include "elem_body_mixed.inc"
 }
+#undef NODE
 #undef NVAL
 #undef CORNER_V
 #undef PX
