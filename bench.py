@@ -321,7 +321,7 @@ def c5_section(args):
     # ratio compares device work alone
     t0 = time.perf_counter()
     lat_seq = []
-    for m in models:
+    for i, m in enumerate(models):
         a = time.perf_counter()
         e = eng.Engine(m, eng.MIXED, reorder=True)
         if args.graph:
@@ -331,6 +331,8 @@ def c5_section(args):
         assert s["err"] == 0.0
         e.close()
         lat_seq.append(time.perf_counter() - a)
+        if (i + 1) % 200 == 0 and i + 1 < len(models):
+            log(f"c5: one at a time (build inside the clock): {i + 1} of {len(models)} done")
     secs_seq = time.perf_counter() - t0
     log(f"c5: one at a time {secs_seq:.3f} s (engine build inside the clock)")
     secs_pre, lat_pre = 0.0, []
@@ -351,6 +353,8 @@ def c5_section(args):
             lat_pre.append(dt)
         for e in group:
             e.close()
+        if (g0 + 40) % 200 == 0 and g0 + 40 < len(models):
+            log(f"c5: one at a time (prebuilt): {g0 + 40} of {len(models)} done")
     log(f"c5: one at a time {secs_pre:.3f} s (engines built before the clock)")
     return {"workload": f"config 5: {len(models)} requests from the config-4 generator, Ne log-uniform "
                         f"[1e3, 5e5], half J2, mixed, admitted by HBM footprint (budget {budget >> 30} GiB, "
