@@ -63,7 +63,8 @@ struct ensc_engine {
     int device = 0, precision = ENSC_FP64, blk_e = 256, blk_n = 256, reorder = 0, staging_opt = 0, waves = 2;
     int fp64_rule = 0;  // ensc_options.fp64_jobs (per-job precision of a mixed engine)
     int fp64_order = 0; // ensc_options.fp64_order (0: block partials, 1: the reference's order)
-    int waves64 = 1;    // occupancy target of the FP64 staged element kernel
+    int waves64 = 1;    // occupancy target of the FP64 staged element kernel (reference-order body)
+    int waves_w64 = 2;  // occupancy target of the FP64 block-assembly element kernel
     bool pipe_opt = true;  // staging option 0: pipelined element pass where it applies
     hipStream_t stream = nullptr;
     // second stream for the staged element blocks of a tick that also runs
@@ -93,11 +94,13 @@ struct ensc_engine {
     char* scratch_base = nullptr;
     uint32_t nblk_pipe = 0, nblk_e = 0, nblk_g = 0;  // element blocks: pipelined, staged, gathered (mixed)
     uint32_t nblk_e64 = 0, nblk_g64 = 0;             // element blocks: staged, gathered (FP64)
+    uint32_t nblk_w64 = 0;  // element blocks of FP64 block-assembly jobs (Walsh-form body)
     uint32_t nblk_n_asm = 0, nblk_n = 0;      // node blocks, mixed jobs: block partials, per-corner forces
     uint32_t nblk_n64_asm = 0, nblk_n64 = 0;  // node blocks, FP64 jobs: block partials, per-corner forces
     bool dirty = true;
     KernelFn ke = nullptr, ke_g = nullptr;      // k_element staged / gathered, mixed jobs
     KernelFn ke64 = nullptr, ke64_g = nullptr;  // the same, FP64 jobs
+    KernelFn ke64w = nullptr;                   // FP64 block-assembly jobs
     KernelFn kn = nullptr, kn_asm = nullptr;      // k_node, mixed jobs: per-corner / block partials
     KernelFn kn64 = nullptr, kn64_asm = nullptr;  // k_node, FP64 jobs
     bool staged = false;      // some live job stages its nodes in LDS
@@ -105,7 +108,7 @@ struct ensc_engine {
     uint32_t n_fp64 = 0;      // live jobs running in FP64
     uint32_t pipe_grid = 0;   // the pipelined kernel's block count (pipe_per_cu per CU)
     int ncu = 0, pipe_per_cu = 2;
-    size_t lds_bytes = 0, lds_bytes64 = 0;  // dynamic LDS of the staged k_element (mixed, FP64)
+    size_t lds_bytes = 0, lds_bytes64 = 0, lds_bytes_w64 = 0;  // dynamic LDS of the staged k_element (mixed, FP64, FP64 block assembly)
     // output snapshots of a single-job engine: files go to out_dir (empty:
     // the caller drains them), numbered from out_count
     std::string out_dir;
@@ -154,6 +157,22 @@ KernelFn pick_e(int blk, bool staged, int waves) {
         case 1: return pick_staged<M, 1>(blk);
         case 3: return pick_staged<M, 3>(blk);
         default: return pick_staged<M, 2>(blk);
+    }
+}
+// Element kernel of FP64 block-assembly jobs (the Walsh-form FP64 body).
+template <int W>
+KernelFn pick_w64_w(int blk) {
+    switch (blk) {
+        case 64: return k_element<false, 64, true, W, true>;
+        case 128: return k_element<false, 128, true, W, true>;
+        default: return k_element<false, 256, true, W, true>;
+    }
+}
+KernelFn pick_w64(int blk, int waves) {
+    switch (waves) {
+        case 1: return pick_w64_w<1>(blk);
+        case 3: return pick_w64_w<3>(blk);
+        default: return pick_w64_w<2>(blk);
     }
 }
 // Node kernel for (precision, block size, force source): ASM sums the block
@@ -264,11 +283,11 @@ ensc_status relayout(ensc_engine* E) {
     };
     // block maps: (job, block of the job, staged range) per launched block,
     // in XCD order, grouped as described at ensc_engine::live
-    std::vector<uint4> lp, ls, lg, ls64, lg64, lna, ln, lna64, ln64, lf;
+    std::vector<uint4> lp, ls, lg, ls64, lw64, lg64, lna, ln, lna64, ln64, lf;
     std::vector<JobDev> tab;
     std::vector<size_t> pe0, pn0;
     size_t ne_blocks = 0, n_blocks = 0;
-    uint32_t lds_nodes = 0, lds_nodes64 = 0;
+    uint32_t lds_nodes = 0, lds_nodes64 = 0, lds_nodes_w64 = 0;
     bool any_asm = false, any_asm64 = false;
     E->n_asm = 0;
     E->n_fp64 = 0;
@@ -283,7 +302,7 @@ ensc_status relayout(ensc_engine* E) {
             const uint32_t nodes = J.blk_ptr_h[b + 1] - J.blk_ptr_h[b];
             std::vector<uint4>* le;
             if (!m)
-                le = staged ? &ls64 : &lg64;
+                le = as ? &lw64 : (staged ? &ls64 : &lg64);
             else
                 le = (pipe && nodes <= (uint32_t)PIPE_CAP) ? &lp : (staged ? &ls : &lg);
             if (le == &ls) {
@@ -292,6 +311,8 @@ ensc_status relayout(ensc_engine* E) {
             } else if (le == &ls64) {
                 lds_nodes64 = std::max(lds_nodes64, nodes);
                 any_asm64 |= as;
+            } else if (le == &lw64) {
+                lds_nodes_w64 = std::max(lds_nodes_w64, nodes);
             }
             le->push_back(make_uint4(i, b, J.blk_ptr_h[b], nodes));
         }
@@ -305,7 +326,7 @@ ensc_status relayout(ensc_engine* E) {
         E->n_fp64 += !m;
     }
     std::vector<uint4> b2e, b2n;
-    for (const std::vector<uint4>* l : {&lp, &ls, &lg, &ls64, &lg64}) {
+    for (const std::vector<uint4>* l : {&lp, &ls, &lg, &ls64, &lw64, &lg64}) {
         const std::vector<uint4> r = xcd_order(*l);
         b2e.insert(b2e.end(), r.begin(), r.end());
     }
@@ -317,6 +338,7 @@ ensc_status relayout(ensc_engine* E) {
     E->nblk_e = (uint32_t)ls.size();
     E->nblk_g = (uint32_t)lg.size();
     E->nblk_e64 = (uint32_t)ls64.size();
+    E->nblk_w64 = (uint32_t)lw64.size();
     E->nblk_g64 = (uint32_t)lg64.size();
     E->nblk_n_asm = (uint32_t)lna.size();
     E->nblk_n = (uint32_t)ln.size();
@@ -379,13 +401,15 @@ ensc_status relayout(ensc_engine* E) {
     if (!lf.empty())
         HIPCHK(hipMemcpyAsync(E->b2f_p, lf.data(), lf.size() * sizeof(uint4), hipMemcpyHostToDevice, E->stream));
     HIPCHK(hipStreamSynchronize(E->stream));
-    E->staged = E->nblk_pipe + E->nblk_e + E->nblk_e64 > 0;
+    E->staged = E->nblk_pipe + E->nblk_e + E->nblk_e64 + E->nblk_w64 > 0;
     E->lds_bytes = (size_t)std::max<uint32_t>(lds_nodes, 1) * (any_asm ? 9 * sizeof(double) + 12 : 9 * sizeof(double));
     E->lds_bytes64 = (size_t)std::max<uint32_t>(lds_nodes64, 1) * (9 * sizeof(double) + (any_asm64 ? 24 : 0));
+    E->lds_bytes_w64 = (size_t)std::max<uint32_t>(lds_nodes_w64, 1) * (9 * sizeof(double) + 24);
     E->ke = pick_e<true>(E->blk_e, true, E->waves);
     E->ke_g = pick_e<true>(E->blk_e, false, E->waves);
     E->ke64 = pick_e<false>(E->blk_e, true, E->waves64);
     E->ke64_g = pick_e<false>(E->blk_e, false, E->waves64);
+    E->ke64w = pick_w64(E->blk_e, E->waves_w64);
     E->kn = pick_n<true, false>(E->blk_n);
     E->kn_asm = pick_n<true, true>(E->blk_n);
     E->kn64 = pick_n<false, false>(E->blk_n);
@@ -416,7 +440,7 @@ hipError_t launch_phase(ensc_engine* E, int phase) {
             // blocks launched beside it (on the side stream) start as its
             // blocks finish and fill its last round instead of following it.
             // The passes cover disjoint element blocks of the tick.
-            const bool fork = E->nblk_pipe && (E->nblk_e || E->nblk_g || E->nblk_e64 || E->nblk_g64);
+            const bool fork = E->nblk_pipe && (E->nblk_e || E->nblk_g || E->nblk_e64 || E->nblk_w64 || E->nblk_g64);
             hipStream_t s2 = E->stream;
             if (fork) {
                 chk(hipEventRecord(E->ev_fork, E->stream));
@@ -436,6 +460,9 @@ hipError_t launch_phase(ensc_engine* E, int phase) {
             if (E->nblk_e64)
                 hipLaunchKernelGGL(E->ke64, dim3(E->nblk_e64), dim3(E->blk_e), E->lds_bytes64, s2, jobs, map);
             map += E->nblk_e64;
+            if (E->nblk_w64)
+                hipLaunchKernelGGL(E->ke64w, dim3(E->nblk_w64), dim3(E->blk_e), E->lds_bytes_w64, s2, jobs, map);
+            map += E->nblk_w64;
             if (E->nblk_g64) hipLaunchKernelGGL(E->ke64_g, dim3(E->nblk_g64), dim3(E->blk_e), 0, s2, jobs, map);
             if (fork) {
                 chk(hipEventRecord(E->ev_join, E->side));
@@ -937,7 +964,7 @@ ensc_status do_advance(ensc_engine* E, int64_t max_steps, int32_t* done) {
         if (n_live_blocks && done_blocks * 4 > n_live_blocks) E->dirty = true;
         if (E->dirty) {
             if ((st = relayout(E))) return st;
-            n_live_blocks = (size_t)E->nblk_pipe + E->nblk_e + E->nblk_g + E->nblk_e64 + E->nblk_g64;
+            n_live_blocks = (size_t)E->nblk_pipe + E->nblk_e + E->nblk_g + E->nblk_e64 + E->nblk_w64 + E->nblk_g64;
         }
         int64_t n = chunk;
         if (output_engine(E)) {
@@ -1279,7 +1306,7 @@ ensc_status autotune(ensc_engine* E, int32_t ticks, int64_t* chosen) {
     if (E->live.empty()) return fail(ENSC_ERR_STATE, "autotune: no job to step");
     SavedState S;
     if ((st = save_state(E, S))) return st;
-    const bool any_pipe = E->nblk_pipe > 0, any_staged = E->nblk_e + E->nblk_g + E->nblk_e64 + E->nblk_g64 > 0;
+    const bool any_pipe = E->nblk_pipe > 0, any_staged = E->nblk_e + E->nblk_g + E->nblk_e64 + E->nblk_w64 + E->nblk_g64 > 0;
     auto trial = [&](double* ms) -> ensc_status {
         ensc_status s2 = time_ticks(E, ticks, ms);
         ensc_status s3 = restore_state(E, S);
@@ -1604,9 +1631,9 @@ ensc_status ensc_engine_layout(ensc_engine* E, int64_t* out) {
     if (st) return st;
     out[ENSC_L_STAGED] = E->staged ? 1 : 0;
     out[ENSC_L_PIPE_BLOCKS] = E->nblk_pipe;
-    out[ENSC_L_ELEM_BLOCKS] = E->nblk_e + E->nblk_g + E->nblk_e64 + E->nblk_g64;
+    out[ENSC_L_ELEM_BLOCKS] = E->nblk_e + E->nblk_g + E->nblk_e64 + E->nblk_w64 + E->nblk_g64;
     out[ENSC_L_NODE_BLOCKS] = E->nblk_n_asm + E->nblk_n + E->nblk_n64_asm + E->nblk_n64;
-    out[ENSC_L_LDS_BYTES] = (int64_t)std::max(E->lds_bytes, E->lds_bytes64);
+    out[ENSC_L_LDS_BYTES] = (int64_t)std::max(E->lds_bytes, std::max(E->lds_bytes64, E->lds_bytes_w64));
     out[ENSC_L_ASM_JOBS] = E->n_asm;
     out[ENSC_L_FP64_JOBS] = E->n_fp64;
     return ENSC_OK;

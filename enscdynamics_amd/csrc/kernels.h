@@ -500,6 +500,13 @@ __device__ __forceinline__ void element_mixed_S(ELEM_ARGS) {
     constexpr bool MIXED = true, ASM_CAP = true;
 #include "elem_body_mixed.inc"
 }
+// FP64 block-assembly jobs: the Walsh-form FP64 body
+template <class R>
+__device__ __forceinline__ void element_fp64_W(ELEM_ARGS) {
+#pragma clang fp contract(fast)
+    constexpr bool MIXED = false, ASM_CAP = true;
+#include "elem_body_walsh64.inc"
+}
 #undef NODE
 #undef NVAL
 #undef DYN_BC_OVERRIDE
@@ -658,7 +665,9 @@ __device__ __forceinline__ void elem_phase_finish(const JobDev& J, Ctl& c, const
     }
 }
 
-template <bool MIXED, int BLK, bool STAGED, int WPE>
+// W64: FP64 block-assembly jobs only (staged, FP64), with the Walsh-form body
+// on every tick (a pre-pass tick computes the min edge only).
+template <bool MIXED, int BLK, bool STAGED, int WPE, bool W64 = false>
 __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(WPE))) void k_element(const JobDev* __restrict__ jobs, const uint4* __restrict__ blkmap) {
 #pragma clang fp contract(off)
     extern __shared__ double s_nodes[];  // STAGED: [block's nodes][x0 x1 x2 v0 v1 v2 u0 u1 u2]
@@ -710,7 +719,26 @@ __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(WPE))) void
     double Le = __builtin_inf(), dts = __builtin_inf();
     double en[3] = {0.0, 0.0, 0.0};
     uint32_t errf = 0;
-    if (asmode) {
+    if constexpr (W64) {
+        static_assert(STAGED && !MIXED, "the Walsh-form FP64 body stages its nodes");
+        // every lane runs the body (see below); a pre-pass stops at the min edge
+        if (asmode || act)
+            element_fp64_W<R>(J, act ? e : J.ne - 1, ES, dc.dt_i_1, dc.kind, dc.time_n, act, asmode, Le, dts, en, errf,
+                              lds, acc);
+        if (asmode) {
+            lds_sync();
+            for (uint32_t i = threadIdx.x; i < nbn; i += BLK) {
+                R* pf = static_cast<R*>(J.part_f) + (size_t)(n0 + i) * 3;
+                gst(pf, 0u) = acc[3 * i];
+                gst(pf, 1u) = acc[3 * i + 1];
+                gst(pf, 2u) = acc[3 * i + 2];
+            }
+        }
+        if (!act) {
+            Le = dts = __builtin_inf();
+            en[0] = en[1] = en[2] = 0.0;
+        }
+    } else if (asmode) {
         // every lane runs the body (padding lanes on the last element) so the
         // whole block meets the assembly barriers; padding results are dropped
         if constexpr (MIXED && STAGED)
