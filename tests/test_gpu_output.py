@@ -49,6 +49,10 @@ def test_fp64_snapshots_equal_the_oracle(c1):
     e.set_output(0.0, slots=2)
     snaps = drain(e)
     assert len(snaps) == o.snapshots()
+    # each field against its largest magnitude over the run: in the first
+    # snapshot (uniform initial velocity, no stress yet) the oracle's
+    # acceleration is rounding noise (~5e-11 against ~1e6 m/s^2 later)
+    scale = {f: max(float(np.abs(o.snapshot(k, f)).max()) for k in range(len(snaps))) for f in NODE_F + ELEM_F}
     worst = 0.0
     for k, s in enumerate(snaps):
         # the step size can differ from the oracle's by an ulp (min dtStable
@@ -56,7 +60,7 @@ def test_fp64_snapshots_equal_the_oracle(c1):
         assert s["time"] == pytest.approx(o.snapshot(k, "time")[0], rel=1e-13, abs=0), k
         for f in NODE_F + ELEM_F:
             ref = o.snapshot(k, f)
-            d = np.abs(s[f] - ref).max() / max(np.abs(ref).max(), 1e-300)
+            d = np.abs(s[f] - ref).max() / max(scale[f], 1e-300)
             worst = max(worst, d)
             assert d <= 1e-12, (k, f, d)
     print(f"[output fp64] {len(snaps)} snapshots, max rel diff {worst:.2e}", flush=True)
@@ -70,6 +74,9 @@ def test_vtu_files_written_during_the_run(c1, tmp_path):
     e.run()
     files = sorted(os.listdir(tmp_path), key=lambda s: int(s.split("_")[1].split(".")[0]))
     assert files == [f"solution_{k}.vtu" for k in range(o.snapshots())]
+    # absolute tolerances from each field's largest magnitude over the run
+    # (the first snapshot's acceleration is rounding noise, see above)
+    nscale = {f: max(float(np.abs(o.snapshot(k, f)).max()) for k in range(len(files))) for f in ("a", "v", "u")}
     for k in (0, len(files) // 2, len(files) - 1):
         r = read_vtu(str(tmp_path / files[k]))
         assert r["time"] == pytest.approx(o.snapshot(k, "time")[0], rel=1e-7)
@@ -81,7 +88,7 @@ def test_vtu_files_written_during_the_run(c1, tmp_path):
         for f in ("a", "v", "u"):
             ref = o.snapshot(k, f)
             assert np.allclose(r["point_data"][f], ref.astype(np.float32), rtol=1e-6,
-                               atol=1e-6 * np.abs(ref).max()), f
+                               atol=1e-6 * nscale[f]), f
         assert not np.any(r["point_data"]["force"])
         assert np.allclose(r["point_data"]["mass"], (1.0 / o.get("invM")).astype(np.float32), rtol=1e-6)
         for f in ELEM_F:

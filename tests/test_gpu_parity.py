@@ -158,9 +158,10 @@ def test_reorder_is_bit_exact():
 
 
 def test_fp64_block_assembly_vs_reference_order():
-    # default FP64 assembly (block partials summed per node) against the
-    # reference's order: the same forces added in another order, so the
-    # results differ by rounding only, renumbered or not
+    # default FP64 path (the Walsh-form element body, block partials summed
+    # per node) against the reference's order: the same forces regrouped and
+    # added in another order, so the results differ by rounding only,
+    # renumbered or not
     m = decks.config3(n=14, steps=400)
     a = eng.Engine(m, eng.FP64, reorder=True, fp64_order=1).run()
     for reorder in (True, False):

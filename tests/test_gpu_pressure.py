@@ -75,8 +75,15 @@ def test_fp64_pressure_matches_oracle(case):
     o = oracle_of(m)
     so, se = o.scalars(), e.scalars()
     assert se["nsteps"] == so["nsteps"]
-    for f in ("u", "v", "sigma"):
-        assert rel(e.get(f), o.get(f), 1e-12) <= TOL_FP64, f
+    # velocities against max(|v|, the particle velocity p / (rho c) of the
+    # largest pressure): once a loaded body has come to rest (GenVirPa, after
+    # 22k steps) its velocity field is rounding noise (~1e-14 mm/s against
+    # p / (rho c) = 21 mm/s), which an engine that groups its sums otherwise
+    # than the oracle does not reproduce digit for digit
+    c = m.material_consts()[3]
+    pmax = max([abs(d.value) for st in m.steps for d in (st.dsload or [])] + [0.0])
+    for f, fl in (("u", 1e-12), ("v", pmax / (m.rho * c)), ("sigma", 1e-12)):
+        assert rel(e.get(f), o.get(f), fl) <= TOL_FP64, f
     assert so["wext"] > 0.0
     scale = max(abs(so["wext"]), so["ke0"])
     for k in ("ke", "ie", "hg", "vis", "wext"):
