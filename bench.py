@@ -90,11 +90,18 @@ def alg_bytes(precision: int, plastic: bool, r: float):
     return elem, node
 
 
+# The engines of the configuration-3 leg: mixed precision; FP64 (the
+# Walsh-form FP64 element body with block partials, the default); FP64 in the
+# reference's operation and summation order (fp64_order=1), which reproduces
+# the reference bit for bit up to libm rounding.
+C3_ENGINES = (("mixed", eng.MIXED, 0), ("fp64", eng.FP64, 0), ("fp64_reference_order", eng.FP64, 1))
+
+
 def c3_section(args, m):
     r = m.n_nodes / m.n_elems
     res = {}
-    for prec, name in ((eng.MIXED, "mixed"), (eng.FP64, "fp64")):
-        e = eng.Engine(m, prec, reorder=True, staging=not args.no_staging)
+    for name, prec, order in C3_ENGINES:
+        e = eng.Engine(m, prec, reorder=True, staging=not args.no_staging, fp64_order=order)
         if args.graph:
             e.set_graph(args.graph)
         e.init()
@@ -124,7 +131,7 @@ def c3_section(args, m):
     u_64 = res["fp64"]["engine"].get("u")
     rel_err = float(np.abs(u_mix - u_64).max() / max(np.abs(u_64).max(), 1e-300))
     out = {}
-    for name, prec in (("mixed", eng.MIXED), ("fp64", eng.FP64)):
+    for name, prec, _ in C3_ENGINES:
         d = res[name]
         be, bn = alg_bytes(prec, m.plastic, r)
         se, sn = survey_bytes(prec, m.plastic, r)
@@ -517,10 +524,15 @@ def cpu_baseline(m, args):
     return port, None
 
 
-KERNEL_KEYS = (("k_element_pipe", "k_element_pipe_mixed"), ("k_element<true", "k_element_mixed"),
-               ("k_element<false", "k_element_fp64"), ("k_node<true, 256, true>", "k_node_mixed_partials"),
-               ("k_node<true", "k_node_mixed"), ("k_node<false", "k_node_fp64"),
-               ("k_fin_element", "k_fin_element"), ("k_fin_node", "k_fin_node"), ("k_copy_probe", "k_copy_probe"))
+# (substrings of the demangled kernel name, key); the first row whose
+# substrings all occur names the kernel
+KERNEL_KEYS = ((("k_element_pipe",), "k_element_pipe_mixed"), (("k_element<true",), "k_element_mixed"),
+               (("k_element<false", ", true>("), "k_element_fp64"),
+               (("k_element<false",), "k_element_fp64_reference_order"),
+               (("k_node<true, 256, true>",), "k_node_mixed_partials"), (("k_node<true",), "k_node_mixed"),
+               (("k_node<false, 256, true>",), "k_node_fp64"), (("k_node<false",), "k_node_fp64_reference_order"),
+               (("k_fin_element",), "k_fin_element"), (("k_fin_node",), "k_fin_node"),
+               (("k_copy_probe",), "k_copy_probe"))
 
 
 def _pmc_rows(d: str, counter: str):
@@ -537,8 +549,8 @@ def _pmc_rows(d: str, counter: str):
                 if row.get("Counter_Name") != counter:
                     continue
                 name = row.get("Kernel_Name", "")
-                for pat, key in KERNEL_KEYS:
-                    if pat in name:
+                for pats, key in KERNEL_KEYS:
+                    if all(p in name for p in pats):
                         rows.setdefault(key, []).append(float(row["Counter_Value"]))
                         break
     return rows
@@ -642,6 +654,8 @@ def main() -> int:
         "mixed": mixed,
         "fp64": fp64,
         "mixed_over_fp64": mixed["value"] / fp64["value"],
+        "fp64_reference_order": c3["fp64_reference_order"],
+        "mixed_over_fp64_reference_order": mixed["value"] / c3["fp64_reference_order"]["value"],
         "rel_disp_err_mixed_vs_fp64": c3["rel_disp_err_mixed_vs_fp64"],
         "copy_probe_gbs": copy_gbs,
     }

@@ -79,15 +79,18 @@ typedef struct ensc_options {
                                1: every job mixed precision;
                                2: every job FP64.
                                FP64 engines run every job in FP64. */
-    int32_t fp64_order;     /* how FP64 jobs sum a node's element forces:
-                               0: each element block sums its elements'
-                               corner forces in LDS and the node adds the
-                               blocks' partials (no per-corner forces through
-                               HBM; <= 1e-10 of the reference, rounding of the
-                               other order); 1: the reference's order, per
-                               corner then ascending element (exDyna3D.cpp:
-                               584-639) through the gather table - bit
-                               identical to the CPU restatement. */
+    int32_t fp64_order;     /* the order of FP64 jobs' arithmetic:
+                               0: the element body regrouped through the
+                               corners' Walsh sign patterns, and each element
+                               block sums its elements' corner forces in LDS,
+                               the node adding the blocks' partials (no
+                               per-corner forces through HBM; <= 1e-10 of the
+                               reference, rounding of the other order);
+                               1: the reference's order, every expression of
+                               the element body as exDyna3D.cpp:838-1109
+                               writes it and the node's sum per corner then
+                               ascending element (584-639) through the gather
+                               table - bit identical to the CPU restatement. */
 } ensc_options;
 
 /* Scalars of one job (ensc_engine_scalars). */
