@@ -185,7 +185,10 @@ __device__ __forceinline__ void load_state(const JobDev& J, uint32_t e, bool ful
     for (int q = 0; q < 6; ++q) S.s[q] = gld(Sp, q * es + e);
 #pragma unroll
     for (int q = 0; q < 12; ++q) S.Q[q] = gld(Qp, q * es + e);
-    S.ep = J.plastic ? gld(static_cast<const R*>(J.epsp), e) : R(0);
+    // elastic jobs read a valid word they never use (the bodies read ep only
+    // for plastic jobs): an unconditional load, so the register is not first
+    // zeroed, which would make the compiler wait for its previous load
+    S.ep = gld(static_cast<const R*>(J.plastic ? J.epsp : J.sigma), e);
 }
 
 // Eigen 3.4's 3x3 determinant (Determinant.h:19-24, 56-65).
@@ -334,6 +337,27 @@ __device__ inline double deal_amp(const JobDev& J, int id, double time, double d
         break;
     }
     return y;
+}
+
+// A job's scalars as one uniform read through the constant address space,
+// so the compiler issues scalar loads together and waits for them alone.
+// Read through generic pointers the fields came one at a time, each load
+// followed by a wait for every outstanding vector memory operation of the
+// wave (in the pipelined element pass: the tile's copies and the previous
+// tile's stores).  Only for kernels that do not write the scalars they read
+// (the element kernels set the sticky error word by atomics; nothing here
+// reads it): the scalars come from an earlier launch, and a launch starts
+// with the scalar cache invalidated.  Used where it measured faster: the
+// pipelined element pass (once per tile) and the nodal update.
+__device__ __forceinline__ Ctl ctl_read(const Ctl* p) {
+    static_assert(sizeof(Ctl) % 4 == 0, "Ctl is read as 32-bit words");
+    typedef const __attribute__((address_space(4))) uint32_t* KWord;
+    const KWord q = (KWord)p;
+    Ctl c;
+    uint32_t* d = reinterpret_cast<uint32_t*>(&c);
+#pragma unroll
+    for (unsigned i = 0; i < sizeof(Ctl) / 4; ++i) d[i] = q[i];
+    return c;
 }
 
 struct Decision {
@@ -679,6 +703,8 @@ __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(WPE))) void
     const uint32_t e = lb * BLK + threadIdx.x;
     const bool act = e < J.ne;
     typedef typename std::conditional<MIXED, float, double>::type R;
+    // generic loads here: with one element block per launched block the
+    // scalar read measured slower (FP64 element pass 128 -> 133 us)
     const Decision dc = decide(*ctl, J.factor, J.wos);
     if (dc.kind == KIND_NONE) return;
     ElemState<R> ES;
@@ -800,13 +826,12 @@ __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(WPE))) void
             r3 += s_red[q][3];
             r4 += s_red[q][4];
         }
-        double* p = J.part_e + lb;
         const uint32_t nb = J.nblk_e;
-        p[0] = r0;
-        p[nb] = r1;
-        p[2 * nb] = r2;
-        p[3 * nb] = r3;
-        p[4 * nb] = r4;
+        gst(J.part_e, lb) = r0;
+        gst(J.part_e, nb + lb) = r1;
+        gst(J.part_e, 2 * nb + lb) = r2;
+        gst(J.part_e, 3 * nb + lb) = r3;
+        gst(J.part_e, 4 * nb + lb) = r4;
         if (s_err) atomicOr(&ctl->err, s_err);
     }
 }
@@ -877,10 +902,17 @@ typedef ENSC_LDS Pair (*PipeBuf)[PIPE_CAP];  // [5][PIPE_CAP]: [X0 X1] [X2 1/m] 
 // (wave's first slot) + lane + 256*j, j < 2 (PIPE_CAP == 512).  pipe_ids
 // loads the node ids of those slots; pipe_copy then issues five 16-byte
 // global->LDS copies per slot, which complete in the background.
+// Slots past the tile's last (and every slot of an empty range, whose n0 is
+// still a valid position) read a valid id that pipe_copy does not use: the
+// loads are unconditional, so no register is first zeroed and then loaded,
+// which would make the compiler wait for the loads of the previous use.
 __device__ __forceinline__ void pipe_ids(const JobDev& J, uint32_t n0, uint32_t nbn, uint32_t g[2]) {
     const uint32_t s0 = threadIdx.x;
 #pragma unroll
-    for (int j = 0; j < 2; ++j) g[j] = (s0 + j * PIPE_BLK < nbn) ? gld(J.blk_nodes, n0 + s0 + j * PIPE_BLK) : 0u;
+    for (int j = 0; j < 2; ++j) {
+        const uint32_t s = s0 + j * PIPE_BLK;
+        g[j] = gld(J.blk_nodes, n0 + (s < nbn ? s : (nbn ? nbn - 1 : 0u)));
+    }
 }
 __device__ __forceinline__ void pipe_copy(const JobDev& J, uint32_t nbn, const uint32_t g[2], PipeBuf buf) {
     const uint32_t lane = threadIdx.x & 63u, w0 = threadIdx.x & ~63u;
@@ -958,6 +990,39 @@ __device__ __forceinline__ void element_mixed_P(const JobDev& J, uint32_t e, con
 #undef ASM_BEGIN
 #undef PREFETCH_HOOK
 
+// A tile's results held in registers until the next tile has started: its
+// partial nodal forces (this thread's staged slots tid and tid + PIPE_BLK)
+// and the block's five values (min edge, min stable dt, three work terms;
+// thread 0).  Stored last in the tile, they would be the youngest memory
+// operations at the next tile's start, whose wait would then also wait for
+// their completion; stored right after that wait they complete during the
+// tile.
+struct PipeOut {
+    float f[2][3];
+    double r[5];
+    float* pf;       // the tile's first partial record (null: nothing to store)
+    double* pe;      // the tile's block partials (stride nb)
+    uint32_t nbn, nb;
+};
+__device__ __forceinline__ void pipe_flush(PipeOut& o) {
+    if (!o.pe) return;
+    if (o.pf) {
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            const uint32_t i = threadIdx.x + j * PIPE_BLK;
+            if (i < o.nbn) {
+                gst(o.pf, 3 * i) = o.f[j][0];
+                gst(o.pf, 3 * i + 1) = o.f[j][1];
+                gst(o.pf, 3 * i + 2) = o.f[j][2];
+            }
+        }
+    }
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int q = 0; q < 5; ++q) gst(o.pe, q * o.nb) = o.r[q];
+    o.pe = nullptr;
+}
+
 // One tile: compute it from `cur` (its node records) and ES (its element
 // state) while the next tile's records are copied into `nxt`; g holds the
 // next tile's node ids.  Fills ESn and gn for the next tile.  The one wait
@@ -969,7 +1034,7 @@ __device__ __forceinline__ void element_mixed_P(const JobDev& J, uint32_t e, con
 __device__ __forceinline__ bool pipe_tile(const JobDev* __restrict__ jobs, const uint4* __restrict__ blkmap,
                                           uint32_t ntiles, uint32_t& t, PipeBuf cur, PipeBuf nxt,
                                           const ElemState<float>& ES, const uint32_t g[2], ElemState<float>& ESn,
-                                          uint32_t gn[2]) {
+                                          uint32_t gn[2], PipeOut& out) {
 #pragma clang fp contract(off)
     const uint4 bm = blkmap[t];  // (job, block of the job, first staged position, staged count)
     const JobDev& J = jobs[bm.x];
@@ -977,7 +1042,7 @@ __device__ __forceinline__ bool pipe_tile(const JobDev* __restrict__ jobs, const
     const uint32_t lb = bm.y, nbn = bm.w;
     const uint32_t e = lb * PIPE_BLK + threadIdx.x;
     const bool act = e < J.ne;
-    const Decision dc = decide(*ctl, J.factor, J.wos);
+    const Decision dc = decide(ctl_read(ctl), J.factor, J.wos);
     const uint32_t tn = t + gridDim.x, tnn = tn + gridDim.x;
     const bool more = tn < ntiles, more2 = tnn < ntiles;
     const uint4 bn = blkmap[more ? tn : t];
@@ -994,6 +1059,7 @@ __device__ __forceinline__ bool pipe_tile(const JobDev* __restrict__ jobs, const
     pn.done = false;
     __builtin_amdgcn_s_waitcnt(0);
     lds_sync();
+    pipe_flush(out);  // the previous tile's results (registers, not LDS)
     if (more) pipe_copy(jobs[bn.x], bn.w, g, nxt);
     if (dc.kind != KIND_NONE) {
         const bool asmode = J.block_asm && dc.kind != KIND_PREPASS;
@@ -1004,15 +1070,20 @@ __device__ __forceinline__ bool pipe_tile(const JobDev* __restrict__ jobs, const
             element_mixed_P(J, act ? e : J.ne - 1, ES, dc.dt_i_1, dc.kind, dc.time_n, act, asmode, Le, dts, en, errf,
                             cur, nbn, pn, ESn, gn);
         if (!pn.done) pipe_prefetch(pn, ESn, gn);
+        out.pf = nullptr;
         if (asmode) {
             const ENSC_LDS float* acc = pipe_acc(cur);
             lds_sync();
-            for (uint32_t i = threadIdx.x; i < nbn; i += PIPE_BLK) {
-                float* pf = static_cast<float*>(J.part_f) + (size_t)(bm.z + i) * 3;
-                gst(pf, 0u) = acc[4 * i];
-                gst(pf, 1u) = acc[4 * i + 1];
-                gst(pf, 2u) = acc[4 * i + 2];
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                const uint32_t i = threadIdx.x + j * PIPE_BLK;
+                const uint32_t k = i < nbn ? i : 0u;
+                out.f[j][0] = acc[4 * k];
+                out.f[j][1] = acc[4 * k + 1];
+                out.f[j][2] = acc[4 * k + 2];
             }
+            out.pf = static_cast<float*>(J.part_f) + (size_t)bm.z * 3;
+            out.nbn = nbn;
         }
         if (!act) {
             Le = dts = __builtin_inf();
@@ -1052,15 +1123,15 @@ __device__ __forceinline__ bool pipe_tile(const JobDev* __restrict__ jobs, const
                 r4 += s_red[q][4];
                 er |= (uint32_t)s_red[q][5];
             }
-            double* p = J.part_e + lb;
-            const uint32_t nb = J.nblk_e;
-            p[0] = r0;
-            p[nb] = r1;
-            p[2 * nb] = r2;
-            p[3 * nb] = r3;
-            p[4 * nb] = r4;
+            out.r[0] = r0;
+            out.r[1] = r1;
+            out.r[2] = r2;
+            out.r[3] = r3;
+            out.r[4] = r4;
             if (er) atomicOr(&ctl->err, er);
         }
+        out.pe = J.part_e + lb;
+        out.nb = J.nblk_e;
     } else {
         pipe_prefetch(pn, ESn, gn);
     }
@@ -1094,9 +1165,12 @@ __global__ __launch_bounds__(PIPE_BLK) __attribute__((amdgpu_waves_per_eu(WPE)))
     // two tiles per trip so each buffer is a distinct LDS object in every
     // access (the compiler then keeps copies into one buffer in flight while
     // the other is read)
-    while (pipe_tile(jobs, blkmap, ntiles, t, bufA, bufB, esA, gB, esB, gA) &&
-           pipe_tile(jobs, blkmap, ntiles, t, bufB, bufA, esB, gA, esA, gB)) {
+    PipeOut out;
+    out.pe = nullptr;
+    while (pipe_tile(jobs, blkmap, ntiles, t, bufA, bufB, esA, gB, esB, gA, out) &&
+           pipe_tile(jobs, blkmap, ntiles, t, bufB, bufA, esB, gA, esA, gB, out)) {
     }
+    pipe_flush(out);  // the block's last tile
 }
 
 // ASM: the launch covers block-assembly jobs only (mixed engine) and sums
@@ -1108,9 +1182,10 @@ __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(ASM ? (MIXE
     const uint4 bm = blkmap[blockIdx.x];  // (job, block of the job, -, -)
     const JobDev& J = jobs[bm.x];
     Ctl* ctl = J.ctl;
-    if (ctl->kind != KIND_STEP) return;
-    const double dt_i = ctl->dt_i, dt_i_1 = ctl->dt_i_1, time = ctl->time;
-    const int32_t snap = ctl->snap_now - 1;  // output slot of this step, or -1
+    const Ctl cs = ctl_read(ctl);
+    if (cs.kind != KIND_STEP) return;
+    const double dt_i = cs.dt_i, dt_i_1 = cs.dt_i_1, time = cs.time;
+    const int32_t snap = cs.snap_now - 1;  // output slot of this step, or -1
     const uint32_t lb = bm.y;
     const uint32_t n = lb * BLK + threadIdx.x;
     double ke = 0.0, pw = 0.0;
@@ -1149,19 +1224,21 @@ __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(ASM ? (MIXE
             typedef typename std::conditional<MIXED, float, double>::type P;
             const uint32_t q0 = gld(J.pos_ptr, n), q1 = gld(J.pos_ptr, n + 1);
             for (uint32_t q = q0; q < q1; q += 4) {
+                // four positions, then four records, each set of loads issued
+                // together (indices past the node's last clamped to it; their
+                // values are dropped)
+                uint32_t pos[4];
+#pragma unroll
+                for (int j = 0; j < 4; ++j) pos[j] = gld(J.pos_list, q + j < q1 ? q + j : q1 - 1);
                 P f[4][3];
 #pragma unroll
                 for (int j = 0; j < 4; ++j) {
-                    f[j][0] = f[j][1] = f[j][2] = P(0);
-                    if (q + j < q1) {
-                        // one record: a single address, three offsets
-                        const uint32_t pos = gld(J.pos_list, q + j);
-                        const ENSC_GLB P* pf =
-                            (const ENSC_GLB P*)((const ENSC_GLB char*)J.part_f + pos * (uint32_t)(3 * sizeof(P)));
-                        f[j][0] = pf[0];
-                        f[j][1] = pf[1];
-                        f[j][2] = pf[2];
-                    }
+                    // one record: a single address, three offsets
+                    const ENSC_GLB P* pf =
+                        (const ENSC_GLB P*)((const ENSC_GLB char*)J.part_f + pos[j] * (uint32_t)(3 * sizeof(P)));
+                    f[j][0] = pf[0];
+                    f[j][1] = pf[1];
+                    f[j][2] = pf[2];
                 }
 #pragma unroll
                 for (int j = 0; j < 4; ++j)
@@ -1238,7 +1315,7 @@ __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(ASM ? (MIXE
             for (int d = 0; d < 3; ++d) a[d] = 0.0 + acc[d];
         }
         if (J.grav_on) {  // apply_gravity (800-826)
-            const double ag = ctl->grav_amp * J.grav_g;
+            const double ag = cs.grav_amp * J.grav_g;
             a[0] += ag * J.grav_dir[0] / im;
             a[1] += ag * J.grav_dir[1] / im;
             a[2] += ag * J.grav_dir[2] / im;
@@ -1357,7 +1434,7 @@ __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(ASM ? (MIXE
         const double half_m = 0.5 / im;
         ke = k2 * half_m;
         pw += kj * half_m;
-        if (J.grav_on) pw += 0.5 * hs * (ctl->grav_amp * J.grav_g) * wg * half_m;
+        if (J.grav_on) pw += 0.5 * hs * (cs.grav_amp * J.grav_g) * wg * half_m;
         if (f1 != f0 || mf >= 0) pw += 0.25 * hs * wp;
         if (MIXED) gst(J.ucomp, n) = cw_new;
         if (!MIXED) {
