@@ -282,9 +282,9 @@ __device__ __forceinline__ uint32_t ucomp_put(double c, double t, int d) {
 __device__ inline double amp_value(const JobDev& J, int id, double t) {
 #pragma clang fp contract(off)
     if (id < 0 || id >= J.n_amp) return 1.0;
-    const int b = J.amp_off[id], n = J.amp_off[id + 1] - b;
-    const double* T = J.amp_t + b;
-    const double* A = J.amp_a + b;
+    const int b = gld(J.amp_off, id), n = gld(J.amp_off, id + 1) - b;
+    const ENSC_GLB double* T = (const ENSC_GLB double*)(J.amp_t + b);
+    const ENSC_GLB double* A = (const ENSC_GLB double*)(J.amp_a + b);
     if (n <= 0) return 1.0;
     if (t <= T[0]) return A[0];
     if (t >= T[n - 1]) return A[n - 1];
@@ -299,9 +299,9 @@ __device__ inline double amp_value(const JobDev& J, int id, double t) {
 __device__ inline double deal_amp(const JobDev& J, int id, double time, double dt) {
 #pragma clang fp contract(off)
     if (id < 0 || id >= J.n_amp) return 1.0;
-    const int b = J.amp_off[id], n = J.amp_off[id + 1] - b;
-    const double* T = J.amp_t + b;
-    const double* A = J.amp_a + b;
+    const int b = gld(J.amp_off, id), n = gld(J.amp_off, id + 1) - b;
+    const ENSC_GLB double* T = (const ENSC_GLB double*)(J.amp_t + b);
+    const ENSC_GLB double* A = (const ENSC_GLB double*)(J.amp_a + b);
     if (n < 2) return 1.0;
     double y = 1.0;
     for (int i = 0; i + 1 < n; ++i) {
@@ -1192,8 +1192,10 @@ __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(ASM ? (MIXE
     if (snap >= 0 && n < J.nn) {
         // output_results' element stress (this step's, already updated)
         typedef typename std::conditional<MIXED, float, double>::type R_;
-        double* se = J.snap_elem + (size_t)snap * J.ne * 6;
-        const R_* sg = static_cast<const R_*>(J.sigma);
+        // global (not generic) accesses: generic ones pending here would make
+        // the compiler's later waits conservative on every path
+        ENSC_GLB double* se = (ENSC_GLB double*)(J.snap_elem + (size_t)snap * J.ne * 6);
+        const ENSC_GLB R_* sg = (const ENSC_GLB R_*)J.sigma;
         for (uint32_t e = n; e < J.ne; e += J.nn)
 #pragma unroll
             for (int q = 0; q < 6; ++q)
@@ -1203,6 +1205,13 @@ __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(ASM ? (MIXE
         // [X0 X1 | X2 im] [v0 v1 | v2 u0 | u1 u2]; the mixed engine never
         // rewrites X (the reference position), so it reads 1/m from its own
         // array instead of the position record
+        // block assembly: the node's range of partials first, so the loads
+        // that depend on it wait for it alone
+        uint32_t q0 = 0, q1 = 0;
+        if (ASM) {
+            q0 = gld(J.pos_ptr, n);
+            q1 = gld(J.pos_ptr, n + 1);
+        }
         const D2 p2 = gld2(J.vu, n * VU_REC), p3 = gld2(J.vu, n * VU_REC + 2), p4 = gld2(J.vu, n * VU_REC + 4);
         double x[3] = {0.0, 0.0, 0.0};
         double im;
@@ -1222,11 +1231,10 @@ __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(ASM ? (MIXE
             // block assembly: the node's partials, one per element block that
             // holds it, in ascending block order
             typedef typename std::conditional<MIXED, float, double>::type P;
-            const uint32_t q0 = gld(J.pos_ptr, n), q1 = gld(J.pos_ptr, n + 1);
             for (uint32_t q = q0; q < q1; q += 4) {
                 // four positions, then four records, each set of loads issued
                 // together (indices past the node's last clamped to it; their
-                // values are dropped)
+                // values are weighted by 0, so no load depends on a branch)
                 uint32_t pos[4];
 #pragma unroll
                 for (int j = 0; j < 4; ++j) pos[j] = gld(J.pos_list, q + j < q1 ? q + j : q1 - 1);
@@ -1241,12 +1249,12 @@ __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(ASM ? (MIXE
                     f[j][2] = pf[2];
                 }
 #pragma unroll
-                for (int j = 0; j < 4; ++j)
-                    if (q + j < q1) {
-                        acc[0] += (double)f[j][0];
-                        acc[1] += (double)f[j][1];
-                        acc[2] += (double)f[j][2];
-                    }
+                for (int j = 0; j < 4; ++j) {
+                    const double w = (q + j < q1) ? 1.0 : 0.0;
+                    acc[0] += (double)f[j][0] * w;
+                    acc[1] += (double)f[j][1] * w;
+                    acc[2] += (double)f[j][2] * w;
+                }
             }
         } else {
         // Element forces of the node in (corner, original element) order;
@@ -1305,11 +1313,11 @@ __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(ASM ? (MIXE
         // apply_fsiSph_nodeForce (1635-1689) runs before add_inForce_to_rhs:
         // a = (sum of the particles' nodal forces) + (the corner sum)
         int32_t mf = -1;
-        if (J.n_fsi_node) mf = J.fsi_node_of[n];
+        if (J.n_fsi_node) mf = gld(J.fsi_node_of, n);
         double a[3];
         if (mf >= 0) {
 #pragma unroll
-            for (int d = 0; d < 3; ++d) a[d] = J.fsi_nforce[(size_t)mf * 3 + d] + acc[d];
+            for (int d = 0; d < 3; ++d) a[d] = gld(J.fsi_nforce, (uint32_t)mf * 3 + d) + acc[d];
         } else {
 #pragma unroll
             for (int d = 0; d < 3; ++d) a[d] = 0.0 + acc[d];
@@ -1325,12 +1333,12 @@ __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(ASM ? (MIXE
         // update loop sums the forces again, so nothing stays live between)
         uint32_t f0 = 0, f1 = 0;
         if (J.n_dsf) {
-            f0 = J.ds_ptr[n];
-            f1 = J.ds_ptr[n + 1];
+            f0 = gld(J.ds_ptr, n);
+            f1 = gld(J.ds_ptr, n + 1);
             for (uint32_t q = f0; q < f1; ++q) {
-                const double* ff = J.dsf_force + (size_t)J.ds_list[q] * 3;
+                const uint32_t fq = gld(J.ds_list, q) * 3;
 #pragma unroll
-                for (int d = 0; d < 3; ++d) a[d] += ff[d];
+                for (int d = 0; d < 3; ++d) a[d] += gld(J.dsf_force, fq + d);
             }
         }
         double wp = 0.0;
@@ -1379,7 +1387,7 @@ __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(ASM ? (MIXE
         double k2 = 0.0;
         const uint32_t cw = MIXED ? gld(J.ucomp, n) : 0u;
         uint32_t cw_new = 0;
-        double* sn = snap >= 0 ? J.snap_node + ((size_t)snap * J.nn + n) * 12 : nullptr;
+        ENSC_GLB double* sn = snap >= 0 ? (ENSC_GLB double*)(J.snap_node + ((size_t)snap * J.nn + n) * 12) : nullptr;
         if (sn) {
             // output_results' node state: x and u before this step's update
             double xs[3] = {x[0], x[1], x[2]};
@@ -1410,10 +1418,10 @@ __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(ASM ? (MIXE
             wg += J.grav_dir[d] * (vb + vd);
             if (f1 != f0) {
                 double fp = 0.0;
-                for (uint32_t q = f0; q < f1; ++q) fp += J.dsf_force[(size_t)J.ds_list[q] * 3 + d];
+                for (uint32_t q = f0; q < f1; ++q) fp += gld(J.dsf_force, gld(J.ds_list, q) * 3 + d);
                 wp += fp * (vb + vd);
             }
-            if (mf >= 0) wp += J.fsi_nforce[(size_t)mf * 3 + d] * (vb + vd);  // the particles' work
+            if (mf >= 0) wp += gld(J.fsi_nforce, (uint32_t)mf * 3 + d) * (vb + vd);  // the particles' work
             const double dx = dt_i * vd;
             if (MIXED) {
                 // update_displacement (651-654), compensated (Kahan); the
