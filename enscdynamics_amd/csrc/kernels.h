@@ -418,6 +418,35 @@ __device__ __forceinline__ void lds_sync() {
     __atomic_signal_fence(__ATOMIC_SEQ_CST);
 }
 
+// Wave reductions of doubles through DPP row moves (no LDS): row shifts by
+// 1, 2, 4 and 8 leave each 16-lane row's total in its lane 15, then
+// row_bcast:15 and row_bcast:31 carry it across rows, so lane 63 holds the
+// wave's total (min or sum).  A lane whose source lies outside its row, or
+// whose row the mask leaves out, receives the identity.  The order of the
+// operations is fixed, so the result is too.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ double dpp_move(double v, double identity) {
+    const long long x = __double_as_longlong(v), o = __double_as_longlong(identity);
+    const int lo = __builtin_amdgcn_update_dpp((int)o, (int)x, CTRL, ROWS, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp((int)(o >> 32), (int)(x >> 32), CTRL, ROWS, 0xF, false);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+template <bool MIN>
+__device__ __forceinline__ double dpp_op(double a, double b) {
+    return MIN ? (b < a ? b : a) : a + b;
+}
+template <bool MIN>
+__device__ __forceinline__ double wave_reduce63(double v) {
+    const double id = MIN ? __builtin_inf() : 0.0;
+    v = dpp_op<MIN>(v, dpp_move<0x111, 0xF>(v, id));  // row_shr:1
+    v = dpp_op<MIN>(v, dpp_move<0x112, 0xF>(v, id));  // row_shr:2
+    v = dpp_op<MIN>(v, dpp_move<0x114, 0xF>(v, id));  // row_shr:4
+    v = dpp_op<MIN>(v, dpp_move<0x118, 0xF>(v, id));  // row_shr:8
+    v = dpp_op<MIN>(v, dpp_move<0x142, 0xA>(v, id));  // row_bcast:15 into rows 1, 3
+    v = dpp_op<MIN>(v, dpp_move<0x143, 0xC>(v, id));  // row_bcast:31 into rows 2, 3
+    return v;                                          // the wave's value in lane 63
+}
+
 template <class T>
 __device__ __forceinline__ T wave_sum(T v) {
 #pragma unroll
@@ -801,15 +830,15 @@ __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(WPE))) void
     __shared__ double s_red[BLK / 64][5];
     __shared__ uint32_t s_err;
     if (threadIdx.x == 0) s_err = 0;
-    Le = wave_min(Le);
-    dts = wave_min(dts);
-    en[0] = wave_sum(en[0]);
-    en[1] = wave_sum(en[1]);
-    en[2] = wave_sum(en[2]);
+    Le = wave_reduce63<true>(Le);
+    dts = wave_reduce63<true>(dts);
+    en[0] = wave_reduce63<false>(en[0]);
+    en[1] = wave_reduce63<false>(en[1]);
+    en[2] = wave_reduce63<false>(en[2]);
     lds_sync();
     if (errf) atomicOr(&s_err, errf);
     const int w = threadIdx.x >> 6;
-    if ((threadIdx.x & 63) == 0) {
+    if ((threadIdx.x & 63) == 63) {
         s_red[w][0] = Le;
         s_red[w][1] = dts;
         s_red[w][2] = en[0];
@@ -1089,13 +1118,14 @@ __device__ __forceinline__ bool pipe_tile(const JobDev* __restrict__ jobs, const
             Le = dts = __builtin_inf();
             en[0] = en[1] = en[2] = 0.0;
         }
-        // block partials: wave shuffles, then the [X2 1/m] row of `cur`
-        // (free once every wave is past the barrier below) across the waves
-        Le = wave_min(Le);
-        dts = wave_min(dts);
-        en[0] = wave_sum(en[0]);
-        en[1] = wave_sum(en[1]);
-        en[2] = wave_sum(en[2]);
+        // block partials: DPP wave reductions (into lane 63), then the
+        // [X2 1/m] row of `cur` (free once every wave is past the barrier
+        // below) across the waves
+        Le = wave_reduce63<true>(Le);
+        dts = wave_reduce63<true>(dts);
+        en[0] = wave_reduce63<false>(en[0]);
+        en[1] = wave_reduce63<false>(en[1]);
+        en[2] = wave_reduce63<false>(en[2]);
         uint32_t werr = 0;
 #pragma unroll
         for (int b = 0; b < 3; ++b)
@@ -1103,7 +1133,7 @@ __device__ __forceinline__ bool pipe_tile(const JobDev* __restrict__ jobs, const
         lds_sync();
         ENSC_LDS double(*s_red)[6] = (ENSC_LDS double(*)[6]) & cur[1][0];
         const int w = threadIdx.x >> 6;
-        if ((threadIdx.x & 63) == 0) {
+        if ((threadIdx.x & 63) == 63) {
             s_red[w][0] = Le;
             s_red[w][1] = dts;
             s_red[w][2] = en[0];
@@ -1454,10 +1484,10 @@ __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(ASM ? (MIXE
         gst2(J.vu, n * VU_REC + 4, u[1], u[2]);
     }
     __shared__ double s_red[BLK / 64][2];
-    ke = wave_sum(ke);
-    pw = wave_sum(pw);
+    ke = wave_reduce63<false>(ke);
+    pw = wave_reduce63<false>(pw);
     const int w = threadIdx.x >> 6;
-    if ((threadIdx.x & 63) == 0) {
+    if ((threadIdx.x & 63) == 63) {
         s_red[w][0] = ke;
         s_red[w][1] = pw;
     }
