@@ -576,8 +576,8 @@ __device__ __forceinline__ void element_fp64_W(ELEM_ARGS) {
 // FOLD_U blocks at a time (one round for jobs of up to FOLD_U * BLK blocks).
 // It reads only the job table, so its loads go out together with those of
 // the job's scalars; sums of a phase that is not open are not used.
-// combine_partials: a fixed xor-shuffle tree per wave, then thread 0 adds
-// the waves in index order.  Same inputs, same result.  Every thread of the
+// combine_partials: a fixed DPP reduction per wave (wave_reduce63), then
+// thread 0 adds the waves in index order.  Same inputs, same result.  Every thread of the
 // block must call both.
 constexpr int FOLD_U = 8;  // blocks per thread whose loads are issued together
 template <int BLK, bool ELEM>
@@ -615,9 +615,9 @@ template <int NV, int BLK>
 __device__ inline void combine_partials(double* acc, const bool* is_min, double (*s_tmp)[NV], double* out) {
     const uint32_t t = threadIdx.x;
 #pragma unroll
-    for (int q = 0; q < NV; ++q) acc[q] = is_min[q] ? wave_min(acc[q]) : wave_sum(acc[q]);
+    for (int q = 0; q < NV; ++q) acc[q] = is_min[q] ? wave_reduce63<true>(acc[q]) : wave_reduce63<false>(acc[q]);
     const int w = t >> 6;
-    if ((t & 63) == 0)
+    if ((t & 63) == 63)
 #pragma unroll
         for (int q = 0; q < NV; ++q) s_tmp[w][q] = acc[q];
     __syncthreads();
@@ -1099,8 +1099,39 @@ __device__ __forceinline__ bool pipe_tile(const JobDev* __restrict__ jobs, const
             element_mixed_P(J, act ? e : J.ne - 1, ES, dc.dt_i_1, dc.kind, dc.time_n, act, asmode, Le, dts, en, errf,
                             cur, nbn, pn, ESn, gn);
         if (!pn.done) pipe_prefetch(pn, ESn, gn);
+        if (!act) {
+            Le = dts = __builtin_inf();
+            en[0] = en[1] = en[2] = 0.0;
+        }
+        // block partials: DPP wave reductions (into lane 63), then the
+        // [X2 1/m] row of `cur` across the waves
+        Le = wave_reduce63<true>(Le);
+        dts = wave_reduce63<true>(dts);
+        en[0] = wave_reduce63<false>(en[0]);
+        en[1] = wave_reduce63<false>(en[1]);
+        en[2] = wave_reduce63<false>(en[2]);
+        uint32_t werr = 0;
+#pragma unroll
+        for (int b = 0; b < 3; ++b)
+            if (__ballot((errf >> b) & 1u)) werr |= 1u << b;
+        ENSC_LDS double(*s_red)[6] = (ENSC_LDS double(*)[6]) & cur[1][0];
+        const int w = threadIdx.x >> 6;
+        auto put_red = [&]() {
+            if ((threadIdx.x & 63) == 63) {
+                s_red[w][0] = Le;
+                s_red[w][1] = dts;
+                s_red[w][2] = en[0];
+                s_red[w][3] = en[1];
+                s_red[w][4] = en[2];
+                s_red[w][5] = (double)werr;
+            }
+        };
         out.pf = nullptr;
         if (asmode) {
+            // the row is free: every wave is past the assembly's barriers, the
+            // first of which follows every corner read; the barrier below
+            // ends the last assembly phase and publishes the wave values
+            put_red();
             const ENSC_LDS float* acc = pipe_acc(cur);
             lds_sync();
 #pragma unroll
@@ -1113,35 +1144,11 @@ __device__ __forceinline__ bool pipe_tile(const JobDev* __restrict__ jobs, const
             }
             out.pf = static_cast<float*>(J.part_f) + (size_t)bm.z * 3;
             out.nbn = nbn;
+        } else {
+            lds_sync();  // every wave's corner reads are done
+            put_red();
+            lds_sync();
         }
-        if (!act) {
-            Le = dts = __builtin_inf();
-            en[0] = en[1] = en[2] = 0.0;
-        }
-        // block partials: DPP wave reductions (into lane 63), then the
-        // [X2 1/m] row of `cur` (free once every wave is past the barrier
-        // below) across the waves
-        Le = wave_reduce63<true>(Le);
-        dts = wave_reduce63<true>(dts);
-        en[0] = wave_reduce63<false>(en[0]);
-        en[1] = wave_reduce63<false>(en[1]);
-        en[2] = wave_reduce63<false>(en[2]);
-        uint32_t werr = 0;
-#pragma unroll
-        for (int b = 0; b < 3; ++b)
-            if (__ballot((errf >> b) & 1u)) werr |= 1u << b;
-        lds_sync();
-        ENSC_LDS double(*s_red)[6] = (ENSC_LDS double(*)[6]) & cur[1][0];
-        const int w = threadIdx.x >> 6;
-        if ((threadIdx.x & 63) == 63) {
-            s_red[w][0] = Le;
-            s_red[w][1] = dts;
-            s_red[w][2] = en[0];
-            s_red[w][3] = en[1];
-            s_red[w][4] = en[2];
-            s_red[w][5] = (double)werr;
-        }
-        lds_sync();
         if (threadIdx.x == 0) {
             double r0 = s_red[0][0], r1 = s_red[0][1], r2 = s_red[0][2], r3 = s_red[0][3], r4 = s_red[0][4];
             uint32_t er = (uint32_t)s_red[0][5];
