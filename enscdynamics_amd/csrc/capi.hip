@@ -37,6 +37,14 @@ namespace {
 // partials (40 B per element block, 16 B per node block) of one engine; 4 MiB
 // hold a single mesh of ~10M elements.
 constexpr size_t STATIC_SCRATCH_BYTES = (size_t)4 << 20;
+// Element finalize: one block for a job of up to FIN_ONE block partials
+// (element or node), else one per FIN_SLICE partials, at most FIN_MAX_G
+// blocks; per job FIN_JOB_BYTES of device memory hold the slices' folds and
+// the ticket.  Measured on configuration 3: the ticket (fences and atomic)
+// costs ~1 us more than the split fold saves at 4k partials, and saves
+// ~3 us at 7.8k (8 blocks).
+constexpr uint32_t FIN_ONE = 4096, FIN_SLICE = 1024, FIN_MAX_G = 8;
+constexpr size_t FIN_JOB_BYTES = FIN_MAX_G * 8 * sizeof(double) + 256;
 
 // Device buffer that grows (contents are not preserved).
 struct DevBuf {
@@ -83,11 +91,12 @@ struct ensc_engine {
     // block-assembly jobs' blocks, the other mixed jobs', the FP64
     // block-assembly jobs' and the other FP64 jobs'
     std::vector<uint32_t> live;
-    DevBuf jobs_buf, b2e_buf, b2n_buf, b2f_buf, pe_buf, pn_buf;
+    DevBuf jobs_buf, b2e_buf, b2n_buf, b2f_buf, pe_buf, pn_buf, b2fin_buf, fin_buf;
     // where the kernels find the job table and block maps: the DevBufs, or
     // the static scratch array (ensc_engine_advance_static)
-    void *jobs_p = nullptr, *b2e_p = nullptr, *b2n_p = nullptr, *b2f_p = nullptr;
-    uint32_t nblk_f = 0;  // face blocks of the surface-pressure kernel
+    void *jobs_p = nullptr, *b2e_p = nullptr, *b2n_p = nullptr, *b2f_p = nullptr, *b2fin_p = nullptr;
+    uint32_t nblk_f = 0;    // face blocks of the surface-pressure kernel
+    uint32_t nblk_fin = 0;  // blocks of the element finalize (fin_g per job)
     uint32_t fsi_np = 0, fsi_nn = 0;  // virtual particles and FSI nodes of the single job
     bool in_scratch = false;      // the call in progress places its tables in scratch
     bool layout_scratch = false;  // the current layout lives in scratch
@@ -131,6 +140,12 @@ namespace {
 // for HBM bandwidth and for scaling hardware byte counters.
 __global__ __launch_bounds__(256) void k_copy_probe(const double2* __restrict__ a, double2* __restrict__ b, size_t n) {
     for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) b[i] = a[i];
+}
+
+// Zero fill of a small device array (the finalize tickets of a new layout);
+// a kernel, not a memset node, so it is safe wherever it is captured.
+__global__ __launch_bounds__(256) void k_zero(double* __restrict__ p, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) p[i] = 0.0;
 }
 
 template <bool M, int W>
@@ -283,7 +298,7 @@ ensc_status relayout(ensc_engine* E) {
     };
     // block maps: (job, block of the job, staged range) per launched block,
     // in XCD order, grouped as described at ensc_engine::live
-    std::vector<uint4> lp, ls, lg, ls64, lw64, lg64, lna, ln, lna64, ln64, lf;
+    std::vector<uint4> lp, ls, lg, ls64, lw64, lg64, lna, ln, lna64, ln64, lf, lfin;
     std::vector<JobDev> tab;
     std::vector<size_t> pe0, pn0;
     size_t ne_blocks = 0, n_blocks = 0;
@@ -321,6 +336,13 @@ ensc_status relayout(ensc_engine* E) {
         for (uint32_t b = 0; b < D.nblk_n; ++b) lnj.push_back(make_uint4(i, b, 0, 0));
         n_blocks += D.nblk_n;
         for (uint32_t b = 0; b < (D.n_dsf + FACE_BLK - 1) / FACE_BLK; ++b) lf.push_back(make_uint4(i, b, 0, 0));
+        // element finalize blocks (the job alone decides, so its sums' order
+        // is the same in any mix)
+        const uint32_t nmax = std::max(D.nblk_e, D.nblk_n);
+        const uint32_t g =
+            nmax <= FIN_ONE ? 1u : std::min<uint32_t>(FIN_MAX_G, (nmax + FIN_SLICE - 1) / FIN_SLICE);
+        J.dev.fin_g = g;
+        for (uint32_t q = 0; q < g; ++q) lfin.push_back(make_uint4(i, q, g, 0));
         J.dev.block_asm = as ? 1 : 0;
         E->n_asm += as;
         E->n_fp64 += !m;
@@ -345,12 +367,16 @@ ensc_status relayout(ensc_engine* E) {
     E->nblk_n64_asm = (uint32_t)lna64.size();
     E->nblk_n64 = (uint32_t)ln64.size();
     E->nblk_f = (uint32_t)lf.size();
-    const size_t sz[6] = {std::max<size_t>(1, E->live.size()) * sizeof(JobDev),
+    E->nblk_fin = (uint32_t)lfin.size();
+    const size_t sz[8] = {std::max<size_t>(1, E->live.size()) * sizeof(JobDev),
                           std::max<size_t>(1, b2e.size()) * sizeof(uint4), std::max<size_t>(1, b2n.size()) * sizeof(uint4),
                           std::max<size_t>(1, ne_blocks) * 5 * sizeof(double),
-                          std::max<size_t>(1, n_blocks) * 2 * sizeof(double), std::max<size_t>(1, lf.size()) * sizeof(uint4)};
+                          std::max<size_t>(1, n_blocks) * 2 * sizeof(double), std::max<size_t>(1, lf.size()) * sizeof(uint4),
+                          std::max<size_t>(1, lfin.size()) * sizeof(uint4),
+                          std::max<size_t>(1, E->live.size()) * FIN_JOB_BYTES};
     void* pe = nullptr;
     void* pn = nullptr;
+    void* fin = nullptr;
     if (E->in_scratch) {
         // the static scratch array: job table, maps and block partials back
         // to back, each 256-byte aligned; nothing is allocated
@@ -362,8 +388,8 @@ ensc_status relayout(ensc_engine* E) {
                                           std::to_string(STATIC_SCRATCH_BYTES) +
                                           "-byte static scratch (use ensc_engine_advance)");
         char* q = E->scratch_base;
-        void* part[6];
-        for (int k = 0; k < 6; ++k) {
+        void* part[8];
+        for (int k = 0; k < 8; ++k) {
             part[k] = q;
             q += round_up(sz[k], 256);
         }
@@ -373,14 +399,19 @@ ensc_status relayout(ensc_engine* E) {
         pe = part[3];
         pn = part[4];
         E->b2f_p = part[5];
+        E->b2fin_p = part[6];
+        fin = part[7];
     } else {
         if (!E->jobs_buf.ensure(sz[0]) || !E->b2e_buf.ensure(sz[1]) || !E->b2n_buf.ensure(sz[2]) ||
-            !E->pe_buf.ensure(sz[3]) || !E->pn_buf.ensure(sz[4]) || !E->b2f_buf.ensure(sz[5]))
+            !E->pe_buf.ensure(sz[3]) || !E->pn_buf.ensure(sz[4]) || !E->b2f_buf.ensure(sz[5]) ||
+            !E->b2fin_buf.ensure(sz[6]) || !E->fin_buf.ensure(sz[7]))
             return fail(ENSC_ERR_MEM, "layout allocation failed");
         E->jobs_p = E->jobs_buf.p;
         E->b2e_p = E->b2e_buf.p;
         E->b2n_p = E->b2n_buf.p;
         E->b2f_p = E->b2f_buf.p;
+        E->b2fin_p = E->b2fin_buf.p;
+        fin = E->fin_buf.p;
         pe = E->pe_buf.p;
         pn = E->pn_buf.p;
     }
@@ -390,6 +421,9 @@ ensc_status relayout(ensc_engine* E) {
         J.dev.ctl = E->d_ctl + E->live[i];
         J.dev.part_e = static_cast<double*>(pe) + pe0[i] * 5;
         J.dev.part_n = static_cast<double*>(pn) + pn0[i] * 2;
+        char* fj = static_cast<char*>(fin) + (size_t)i * FIN_JOB_BYTES;
+        J.dev.fin_part = reinterpret_cast<double*>(fj);
+        J.dev.fin_ticket = reinterpret_cast<uint32_t*>(fj + FIN_MAX_G * 8 * sizeof(double));
         tab.push_back(J.dev);
     }
     if (!tab.empty())
@@ -400,6 +434,12 @@ ensc_status relayout(ensc_engine* E) {
         HIPCHK(hipMemcpyAsync(E->b2n_p, b2n.data(), b2n.size() * sizeof(uint4), hipMemcpyHostToDevice, E->stream));
     if (!lf.empty())
         HIPCHK(hipMemcpyAsync(E->b2f_p, lf.data(), lf.size() * sizeof(uint4), hipMemcpyHostToDevice, E->stream));
+    if (!lfin.empty())
+        HIPCHK(hipMemcpyAsync(E->b2fin_p, lfin.data(), lfin.size() * sizeof(uint4), hipMemcpyHostToDevice, E->stream));
+    // the finalize tickets start at zero (a kernel, not a memset node)
+    hipLaunchKernelGGL(k_zero, dim3(1), dim3(256), 0, E->stream, static_cast<double*>(fin),
+                       sz[7] / sizeof(double));
+    HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(E->stream));
     E->staged = E->nblk_pipe + E->nblk_e + E->nblk_e64 + E->nblk_w64 > 0;
     E->lds_bytes = (size_t)std::max<uint32_t>(lds_nodes, 1) * (any_asm ? 9 * sizeof(double) + 12 : 9 * sizeof(double));
@@ -470,7 +510,10 @@ hipError_t launch_phase(ensc_engine* E, int phase) {
             }
             break;
         }
-        case 1: hipLaunchKernelGGL(k_fin_element<FIN_BLK>, dim3(nj), dim3(FIN_BLK), 0, E->stream, jobs); break;
+        case 1:
+            hipLaunchKernelGGL(k_fin_element<FIN_BLK>, dim3(E->nblk_fin), dim3(FIN_BLK), 0, E->stream, jobs,
+                               static_cast<const uint4*>(E->b2fin_p));
+            break;
         case 2: {
             // surface-pressure forces and the virtual particles' nodal forces
             // of the step, then the nodal update, then the particles follow
@@ -1895,7 +1938,7 @@ int64_t ensc_engine_device_bytes(ensc_engine* E) {
     for (auto& j : E->slots)
         if (j) t += (int64_t)j->arena.total;
     return t + (int64_t)(E->jobs_buf.cap + E->b2e_buf.cap + E->b2n_buf.cap + E->b2f_buf.cap + E->pe_buf.cap +
-                         E->pn_buf.cap) +
+                         E->pn_buf.cap + E->b2fin_buf.cap + E->fin_buf.cap) +
            (int64_t)E->ctl_cap * (int64_t)sizeof(Ctl);
 }
 

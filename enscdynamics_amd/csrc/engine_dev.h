@@ -169,6 +169,13 @@ struct JobDev {
     Ctl* ctl;
     double* part_e;  // [5][nblk_e]: min L, min dtStable, internal, hourglass, viscous work
     double* part_n;  // [2][nblk_n]: kinetic energy, external work of the step
+    // element finalize spread over fin_g blocks (jobs with many block
+    // partials): each block folds a fixed slice into fin_part[slice][0..6]
+    // and counts itself on fin_ticket; the last one folds the slices in
+    // slice order and resets the ticket
+    double* fin_part;     // [fin_g][8]
+    uint32_t* fin_ticket;
+    uint32_t fin_g, pad_fin;
 };
 
 }  // namespace ensc

@@ -580,12 +580,21 @@ __device__ __forceinline__ void element_fp64_W(ELEM_ARGS) {
 // thread 0 adds the waves in index order.  Same inputs, same result.  Every thread of the
 // block must call both.
 constexpr int FOLD_U = 8;  // blocks per thread whose loads are issued together
+// The slice [slice, slice + 1) of `slices` equal parts of the partials (in
+// ascending block order) is folded; one slice folds them all.
 template <int BLK, bool ELEM>
-__device__ inline void fold_partials(const JobDev& J, const bool* is_min, double* acc) {
+__device__ inline void fold_partials(const JobDev& J, const bool* is_min, double* acc, uint32_t slice = 0,
+                                     uint32_t slices = 1) {
     const uint32_t t = threadIdx.x;
 #pragma unroll
     for (int q = 0; q < 7; ++q) acc[q] = is_min[q] ? __builtin_inf() : 0.0;
-    const uint32_t nn = J.nblk_n, ne = ELEM ? J.nblk_e : 0u;
+    const uint32_t nn_all = J.nblk_n, ne_all = ELEM ? J.nblk_e : 0u;
+    const uint32_t sn = (nn_all + slices - 1) / slices, se = (ne_all + slices - 1) / slices;
+    const uint32_t n0 = sn * slice, e0 = se * slice;
+    const uint32_t nn = nn_all > n0 ? (nn_all - n0 < sn ? nn_all - n0 : sn) : 0u;  // this slice's counts
+    const uint32_t ne = ne_all > e0 ? (ne_all - e0 < se ? ne_all - e0 : se) : 0u;
+    const ENSC_GLB double* pn = (const ENSC_GLB double*)J.part_n;
+    const ENSC_GLB double* pe = (const ENSC_GLB double*)J.part_e;
     const uint32_t nmax = nn > ne ? nn : ne;
     for (uint32_t b0 = t; b0 < nmax; b0 += FOLD_U * BLK) {
         double v[FOLD_U][7];
@@ -594,11 +603,13 @@ __device__ inline void fold_partials(const JobDev& J, const bool* is_min, double
 #pragma unroll
         for (int u = 0; u < FOLD_U; ++u) {
             const uint32_t b = b0 + u * BLK;
-            const uint32_t bn = nn ? (b < nn ? b : nn - 1) : 0u, be = ne ? (b < ne ? b : ne - 1) : 0u;
+            const uint32_t bn = n0 + (nn ? (b < nn ? b : nn - 1) : 0u), be = e0 + (ne ? (b < ne ? b : ne - 1) : 0u);
+            const uint32_t bn_ok = nn_all ? bn : 0u, be_ok = ne_all ? be : 0u;
 #pragma unroll
-            for (int q = 0; q < 2; ++q) v[u][q] = J.part_n[(size_t)q * nn + bn];
+            for (int q = 0; q < 2; ++q) v[u][q] = pn[(size_t)q * nn_all + (nn_all ? (bn_ok < nn_all ? bn_ok : nn_all - 1) : 0u)];
 #pragma unroll
-            for (int q = 2; q < 7; ++q) v[u][q] = J.part_e[(size_t)(q - 2) * ne + be];
+            for (int q = 2; q < 7; ++q)
+                v[u][q] = pe[(size_t)(q - 2) * ne_all + (ne_all ? (be_ok < ne_all ? be_ok : ne_all - 1) : 0u)];
 #pragma unroll
             for (int q = 0; q < 2; ++q) v[u][q] = (b < nn) ? v[u][q] : 0.0;
 #pragma unroll
@@ -865,32 +876,66 @@ __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(WPE))) void
     }
 }
 
-// Per-job end of the element pass: one block per job in the job table.  It
-// closes the previous nodal update if still open (node_phase_finish), and
-// reduces the job's element block partials (written by k_element; the
-// launch boundary orders them) to advance the job's time-step scalars
-// exactly as update_minVertex_perMaterial_and_dt and the run-loop clamps do
-// (exDyna3D.cpp:663-686, 90-97, 775-780).  Both sets of partials are folded
-// in one pass so their loads overlap.
+// Per-job end of the element pass: fin_g blocks per job (the map holds
+// (job, slice, fin_g)).  It closes the previous nodal update if still open
+// (node_phase_finish), and reduces the job's element block partials
+// (written by k_element; the launch boundary orders them) to advance the
+// job's time-step scalars exactly as update_minVertex_perMaterial_and_dt and
+// the run-loop clamps do (exDyna3D.cpp:663-686, 90-97, 775-780).  Both sets
+// of partials are folded in one pass so their loads overlap.  A job with
+// many partials spreads the fold over several blocks (one CU's loads bound
+// it otherwise): each folds its slice, publishes it and counts itself on the
+// job's ticket (device-scope fences around the atomic); the block that
+// counts last folds the slices in slice order, so the result does not
+// depend on which block that is, and decides.
 template <int BLK>
-__global__ __launch_bounds__(BLK) void k_fin_element(const JobDev* __restrict__ jobs) {
+__global__ __launch_bounds__(BLK) void k_fin_element(const JobDev* __restrict__ jobs,
+                                                     const uint4* __restrict__ finmap) {
 #pragma clang fp contract(off)
-    const JobDev& J = jobs[blockIdx.x];
+    const uint4 fm = finmap[blockIdx.x];  // (job, slice, slices, -)
+    const JobDev& J = jobs[fm.x];
+    const uint32_t slice = fm.y, slices = fm.z;
     Ctl* ctl = J.ctl;
     // one copy of the job's scalars, loaded at once; nothing else writes them
-    // while this kernel runs, and thread 0 updates the copy and stores it
+    // while this kernel runs (the job's deciding block stores them once every
+    // block has counted itself, i.e. after every block has read them)
     const Ctl c0 = *ctl;
     // values 0-1: node partials (sums); 2-6: element partials (2 minima, 3 sums)
     const bool is_min[7] = {false, false, true, true, false, false, false};
     double acc[7];
-    fold_partials<BLK, true>(J, is_min, acc);
+    fold_partials<BLK, true>(J, is_min, acc, slice, slices);
     const bool node_open = c0.kind == KIND_STEP;
     const Decision dc = decide(c0, J.factor, J.wos);
     const bool elem = dc.kind != KIND_NONE;
-    if (!node_open && !elem) return;
+    if (!node_open && !elem) return;  // alike in every block of the job
     __shared__ double s_red[BLK / 64][7];
     double tot[7];
     combine_partials<7, BLK>(acc, is_min, s_red, tot);
+    if (slices > 1) {
+        __shared__ uint32_t s_last;
+        if (threadIdx.x == 0) {
+            ENSC_GLB double* fp = (ENSC_GLB double*)J.fin_part;
+#pragma unroll
+            for (int q = 0; q < 7; ++q) fp[slice * 8 + q] = tot[q];
+            __threadfence();  // the slice before the count
+            s_last = atomicAdd(J.fin_ticket, 1u) == slices - 1 ? 1u : 0u;
+        }
+        __syncthreads();
+        if (!s_last) return;
+        if (threadIdx.x == 0) {
+            __threadfence();  // every counted slice after the count
+            const ENSC_GLB double* fp = (const ENSC_GLB double*)J.fin_part;
+#pragma unroll
+            for (int q = 0; q < 7; ++q) tot[q] = fp[q];
+            for (uint32_t g = 1; g < slices; ++g)
+#pragma unroll
+                for (int q = 0; q < 7; ++q) {
+                    const double x = fp[g * 8 + q];
+                    tot[q] = is_min[q] ? (x < tot[q] ? x : tot[q]) : tot[q] + x;
+                }
+            *J.fin_ticket = 0u;  // the next launch's count starts from zero
+        }
+    }
     if (threadIdx.x == 0) {
         Ctl c = c0;
         if (node_open) node_phase_finish(J, c, tot[0], tot[1]);

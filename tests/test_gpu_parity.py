@@ -304,6 +304,36 @@ def test_full_size_cube_fp64_vs_oracle_few_steps():
         assert err <= TOL_FP64, f
 
 
+def test_fp64_full_size_cube_split_finalize_is_deterministic():
+    # the 1M-element cube in FP64 has 7813 element blocks, so its element
+    # finalize folds the partials over 8 blocks and the last block to count
+    # itself decides; which block that is varies from run to run, the
+    # result must not: two runs agree bit for bit, scalars included, and
+    # dt and the energies agree with the oracle over the same steps
+    m = decks.config3(n=100, steps=3000)
+    runs = []
+    for _ in range(2):
+        e = eng.Engine(m, eng.FP64, reorder=True)
+        e.run_steps(40)
+        runs.append(e)
+    a, b = runs
+    assert a.layout()["elem_blocks"] > 4096
+    for f in ("u", "v", "sigma", "qia", "epsp"):
+        assert np.array_equal(a.get(f), b.get(f)), f
+    sa, sb = a.scalars(), b.scalars()
+    for k in ("time", "dt_i", "dt_i_1", "dts_min", "ke", "ie", "hg", "vis", "wext"):
+        assert sa[k] == sb[k], k
+    o = Oracle(m, 0).init()
+    o.run(40, [], 0)
+    so = o.scalars()
+    for k in ("time", "dt_i"):
+        assert sa[k] == pytest.approx(so[k], rel=1e-12), k
+    for k in ("ie", "hg", "vis", "wext", "ke"):
+        assert sa[k] == pytest.approx(so[k], rel=1e-9, abs=1e-9 * max(abs(so["wext"]), so["ke0"])), k
+    print(f"[c3 fp64 split finalize] 40 steps, two runs bit-identical, dt {sa['dt_i']:.6e} "
+          f"(oracle {so['dt_i']:.6e})", flush=True)
+
+
 def test_mixed_vs_fp64_full_plastic_cube_3000_steps():
     # configuration 3 as specified: 1M elements, J2, run to the end time of
     # 3000 initial stable steps (the step shrinks as the cube deforms, so a
