@@ -421,30 +421,32 @@ __device__ __forceinline__ void lds_sync() {
 // Wave reductions of doubles through DPP row moves (no LDS): row shifts by
 // 1, 2, 4 and 8 leave each 16-lane row's total in its lane 15, then
 // row_bcast:15 and row_bcast:31 carry it across rows, so lane 63 holds the
-// wave's total (min or sum).  A lane whose source lies outside its row, or
-// whose row the mask leaves out, receives the identity.  The order of the
-// operations is fixed, so the result is too.
+// wave's total (min or sum).  Lane 63's value depends only on lanes whose
+// source lies inside their row at every step (15 <- 14, 13, 11, 7 and
+// their own chains), so the moves need no identity: a lane without a
+// source reads 0 (bound control) or keeps its register, and its result,
+// wrong, feeds no lane that lane 63 reads.  The order of the operations is
+// fixed, so the result is too.
 template <int CTRL, int ROWS>
-__device__ __forceinline__ double dpp_move(double v, double identity) {
-    const long long x = __double_as_longlong(v), o = __double_as_longlong(identity);
-    const int lo = __builtin_amdgcn_update_dpp((int)o, (int)x, CTRL, ROWS, 0xF, false);
-    const int hi = __builtin_amdgcn_update_dpp((int)(o >> 32), (int)(x >> 32), CTRL, ROWS, 0xF, false);
+__device__ __forceinline__ double dpp_move(double v) {
+    const long long x = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_mov_dpp((int)x, CTRL, ROWS, 0xF, true);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(x >> 32), CTRL, ROWS, 0xF, true);
     return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
 }
 template <bool MIN>
 __device__ __forceinline__ double dpp_op(double a, double b) {
-    return MIN ? (b < a ? b : a) : a + b;
+    return MIN ? __builtin_fmin(a, b) : a + b;
 }
 template <bool MIN>
 __device__ __forceinline__ double wave_reduce63(double v) {
-    const double id = MIN ? __builtin_inf() : 0.0;
-    v = dpp_op<MIN>(v, dpp_move<0x111, 0xF>(v, id));  // row_shr:1
-    v = dpp_op<MIN>(v, dpp_move<0x112, 0xF>(v, id));  // row_shr:2
-    v = dpp_op<MIN>(v, dpp_move<0x114, 0xF>(v, id));  // row_shr:4
-    v = dpp_op<MIN>(v, dpp_move<0x118, 0xF>(v, id));  // row_shr:8
-    v = dpp_op<MIN>(v, dpp_move<0x142, 0xA>(v, id));  // row_bcast:15 into rows 1, 3
-    v = dpp_op<MIN>(v, dpp_move<0x143, 0xC>(v, id));  // row_bcast:31 into rows 2, 3
-    return v;                                          // the wave's value in lane 63
+    v = dpp_op<MIN>(v, dpp_move<0x111, 0xF>(v));  // row_shr:1
+    v = dpp_op<MIN>(v, dpp_move<0x112, 0xF>(v));  // row_shr:2
+    v = dpp_op<MIN>(v, dpp_move<0x114, 0xF>(v));  // row_shr:4
+    v = dpp_op<MIN>(v, dpp_move<0x118, 0xF>(v));  // row_shr:8
+    v = dpp_op<MIN>(v, dpp_move<0x142, 0xA>(v));  // row_bcast:15 into rows 1, 3
+    v = dpp_op<MIN>(v, dpp_move<0x143, 0xC>(v));  // row_bcast:31 into rows 2, 3
+    return v;                                      // the wave's value in lane 63
 }
 
 template <class T>
