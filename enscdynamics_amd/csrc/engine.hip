@@ -443,6 +443,26 @@ ensc_status prepare_job(Job& J, int engine_precision, int reorder, int blk_e, in
             }
         }
     }
+    // Block assembly adds corner k's forces in phase k = 0..7; the first
+    // (element, corner) of a block to reach a node in that order stores its
+    // force instead of adding it, so the accumulators need no clearing.  It
+    // is marked by bit 15 of the corner's slot (slots stay below 2^15).
+    if (J.asm_capable) {
+        const size_t nb = blk_ptr.size() - 1;
+        std::vector<uint8_t> seen(J.lds_nodes + 1, 0);
+        for (size_t b = 0; b < nb; ++b) {
+            const size_t e0 = b * blk_e, e1 = std::min(Ne, e0 + blk_e);
+            std::fill(seen.begin(), seen.end(), 0);
+            for (int k = 0; k < 8; ++k)
+                for (size_t e = e0; e < e1; ++e) {
+                    uint16_t& l = lconn[(size_t)k * es + e];
+                    if (!seen[l]) {
+                        seen[l] = 1;
+                        l |= 0x8000u;
+                    }
+                }
+        }
+    }
     // each node's positions in blk_nodes (one per element block holding it),
     // ascending: the order the nodal update adds the block partials
     std::vector<uint32_t> pos_ptr(Nn + 1, 0), pos_list(blk_nodes.size());

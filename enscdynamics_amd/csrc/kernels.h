@@ -505,6 +505,7 @@ __device__ __noinline__ V3 bc_override(const JobDev& J, uint32_t g, double t, V3
         bool asmode, double &Le, double &dts, double *en, uint32_t &errf, const ENSC_LDS double *lds,     \
         ENSC_LDS R *acc
 #define NODE(k) ES.nd[k]
+#define FIRST(k) 0u  // gathered corners: no block assembly
 #define NVAL(k, i)                                                                                      \
     ((i) < 3 ? (MIXED ? gld((const double*)J.pos, NODE(k) * POS_REC + (i)) +                           \
                             gld((const double*)J.vu, NODE(k) * VU_REC + 3 + (i))                           \
@@ -532,13 +533,17 @@ __device__ __forceinline__ void element_mixed_G(ELEM_ARGS) {
 #include "elem_body_mixed.inc"
 }
 #undef NODE
+#undef FIRST
 #undef NVAL
 #undef DYN_BC_OVERRIDE
 #undef CORNER_V
 
 // Staged record in LDS: [x0 x1 x2 v0 v1 v2 u0 u1 u2] as copied from the node
-// record (x0..2 = the record's X).
-#define NODE(k) (((k)&1) ? (ES.nd[(k) >> 1] >> 16) : (ES.nd[(k) >> 1] & 0xFFFFu))
+// record (x0..2 = the record's X).  A corner's 16-bit slot carries in bit 15
+// whether it is the first contribution to its node in the assembly's phase
+// order (FIRST(k)).
+#define NODE(k) (((k)&1) ? ((ES.nd[(k) >> 1] >> 16) & 0x7FFFu) : (ES.nd[(k) >> 1] & 0x7FFFu))
+#define FIRST(k) (((k)&1) ? (ES.nd[(k) >> 1] >> 31) : ((ES.nd[(k) >> 1] >> 15) & 1u))
 #define NVAL(k, i) \
     ((MIXED && (i) < 3) ? lds[NODE(k) * 9 + (i)] + lds[NODE(k) * 9 + 6 + (i)] : lds[NODE(k) * 9 + (i)])
 #define DYN_BC_OVERRIDE(k, v)  // applied once per node while staging
@@ -563,6 +568,7 @@ __device__ __forceinline__ void element_fp64_W(ELEM_ARGS) {
 #include "elem_body_walsh64.inc"
 }
 #undef NODE
+#undef FIRST
 #undef NVAL
 #undef DYN_BC_OVERRIDE
 #undef CORNER_V
@@ -1007,16 +1013,9 @@ __device__ __forceinline__ void pipe_copy(const JobDev& J, uint32_t nbn, const u
     }
 }
 // Accumulators of the current tile in its [X0 X1] row, 4 floats per node,
-// zeroed after every wave has read its corner values.
+// written once every wave has read its corner values (each node's first
+// contribution stores, so they need no clearing).
 __device__ __forceinline__ ENSC_LDS float* pipe_acc(PipeBuf buf) { return (ENSC_LDS float*)&buf[0][0]; }
-__device__ __forceinline__ void pipe_acc_clear(ENSC_LDS float* acc, uint32_t nbn) {
-    lds_sync();
-    for (uint32_t i = threadIdx.x; i < nbn; i += PIPE_BLK) {
-        acc[4 * i] = 0.f;
-        acc[4 * i + 1] = 0.f;
-        acc[4 * i + 2] = 0.f;
-    }
-}
 
 // What a tile prefetches for the next one while its own corner forces and
 // assembly run: the next tile's element state (into the registers the
@@ -1039,12 +1038,13 @@ __device__ __forceinline__ void pipe_prefetch(PipeNext& p, ElemState<float>& ESn
 #define PX(n, i) ((i) == 0 ? buf[0][n].a : ((i) == 1 ? buf[0][n].b : buf[1][n].a))
 #define PV(n, i) ((i) == 0 ? buf[2][n].a : ((i) == 1 ? buf[2][n].b : buf[3][n].a))
 #define PU(n, i) ((i) == 0 ? buf[3][n].b : ((i) == 1 ? buf[4][n].a : buf[4][n].b))
-#define NODE(k) (((k)&1) ? (ES.nd[(k) >> 1] >> 16) : (ES.nd[(k) >> 1] & 0xFFFFu))
+#define NODE(k) (((k)&1) ? ((ES.nd[(k) >> 1] >> 16) & 0x7FFFu) : (ES.nd[(k) >> 1] & 0x7FFFu))
+#define FIRST(k) (((k)&1) ? (ES.nd[(k) >> 1] >> 31) : ((ES.nd[(k) >> 1] >> 15) & 1u))
 #define NVAL(k, i) \
     ((i) < 3 ? PX(NODE(k), (i)) + PU(NODE(k), (i)) : ((i) < 6 ? PV(NODE(k), (i)-3) : PU(NODE(k), (i)-6)))
 #define CORNER_V(k, i) NVAL(k, 3 + (i))
 #define ACC_W 4
-#define ASM_BEGIN pipe_acc_clear(acc, nbn)
+#define ASM_BEGIN ((void)0)  // first contributions store (FIRST), nothing to clear
 #define PREFETCH_HOOK pipe_prefetch(pn, ESn, gn)
 __device__ __forceinline__ void element_mixed_P(const JobDev& J, uint32_t e, const ElemState<float>& ES, double dt,
                                                 int32_t kind, double time_n, bool act, bool asmode, double& Le,
@@ -1057,6 +1057,7 @@ __device__ __forceinline__ void element_mixed_P(const JobDev& J, uint32_t e, con
 #include "elem_body_mixed.inc"
 }
 #undef NODE
+#undef FIRST
 #undef NVAL
 #undef CORNER_V
 #undef PX
