@@ -663,9 +663,11 @@ ensc_status engine_new(const ensc_options* opts, std::unique_ptr<ensc_engine>& E
     E->staging_opt = o.staging;
     auto okblk = [](int b) { return b == 64 || b == 128 || b == 256; };
     // defaults from block-size / occupancy sweeps on MI355X (configuration 3):
-    // FP64 128 threads at one wave per SIMD (the body needs > 256 registers),
-    // mixed 256 threads at two
-    E->blk_e = okblk(o.block_elem) ? o.block_elem : (o.precision == ENSC_FP64 ? 128 : 256);
+    // 256 elements per block (two waves per SIMD) for the mixed and the
+    // default FP64 bodies; 128 for FP64 in the reference's order (its body
+    // needs > 256 registers: one wave per SIMD)
+    const bool ref_order = o.precision == ENSC_FP64 && o.fp64_order == 1;
+    E->blk_e = okblk(o.block_elem) ? o.block_elem : (ref_order ? 128 : 256);
     E->blk_n = okblk(o.block_node) ? o.block_node : 256;
     if (o.waves < 0 || o.waves > 3) return fail(ENSC_ERR_ARG, "waves must be 0 (default), 1, 2 or 3");
     if (o.fp64_jobs < 0 || o.fp64_jobs > 2) return fail(ENSC_ERR_ARG, "fp64_jobs must be 0 (auto), 1 or 2");

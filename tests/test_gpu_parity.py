@@ -305,15 +305,16 @@ def test_full_size_cube_fp64_vs_oracle_few_steps():
 
 
 def test_fp64_full_size_cube_split_finalize_is_deterministic():
-    # the 1M-element cube in FP64 has 7813 element blocks, so its element
-    # finalize folds the partials over 8 blocks and the last block to count
-    # itself decides; which block that is varies from run to run, the
-    # result must not: two runs agree bit for bit, scalars included, and
-    # dt and the energies agree with the oracle over the same steps
+    # the 1M-element cube in FP64 with 128-element blocks has 7813 element
+    # blocks, so its element finalize folds the partials over 8 blocks and
+    # the last block to count itself decides; which block that is varies
+    # from run to run, the result must not: two runs agree bit for bit,
+    # scalars included, and dt and the energies agree with the oracle over
+    # the same steps
     m = decks.config3(n=100, steps=3000)
     runs = []
     for _ in range(2):
-        e = eng.Engine(m, eng.FP64, reorder=True)
+        e = eng.Engine(m, eng.FP64, reorder=True, block_elem=128)
         e.run_steps(40)
         runs.append(e)
     a, b = runs
