@@ -235,6 +235,23 @@ __device__ __forceinline__ float rcp_nr(float x) {
     return __builtin_fmaf(r, __builtin_fmaf(-x, r, 1.0f), r);
 }
 
+// FP64 square root for the mixed-precision body's dt terms, whose inputs
+// carry FP32 accuracy: the hardware reciprocal square root refined by one
+// coupled Newton step for sqrt(x) and 1 / (2 sqrt(x)), then one correction of
+// sqrt(x) (about 1 ulp; the library's correctly rounded root adds a second
+// correction and range scaling).  Zero, infinity and NaN are returned as
+// they are (the roots of 0 and +inf), like every other value outside the
+// positive normals, which these inputs never are.
+__device__ __forceinline__ double sqrt_nr(double x) {
+    const double y = __builtin_amdgcn_rsq(x);
+    double s = x * y, h = 0.5 * y;
+    const double r = __builtin_fma(-s, h, 0.5);
+    s = __builtin_fma(s, r, s);
+    h = __builtin_fma(h, r, h);
+    s = __builtin_fma(__builtin_fma(-s, s, x), h, s);
+    return __builtin_amdgcn_class(x, 0x100) ? s : x;  // class bit 8: positive normal
+}
+
 // The cofactor inverse for the mixed-precision body: contracted products and
 // one reciprocal (rcp_nr) in place of the IEEE division.  Returns the
 // determinant (first-column expansion); *invdet gets its reciprocal.
@@ -1069,14 +1086,14 @@ __device__ __forceinline__ void element_mixed_P(const JobDev& J, uint32_t e, con
 
 // A tile's results held in registers until the next tile has started: its
 // partial nodal forces (this thread's staged slots tid and tid + PIPE_BLK)
-// and the block's five values (min edge, min stable dt, three work terms;
-// thread 0).  Stored last in the tile, they would be the youngest memory
-// operations at the next tile's start, whose wait would then also wait for
-// their completion; stored right after that wait they complete during the
-// tile.
+// and the block's five values (min edge, min stable dt, three work terms),
+// value w in lane 63 of wave w and value 4 in lane 63 of wave 0 as well.
+// Stored last in the tile, they would be the youngest memory operations at
+// the next tile's start, whose wait would then also wait for their
+// completion; stored right after that wait they complete during the tile.
 struct PipeOut {
     float f[2][3];
-    double r[5];
+    double r[2];     // lane 63 of wave w: value w, and value 4 in wave 0
     float* pf;       // the tile's first partial record (null: nothing to store)
     double* pe;      // the tile's block partials (stride nb)
     uint32_t nbn, nb;
@@ -1094,9 +1111,11 @@ __device__ __forceinline__ void pipe_flush(PipeOut& o) {
             }
         }
     }
-    if (threadIdx.x == 0)
-#pragma unroll
-        for (int q = 0; q < 5; ++q) gst(o.pe, q * o.nb) = o.r[q];
+    if ((threadIdx.x & 63) == 63) {
+        const uint32_t w = threadIdx.x >> 6;
+        gst(o.pe, w * o.nb) = o.r[0];
+        if (w == 0) gst(o.pe, 4 * o.nb) = o.r[1];
+    }
     o.pe = nullptr;
 }
 
@@ -1151,34 +1170,30 @@ __device__ __forceinline__ bool pipe_tile(const JobDev* __restrict__ jobs, const
             Le = dts = __builtin_inf();
             en[0] = en[1] = en[2] = 0.0;
         }
-        // block partials: DPP wave reductions (into lane 63), then the
-        // [X2 1/m] row of `cur` across the waves
-        Le = wave_reduce63<true>(Le);
-        dts = wave_reduce63<true>(dts);
-        en[0] = wave_reduce63<false>(en[0]);
-        en[1] = wave_reduce63<false>(en[1]);
-        en[2] = wave_reduce63<false>(en[2]);
+        // block partials: every thread's five values go to the [X2 1/m]
+        // and [v0 v1] rows of `cur` ([5][PIPE_BLK] doubles); then wave w
+        // folds value w (wave 0 also value 4) over the four waves' lanes in
+        // wave order and reduces the wave through DPP into lane 63: one
+        // wave reduction per value instead of one per value in every wave
         uint32_t werr = 0;
 #pragma unroll
         for (int b = 0; b < 3; ++b)
             if (__ballot((errf >> b) & 1u)) werr |= 1u << b;
-        ENSC_LDS double(*s_red)[6] = (ENSC_LDS double(*)[6]) & cur[1][0];
-        const int w = threadIdx.x >> 6;
+        if (werr && (threadIdx.x & 63) == 0) atomicOr(&ctl->err, werr);
+        ENSC_LDS double(*s_val)[PIPE_BLK] = (ENSC_LDS double(*)[PIPE_BLK]) & cur[1][0];
         auto put_red = [&]() {
-            if ((threadIdx.x & 63) == 63) {
-                s_red[w][0] = Le;
-                s_red[w][1] = dts;
-                s_red[w][2] = en[0];
-                s_red[w][3] = en[1];
-                s_red[w][4] = en[2];
-                s_red[w][5] = (double)werr;
-            }
+            const uint32_t i = threadIdx.x;
+            s_val[0][i] = Le;
+            s_val[1][i] = dts;
+            s_val[2][i] = en[0];
+            s_val[3][i] = en[1];
+            s_val[4][i] = en[2];
         };
         out.pf = nullptr;
         if (asmode) {
-            // the row is free: every wave is past the assembly's barriers, the
-            // first of which follows every corner read; the barrier below
-            // ends the last assembly phase and publishes the wave values
+            // the rows are free: every wave is past the assembly's barriers,
+            // the first of which follows every corner read; the barrier below
+            // ends the last assembly phase and publishes the values
             put_red();
             const ENSC_LDS float* acc = pipe_acc(cur);
             lds_sync();
@@ -1197,23 +1212,24 @@ __device__ __forceinline__ bool pipe_tile(const JobDev* __restrict__ jobs, const
             put_red();
             lds_sync();
         }
-        if (threadIdx.x == 0) {
-            double r0 = s_red[0][0], r1 = s_red[0][1], r2 = s_red[0][2], r3 = s_red[0][3], r4 = s_red[0][4];
-            uint32_t er = (uint32_t)s_red[0][5];
-            for (int q = 1; q < PIPE_BLK / 64; ++q) {
-                r0 = s_red[q][0] < r0 ? s_red[q][0] : r0;
-                r1 = s_red[q][1] < r1 ? s_red[q][1] : r1;
-                r2 += s_red[q][2];
-                r3 += s_red[q][3];
-                r4 += s_red[q][4];
-                er |= (uint32_t)s_red[q][5];
+        {
+            static_assert(PIPE_BLK == 256, "four waves, one per block value (wave 0 two)");
+            const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;  // w is uniform in the wave
+            auto fold = [&](uint32_t q, bool mn) {
+                double r = s_val[q][lane];
+#pragma unroll
+                for (int x = 1; x < PIPE_BLK / 64; ++x) {
+                    const double v = s_val[q][lane + 64 * x];
+                    r = mn ? (v < r ? v : r) : r + v;
+                }
+                return mn ? wave_reduce63<true>(r) : wave_reduce63<false>(r);
+            };
+            if (w < 2) {
+                out.r[0] = fold(w, true);  // min edge, min stable dt
+            } else {
+                out.r[0] = fold(w, false);  // internal and hourglass work
             }
-            out.r[0] = r0;
-            out.r[1] = r1;
-            out.r[2] = r2;
-            out.r[3] = r3;
-            out.r[4] = r4;
-            if (er) atomicOr(&ctl->err, er);
+            if (w == 0) out.r[1] = fold(4, false);  // viscous work
         }
         out.pe = J.part_e + lb;
         out.nb = J.nblk_e;
