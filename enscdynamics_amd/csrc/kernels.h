@@ -992,6 +992,9 @@ __global__ __launch_bounds__(BLK) void k_fin_element(const JobDev* __restrict__ 
 // [X2 1/m] row.
 constexpr int PIPE_BLK = 256;
 constexpr int PIPE_CAP = 512;  // staged nodes per tile; larger tiles use k_element
+constexpr int PIPE_NW = PIPE_BLK / 64;                // waves per block
+constexpr int PIPE_NV = (5 + PIPE_NW - 1) / PIPE_NW;  // block values reduced per wave (at most)
+static_assert(PIPE_CAP == 2 * PIPE_BLK, "two staged slots per thread");
 struct alignas(16) Pair {
     double a, b;
 };
@@ -1086,14 +1089,14 @@ __device__ __forceinline__ void element_mixed_P(const JobDev& J, uint32_t e, con
 
 // A tile's results held in registers until the next tile has started: its
 // partial nodal forces (this thread's staged slots tid and tid + PIPE_BLK)
-// and the block's five values (min edge, min stable dt, three work terms),
-// value w in lane 63 of wave w and value 4 in lane 63 of wave 0 as well.
+// and the block's five values (min edge, min stable dt, three work terms):
+// value q in lane 63 of wave q mod PIPE_NW, as its r[q / PIPE_NW].
 // Stored last in the tile, they would be the youngest memory operations at
 // the next tile's start, whose wait would then also wait for their
 // completion; stored right after that wait they complete during the tile.
 struct PipeOut {
     float f[2][3];
-    double r[2];     // lane 63 of wave w: value w, and value 4 in wave 0
+    double r[PIPE_NV];
     float* pf;       // the tile's first partial record (null: nothing to store)
     double* pe;      // the tile's block partials (stride nb)
     uint32_t nbn, nb;
@@ -1113,8 +1116,11 @@ __device__ __forceinline__ void pipe_flush(PipeOut& o) {
     }
     if ((threadIdx.x & 63) == 63) {
         const uint32_t w = threadIdx.x >> 6;
-        gst(o.pe, w * o.nb) = o.r[0];
-        if (w == 0) gst(o.pe, 4 * o.nb) = o.r[1];
+#pragma unroll
+        for (int j = 0; j < PIPE_NV; ++j) {
+            const uint32_t q = w + j * PIPE_NW;
+            if (q < 5) gst(o.pe, q * o.nb) = o.r[j];
+        }
     }
     o.pe = nullptr;
 }
@@ -1172,7 +1178,7 @@ __device__ __forceinline__ bool pipe_tile(const JobDev* __restrict__ jobs, const
         }
         // block partials: every thread's five values go to the [X2 1/m]
         // and [v0 v1] rows of `cur` ([5][PIPE_BLK] doubles); then wave w
-        // folds value w (wave 0 also value 4) over the four waves' lanes in
+        // folds the values q with q mod PIPE_NW = w over the waves' lanes in
         // wave order and reduces the wave through DPP into lane 63: one
         // wave reduction per value instead of one per value in every wave
         uint32_t werr = 0;
@@ -1213,23 +1219,24 @@ __device__ __forceinline__ bool pipe_tile(const JobDev* __restrict__ jobs, const
             lds_sync();
         }
         {
-            static_assert(PIPE_BLK == 256, "four waves, one per block value (wave 0 two)");
             const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;  // w is uniform in the wave
             auto fold = [&](uint32_t q, bool mn) {
                 double r = s_val[q][lane];
 #pragma unroll
-                for (int x = 1; x < PIPE_BLK / 64; ++x) {
+                for (int x = 1; x < PIPE_NW; ++x) {
                     const double v = s_val[q][lane + 64 * x];
                     r = mn ? (v < r ? v : r) : r + v;
                 }
                 return mn ? wave_reduce63<true>(r) : wave_reduce63<false>(r);
             };
-            if (w < 2) {
-                out.r[0] = fold(w, true);  // min edge, min stable dt
-            } else {
-                out.r[0] = fold(w, false);  // internal and hourglass work
+#pragma unroll
+            for (int j = 0; j < PIPE_NV; ++j) {
+                const uint32_t q = w + j * PIPE_NW;
+                if (q < 2)
+                    out.r[j] = fold(q, true);  // min edge, min stable dt
+                else if (q < 5)
+                    out.r[j] = fold(q, false);  // internal, hourglass and viscous work
             }
-            if (w == 0) out.r[1] = fold(4, false);  // viscous work
         }
         out.pe = J.part_e + lb;
         out.nb = J.nblk_e;
