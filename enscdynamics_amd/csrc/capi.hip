@@ -37,13 +37,15 @@ namespace {
 // partials (40 B per element block, 16 B per node block) of one engine; 4 MiB
 // hold a single mesh of ~10M elements.
 constexpr size_t STATIC_SCRATCH_BYTES = (size_t)4 << 20;
-// Element finalize: one block for a job of up to FIN_ONE block partials
-// (element or node), else one per FIN_SLICE partials, at most FIN_MAX_G
-// blocks; per job FIN_JOB_BYTES of device memory hold the slices' folds and
-// the ticket.  Measured on configuration 3: the ticket (fences and atomic)
-// costs ~1 us more than the split fold saves at 4k partials, and saves
-// ~3 us at 7.8k (8 blocks).
-constexpr uint32_t FIN_ONE = 4096, FIN_SLICE = 1024, FIN_MAX_G = 8;
+// Element finalize: for a job of up to FIN_ONE block partials (element or
+// node), four blocks that each fold the partials of the scalars only they
+// write (FIN_ROLES); else one block per FIN_SLICE partials, at most FIN_MAX_G,
+// each folding a slice of every value, the last to count deciding; per job
+// FIN_JOB_BYTES of device memory hold the slices' folds and the ticket.
+// Measured on configuration 3: the ticket (fences and atomic) costs ~1 us
+// more than the split fold saves at 4k partials, and saves ~3 us at 7.8k
+// (8 blocks).
+constexpr uint32_t FIN_ONE = 4096, FIN_SLICE = 1024, FIN_MAX_G = 8, FIN_ROLES = 4;
 constexpr size_t FIN_JOB_BYTES = FIN_MAX_G * 8 * sizeof(double) + 256;
 
 // Device buffer that grows (contents are not preserved).
@@ -96,7 +98,7 @@ struct ensc_engine {
     // the static scratch array (ensc_engine_advance_static)
     void *jobs_p = nullptr, *b2e_p = nullptr, *b2n_p = nullptr, *b2f_p = nullptr, *b2fin_p = nullptr;
     uint32_t nblk_f = 0;    // face blocks of the surface-pressure kernel
-    uint32_t nblk_fin = 0;  // blocks of the element finalize (fin_g per job)
+    uint32_t nblk_fin = 0;  // blocks of the element finalize (FIN_ROLES or fin_g per job)
     uint32_t fsi_np = 0, fsi_nn = 0;  // virtual particles and FSI nodes of the single job
     bool in_scratch = false;      // the call in progress places its tables in scratch
     bool layout_scratch = false;  // the current layout lives in scratch
@@ -342,7 +344,10 @@ ensc_status relayout(ensc_engine* E) {
         const uint32_t g =
             nmax <= FIN_ONE ? 1u : std::min<uint32_t>(FIN_MAX_G, (nmax + FIN_SLICE - 1) / FIN_SLICE);
         J.dev.fin_g = g;
-        for (uint32_t q = 0; q < g; ++q) lfin.push_back(make_uint4(i, q, g, 0));
+        if (g == 1)
+            for (uint32_t r = 1; r <= FIN_ROLES; ++r) lfin.push_back(make_uint4(i, 0, 1, r));  // FIN_DECIDE .. FIN_WORK_B
+        else
+            for (uint32_t q = 0; q < g; ++q) lfin.push_back(make_uint4(i, q, g, 0));  // FIN_ALL
         J.dev.block_asm = as ? 1 : 0;
         E->n_asm += as;
         E->n_fp64 += !m;

@@ -73,6 +73,13 @@ struct Ctl {
     // is 1 + the slot this tick's nodal update fills (0: none)
     double t_out, t_int;
     int32_t out_on, snap_now, snap_head, snap_tail, snap_cap, pad2;
+    // what the tick's passes did, for the element finalize's blocks that
+    // must not read the fields its deciding block rewrites: tick_nsteps is
+    // nsteps as the last nodal update saw it when that update ran a step
+    // (its history row is tick_nsteps - 1), 0 when none is open; tick_elem
+    // is the kind of the tick's element pass
+    long long tick_nsteps;
+    int32_t tick_elem, pad3;
 };
 
 struct JobDev {

@@ -595,25 +595,26 @@ __device__ __forceinline__ void element_fp64_W(ELEM_ARGS) {
 #undef PREFETCH_HOOK
 
 // Fixed-order reduction of a job's block partials (value q of block b at
-// part[q * nblk + b]).  fold_partials: thread t folds blocks t, t+BLK,
-// t+2*BLK, ... in ascending order, the node partials (values 0-1) and, with
-// ELEM, the element partials (values 2-6) in the same loop, loads issued
-// FOLD_U blocks at a time (one round for jobs of up to FOLD_U * BLK blocks).
-// It reads only the job table, so its loads go out together with those of
-// the job's scalars; sums of a phase that is not open are not used.
-// combine_partials: a fixed DPP reduction per wave (wave_reduce63), then
-// thread 0 adds the waves in index order.  Same inputs, same result.  Every thread of the
-// block must call both.
+// part[q * nblk + b]; values 0-1 are the node partials, 2-6 the element
+// partials).  fold_partials: thread t folds blocks t, t+BLK, t+2*BLK, ... in
+// ascending order, for the values in VM (bit q: value q; the others keep
+// their identity), loads issued FOLD_U blocks at a time (one round for jobs
+// of up to FOLD_U * BLK blocks).  It reads only the job table, so its loads
+// go out together with those of the job's scalars; sums of a phase that is
+// not open are not used.  combine_partials: a fixed DPP reduction per wave
+// (wave_reduce63) of the values in VM, then thread 0 adds the waves in index
+// order.  Same inputs, same result.  Every thread of the block must call
+// both.
 constexpr int FOLD_U = 8;  // blocks per thread whose loads are issued together
+__host__ __device__ constexpr bool fold_min(int q) { return q == 2 || q == 3; }  // values 2-3 are minima
 // The slice [slice, slice + 1) of `slices` equal parts of the partials (in
 // ascending block order) is folded; one slice folds them all.
-template <int BLK, bool ELEM>
-__device__ inline void fold_partials(const JobDev& J, const bool* is_min, double* acc, uint32_t slice = 0,
-                                     uint32_t slices = 1) {
+template <int BLK, uint32_t VM>
+__device__ inline void fold_partials(const JobDev& J, double* acc, uint32_t slice = 0, uint32_t slices = 1) {
     const uint32_t t = threadIdx.x;
 #pragma unroll
-    for (int q = 0; q < 7; ++q) acc[q] = is_min[q] ? __builtin_inf() : 0.0;
-    const uint32_t nn_all = J.nblk_n, ne_all = ELEM ? J.nblk_e : 0u;
+    for (int q = 0; q < 7; ++q) acc[q] = fold_min(q) ? __builtin_inf() : 0.0;
+    const uint32_t nn_all = (VM & 3u) ? J.nblk_n : 0u, ne_all = (VM & 0x7Cu) ? J.nblk_e : 0u;
     const uint32_t sn = (nn_all + slices - 1) / slices, se = (ne_all + slices - 1) / slices;
     const uint32_t n0 = sn * slice, e0 = se * slice;
     const uint32_t nn = nn_all > n0 ? (nn_all - n0 < sn ? nn_all - n0 : sn) : 0u;  // this slice's counts
@@ -631,40 +632,55 @@ __device__ inline void fold_partials(const JobDev& J, const bool* is_min, double
             const uint32_t bn = n0 + (nn ? (b < nn ? b : nn - 1) : 0u), be = e0 + (ne ? (b < ne ? b : ne - 1) : 0u);
             const uint32_t bn_ok = nn_all ? bn : 0u, be_ok = ne_all ? be : 0u;
 #pragma unroll
-            for (int q = 0; q < 2; ++q) v[u][q] = pn[(size_t)q * nn_all + (nn_all ? (bn_ok < nn_all ? bn_ok : nn_all - 1) : 0u)];
-#pragma unroll
-            for (int q = 2; q < 7; ++q)
-                v[u][q] = pe[(size_t)(q - 2) * ne_all + (ne_all ? (be_ok < ne_all ? be_ok : ne_all - 1) : 0u)];
-#pragma unroll
-            for (int q = 0; q < 2; ++q) v[u][q] = (b < nn) ? v[u][q] : 0.0;
-#pragma unroll
-            for (int q = 2; q < 7; ++q) v[u][q] = (b < ne) ? v[u][q] : (is_min[q] ? __builtin_inf() : 0.0);
+            for (int q = 0; q < 7; ++q) {
+                if (!((VM >> q) & 1u)) {
+                    v[u][q] = fold_min(q) ? __builtin_inf() : 0.0;
+                } else if (q < 2) {
+                    v[u][q] = pn[(size_t)q * nn_all + (nn_all ? (bn_ok < nn_all ? bn_ok : nn_all - 1) : 0u)];
+                    v[u][q] = (b < nn) ? v[u][q] : 0.0;
+                } else {
+                    v[u][q] = pe[(size_t)(q - 2) * ne_all + (ne_all ? (be_ok < ne_all ? be_ok : ne_all - 1) : 0u)];
+                    v[u][q] = (b < ne) ? v[u][q] : (fold_min(q) ? __builtin_inf() : 0.0);
+                }
+            }
         }
 #pragma unroll
         for (int u = 0; u < FOLD_U; ++u)
 #pragma unroll
-            for (int q = 0; q < 7; ++q) acc[q] = is_min[q] ? (v[u][q] < acc[q] ? v[u][q] : acc[q]) : acc[q] + v[u][q];
+            for (int q = 0; q < 7; ++q)
+                if ((VM >> q) & 1u) acc[q] = fold_min(q) ? (v[u][q] < acc[q] ? v[u][q] : acc[q]) : acc[q] + v[u][q];
     }
 }
 
-template <int NV, int BLK>
-__device__ inline void combine_partials(double* acc, const bool* is_min, double (*s_tmp)[NV], double* out) {
+template <int BLK, uint32_t VM>
+__device__ inline void combine_partials(double* acc, double (*s_tmp)[7], double* out) {
     const uint32_t t = threadIdx.x;
 #pragma unroll
-    for (int q = 0; q < NV; ++q) acc[q] = is_min[q] ? wave_reduce63<true>(acc[q]) : wave_reduce63<false>(acc[q]);
+    for (int q = 0; q < 7; ++q)
+        if ((VM >> q) & 1u) acc[q] = fold_min(q) ? wave_reduce63<true>(acc[q]) : wave_reduce63<false>(acc[q]);
     const int w = t >> 6;
     if ((t & 63) == 63)
 #pragma unroll
-        for (int q = 0; q < NV; ++q) s_tmp[w][q] = acc[q];
+        for (int q = 0; q < 7; ++q)
+            if ((VM >> q) & 1u) s_tmp[w][q] = acc[q];
     __syncthreads();
     if (t == 0) {
 #pragma unroll
-        for (int q = 0; q < NV; ++q) {
+        for (int q = 0; q < 7; ++q) {
+            if (!((VM >> q) & 1u)) {
+                out[q] = fold_min(q) ? __builtin_inf() : 0.0;
+                continue;
+            }
             double r = s_tmp[0][q];
-            for (int x = 1; x < BLK / 64; ++x) r = is_min[q] ? (s_tmp[x][q] < r ? s_tmp[x][q] : r) : r + s_tmp[x][q];
+            for (int x = 1; x < BLK / 64; ++x) r = fold_min(q) ? (s_tmp[x][q] < r ? s_tmp[x][q] : r) : r + s_tmp[x][q];
             out[q] = r;
         }
     }
+}
+
+// Row `row` of the job's history, or null (no history, or past its end).
+__device__ __forceinline__ double* hist_row(const JobDev& J, long long row) {
+    return (J.hist && row >= 0 && row < J.hist_cap) ? J.hist + row * (ENSC_H_BASE + 3 * J.n_probe) : nullptr;
 }
 
 // End of a job's nodal update (thread 0): kinetic energy, external work (w:
@@ -677,9 +693,7 @@ __device__ __forceinline__ void node_phase_finish(const JobDev& J, Ctl& c, doubl
 #pragma clang fp contract(off)
     c.ke = ke;
     c.wext += w;
-    const long long row = c.nsteps - 1;
-    if (J.hist && row >= 0 && row < J.hist_cap) {
-        double* h = J.hist + row * (ENSC_H_BASE + 3 * J.n_probe);
+    if (double* h = hist_row(J, c.nsteps - 1)) {
         h[ENSC_H_TIME] = c.time;
         h[ENSC_H_DT_I] = c.dt_i;
         h[ENSC_H_DT_I_1] = c.dt_i_1;
@@ -697,6 +711,7 @@ __device__ __forceinline__ void node_phase_finish(const JobDev& J, Ctl& c, doubl
         }
     }
     c.kind = KIND_NONE;
+    c.tick_nsteps = 0;
 }
 
 // End of a job's element pass (thread 0): tot = min edge, min dtStable,
@@ -771,7 +786,10 @@ __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(WPE))) void
     // generic loads here: with one element block per launched block the
     // scalar read measured slower (FP64 element pass 128 -> 133 us)
     const Decision dc = decide(*ctl, J.factor, J.wos);
-    if (dc.kind == KIND_NONE) return;
+    if (dc.kind == KIND_NONE) {
+        if (lb == 0 && threadIdx.x == 0) gst(&ctl->tick_elem, 0u) = KIND_NONE;  // the tick record
+        return;
+    }
     ElemState<R> ES;
     load_state<R, STAGED>(J, act ? e : J.ne - 1, dc.kind != KIND_PREPASS, ES);
     ENSC_LDS double* lds = (ENSC_LDS double*)s_nodes;
@@ -898,44 +916,146 @@ __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(WPE))) void
         gst(J.part_e, 3 * nb + lb) = r3;
         gst(J.part_e, 4 * nb + lb) = r4;
         if (s_err) atomicOr(&ctl->err, s_err);
+        if (lb == 0) gst(&ctl->tick_elem, 0u) = dc.kind;  // the tick record
     }
 }
 
-// Per-job end of the element pass: fin_g blocks per job (the map holds
-// (job, slice, fin_g)).  It closes the previous nodal update if still open
-// (node_phase_finish), and reduces the job's element block partials
-// (written by k_element; the launch boundary orders them) to advance the
-// job's time-step scalars exactly as update_minVertex_perMaterial_and_dt and
-// the run-loop clamps do (exDyna3D.cpp:663-686, 90-97, 775-780).  Both sets
-// of partials are folded in one pass so their loads overlap.  A job with
-// many partials spreads the fold over several blocks (one CU's loads bound
-// it otherwise): each folds its slice, publishes it and counts itself on the
-// job's ticket (device-scope fences around the atomic); the block that
-// counts last folds the slices in slice order, so the result does not
-// depend on which block that is, and decides.
+// Per-job end of the element pass.  It closes the previous nodal update if
+// still open (node_phase_finish), and reduces the job's element block
+// partials (written by k_element; the launch boundary orders them) to
+// advance the job's time-step scalars exactly as
+// update_minVertex_perMaterial_and_dt and the run-loop clamps do
+// (exDyna3D.cpp:663-686, 90-97, 775-780).
+//
+// The map holds (job, slice, slices, role) per block.  A job with up to
+// FIN_ONE partials per set has four blocks, which run side by side, each
+// folding the partials of the scalars only it writes (one CU's loads bound
+// the fold, so four CUs fold a quarter each):
+//   FIN_DECIDE: min edge and min stable dt -> the decision, the time-step
+//     scalars and the history row's time columns;
+//   FIN_NODE:   kinetic energy and the step's external work of the nodal
+//     update being closed -> ke, wext and their history columns;
+//   FIN_WORK_A / FIN_WORK_B: internal and hourglass work / viscous work ->
+//     ie, hg / vis and their history columns (the values before this
+//     pass's work, as node_phase_finish records them).
+// The deciding block rewrites kind and nsteps, so the other three learn
+// what the tick did from the tick record (tick_nsteps, tick_elem), written
+// by the nodal update and the element pass before this launch; each block
+// stores its own fields only.
+// A job with more partials has fin_g blocks of role FIN_ALL: each folds its
+// slice of all seven values, publishes it and counts itself on the job's
+// ticket (device-scope fences around the atomic); the block that counts
+// last folds the slices in slice order, so the result does not depend on
+// which block that is, and finishes both phases.
+enum : uint32_t { FIN_ALL = 0, FIN_DECIDE = 1, FIN_NODE = 2, FIN_WORK_A = 3, FIN_WORK_B = 4 };
 template <int BLK>
 __global__ __launch_bounds__(BLK) void k_fin_element(const JobDev* __restrict__ jobs,
                                                      const uint4* __restrict__ finmap) {
 #pragma clang fp contract(off)
-    const uint4 fm = finmap[blockIdx.x];  // (job, slice, slices, -)
+    const uint4 fm = finmap[blockIdx.x];  // (job, slice, slices, role)
     const JobDev& J = jobs[fm.x];
-    const uint32_t slice = fm.y, slices = fm.z;
+    const uint32_t slice = fm.y, slices = fm.z, role = fm.w;
     Ctl* ctl = J.ctl;
-    // one copy of the job's scalars, loaded at once; nothing else writes them
-    // while this kernel runs (the job's deciding block stores them once every
-    // block has counted itself, i.e. after every block has read them)
+    // one copy of the job's scalars, loaded at once.  FIN_ALL: nothing else
+    // writes them while this kernel runs (the job's deciding block stores
+    // them once every block has counted itself, i.e. after every block has
+    // read them).  The roles: each reads only fields that no other block of
+    // the job writes in this launch.
     const Ctl c0 = *ctl;
-    // values 0-1: node partials (sums); 2-6: element partials (2 minima, 3 sums)
-    const bool is_min[7] = {false, false, true, true, false, false, false};
-    double acc[7];
-    fold_partials<BLK, true>(J, is_min, acc, slice, slices);
+    __shared__ double s_red[BLK / 64][7];
+    double acc[7], tot[7];
+    if (role == FIN_NODE) {
+        if (!c0.tick_nsteps) return;
+        fold_partials<BLK, 0x03u>(J, acc);
+        combine_partials<BLK, 0x03u>(acc, s_red, tot);
+        if (threadIdx.x == 0) {
+            const double ke = tot[0], wext = c0.wext + tot[1];
+            ctl->ke = ke;
+            ctl->wext = wext;
+            if (double* h = hist_row(J, c0.tick_nsteps - 1)) {
+                h[ENSC_H_KE] = ke;
+                h[ENSC_H_WEXT] = wext;
+            }
+        }
+        return;
+    }
+    if (role == FIN_WORK_A || role == FIN_WORK_B) {
+        const bool step = c0.tick_elem == KIND_STEP;
+        if (!step && !c0.tick_nsteps) return;
+        if (role == FIN_WORK_A) {
+            fold_partials<BLK, 0x30u>(J, acc);
+            combine_partials<BLK, 0x30u>(acc, s_red, tot);
+        } else {
+            fold_partials<BLK, 0x40u>(J, acc);
+            combine_partials<BLK, 0x40u>(acc, s_red, tot);
+        }
+        if (threadIdx.x == 0) {
+            double* h = c0.tick_nsteps ? hist_row(J, c0.tick_nsteps - 1) : nullptr;
+            if (role == FIN_WORK_A) {
+                if (h) {
+                    h[ENSC_H_IE] = c0.ie;
+                    h[ENSC_H_HG] = c0.hg;
+                }
+                if (step) {
+                    ctl->ie = c0.ie + tot[4];
+                    ctl->hg = c0.hg + tot[5];
+                }
+            } else {
+                if (h) h[ENSC_H_VIS] = c0.vis;
+                if (step) ctl->vis = c0.vis + tot[6];
+            }
+        }
+        return;
+    }
     const bool node_open = c0.kind == KIND_STEP;
     const Decision dc = decide(c0, J.factor, J.wos);
     const bool elem = dc.kind != KIND_NONE;
+    if (role == FIN_DECIDE) {
+        if (!node_open && !elem) return;
+        fold_partials<BLK, 0x0Cu>(J, acc);
+        combine_partials<BLK, 0x0Cu>(acc, s_red, tot);
+        if (threadIdx.x == 0) {
+            Ctl c = c0;
+            if (node_open) {
+                // node_phase_finish's columns of this block's fields
+                if (double* h = hist_row(J, c.nsteps - 1)) {
+                    h[ENSC_H_TIME] = c.time;
+                    h[ENSC_H_DT_I] = c.dt_i;
+                    h[ENSC_H_DT_I_1] = c.dt_i_1;
+                    h[ENSC_H_DTS_MIN] = c.dts_min;
+                    for (int p = 0; p < J.n_probe; ++p) {
+                        const double* r = J.vu + (size_t)J.probe[p] * VU_REC + 3;
+                        h[ENSC_H_BASE + 3 * p + 0] = r[0];
+                        h[ENSC_H_BASE + 3 * p + 1] = r[1];
+                        h[ENSC_H_BASE + 3 * p + 2] = r[2];
+                    }
+                }
+                c.kind = KIND_NONE;
+            }
+            // the work sums (tot[4..6]) are the identity here and ie, hg,
+            // vis are not stored: FIN_WORK_A and FIN_WORK_B own them
+            if (elem) elem_phase_finish(J, c, dc, tot + 2);
+            ctl->time = c.time;
+            ctl->dt_i = c.dt_i;
+            ctl->dt_i_1 = c.dt_i_1;
+            ctl->dts_min = c.dts_min;
+            ctl->lmin_pre = c.lmin_pre;
+            ctl->grav_amp = c.grav_amp;
+            ctl->nsteps = c.nsteps;
+            ctl->kind = c.kind;
+            ctl->have_lpre = c.have_lpre;
+            ctl->init_pending = c.init_pending;
+            ctl->err = c.err;
+            ctl->t_out = c.t_out;
+            ctl->snap_now = c.snap_now;
+            ctl->snap_head = c.snap_head;
+        }
+        return;
+    }
+    // FIN_ALL
+    fold_partials<BLK, 0x7Fu>(J, acc, slice, slices);
     if (!node_open && !elem) return;  // alike in every block of the job
-    __shared__ double s_red[BLK / 64][7];
-    double tot[7];
-    combine_partials<7, BLK>(acc, is_min, s_red, tot);
+    combine_partials<BLK, 0x7Fu>(acc, s_red, tot);
     if (slices > 1) {
         __shared__ uint32_t s_last;
         if (threadIdx.x == 0) {
@@ -956,7 +1076,7 @@ __global__ __launch_bounds__(BLK) void k_fin_element(const JobDev* __restrict__ 
 #pragma unroll
                 for (int q = 0; q < 7; ++q) {
                     const double x = fp[g * 8 + q];
-                    tot[q] = is_min[q] ? (x < tot[q] ? x : tot[q]) : tot[q] + x;
+                    tot[q] = fold_min(q) ? (x < tot[q] ? x : tot[q]) : tot[q] + x;
                 }
             *J.fin_ticket = 0u;  // the next launch's count starts from zero
         }
@@ -1243,6 +1363,7 @@ __device__ __forceinline__ bool pipe_tile(const JobDev* __restrict__ jobs, const
     } else {
         pipe_prefetch(pn, ESn, gn);
     }
+    if (lb == 0 && threadIdx.x == 0) gst(&ctl->tick_elem, 0u) = dc.kind;  // the tick record
     t = tn;
     return more;
 }
@@ -1291,7 +1412,10 @@ __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(ASM ? (MIXE
     const JobDev& J = jobs[bm.x];
     Ctl* ctl = J.ctl;
     const Ctl cs = ctl_read(ctl);
-    if (cs.kind != KIND_STEP) return;
+    if (cs.kind != KIND_STEP) {
+        if (bm.y == 0 && threadIdx.x == 0) gst(&ctl->tick_nsteps, 0u) = 0;  // the tick record
+        return;
+    }
     const double dt_i = cs.dt_i, dt_i_1 = cs.dt_i_1, time = cs.time;
     const int32_t snap = cs.snap_now - 1;  // output slot of this step, or -1
     const uint32_t lb = bm.y;
@@ -1579,6 +1703,7 @@ __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(ASM ? (MIXE
         double* p = J.part_n + lb;
         p[0] = r0;
         p[J.nblk_n] = r1;
+        if (lb == 0) gst(&ctl->tick_nsteps, 0u) = cs.nsteps;  // the tick record: an update is open
     }
 }
 
@@ -1700,12 +1825,11 @@ template <int BLK>
 __global__ __launch_bounds__(BLK) void k_fin_node(const JobDev* __restrict__ jobs) {
     const JobDev& J = jobs[blockIdx.x];
     if (J.ctl->kind != KIND_STEP) return;
-    const bool is_min[7] = {false, false, true, true, false, false, false};
     double acc[7];
-    fold_partials<BLK, false>(J, is_min, acc);
+    fold_partials<BLK, 0x03u>(J, acc);
     __shared__ double s_red[BLK / 64][7];
     double tot[7];
-    combine_partials<7, BLK>(acc, is_min, s_red, tot);
+    combine_partials<BLK, 0x03u>(acc, s_red, tot);
     // once per run of ticks: the device copy is updated in place
     if (threadIdx.x == 0) node_phase_finish(J, *J.ctl, tot[0], tot[1]);
 }
