@@ -526,6 +526,52 @@ def test_fp64_energies_match_oracle(case):
           f"{EC.balance(se) / scale:.2e} of the energy scale", flush=True)
 
 
+@pytest.mark.parametrize("case", sorted(ENERGY_CASES))
+def test_fp64_history_rows_match_oracle(case):
+    # every column of every history row against the oracle's rows: the time
+    # columns come from the element finalize's deciding block, the energy
+    # columns from the finalize blocks that own those scalars (the last row
+    # from the node finalize that ends the run)
+    m = ENERGY_CASES[case]()
+    probe = m.x0.shape[0] - 1
+    e = eng.Engine(m, eng.FP64, probes=[probe], hist_cap=100000).run()
+    n = int(e.scalars()["nsteps"])
+    o = Oracle(m, 0).init()
+    no, H = o.run(0, [probe], n)
+    h = e.history()
+    assert no == n and h.shape == H.shape == (n, HIST_BASE + 3)
+    so = o.scalars()
+    scale = max(abs(so["wext"]), so["ke0"])
+    assert rel(h[:, 0], H[:, 0]) <= 1e-13
+    assert rel(h[:, 1:4], H[:, 1:4]) <= 1e-11
+    de = float(np.abs(h[:, 4:HIST_BASE] - H[:, 4:HIST_BASE]).max())
+    assert de <= 1e-9 * scale, de
+    assert rel(h[:, HIST_BASE:], H[:, HIST_BASE:], 1e-12) <= TOL_FP64
+    print(f"[history] {case}: {n} rows, energy columns within {de / scale:.1e} of the energy scale", flush=True)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_history_alike_one_step_per_launch(prec):
+    # a continuous run's rows are written by the element finalize's blocks,
+    # a run of one-step launches' rows by the node finalize that ends each
+    # launch: the two histories, and the final scalars, agree bit for bit
+    p = eng.FP64 if prec == "fp64" else eng.MIXED
+    m = EC.driven_column()
+    a = eng.Engine(m, p, probes=[0], hist_cap=100000).run()
+    b = eng.Engine(m, p, probes=[0], hist_cap=100000).init()
+    for st in range(1, len(m.steps)):
+        b.set_current_step(st)
+        while not b.advance(1):
+            pass
+    ha, hb = a.history(), b.history()
+    print(f"[history chunked] {prec}: {ha.shape[0]} rows", flush=True)
+    assert ha.shape == hb.shape and ha.shape[0] > 100
+    assert np.array_equal(ha, hb)
+    sa, sb = a.scalars(), b.scalars()
+    for k in ("time", "dt_i", "ke", "ie", "hg", "vis", "wext"):
+        assert sa[k] == sb[k], k
+
+
 def test_mixed_energies_close_to_fp64_ramp_and_gravity():
     # mixed engine: the same external work within 1% (the mixed bar on energy)
     for m in (EC.ramped_block(), EC.free_fall(steps=200)):
