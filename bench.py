@@ -446,16 +446,17 @@ def configs_section(args):
                 except (RuntimeError, OSError, subprocess.SubprocessError) as ex:
                     log(f"{name}: cpu reference not timed: {ex}")
             log(f"{name}: cpu restatement {out[name]['cpu_restatement']['value']:.3e} elem-steps/s")
-        # C4: 20 steps per mesh over all 256 meshes, the restatement (one
-        # reference process per mesh would time its deck parsing instead)
+        # C4: a fixed 100 steps per mesh over all 256 meshes, the
+        # restatement (one reference process per mesh would time its deck
+        # parsing instead)
         models = decks.config4(args.c4_jobs)
         tot, secs = 0, 0.0
         for m in models:
             o = Oracle(m, threads).init()
-            secs += o.time_steps(20)
-            tot += m.n_elems * 20
+            secs += o.time_steps(100)
+            tot += m.n_elems * 100
         out["c4"] = {"cpu_restatement": {"value": tot / secs, "cores": threads,
-                                         "sample": f"20 steps of each of the {len(models)} meshes"}}
+                                         "sample": f"100 steps of each of the {len(models)} meshes"}}
         log(f"c4: cpu restatement {tot / secs:.3e} elem-steps/s")
     return out
 
