@@ -284,7 +284,7 @@ ensc_status relayout(ensc_engine* E) {
     //  - block assembly: staged and free of corner conflicts (FP64 jobs:
     //    unless the reference's summation order is asked for);
     //  - pipelined pass: block assembly, no amplitude velocity BCs and tiles
-    //    of PIPE_BLK elements; each such tile with at most PIPE_CAP nodes
+    //    of PIPE_BLK elements; each such tile with fewer than PIPE_CAP nodes
     //    (the rest of the job's tiles use the staged pass).  The split
     //    follows from the job's own tiles, so it too is the same in any mix.
     auto asm_wanted = [&](const Job& J) { return J.precision == ENSC_MIXED || E->fp64_order == 0; };
@@ -321,7 +321,7 @@ ensc_status relayout(ensc_engine* E) {
             if (!m)
                 le = as ? &lw64 : (staged ? &ls64 : &lg64);
             else
-                le = (pipe && nodes <= (uint32_t)PIPE_CAP) ? &lp : (staged ? &ls : &lg);
+                le = (pipe && nodes < (uint32_t)PIPE_CAP) ? &lp : (staged ? &ls : &lg);
             if (le == &ls) {
                 lds_nodes = std::max(lds_nodes, nodes);
                 any_asm |= as;

@@ -1102,7 +1102,7 @@ __global__ __launch_bounds__(BLK) void k_fin_element(const JobDev* __restrict__ 
 // instantiations may contract multiply-adds differently, so each tile is
 // given one of the two passes by its job's properties and its own node
 // count, and keeps it in any batch.  Jobs with amplitude velocity BCs
-// (applied to staged velocities) and tiles of more than PIPE_CAP nodes use
+// (applied to staged velocities) and tiles of PIPE_CAP nodes or more use
 // k_element.
 //
 // LDS per block is exactly the two buffers, 2 x 5 x 16 B x PIPE_CAP = 80 KiB,
@@ -1184,6 +1184,10 @@ __device__ __forceinline__ void pipe_prefetch(PipeNext& p, ElemState<float>& ESn
     ((i) < 3 ? PX(NODE(k), (i)) + PU(NODE(k), (i)) : ((i) < 6 ? PV(NODE(k), (i)-3) : PU(NODE(k), (i)-6)))
 #define CORNER_V(k, i) NVAL(k, 3 + (i))
 #define ACC_W 4
+// a node's first contribution adds to the zero slot's value instead of its
+// stale accumulator (the tile's last staged slot, never a node of the tile:
+// pipelined tiles hold fewer than PIPE_CAP nodes; zeroed at the tile start)
+#define ACC_ZERO (acc + (PIPE_CAP - 1) * ACC_W)
 #define ASM_BEGIN ((void)0)  // first contributions store (FIRST), nothing to clear
 #define PREFETCH_HOOK pipe_prefetch(pn, ESn, gn)
 __device__ __forceinline__ void element_mixed_P(const JobDev& J, uint32_t e, const ElemState<float>& ES, double dt,
@@ -1204,6 +1208,7 @@ __device__ __forceinline__ void element_mixed_P(const JobDev& J, uint32_t e, con
 #undef PV
 #undef PU
 #undef ACC_W
+#undef ACC_ZERO
 #undef ASM_BEGIN
 #undef PREFETCH_HOOK
 
@@ -1282,6 +1287,11 @@ __device__ __forceinline__ bool pipe_tile(const JobDev* __restrict__ jobs, const
     __builtin_amdgcn_s_waitcnt(0);
     lds_sync();
     pipe_flush(out);  // the previous tile's results (registers, not LDS)
+    if (threadIdx.x == 0) {
+        // the assembly's zero slot (ACC_ZERO), before its first phase
+        ENSC_LDS float* z = pipe_acc(cur) + (PIPE_CAP - 1) * 4;
+        z[0] = z[1] = z[2] = 0.0f;
+    }
     if (more) pipe_copy(jobs[bn.x], bn.w, g, nxt);
     if (dc.kind != KIND_NONE) {
         const bool asmode = J.block_asm && dc.kind != KIND_PREPASS;
