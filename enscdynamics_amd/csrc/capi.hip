@@ -37,15 +37,18 @@ namespace {
 // partials (40 B per element block, 16 B per node block) of one engine; 4 MiB
 // hold a single mesh of ~10M elements.
 constexpr size_t STATIC_SCRATCH_BYTES = (size_t)4 << 20;
-// Element finalize: for a job of up to FIN_ONE block partials (element or
-// node), four blocks that each fold the partials of the scalars only they
-// write (FIN_ROLES); else one block per FIN_SLICE partials, at most FIN_MAX_G,
-// each folding a slice of every value, the last to count deciding; per job
-// FIN_JOB_BYTES of device memory hold the slices' folds and the ticket.
-// Measured on configuration 3: the ticket (fences and atomic) costs ~1 us
-// more than the split fold saves at 4k partials, and saves ~3 us at 7.8k
-// (8 blocks).
-constexpr uint32_t FIN_ONE = 4096, FIN_SLICE = 1024, FIN_MAX_G = 8, FIN_ROLES = 4;
+// Element finalize: one block for a job of up to FIN_ROLE_MIN block partials
+// (element or node); up to FIN_ONE, four blocks that each fold the partials
+// of the scalars only they write (FIN_ROLES); else one block per FIN_SLICE
+// partials, at most FIN_MAX_G, each folding a slice of every value, the last
+// to count deciding; per job FIN_JOB_BYTES of device memory hold the slices'
+// folds and the ticket.  Measured on configuration 3: the ticket (fences and
+// atomic) costs ~1 us more than the split fold saves at 4k partials, and
+// saves ~3 us at 7.8k (8 blocks); the four role blocks save ~2.5 us at 4k.
+// Small jobs keep one block: a 512-thread finalize block takes a whole CU
+// (160 VGPRs), so four per job in a batch of 256 small jobs ran in four
+// rounds (configuration 4: 7.8 -> 11 us per tick).
+constexpr uint32_t FIN_ROLE_MIN = 1024, FIN_ONE = 4096, FIN_SLICE = 1024, FIN_MAX_G = 8, FIN_ROLES = 4;
 constexpr size_t FIN_JOB_BYTES = FIN_MAX_G * 8 * sizeof(double) + 256;
 
 // Device buffer that grows (contents are not preserved).
@@ -344,7 +347,7 @@ ensc_status relayout(ensc_engine* E) {
         const uint32_t g =
             nmax <= FIN_ONE ? 1u : std::min<uint32_t>(FIN_MAX_G, (nmax + FIN_SLICE - 1) / FIN_SLICE);
         J.dev.fin_g = g;
-        if (g == 1)
+        if (g == 1 && nmax > FIN_ROLE_MIN)
             for (uint32_t r = 1; r <= FIN_ROLES; ++r) lfin.push_back(make_uint4(i, 0, 1, r));  // FIN_DECIDE .. FIN_WORK_B
         else
             for (uint32_t q = 0; q < g; ++q) lfin.push_back(make_uint4(i, q, g, 0));  // FIN_ALL

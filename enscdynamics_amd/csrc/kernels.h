@@ -927,10 +927,10 @@ __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(WPE))) void
 // update_minVertex_perMaterial_and_dt and the run-loop clamps do
 // (exDyna3D.cpp:663-686, 90-97, 775-780).
 //
-// The map holds (job, slice, slices, role) per block.  A job with up to
-// FIN_ONE partials per set has four blocks, which run side by side, each
-// folding the partials of the scalars only it writes (one CU's loads bound
-// the fold, so four CUs fold a quarter each):
+// The map holds (job, slice, slices, role) per block.  A job with more than
+// 1024 and up to 4096 partials per set has four blocks, which run side by
+// side, each folding the partials of the scalars only it writes (one CU's
+// loads bound the fold, so four CUs fold a quarter each):
 //   FIN_DECIDE: min edge and min stable dt -> the decision, the time-step
 //     scalars and the history row's time columns;
 //   FIN_NODE:   kinetic energy and the step's external work of the nodal
@@ -942,11 +942,13 @@ __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(WPE))) void
 // what the tick did from the tick record (tick_nsteps, tick_elem), written
 // by the nodal update and the element pass before this launch; each block
 // stores its own fields only.
-// A job with more partials has fin_g blocks of role FIN_ALL: each folds its
-// slice of all seven values, publishes it and counts itself on the job's
-// ticket (device-scope fences around the atomic); the block that counts
-// last folds the slices in slice order, so the result does not depend on
-// which block that is, and finishes both phases.
+// Other jobs have fin_g blocks of role FIN_ALL: one for a small job, which
+// folds all seven values and finishes both phases; for a job of more than
+// 4096 partials each folds its slice of all seven values, publishes it and
+// counts itself on the job's ticket (device-scope fences around the
+// atomic); the block that counts last folds the slices in slice order, so
+// the result does not depend on which block that is, and finishes both
+// phases.
 enum : uint32_t { FIN_ALL = 0, FIN_DECIDE = 1, FIN_NODE = 2, FIN_WORK_A = 3, FIN_WORK_B = 4 };
 template <int BLK>
 __global__ __launch_bounds__(BLK) void k_fin_element(const JobDev* __restrict__ jobs,

@@ -572,6 +572,51 @@ def test_history_alike_one_step_per_launch(prec):
         assert sa[k] == sb[k], k
 
 
+def test_history_rows_with_four_finalize_blocks():
+    # a job of more than 1024 element blocks: its element finalize runs as
+    # four blocks (the decision, the nodal update's close, two work blocks),
+    # each writing its own history columns.  60 steps of the J2 cube of 70^3
+    # elements: the FP64 rows against the oracle's, and in both precisions a
+    # continuous run against one-step launches (whose rows the node finalize
+    # writes), bit for bit
+    m = decks.config3(n=70, steps=3000)
+    probe = m.x0.shape[0] - 1
+    n = 60
+
+    def run(p, chunked):
+        e = eng.Engine(m, p, probes=[probe], hist_cap=n + 8)
+        e.init()
+        e.set_current_step(1)
+        if chunked:
+            for _ in range(n):
+                e.advance(1)
+        else:
+            e.advance(n)
+        return e
+
+    a = run(eng.FP64, False)
+    assert a.layout()["elem_blocks"] > 1024
+    o = Oracle(m, 0).init()
+    no, H = o.run(n, [probe], n)
+    h = a.history()
+    assert no == n and h.shape == H.shape == (n, HIST_BASE + 3)
+    so = o.scalars()
+    scale = max(abs(so["wext"]), so["ke0"])
+    assert rel(h[:, 0], H[:, 0]) <= 1e-13
+    assert rel(h[:, 1:4], H[:, 1:4]) <= 1e-11
+    de = float(np.abs(h[:, 4:HIST_BASE] - H[:, 4:HIST_BASE]).max())
+    assert de <= 1e-9 * scale, de
+    assert rel(h[:, HIST_BASE:], H[:, HIST_BASE:], 1e-12) <= TOL_FP64
+    print(f"[history 4-block finalize] fp64 vs oracle: energy columns within {de / scale:.1e} of the scale",
+          flush=True)
+    for p, cont in ((eng.FP64, a), (eng.MIXED, run(eng.MIXED, False))):
+        ch = run(p, True)
+        assert np.array_equal(cont.history(), ch.history()), p
+        sa, sb = cont.scalars(), ch.scalars()
+        for k in ("time", "dt_i", "ke", "ie", "hg", "vis", "wext"):
+            assert sa[k] == sb[k], (p, k)
+
+
 def test_mixed_energies_close_to_fp64_ramp_and_gravity():
     # mixed engine: the same external work within 1% (the mixed bar on energy)
     for m in (EC.ramped_block(), EC.free_fall(steps=200)):
