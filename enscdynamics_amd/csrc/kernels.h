@@ -689,26 +689,30 @@ __device__ __forceinline__ double* hist_row(const JobDev& J, long long row) {
 // job's Ctl or the caller's register copy of it (written back once): updating
 // the device copy field by field makes every later read of it wait for the
 // stores before it, which the per-tick finalize avoids.
+// A history row's time-step columns and probe displacements.
+__device__ __forceinline__ void hist_time_columns(const JobDev& J, const Ctl& c, double* h) {
+    h[ENSC_H_TIME] = c.time;
+    h[ENSC_H_DT_I] = c.dt_i;
+    h[ENSC_H_DT_I_1] = c.dt_i_1;
+    h[ENSC_H_DTS_MIN] = c.dts_min;
+    for (int p = 0; p < J.n_probe; ++p) {
+        const double* r = J.vu + (size_t)J.probe[p] * VU_REC + 3;
+        h[ENSC_H_BASE + 3 * p + 0] = r[0];
+        h[ENSC_H_BASE + 3 * p + 1] = r[1];
+        h[ENSC_H_BASE + 3 * p + 2] = r[2];
+    }
+}
 __device__ __forceinline__ void node_phase_finish(const JobDev& J, Ctl& c, double ke, double w) {
 #pragma clang fp contract(off)
     c.ke = ke;
     c.wext += w;
     if (double* h = hist_row(J, c.nsteps - 1)) {
-        h[ENSC_H_TIME] = c.time;
-        h[ENSC_H_DT_I] = c.dt_i;
-        h[ENSC_H_DT_I_1] = c.dt_i_1;
-        h[ENSC_H_DTS_MIN] = c.dts_min;
+        hist_time_columns(J, c, h);
         h[ENSC_H_KE] = c.ke;
         h[ENSC_H_IE] = c.ie;
         h[ENSC_H_HG] = c.hg;
         h[ENSC_H_VIS] = c.vis;
         h[ENSC_H_WEXT] = c.wext;
-        for (int p = 0; p < J.n_probe; ++p) {
-            const double* r = J.vu + (size_t)J.probe[p] * VU_REC + 3;
-            h[ENSC_H_BASE + 3 * p + 0] = r[0];
-            h[ENSC_H_BASE + 3 * p + 1] = r[1];
-            h[ENSC_H_BASE + 3 * p + 2] = r[2];
-        }
     }
     c.kind = KIND_NONE;
     c.tick_nsteps = 0;
@@ -1020,18 +1024,7 @@ __global__ __launch_bounds__(BLK) void k_fin_element(const JobDev* __restrict__ 
             Ctl c = c0;
             if (node_open) {
                 // node_phase_finish's columns of this block's fields
-                if (double* h = hist_row(J, c.nsteps - 1)) {
-                    h[ENSC_H_TIME] = c.time;
-                    h[ENSC_H_DT_I] = c.dt_i;
-                    h[ENSC_H_DT_I_1] = c.dt_i_1;
-                    h[ENSC_H_DTS_MIN] = c.dts_min;
-                    for (int p = 0; p < J.n_probe; ++p) {
-                        const double* r = J.vu + (size_t)J.probe[p] * VU_REC + 3;
-                        h[ENSC_H_BASE + 3 * p + 0] = r[0];
-                        h[ENSC_H_BASE + 3 * p + 1] = r[1];
-                        h[ENSC_H_BASE + 3 * p + 2] = r[2];
-                    }
-                }
+                if (double* h = hist_row(J, c.nsteps - 1)) hist_time_columns(J, c, h);
                 c.kind = KIND_NONE;
             }
             // the work sums (tot[4..6]) are the identity here and ie, hg,
