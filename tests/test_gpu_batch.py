@@ -47,6 +47,27 @@ def test_batch_alone_equals_in_mix(precision, staging):
     print(f"[batch] {len(jobs)} jobs equal alone and in the mix", flush=True)
 
 
+@pytest.mark.parametrize("precision", [eng.FP64, eng.MIXED])
+def test_batch_of_large_and_small_jobs_alone_equals_in_mix(precision):
+    # a job of more than 1024 element blocks (its element finalize runs as
+    # four blocks, one per group of scalars) beside small jobs (one block
+    # each): every job equals its own run, bit for bit, energies included
+    big = decks.config3(n=70, steps=600)
+    jobs = [big] + small_jobs(6, seed=5, plastic_p=0.5)
+    b = Batch(jobs, precision, reorder=True).run()
+    for i, m in enumerate(jobs):
+        s = b.scalars(i)
+        assert s["done"] == 1.0 and s["err"] == 0.0
+        a = eng.Engine(m, precision, reorder=True).run()
+        for f in ("u", "v", "sigma", "qia"):
+            assert np.array_equal(b.get(i, f), a.get(f)), (i, f)
+        sa = a.scalars()
+        for k in ("nsteps", "time", "dt_i", "ke", "ie", "hg", "vis", "wext"):
+            assert s[k] == sa[k], (i, k)
+    print(f"[batch] a {big.n_elems}-element job and {len(jobs) - 1} small ones equal alone and in the mix",
+          flush=True)
+
+
 def test_mixed_batch_of_staged_and_gathered_jobs_alone_equals_in_mix():
     # without renumbering, a job with shuffled elements has element blocks
     # whose node lists overflow the LDS budget and gathers its corners from
