@@ -107,6 +107,7 @@ struct ensc_engine {
     bool layout_scratch = false;  // the current layout lives in scratch
     char* scratch_base = nullptr;
     uint32_t nblk_pipe = 0, nblk_e = 0, nblk_g = 0;  // element blocks: pipelined, staged, gathered (mixed)
+    bool pipe_bc = false;  // the pipelined launch holds tiles of a job with amplitude velocity BCs
     uint32_t nblk_e64 = 0, nblk_g64 = 0;             // element blocks: staged, gathered (FP64)
     uint32_t nblk_w64 = 0;  // element blocks of FP64 block-assembly jobs (Walsh-form body)
     uint32_t nblk_n_asm = 0, nblk_n = 0;      // node blocks, mixed jobs: block partials, per-corner forces
@@ -286,8 +287,10 @@ ensc_status relayout(ensc_engine* E) {
     //    node, plus the block assembly accumulators: 12 B mixed, 24 B FP64);
     //  - block assembly: staged and free of corner conflicts (FP64 jobs:
     //    unless the reference's summation order is asked for);
-    //  - pipelined pass: block assembly, no amplitude velocity BCs and tiles
-    //    of PIPE_BLK elements; each such tile with fewer than PIPE_CAP nodes
+    //  - pipelined pass: block assembly and tiles of PIPE_BLK elements; each
+    //    such tile with fewer than PIPE_CAP nodes (for a job with amplitude
+    //    velocity BCs, the pass overrides the staged velocities of the tiles
+    //    that hold a constrained node)
     //    (the rest of the job's tiles use the staged pass).  The split
     //    follows from the job's own tiles, so it too is the same in any mix.
     auto asm_wanted = [&](const Job& J) { return J.precision == ENSC_MIXED || E->fp64_order == 0; };
@@ -299,7 +302,7 @@ ensc_status relayout(ensc_engine* E) {
     };
     auto job_asm = [&](const Job& J) { return asm_wanted(J) && J.asm_capable && job_staged(J); };
     auto job_pipe = [&](const Job& J) {
-        return E->pipe_opt && job_asm(J) && !J.dev.dyn_bc && E->blk_e == PIPE_BLK;
+        return E->pipe_opt && job_asm(J) && E->blk_e == PIPE_BLK;
     };
     // block maps: (job, block of the job, staged range) per launched block,
     // in XCD order, grouped as described at ensc_engine::live
@@ -365,6 +368,8 @@ ensc_status relayout(ensc_engine* E) {
         b2n.insert(b2n.end(), r.begin(), r.end());
     }
     E->nblk_pipe = (uint32_t)lp.size();
+    E->pipe_bc = false;
+    for (const uint4& q : lp) E->pipe_bc |= E->slots[E->live[q.x]]->dev.dyn_bc != 0;
     E->nblk_e = (uint32_t)ls.size();
     E->nblk_g = (uint32_t)lg.size();
     E->nblk_e64 = (uint32_t)ls64.size();
@@ -497,8 +502,9 @@ hipError_t launch_phase(ensc_engine* E, int phase) {
             }
             const uint4* map = static_cast<const uint4*>(E->b2e_p);
             if (E->nblk_pipe)
-                hipLaunchKernelGGL(k_element_pipe<2>, dim3(std::min(E->nblk_pipe, E->pipe_grid)), dim3(PIPE_BLK), 0,
-                                   E->stream, jobs, map, E->nblk_pipe);
+                hipLaunchKernelGGL((E->pipe_bc ? k_element_pipe<2, true> : k_element_pipe<2, false>),
+                                   dim3(std::min(E->nblk_pipe, E->pipe_grid)), dim3(PIPE_BLK), 0, E->stream, jobs, map,
+                                   E->nblk_pipe);
             map += E->nblk_pipe;
             if (E->nblk_e)
                 hipLaunchKernelGGL(E->ke, dim3(E->nblk_e), dim3(E->blk_e), E->lds_bytes, s2, jobs, map);

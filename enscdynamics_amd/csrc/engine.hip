@@ -675,6 +675,29 @@ ensc_status upload_job(Job& J) {
     for (const auto& s : M.steps)
         for (const auto& b : s.vel)
             if (b.amp >= 0) D.dyn_bc = 1;
+    // element blocks holding a node that some step constrains (velocity or
+    // SPC lines): the blocks whose staged velocities the pipelined pass
+    // overrides on such jobs' step ticks
+    if (D.dyn_bc) {
+        std::vector<uint8_t> blk_bc(D.nblk_e, 0);
+        std::vector<uint8_t> bc(Nn, 0);
+        for (const auto& s : M.steps) {
+            for (const auto& b : s.vel)
+                for (uint32_t no : b.nodes) bc[J.n_old2new[no]] = 1;
+            for (const auto& b : s.spc)
+                if (b.cond == 1)
+                    for (uint32_t no : b.nodes) bc[J.n_old2new[no]] = 1;
+        }
+        for (uint32_t k = 0; k < D.nblk_e; ++k)
+            for (uint32_t q = I.blk_ptr[k]; q < I.blk_ptr[k + 1]; ++q)
+                if (bc[I.blk_nodes[q]]) {
+                    blk_bc[k] = 1;
+                    break;
+                }
+        uint8_t* d_bbc;
+        if ((st = upload(&d_bbc, blk_bc, blk_bc.size(), A))) return st;
+        D.blk_bc = d_bbc;
+    }
     J.grav.resize(M.steps.size());
     for (size_t s = 0; s < M.steps.size(); ++s) {
         J.grav[s].on = M.steps[s].grav_on;

@@ -161,6 +161,10 @@ struct JobDev {
         double inv_wos, cq2_wos;             // 1 / wos;  cvisq^2 / wos
     } mc;
     int32_t plastic, dyn_bc, ell_w, n_probe;
+    // dyn_bc jobs: 1 for each element block holding a node that some step
+    // constrains (the blocks whose staged velocities the pipelined pass
+    // overrides on step ticks); null otherwise
+    const uint8_t* blk_bc;
     int32_t block_asm, pad_asm;  // 1: block assembly (staged mixed job, conflict-free blocks)
     uint32_t ne, nn, ne_s, nn_s;
     uint32_t nblk_e, nblk_n;

@@ -496,7 +496,7 @@ __device__ __noinline__ double bc_override_c(const JobDev& J, uint32_t g, double
     if (mk & (BC_SPC0 << d)) v = 0.0;
     return v;
 }
-__device__ __noinline__ V3 bc_override(const JobDev& J, uint32_t g, double t, V3 in) {
+__device__ __forceinline__ V3 bc_override_inl(const JobDev& J, uint32_t g, double t, V3 in) {
 #pragma clang fp contract(off)
     const uint8_t mk = gld(J.bcmask, g);
     if (!mk) return in;
@@ -506,6 +506,9 @@ __device__ __noinline__ V3 bc_override(const JobDev& J, uint32_t g, double t, V3
         if (mk & (BC_SPC0 << d)) v[d] = 0.0;
     }
     return V3{v[0], v[1], v[2]};
+}
+__device__ __noinline__ V3 bc_override(const JobDev& J, uint32_t g, double t, V3 in) {
+    return bc_override_inl(J, g, t, in);
 }
 
 #define CORNER_FENCE
@@ -1096,9 +1099,10 @@ __global__ __launch_bounds__(BLK) void k_fin_element(const JobDev* __restrict__ 
 // is element_mixed_S's (same operations, same assembly order); the two
 // instantiations may contract multiply-adds differently, so each tile is
 // given one of the two passes by its job's properties and its own node
-// count, and keeps it in any batch.  Jobs with amplitude velocity BCs
-// (applied to staged velocities) and tiles of PIPE_CAP nodes or more use
-// k_element.
+// count, and keeps it in any batch.  Tiles of PIPE_CAP nodes or more use
+// k_element.  In jobs with amplitude velocity BCs, the tiles that hold a
+// constrained node (JobDev::blk_bc) override their staged velocities in LDS
+// on step ticks before the corner reads, as k_element does while staging.
 //
 // LDS per block is exactly the two buffers, 2 x 5 x 16 B x PIPE_CAP = 80 KiB,
 // so two blocks fit a CU's 160 KiB: once every corner value of the tile has
@@ -1147,6 +1151,22 @@ __device__ __forceinline__ void pipe_copy(const JobDev& J, uint32_t nbn, const u
         }
     }
 }
+// The BC override (bc_override) of the staged velocities of a tile's nodes:
+// thread slots tid and tid + PIPE_BLK, each reading its node id again.
+__device__ __forceinline__ void pipe_bc_patch(const JobDev& J, uint32_t n0, uint32_t nbn, double t, PipeBuf buf) {
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+        const uint32_t s = threadIdx.x + j * PIPE_BLK;
+        if (s < nbn) {
+            const uint32_t g = gld(J.blk_nodes, n0 + s);
+            const V3 v = bc_override_inl(J, g, t, V3{buf[2][s].a, buf[2][s].b, buf[3][s].a});
+            buf[2][s].a = v.x;
+            buf[2][s].b = v.y;
+            buf[3][s].a = v.z;
+        }
+    }
+}
+
 // Accumulators of the current tile in its [X0 X1] row, 4 floats per node,
 // written once every wave has read its corner values (each node's first
 // contribution stores, so they need no clearing).
@@ -1253,6 +1273,7 @@ __device__ __forceinline__ void pipe_flush(PipeOut& o) {
 // previous tile's stores; the barrier after it also means every wave is
 // done with `nxt` (the previous tile's buffer).  Returns false after the
 // block's last tile.
+template <bool BC>
 __device__ __forceinline__ bool pipe_tile(const JobDev* __restrict__ jobs, const uint4* __restrict__ blkmap,
                                           uint32_t ntiles, uint32_t& t, PipeBuf cur, PipeBuf nxt,
                                           const ElemState<float>& ES, const uint32_t g[2], ElemState<float>& ESn,
@@ -1286,6 +1307,14 @@ __device__ __forceinline__ bool pipe_tile(const JobDev* __restrict__ jobs, const
         // the assembly's zero slot (ACC_ZERO), before its first phase
         ENSC_LDS float* z = pipe_acc(cur) + (PIPE_CAP - 1) * 4;
         z[0] = z[1] = z[2] = 0.0f;
+    }
+    if (BC && J.dyn_bc && dc.kind == KIND_STEP && gld(J.blk_bc, lb)) {
+        // the velocity BCs at this step's time on the tile's constrained
+        // nodes, as the staged pass applies them while staging; before the
+        // next tile's copies are issued, so the wait for these loads does
+        // not also wait for those (uniform branch; few tiles take it)
+        pipe_bc_patch(J, bm.z, nbn, dc.time_n, cur);
+        lds_sync();
     }
     if (more) pipe_copy(jobs[bn.x], bn.w, g, nxt);
     if (dc.kind != KIND_NONE) {
@@ -1373,7 +1402,10 @@ __device__ __forceinline__ bool pipe_tile(const JobDev* __restrict__ jobs, const
     return more;
 }
 
-template <int WPE>
+// BC: the launch holds tiles of jobs with amplitude velocity BCs (the
+// override of constrained tiles' velocities is compiled in; without it the
+// tile loop is leaner).  Both instantiations run the same arithmetic.
+template <int WPE, bool BC>
 __global__ __launch_bounds__(PIPE_BLK) __attribute__((amdgpu_waves_per_eu(WPE))) void k_element_pipe(
     const JobDev* __restrict__ jobs, const uint4* __restrict__ blkmap, uint32_t ntiles) {
     __shared__ Pair s_bufA[5][PIPE_CAP];
@@ -1401,8 +1433,8 @@ __global__ __launch_bounds__(PIPE_BLK) __attribute__((amdgpu_waves_per_eu(WPE)))
     // the other is read)
     PipeOut out;
     out.pe = nullptr;
-    while (pipe_tile(jobs, blkmap, ntiles, t, bufA, bufB, esA, gB, esB, gA, out) &&
-           pipe_tile(jobs, blkmap, ntiles, t, bufB, bufA, esB, gA, esA, gB, out)) {
+    while (pipe_tile<BC>(jobs, blkmap, ntiles, t, bufA, bufB, esA, gB, esB, gA, out) &&
+           pipe_tile<BC>(jobs, blkmap, ntiles, t, bufB, bufA, esB, gA, esA, gB, out)) {
     }
     pipe_flush(out);  // the block's last tile
 }

@@ -24,15 +24,39 @@ def small_jobs(n, seed=7, plastic_p=0.0):
     return [decks.random_job(rng, 300, 4000, plastic_p, k_lo=150, k_hi=400, name=f"t{i}") for i in range(n)]
 
 
+def thin_slab(steps=300):
+    """One element thick: its 256-element blocks hold 578 nodes, more than
+    the pipelined pass stages, so its element blocks run the
+    one-block-per-tile pass."""
+    from enscdynamics_amd.model import Bc, Model, Step
+    h = 0.01
+    x0, conn, ijk = decks.block_mesh(32, 32, 1, 32 * h, 32 * h, h)
+    fixed = decks.face_nodes(ijk, 0, 0, 32)
+    rest = np.setdiff1d(np.arange(x0.shape[0], dtype=np.uint32), fixed)
+    m = Model(x0=x0, conn=conn, E=70e9, nu=0.33, rho=2700.0, name="thin_slab_32x32x1")
+    m.node_sets = {"Set-1": fixed, "Set-2": rest}
+    v = np.zeros_like(x0)
+    v[rest, 2] = 3.0
+    m.v_init = v
+    m.ic = [("Set-2", 2, 3.0)]
+    T = steps * decks.stable_dt0(m)
+    m.steps = [Step("Initial", 0.0, spc=[Bc("spc", fixed, 0, 2, 1, name="Set-1")]), Step("Step-1", T)]
+    m.total_time = T
+    return m
+
+
 @pytest.mark.parametrize("precision,staging", [(eng.FP64, 0), (eng.MIXED, 0), (eng.MIXED, 1)])
 def test_batch_alone_equals_in_mix(precision, staging):
     # staging 0: element blocks that fit the pipelined pass run it, the
     # others the one-block-per-tile pass, decided from each job alone;
     # staging 1: every job the one-block-per-tile pass
     jobs = small_jobs(10, seed=3, plastic_p=0.5)
-    # a ramped velocity BC (amplitude) keeps this job on the one-block-per-
-    # tile pass, so a mixed batch at staging 0 runs both passes per step
+    # a ramped velocity BC (amplitude): the pipelined pass overrides the
+    # staged velocities of its constrained tiles; the thin slab's blocks run
+    # the one-block-per-tile pass, so a mixed batch at staging 0 runs both
+    # passes per step
     jobs.append(decks.config2(nx=12, ny=6, nz=6, steps=300))
+    jobs.append(thin_slab())
     b = Batch(jobs, precision, reorder=True, staging=staging).run()
     for i, m in enumerate(jobs):
         s = b.scalars(i)
@@ -111,12 +135,14 @@ def test_batch_fp64_matches_oracle():
 
 
 def test_graph_replay_matches_eager():
-    # Both element passes run in the captured step (the ramped-BC job's
-    # blocks take the staged pass on the side stream beside the pipelined
-    # one).  The graph is replayed many times, with host reads and syncs
-    # between the replays, and every checkpoint equals the eager run.
+    # Both element passes run in the captured step (the thin slab's blocks
+    # take the staged pass on the side stream beside the pipelined one; the
+    # ramped-BC job's constrained tiles override their staged velocities).
+    # The graph is replayed many times, with host reads and syncs between
+    # the replays, and every checkpoint equals the eager run.
     jobs = small_jobs(6, seed=9)
     jobs.append(decks.config2(nx=12, ny=6, nz=6, steps=300))
+    jobs.append(thin_slab())
     a = Batch(jobs, eng.MIXED, graph_ticks=0).init()
     g = Batch(jobs, eng.MIXED, graph_ticks=16).init()
     a.set_current_step(1)
@@ -191,7 +217,8 @@ def test_autotune_leaves_the_solution_unchanged(precision):
     # for bit; kinetic energy and external work may differ in their last
     # bits (the nodal update's block size regroups their block sums).
     jobs = small_jobs(6, seed=31, plastic_p=0.5)
-    jobs.append(decks.config2(nx=12, ny=6, nz=6, steps=300))  # staged pass (amplitude BC)
+    jobs.append(decks.config2(nx=12, ny=6, nz=6, steps=300))  # amplitude BC
+    jobs.append(thin_slab())  # staged pass
     ref = Batch(jobs, precision, reorder=True).run()
     b = Batch(jobs, precision, reorder=True, graph_ticks=4)
     b.init()

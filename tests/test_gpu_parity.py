@@ -157,6 +157,31 @@ def test_reorder_is_bit_exact():
         assert rel(a.get(f), o.get(f), 1e-12) <= 1e-12, f
 
 
+def test_mixed_amplitude_bc_job_in_the_pipelined_pass():
+    # a velocity BC ramped by an amplitude: the pipelined pass overrides the
+    # staged velocities of the tiles that hold constrained nodes with the
+    # value at each step's time.  Against the one-block-per-tile pass
+    # (staging 1), which applies the same override while staging (the two
+    # differ by contraction only), and both against FP64
+    m = decks.config2(nx=40, ny=12, nz=12, steps=600)
+    a = eng.Engine(m, eng.FP64, reorder=True).run()
+    b = eng.Engine(m, eng.MIXED, reorder=True)
+    b.init()
+    b.set_current_step(1)
+    b.advance(3)
+    lay = b.layout()
+    b.advance(0)
+    assert lay["pipe_blocks"] == (m.n_elems + 255) // 256 and lay["elem_blocks"] == 0, lay
+    c = eng.Engine(m, eng.MIXED, reorder=True, staging=1).run()
+    assert b.scalars()["done"] == 1.0 and b.scalars()["err"] == 0.0
+    err_b, err_c = rel(b.get("u"), a.get("u")), rel(c.get("u"), a.get("u"))
+    err_bc = rel(b.get("u"), c.get("u"))
+    print(f"[amplitude BC pipelined] mixed vs fp64: pipelined {err_b:.2e}, staged {err_c:.2e}; "
+          f"pipelined vs staged {err_bc:.2e}", flush=True)
+    assert err_b <= TOL_MIXED and err_c <= TOL_MIXED
+    assert err_bc <= 1e-5
+
+
 def test_fp64_block_assembly_vs_reference_order():
     # default FP64 path (the Walsh-form element body, block partials summed
     # per node) against the reference's order: the same forces regrouped and
