@@ -112,9 +112,15 @@ static bool setup_geometry(const std::vector<double>& x0, const std::vector<uint
 }
 
 // Space-filling (Morton) element order from element centroids, quantized
-// with one scale for all three axes (the longest extent), so that runs of
-// consecutive elements are compact in space for any aspect ratio; nodes are
-// then numbered in first-touch order of the reordered elements.
+// with one scale for all three axes, so that runs of consecutive elements
+// are compact in space for any aspect ratio; nodes are then numbered in
+// first-touch order of the reordered elements.  The quantization cell is
+// the mean element size (the cube root of the bounding box volume per
+// element): on a structured mesh each element then has its own integer
+// cell, the Z-curve's cells line up with the element grid, and a run of 256
+// elements is an 8 x 8 x 4 box (405 nodes); a finer scale, unaligned with
+// the grid, cuts element columns and makes the runs ragged (the 100^3 cube:
+// 444 nodes per run on average; a 200 x 40 x 40 block: 470, some over 512).
 static void morton_order(const std::vector<double>& x0, const std::vector<uint32_t>& conn, size_t Ne, size_t Nn,
                          std::vector<uint32_t>& e_new2old, std::vector<uint32_t>& n_new2old) {
     double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
@@ -133,6 +139,10 @@ static void morton_order(const std::vector<double>& x0, const std::vector<uint32
         return v;
     };
     const double span = std::max(hi[0] - lo[0], std::max(hi[1] - lo[1], hi[2] - lo[2]));
+    double h = std::cbrt((hi[0] - lo[0]) * (hi[1] - lo[1]) * (hi[2] - lo[2]) / (double)std::max<size_t>(Ne, 1));
+    if (!(h > 0.0) || !std::isfinite(h)) h = 0.0;
+    // at most 2^21 cells along the longest extent (21 bits per axis)
+    const double cell = std::max(h, span / 2097151.0);
     std::vector<std::pair<uint64_t, uint32_t>> key(Ne);
     for (size_t e = 0; e < Ne; ++e) {
         double c[3] = {0, 0, 0};
@@ -140,8 +150,8 @@ static void morton_order(const std::vector<double>& x0, const std::vector<uint32
             for (int d = 0; d < 3; ++d) c[d] += x0[3 * (size_t)conn[8 * e + k] + d];
         uint64_t q[3];
         for (int d = 0; d < 3; ++d) {
-            const double t = span > 0 ? (c[d] / 8.0 - lo[d]) / span : 0.0;
-            q[d] = (uint64_t)std::min(2097151.0, std::max(0.0, t * 2097151.0));
+            const double t = cell > 0.0 ? std::floor((c[d] / 8.0 - lo[d]) / cell) : 0.0;
+            q[d] = (uint64_t)std::min(2097151.0, std::max(0.0, t));
         }
         key[e] = {spread(q[0]) | spread(q[1]) << 1 | spread(q[2]) << 2, (uint32_t)e};
     }
