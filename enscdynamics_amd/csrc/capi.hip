@@ -42,12 +42,12 @@ constexpr size_t STATIC_SCRATCH_BYTES = (size_t)4 << 20;
 // of the scalars only they write (FIN_ROLES); else one block per FIN_SLICE
 // partials, at most FIN_MAX_G, each folding a slice of every value, the last
 // to count deciding; per job FIN_JOB_BYTES of device memory hold the slices'
-// folds and the ticket.  Measured on configuration 3: the ticket (fences and
-// atomic) costs ~1 us more than the split fold saves at 4k partials, and
-// saves ~3 us at 7.8k (8 blocks); the four role blocks save ~2.5 us at 4k.
-// Small jobs keep one block: a 512-thread finalize block takes a whole CU
-// (160 VGPRs), so four per job in a batch of 256 small jobs ran in four
-// rounds (configuration 4: 7.8 -> 11 us per tick).
+// folds and the ticket.  Measured on the 1M-element J2 cube: the ticket
+// (fences and atomic) costs ~1 us more than the split fold saves at 4k
+// partials, and saves ~3 us at 7.8k (8 blocks); the four role blocks save
+// ~2.5 us at 4k.  Small jobs keep one block: a 512-thread finalize block
+// takes a whole CU (160 VGPRs), so four per job in a batch of 256 meshes of
+// 1k-50k elements ran in four rounds (7.8 -> 11 us per tick).
 constexpr uint32_t FIN_ROLE_MIN = 1024, FIN_ONE = 4096, FIN_SLICE = 1024, FIN_MAX_G = 8, FIN_ROLES = 4;
 constexpr size_t FIN_JOB_BYTES = FIN_MAX_G * 8 * sizeof(double) + 256;
 
@@ -676,7 +676,8 @@ ensc_status engine_new(const ensc_options* opts, std::unique_ptr<ensc_engine>& E
     E->reorder = o.reorder;
     E->staging_opt = o.staging;
     auto okblk = [](int b) { return b == 64 || b == 128 || b == 256; };
-    // defaults from block-size / occupancy sweeps on MI355X (configuration 3):
+    // defaults from block-size / occupancy sweeps on MI355X (the 1M-element
+    // J2 cube):
     // 256 elements per block (two waves per SIMD) for the mixed and the
     // default FP64 bodies; 128 for FP64 in the reference's order (its body
     // needs > 256 registers: one wave per SIMD)
